@@ -1,0 +1,498 @@
+/* CPU restatement of CTEMsoft's EBSD master-pattern beam loop: the oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, the package's smoke check and bench.py's CPU
+ * baseline load this file's library, and only as the checker. The product (the HIP library
+ * under ctemsoft_amd/) never links or calls it.
+ *
+ * Float32 steps are written with float variables in the reference's operation order so that
+ * results match the reference bit for bit where it computes in single precision; this file is
+ * compiled with -ffp-contract=off so that no multiply-add is fused. acosf/cosf/cos/sin/exp
+ * are the C library's, which is what the reference's compiled code calls.
+ */
+#include "ctem_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define CPI 3.141592653589793238 /* constants.f90:70 */
+
+/* CalcDotSingle 'r' (crystal.f90:411-429): dot_product(p, matmul(rmt, q)) with the double
+ * metric, result stored in single precision */
+static float dot_r_single(const float p[3], const float q[3], const double rmt[9]) {
+  double v[3];
+  for (int i = 0; i < 3; i++) {
+    double s = 0.0;
+    for (int j = 0; j < 3; j++) s += rmt[i * 3 + j] * (double)q[j];
+    v[i] = s;
+  }
+  double c = 0.0;
+  for (int i = 0; i < 3; i++) c += (double)p[i] * v[i];
+  return (float)c;
+}
+
+/* CalcLengthSingle 'r' (crystal.f90:564-577) */
+static float len_r_single(const float p[3], const double rmt[9]) {
+  return sqrtf(dot_r_single(p, p, rmt));
+}
+
+/* CalcAngleSingle 'r' (crystal.f90:632-666) */
+static float angle_r_single(const float p[3], const float q[3], const double rmt[9]) {
+  float x = dot_r_single(p, q, rmt);
+  float y = len_r_single(p, rmt);
+  float z = len_r_single(q, rmt);
+  float yz = y * z;
+  float t = x / yz;
+  if (t >= 1.0f) return 0.0f;
+  if (t <= -1.0f) return (float)CPI;
+  return acosf(t);
+}
+
+/* CalcsgSingle (diffraction.f90:649-674) */
+float orc_calcsg(const float g[3], const float k[3], const float fn[3], const double rmt[9]) {
+  float kpg[3], tkpg[3];
+  for (int i = 0; i < 3; i++) {
+    kpg[i] = k[i] + g[i];
+    float t2 = 2.0f * k[i];
+    tkpg[i] = t2 + g[i];
+  }
+  float xnom = -dot_r_single(g, tkpg, rmt);
+  float q1 = len_r_single(kpg, rmt);
+  float q2 = angle_r_single(kpg, fn, rmt);
+  float t = 2.0f * q1;
+  float xden = t * cosf(q2);
+  return xnom / xden;
+}
+
+/* Compute_DynMat 'BLOCHBETHE' beam selection and ordering (gvectors.f90:988-1120):
+ * |s_g| against weakcutoff*lambda|U_g| (strong) and cutoff*lambda|U_g| (weak), with the
+ * sg cutoff for double-diffraction reflections; 000 is always strong and first. */
+int orc_select_beams(int nref, const int32_t* hkl, const double* ucg, const uint8_t* dbdiff,
+                     double mlambda, float weakcutoff, float cutoff, float sgcutoff,
+                     const double rmt[9], const double kvec[3], int32_t* strong,
+                     float* strongsg, int32_t* nns, int32_t* weak, float* weaksg,
+                     int32_t* nnw) {
+  float k[3] = {(float)kvec[0], (float)kvec[1], (float)kvec[2]}; /* sngl(kk), DynFN */
+  int ns = 0, nw = 0;
+  strong[ns] = 0;
+  strongsg[ns] = 0.0f;
+  ns++;
+  for (int ig = 1; ig < nref; ig++) {
+    float g[3] = {(float)hkl[3 * ig], (float)hkl[3 * ig + 1], (float)hkl[3 * ig + 2]};
+    float sg = orc_calcsg(g, k, k, rmt);
+    float sgp = fabsf(sg);
+    float lug = (float)(cabs(ucg[2 * ig] + I * ucg[2 * ig + 1]) * mlambda);
+    float cut1 = cutoff * lug;
+    float cut2 = weakcutoff * lug;
+    if (dbdiff[ig]) {
+      if (sgp <= sgcutoff) {
+        strong[ns] = ig;
+        strongsg[ns] = sg;
+        ns++;
+      }
+    } else if (sgp <= cut1) {
+      if (sgp <= cut2) {
+        strong[ns] = ig;
+        strongsg[ns] = sg;
+        ns++;
+      } else {
+        weak[nw] = ig;
+        weaksg[nw] = sg;
+        nw++;
+      }
+    }
+  }
+  *nns = ns;
+  *nnw = nw;
+  return 0;
+}
+
+static inline double complex lut_at(const double* lut, const int32_t lo[3], const int32_t dim[3],
+                                    int h, int k, int l) {
+  size_t i = (size_t)(h - lo[0]) + (size_t)dim[0] * ((size_t)(k - lo[1]) + (size_t)dim[1] * (size_t)(l - lo[2]));
+  return lut[2 * i] + I * lut[2 * i + 1];
+}
+
+/* Bethe-corrected DynMat (gvectors.f90:1123-1173) */
+int orc_dynmat(int nns, const int32_t* strong, const float* strongsg, int nnw,
+               const int32_t* weak, const float* weaksg, const int32_t* hkl,
+               const double* lut, const int32_t lo[3], const int32_t dim[3], double mlambda,
+               float dynupz, double* dynmat) {
+  for (int ir = 0; ir < nns; ir++) {
+    const int32_t* gr = hkl + 3 * strong[ir];
+    for (int ic = 0; ic < nns; ic++) {
+      const int32_t* gc = hkl + 3 * strong[ic];
+      double complex v;
+      if (ic != ir) {
+        v = lut_at(lut, lo, dim, gr[0] - gc[0], gr[1] - gc[1], gr[2] - gc[2]);
+        double wr = 0.0, wi = 0.0;
+        for (int iw = 0; iw < nnw; iw++) {
+          const int32_t* gw = hkl + 3 * weak[iw];
+          double complex a = lut_at(lut, lo, dim, gr[0] - gw[0], gr[1] - gw[1], gr[2] - gw[2]);
+          double complex b = lut_at(lut, lo, dim, gw[0] - gc[0], gw[1] - gc[1], gw[2] - gc[2]);
+          double pr = creal(a) * creal(b) - cimag(a) * cimag(b);
+          double pi = creal(a) * cimag(b) + cimag(a) * creal(b);
+          double inv = 1.0 / (double)weaksg[iw];
+          wr += pr * inv;
+          wi += pi * inv;
+        }
+        double c = 0.5 * mlambda;
+        v = (creal(v) - c * wr) + I * (cimag(v) - c * wi);
+      } else {
+        double s = 0.0;
+        for (int iw = 0; iw < nnw; iw++) {
+          const int32_t* gw = hkl + 3 * weak[iw];
+          double complex a = lut_at(lut, lo, dim, gr[0] - gw[0], gr[1] - gw[1], gr[2] - gw[2]);
+          double m = cabs(a);
+          s += m * m / (double)weaksg[iw];
+        }
+        s = s * mlambda / 2.0;
+        v = (2.0 * (double)strongsg[ir] / mlambda - s) + I * (double)dynupz;
+      }
+      dynmat[2 * (ir * nns + ic)] = creal(v);
+      dynmat[2 * (ir * nns + ic) + 1] = cimag(v);
+    }
+  }
+  return 0;
+}
+
+/* Sgh for one site (CTEMEBSDmaster.f90:418-449): sum over the orbit of
+ * cmplx(Z^2 DBWF) * cmplx(cos, sin), each factor and the product in float32, summed in
+ * double. No occupancy factor, as in the reference. */
+int orc_sgh(int nns, const int32_t* strong, const int32_t* hkl, const double rmt[9], int32_t z,
+            float b_nm2, int norbit, const double* orbit, double* sgh) {
+  const double tpi = 2.0 * CPI;
+  float zf = (float)z;
+  double znsq = (double)(zf * zf);
+  for (int ir = 0; ir < nns; ir++) {
+    for (int ic = 0; ic < nns; ic++) {
+      int kk[3];
+      for (int t = 0; t < 3; t++) kk[t] = hkl[3 * strong[ir] + t] - hkl[3 * strong[ic] + t];
+      float kf[3] = {(float)kk[0], (float)kk[1], (float)kk[2]};
+      float L = len_r_single(kf, rmt);
+      float L2 = L * L;
+      double kkl = (double)(0.25f * L2);
+      double sr = 0.0, si = 0.0;
+      for (int a = 0; a < norbit; a++) {
+        double s = 0.0;
+        for (int t = 0; t < 3; t++) s += (double)kk[t] * orbit[3 * a + t];
+        double arg = tpi * s;
+        double dbwf = exp(-((double)b_nm2 * kkl));
+        float pre = (float)(znsq * dbwf);
+        float c = (float)cos(arg), sn = (float)sin(arg);
+        float re = pre * c;
+        float im = pre * sn;
+        sr += (double)re;
+        si += (double)im;
+      }
+      sgh[2 * (ir * nns + ic)] = sr;
+      sgh[2 * (ir * nns + ic) + 1] = si;
+    }
+  }
+  return 0;
+}
+
+/* ---------------- extended-precision Lgh (scaling and squaring) ---------------------- */
+typedef long double complex lcx;
+
+static void lmatmul(int n, const lcx* a, const lcx* b, lcx* c) {
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) c[i * n + j] = 0;
+    for (int k = 0; k < n; k++) {
+      lcx aik = a[i * n + k];
+      if (aik == 0) continue;
+      const lcx* bk = b + k * n;
+      lcx* ci = c + i * n;
+      for (int j = 0; j < n; j++) ci[j] += aik * bk[j];
+    }
+  }
+}
+
+int orc_lgh(int n, const double* dynmat, double kn, double dz, double thick, int nz,
+            const float* lam, double* lgh) {
+  size_t nn = (size_t)n * n;
+  lcx* A = malloc(nn * sizeof(lcx));
+  lcx* E = malloc(nn * sizeof(lcx));
+  lcx* T = malloc(nn * sizeof(lcx));
+  lcx* L = calloc(nn, sizeof(lcx));
+  lcx* psi = malloc(n * sizeof(lcx));
+  lcx* tmp = malloc(n * sizeof(lcx));
+  if (!A || !E || !T || !L || !psi || !tmp) return -1;
+  /* A = 2 pi i dz DynMat / fl32(2 kn)  (CTEMEBSDmaster.f90:693,703-716) */
+  long double d2kn = (long double)(float)(2.0 * kn);
+  lcx pre = (2.0L * 3.141592653589793238462643383279L * (long double)dz / d2kn) * I;
+  long double norm1 = 0;
+  for (int j = 0; j < n; j++) {
+    long double cs = 0;
+    for (int i = 0; i < n; i++) {
+      A[i * n + j] = pre * ((long double)dynmat[2 * (i * n + j)] + I * (long double)dynmat[2 * (i * n + j) + 1]);
+      cs += cabsl(A[i * n + j]);
+    }
+    if (cs > norm1) norm1 = cs;
+  }
+  int s = 0;
+  while (norm1 > 0.25L) {
+    norm1 *= 0.5L;
+    s++;
+  }
+  long double sc = ldexpl(1.0L, -s);
+  for (size_t i = 0; i < nn; i++) A[i] *= sc;
+  /* Taylor series, Horner form, 18 terms: E = I + A(I + A/2(I + A/3(...))) */
+  for (size_t i = 0; i < nn; i++) E[i] = 0;
+  for (int i = 0; i < n; i++) E[i * n + i] = 1;
+  for (int m = 18; m >= 1; m--) {
+    lmatmul(n, A, E, T);
+    for (size_t i = 0; i < nn; i++) E[i] = T[i] / (long double)m;
+    for (int i = 0; i < n; i++) E[i * n + i] += 1;
+  }
+  for (int q = 0; q < s; q++) {
+    lmatmul(n, E, E, T);
+    memcpy(E, T, nn * sizeof(lcx));
+  }
+  /* depth recursion psi_{k+1} = E psi_k, psi_0 = e_1; accumulate lam_k psi_k psi_k^H */
+  for (int i = 0; i < n; i++) psi[i] = (i == 0) ? 1 : 0;
+  for (int k = 0; k < nz; k++) {
+    for (int i = 0; i < n; i++) {
+      lcx acc = 0;
+      for (int j = 0; j < n; j++) acc += E[i * n + j] * psi[j];
+      tmp[i] = acc;
+    }
+    memcpy(psi, tmp, n * sizeof(lcx));
+    long double w = (long double)lam[k];
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) L[i * n + j] += w * psi[i] * conjl(psi[j]);
+  }
+  long double dzt = (long double)dz / (long double)thick;
+  for (size_t i = 0; i < nn; i++) {
+    lcx v = L[i] * dzt;
+    lgh[2 * i] = (double)creall(v);
+    lgh[2 * i + 1] = (double)cimagl(v);
+  }
+  free(A); free(E); free(T); free(L); free(psi); free(tmp);
+  return 0;
+}
+
+double orc_lgh_dot_sgh(int n, const double* lgh, const double* sgh) {
+  double s = 0.0;
+  for (size_t i = 0; i < (size_t)n * n; i++) s += lgh[2 * i] * sgh[2 * i] - lgh[2 * i + 1] * sgh[2 * i + 1];
+  return s;
+}
+
+/* ---------------- RoscaLambert incident directions ----------------------------------- */
+static int get_sextant(double x, double y) { /* kvectors.f90:781-817 */
+  const double srt = (double)1.732050808f; /* a default-real literal in the reference */
+  double xx = fabs(x * srt);
+  if (y >= 0) {
+    if (y >= xx) return 0;
+    return (x > 0.0) ? 1 : 5;
+  }
+  if (fabs(y) >= xx) return 3;
+  return (x > 0.0) ? 2 : 4;
+}
+
+typedef struct {
+  int cap, n;
+  int32_t* ij;
+  double* k;
+  double* kn;
+  const double* dsm;
+  double mlambda;
+} klist;
+
+static void push_k(klist* L, int i, int j, const double kstar_in[3]) {
+  if (L->n >= L->cap) {
+    L->n++;
+    return;
+  }
+  double ks[3] = {kstar_in[0], kstar_in[1], kstar_in[2]};
+  double x = sqrt(ks[0] * ks[0] + ks[1] * ks[1] + ks[2] * ks[2]); /* NormVec 'c' */
+  if (x != 0.0)
+    for (int t = 0; t < 3; t++) ks[t] = ks[t] / x;
+  for (int t = 0; t < 3; t++) ks[t] = ks[t] / L->mlambda;
+  for (int a = 0; a < 3; a++) { /* matmul(transpose(dsm), kstar) */
+    double s = 0.0;
+    for (int b = 0; b < 3; b++) s += L->dsm[b * 3 + a] * ks[b];
+    L->k[3 * L->n + a] = s;
+  }
+  L->ij[2 * L->n] = i;
+  L->ij[2 * L->n + 1] = j;
+  L->kn[L->n] = 1.0 / L->mlambda;
+  L->n++;
+}
+
+/* AddkVector (kvectors.f90:839-947) */
+static void add_k(klist* L, double delta, int i, int j, int usehex) {
+  const double srt = 0.86602540, rtt = 1.7320508076, prea = 0.525037568, preb = 1.050075136,
+               prec = 0.90689968, pred = 2.09439510;
+  const double ipi = 1.0 / CPI;
+  double x, y;
+  if (usehex) {
+    float jh = (float)j * 0.5f;
+    x = (double)((float)i - jh) * delta;
+    y = (double)j * delta * srt;
+  } else {
+    x = i * delta;
+    y = j * delta;
+  }
+  if (i == 0 && j == 0) return;
+  double ks[3];
+  if (usehex) {
+    double XX = 0, YY = 0, xp, yp;
+    switch (get_sextant(x, y)) {
+      case 0: case 3:
+        XX = preb * y * cos(x * prec / y);
+        YY = preb * y * sin(x * prec / y);
+        break;
+      case 1: case 4:
+        xp = y + rtt * x;
+        yp = y * pred / xp;
+        XX = prea * xp * sin(yp);
+        YY = prea * xp * cos(yp);
+        break;
+      default:
+        xp = y - rtt * x;
+        yp = y * pred / xp;
+        XX = prea * xp * sin(yp);
+        YY = -prea * xp * cos(yp);
+        break;
+    }
+    double q = XX * XX + YY * YY;
+    ks[0] = 0.5 * XX * sqrt(4.0 - q);
+    ks[1] = 0.5 * YY * sqrt(4.0 - q);
+    ks[2] = 1.0 - 0.5 * q;
+    /* hexagonal -> square-array index; integer division as in the reference (:934) */
+    push_k(L, i - j / 2 + (j % 2) / 2, j, ks);
+  } else {
+    if (fabs(x) <= fabs(y)) {
+      double q = 2.0 * y * ipi * sqrt(CPI - y * y);
+      double ang = x * CPI * 0.25 / y;
+      ks[0] = q * sin(ang);
+      ks[1] = q * cos(ang);
+      ks[2] = 1.0 - 2.0 * y * y * ipi;
+    } else {
+      double q = 2.0 * x * ipi * sqrt(CPI - x * x);
+      double ang = y * CPI * 0.25 / x;
+      ks[0] = q * cos(ang);
+      ks[1] = q * sin(ang);
+      ks[2] = 1.0 - 2.0 * x * x * ipi;
+    }
+    push_k(L, i, j, ks);
+  }
+}
+
+/* Calckvectors 'RoscaLambert' (kvectors.f90:421-542) */
+int orc_kvectors(int npx, int npy, int isym, int usehex, const double dsm[9], double mlambda,
+                 int cap, int32_t* ij, double* k, double* kn) {
+  klist L = {cap, 0, ij, k, kn, dsm, mlambda};
+  double delta = usehex ? 2.0 * sqrt(CPI) / pow(3.0, 0.75) / (double)npx
+                        : sqrt(CPI * 0.5) / (double)npx;
+  double head[3] = {0.0, 0.0, 1.0};
+  push_k(&L, 0, 0, head);
+  switch (isym) {
+    case 1:
+      for (int j = -npy; j <= npy; j++)
+        for (int i = -npx; i <= npx; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 2:
+      for (int j = 0; j <= npy; j++)
+        for (int i = -npx; i <= npx; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 3: case 4: case 10:
+      for (int j = 0; j <= npy; j++)
+        for (int i = 0; i <= npx; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 5: case 11:
+      for (int i = 0; i <= npx; i++)
+        for (int j = 0; j <= i; j++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 6: case 7:
+      for (int j = 0; j <= npy; j++)
+        for (int i = 0; i <= npy; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 8:
+      for (int j = 0; j <= npy; j++)
+        for (int i = j; i <= npy; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 9:
+      for (int j = 0; j <= npx; j++)
+        for (int i = 2 * j; i <= npx; i++) add_k(&L, delta, i, j, usehex);
+      break;
+    case 12:
+      for (int j = 0; j <= npy / 2; j++)
+        for (int i = 2 * j; i <= npy; i++) {
+          add_k(&L, delta, i, j, usehex);
+          add_k(&L, delta, i - j, -j, usehex);
+        }
+      break;
+    default:
+      return -2;
+  }
+  return (L.n > cap) ? -1 : L.n;
+}
+
+/* Apply2DLaueSymmetry (Lambert.f90:1942-2046) */
+int orc_laue_equiv(int ipx, int ipy, int isym, int32_t* q) {
+#define SET(n, a, b) do { q[2 * (n)] = (a); q[2 * (n) + 1] = (b); } while (0)
+  memset(q, 0, 24 * sizeof(int32_t));
+  SET(0, ipx, ipy);
+  int ne = 1;
+  if (ipx == 0 && ipy == 0) return 1;
+  int fly = (int)floorf((float)ipy / 2.0f);
+#define FL(v) ((int)floorf((float)(v) / 2.0f))
+  float my;
+  switch (isym) {
+    case 2: SET(1, ipx, -ipy); ne = 2; break;
+    case 4: SET(1, -ipy, ipx); SET(2, ipy, -ipx); SET(3, -ipx, -ipy); ne = 4; break;
+    case 3: case 10: SET(1, -ipx, ipy); SET(2, ipx, -ipy); SET(3, -ipx, -ipy); ne = 4; break;
+    case 5: case 11:
+      SET(1, -ipx, ipy); SET(2, ipx, -ipy); SET(3, -ipx, -ipy); SET(4, ipy, ipx);
+      SET(5, -ipy, ipx); SET(6, ipy, -ipx); SET(7, -ipy, -ipx); ne = 8; break;
+    case 6: case 7:
+      SET(1, ipy - 2 * fly - FL(ipx + 3 * ipy - 3 * fly), ipx - ipy + fly);
+      SET(2, ipy - fly - FL(ipx - fly), -ipx - fly);
+      ne = 3; break;
+    case 13:
+      my = (float)(ipy % 2) / 2.0f;
+      SET(1, ipy - FL(ipx + fly), (int)((float)(ipx + ipy / 2) - my));
+      SET(2, ipy - 2 * fly - FL(ipx + 3 * ipy - 3 * fly), ipx - ipy + fly);
+      SET(3, ipx, -ipy);
+      SET(4, ipy - fly - FL(ipx - fly), -ipx - fly);
+      SET(5, -fly - FL(ipx + ipy - fly), (int)((float)(-ipx + ipy / 2) + my));
+      ne = 6; break;
+    case 8:
+      my = (float)(ipy % 2);
+      SET(1, ipx - fly - FL(ipx + 2 * ipy - 3 * fly), ipx + fly);
+      SET(2, ipy - 2 * fly - FL(ipx + 3 * ipy - 3 * fly), ipx - ipy + fly);
+      SET(3, (int)((float)(-ipx) + my), -ipy);
+      SET(4, ipy - fly - FL(ipx - fly), -ipx - fly);
+      SET(5, ipx + ipy - FL(ipx + ipy - fly), (int)((float)(-ipx + ipy / 2) + my / 2.0f));
+      ne = 6; break;
+    case 9:
+      my = (float)(ipy % 2) / 2.0f;
+      SET(1, fly + FL(ipx + ipy - fly), (int)((float)(ipx - ipy / 2) - my));
+      SET(2, ipx - fly - FL(ipx + 2 * ipy - 3 * fly), ipx + fly);
+      SET(3, ipx, -ipy);
+      SET(4, ipy - 2 * fly - FL(ipx + 3 * ipy - 3 * fly), ipx - ipy + fly);
+      SET(5, ipx - FL(ipx + 2 * ipy - fly), -ipx - fly);
+      SET(6, (int)((float)(-ipx) + 2.0f * my), -ipy);
+      SET(7, -fly - FL(ipx + ipy - fly), (int)((float)(-ipx + ipy / 2) + my));
+      SET(8, ipy - fly - FL(ipx - fly), -ipx - fly);
+      SET(9, (int)((float)(-ipx) + 2.0f * my), ipy);
+      SET(10, ipx + ipy - FL(ipx + ipy - fly), (int)(((float)(-2 * ipx + ipy) + 2.0f * my) / 2.0f));
+      SET(11, ipy - 2 * fly - FL(ipx - 3 * fly), ipx + fly);
+      ne = 12; break;
+    case 12:
+      my = (float)(ipy % 2);
+      SET(1, ipx + fly - FL(ipx - ipy + fly), ipx - ipy + fly);
+      SET(2, ipy - 2 * fly - FL(ipx + 3 * ipy - 3 * fly), ipx - ipy + fly);
+      SET(3, ipx - FL(ipx + 2 * ipy - fly), -ipx - fly);
+      SET(4, ipy - fly - FL(ipx - fly), -ipx - fly);
+      SET(5, (int)((float)(-ipx) + my), ipy);
+      ne = 6; break;
+    default: break;
+  }
+#undef SET
+#undef FL
+  return ne;
+}

@@ -1,0 +1,64 @@
+/* CPU restatement of CTEMsoft's EBSD master-pattern beam loop (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: this library is the checker that tests/, the package's
+ * smoke check and bench.py's CPU-baseline leg compare the HIP kernels against. It is never
+ * linked into, loaded by or called from the product library.
+ *
+ * Every function cites the reference routine it restates. Conventions: 3x3 metric and
+ * structure matrices are row-major (m[i*3+j] = M(i,j)); complex matrices are row-major
+ * interleaved (re,im) doubles, D[2*(i*n+j)] = Re M(i,j); the potential look-up table is in
+ * the reference's Fortran order LUT(h,k,l) with lower bounds lo[3] and extents dim[3].
+ */
+#ifndef CTEM_ORACLE_H
+#define CTEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* excitation error s_g in float32: CalcsgSingle, diffraction.f90:649-674 */
+float orc_calcsg(const float g[3], const float k[3], const float fn[3], const double rmt[9]);
+
+/* beam selection for one incident direction: Compute_DynMat 'BLOCHBETHE' selection and
+ * ordering, gvectors.f90:988-1120. Returns 0, fills strong/weak reflection-list indices
+ * (0-based, list order; strong[0] = 0 is the transmitted beam) and their float32 s_g. */
+int orc_select_beams(int nref, const int32_t* hkl, const double* ucg /* 2*nref */,
+                     const uint8_t* dbdiff /* per list entry */, double mlambda,
+                     float weakcutoff, float cutoff, float sgcutoff, const double rmt[9],
+                     const double kvec[3], int32_t* strong, float* strongsg, int32_t* nns,
+                     int32_t* weak, float* weaksg, int32_t* nnw);
+
+/* Bethe-corrected dynamical matrix (n x n complex): gvectors.f90:1123-1173 */
+int orc_dynmat(int nns, const int32_t* strong, const float* strongsg, int nnw,
+               const int32_t* weak, const float* weaksg, const int32_t* hkl,
+               const double* lut, const int32_t lo[3], const int32_t dim[3], double mlambda,
+               float dynupz, double* dynmat);
+
+/* structure matrix Sgh for one site: CTEMEBSDmaster.f90:418-449 (float32 terms) */
+int orc_sgh(int nns, const int32_t* strong, const int32_t* hkl, const double rmt[9],
+            int32_t z, float b_nm2, int norbit, const double* orbit /* norbit x 3 */,
+            double* sgh);
+
+/* depth-integrated Lgh = (dz/thick) sum_k lam_k psi_k psi_k^H with
+ * psi_k = expm(2 pi i k dz DynMat / fl32(2 kn)) e_1, evaluated in extended precision by
+ * scaling and squaring; equal to CalcLgh3 (CTEMEBSDmaster.f90:610-722) up to rounding. */
+int orc_lgh(int n, const double* dynmat, double kn, double dz, double thick, int nz,
+            const float* lam, double* lgh);
+
+/* Re sum_gh Lgh(g,h) Sgh(g,h): the double sum before the float32 store, :456-459 */
+double orc_lgh_dot_sgh(int n, const double* lgh, const double* sgh);
+
+/* RoscaLambert incident directions: kvectors.f90:421-542 (+AddkVector :839-947). Writes up
+ * to cap directions; returns numk (or -1 if cap is too small). */
+int orc_kvectors(int npx, int npy, int isym, int usehex, const double dsm[9], double mlambda,
+                 int cap, int32_t* ij /* 2*cap */, double* k /* 3*cap */, double* kn);
+
+/* Laue-equivalent Lambert pixels: Apply2DLaueSymmetry, Lambert.f90:1942-2046 */
+int orc_laue_equiv(int ipx, int ipy, int isym, int32_t* iequiv /* 2*12 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

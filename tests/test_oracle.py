@@ -1,0 +1,201 @@
+"""Pins for the oracle (oracle/ctem_oracle.c) before anything is compared against it.
+
+1. Against the compiled reference (golden fixtures from tests/golden/make_golden.py):
+   beam sets, DynMat, Lgh, k-vectors and the whole float32 master pattern.
+2. Against mathematics that fixes the answer independently of the reference: the one-beam
+   closed form, norm conservation without absorption, Hermitian PSD structure, a
+   library matrix exponential and eigendecomposition, and brute-force RK4 integration.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import load_golden
+
+FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"]
+
+
+def _split(fx, j, ns, nw):
+    so = np.concatenate([[0], np.cumsum(ns)])
+    wo = np.concatenate([[0], np.cumsum(nw)])
+    return (fx["beams_strong"][so[j]:so[j + 1]], fx["beams_strongsg"][so[j]:so[j + 1]],
+            fx["beams_weak"][wo[j]:wo[j + 1]], fx["beams_weaksg"][wo[j]:wo[j + 1]])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_beam_sets_bit_exact(name, oracle_mod):
+    fx = load_golden(name)
+    cr = oracle_mod.Crystal(fx)
+    ns, nw = fx["beams_nns"], fx["beams_nnw"]
+    bad = []
+    for j, k in enumerate(fx["beams_ik"]):
+        st, ssg, wk, wsg = oracle_mod.select_beams(cr, fx["kvec"][k])
+        rs, rssg, rw, rwsg = _split(fx, j, ns, nw)
+        ok = (np.array_equal(st, rs) and np.array_equal(wk, rw)
+              and np.array_equal(ssg.view(np.int32), rssg.view(np.int32))
+              and np.array_equal(wsg.view(np.int32), rwsg.view(np.int32)))
+        if not ok:
+            bad.append(int(k))
+    assert not bad, f"{len(bad)} directions with different beam sets: {bad[:10]}"
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_dynmat_and_lgh_match_reference(name, oracle_mod):
+    fx = load_golden(name)
+    cr = oracle_mod.Crystal(fx)
+    for j, k in enumerate(fx["dyn_ik"]):
+        st, ssg, wk, wsg = oracle_mod.select_beams(cr, fx["kvec"][k])
+        dm = oracle_mod.dynmat(cr, st, ssg, wk, wsg)
+        ref = fx[f"dyn{j}_dynmat"]
+        assert np.array_equal(dm, ref), f"DynMat differs at direction {k}"
+        L = oracle_mod.lgh(ref, fx["kn"][k], float(fx["dz"]), float(fx["thick"]), fx["lam"])
+        Lr = fx[f"dyn{j}_lgh"]
+        err = np.linalg.norm(L - Lr) / np.linalg.norm(Lr)
+        # the reference's Lgh comes from zgeev/zgetri, whose error is ~cond(C) * eps
+        assert err < 1e-12, (k, err)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_kvectors_bit_exact(name, oracle_mod):
+    fx = load_golden(name)
+    npx = int(fx["kij"][:, 0].max())
+    ij, k, kn = oracle_mod.kvectors(npx, npx, int(fx["isym"]), int(fx["usehex"]), fx["dsm"],
+                                    float(fx["mLambda"]))
+    assert np.array_equal(ij, fx["kij"])
+    assert np.array_equal(k, fx["kvec"])
+    assert np.array_equal(kn, fx["kn"])
+
+
+@pytest.mark.parametrize("name,nsample", [("ni_c0", None), ("ni_c0w", 12)])
+def test_master_pattern_bit_exact(name, nsample, oracle_mod):
+    """The float32 master pattern of the reference program, pixel for pixel."""
+    fx = load_golden(name)
+    cr = oracle_mod.Crystal(fx)
+    sr = fx["master_sr"][0, 0]
+    npx = (sr.shape[1] - 1) // 2
+    nat = sum(len(s["orbit"]) for s in cr.sites)
+    ks = range(len(fx["kij"])) if nsample is None else \
+        np.random.default_rng(7).choice(len(fx["kij"]), nsample, replace=False)
+    mism = 0
+    for kk in ks:
+        dots, _, _ = oracle_mod.master_direction(cr, fx["kvec"][kk], fx["kn"][kk], fx["lam"],
+                                                  float(fx["dz"]), float(fx["thick"]))
+        v = oracle_mod.svals_f32(dots[0], nat)
+        for a, b in oracle_mod.laue_equiv(int(fx["kij"][kk, 0]), int(fx["kij"][kk, 1]),
+                                          int(fx["isym"])):
+            mism += int(np.float32(v).view(np.int32) != sr[b + npx, a + npx].view(np.int32))
+    assert mism == 0
+
+
+# ------------------------------- mathematical pins ------------------------------------
+def _random_dynmat(n, rng, absorb=True, herm=False):
+    a = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 2.0
+    if herm:
+        a = 0.5 * (a + a.conj().T)
+    d = np.diag(rng.uniform(-40, 40, n)).astype(complex)
+    m = a + d
+    if absorb:
+        m = m + 1j * np.diag(np.full(n, 1.5))
+    return m
+
+
+def test_one_beam_closed_form(oracle_mod):
+    fx = load_golden("ni_c0")
+    cr = oracle_mod.Crystal(fx)
+    k = int(np.flatnonzero(fx["beams_nns"] == 1)[0])
+    k = int(fx["beams_ik"][k])
+    st, ssg, wk, wsg = oracle_mod.select_beams(cr, fx["kvec"][k])
+    assert len(st) == 1
+    dm = oracle_mod.dynmat(cr, st, ssg, wk, wsg)
+    lam, dz, thick, kn = fx["lam"].astype(float), float(fx["dz"]), float(fx["thick"]), fx["kn"][k]
+    d = float(np.float32(2.0 * kn))
+    z = dz * np.arange(1, len(lam) + 1)
+    closed = dz / thick * np.sum(lam * np.exp(-4 * np.pi * z * dm[0, 0].imag / d))
+    L = oracle_mod.lgh(dm, kn, dz, thick, lam)
+    assert abs(L[0, 0] - closed) / closed < 1e-14
+    assert abs(L[0, 0].imag) <= 1e-18 * abs(L[0, 0])
+
+
+def test_lgh_against_scipy_expm_and_eig(oracle_mod):
+    rng = np.random.default_rng(3)
+    for n in (2, 5, 17, 40):
+        m = _random_dynmat(n, rng)
+        kn, dz, thick = 116.5, 0.5, 25.0
+        lam = rng.uniform(0, 1, 50).astype(np.float32)
+        L = oracle_mod.lgh(m, kn, dz, thick, lam)
+        d = float(np.float32(2 * kn))
+        E = scipy.linalg.expm(2j * np.pi * dz * m / d)
+        psi = np.zeros(n, complex); psi[0] = 1
+        ref = np.zeros((n, n), complex)
+        for w in lam.astype(float):
+            psi = E @ psi
+            ref += w * np.outer(psi, psi.conj())
+        ref *= dz / thick
+        assert np.linalg.norm(L - ref) / np.linalg.norm(ref) < 1e-12
+        # eigen route as in CalcLgh3: psi_k = C exp(2 pi i z W) C^-1 e1
+        W, Cm = np.linalg.eig(m / d)
+        alpha = np.linalg.solve(Cm, np.eye(n)[:, 0])
+        ref2 = np.zeros((n, n), complex)
+        for i, w in enumerate(lam.astype(float)):
+            p = Cm @ (np.exp(2j * np.pi * dz * (i + 1) * W) * alpha)
+            ref2 += w * np.outer(p, p.conj())
+        ref2 *= dz / thick
+        assert np.linalg.norm(L - ref2) / np.linalg.norm(ref2) < 1e-10
+
+
+def test_lgh_hermitian_psd_and_trace(oracle_mod):
+    rng = np.random.default_rng(5)
+    n = 12
+    m = _random_dynmat(n, rng, absorb=False, herm=True)
+    lam = rng.uniform(0, 1, 30).astype(np.float32)
+    L = oracle_mod.lgh(m, 116.0, 1.0, 30.0, lam)
+    assert np.allclose(L, L.conj().T, atol=1e-15)
+    assert np.linalg.eigvalsh(L).min() > -1e-14
+    # no absorption, Hermitian DynMat: |psi_k| = 1, so tr L = (dz/thick) sum(lam)
+    assert abs(np.trace(L).real - np.sum(lam.astype(float)) / 30.0) < 1e-13
+
+
+def test_lgh_against_rk4(oracle_mod):
+    """Brute force: integrate d psi/dz = 2 pi i M' psi with many RK4 steps."""
+    rng = np.random.default_rng(11)
+    n = 3
+    m = _random_dynmat(n, rng)
+    kn, dz, thick = 100.0, 1.0, 4.0
+    lam = np.array([0.4, 0.3, 0.2, 0.1], np.float32)
+    d = float(np.float32(2 * kn))
+    A = 2j * np.pi * m / d
+    psi = np.zeros(n, complex); psi[0] = 1
+    h = dz / 4000
+    ref = np.zeros((n, n), complex)
+    for w in lam.astype(float):
+        for _ in range(4000):
+            k1 = A @ psi; k2 = A @ (psi + 0.5 * h * k1); k3 = A @ (psi + 0.5 * h * k2); k4 = A @ (psi + h * k3)
+            psi = psi + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        ref += w * np.outer(psi, psi.conj())
+    ref *= dz / thick
+    L = oracle_mod.lgh(m, kn, dz, thick, lam)
+    assert np.linalg.norm(L - ref) / np.linalg.norm(ref) < 1e-10
+
+
+@pytest.mark.parametrize("isym,order", [(2, 2), (3, 4), (4, 4), (5, 8), (11, 8)])
+def test_laue_equivalents_form_orbits(oracle_mod, isym, order):
+    """Square-grid Laue operations: the image set of a generic pixel has the group order
+    and is the same set from every member (closure)."""
+    q = oracle_mod.laue_equiv(7, 3, isym)
+    assert len(q) == order and len({tuple(x) for x in q}) == order
+    s = {tuple(x) for x in q}
+    for a, b in q:
+        assert {tuple(x) for x in oracle_mod.laue_equiv(int(a), int(b), isym)} == s
+
+
+def test_calcsg_ewald_identity(oracle_mod):
+    """With the foil normal along k the excitation error is (k^2 - |k+g|^2)/(2|k+g|cos a),
+    cos a = (k+g).k/(|k+g||k|); check against a float64 evaluation."""
+    rmt = np.diag([1 / 0.3**2] * 3)
+    k = np.array([0.3, 1.1, 38.0])
+    for g in ([1, 1, 0], [2, 0, 0], [1, -1, 2], [0, 0, -2]):
+        s = oracle_mod.calcsg(g, k, k, rmt)
+        kpg = (k + g) / 0.3; kk = k / 0.3
+        cosa = kpg @ kk / (np.linalg.norm(kpg) * np.linalg.norm(kk))
+        ref = (kk @ kk - kpg @ kpg) / (2 * np.linalg.norm(kpg) * cosa)
+        assert abs(s - ref) <= 2e-6 * max(1.0, abs(ref))
