@@ -1,0 +1,887 @@
+// C-ABI of the EBSD master-pattern path (include/ctem_ebsd.h): contexts, crystal upload,
+// incident directions, batched plans and the CalcLgh3 drop-in. Host orchestration only;
+// the arithmetic runs in the kernels of ctem_kernels.hip and ctem_mixed.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ctem_ebsd.h"
+#include "ctem_internal.h"
+#include "ctem_laue.h"
+#include "ctem_sg.h"
+
+using namespace ctem;
+
+namespace {
+
+thread_local std::string g_err;
+
+int32_t fail(int32_t code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(CTEM_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+const double kPi = 3.141592653589793238;
+
+// Device allocation owned by a context or plan.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+  int32_t ensure(size_t b) {
+    if (b <= bytes && p) return CTEM_OK;
+    release();
+    if (b == 0) return CTEM_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) { p = nullptr; return fail(CTEM_E_HIP, "hipMalloc(%zu): %s", b, hipGetErrorString(e)); }
+    bytes = b;
+    return CTEM_OK;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+template <class T>
+int32_t upload(DBuf& d, const T* h, size_t n, hipStream_t s) {
+  int32_t rc = d.ensure(n * sizeof(T));
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return CTEM_OK;
+}
+
+// Taylor coefficients 1/j!
+double inv_fact(int j) { double f = 1.0; for (int i = 2; i <= j; i++) f *= i; return 1.0 / f; }
+
+}  // namespace
+
+struct ctem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool have_crystal = false;
+  // crystal on the device
+  DBuf g, hkl, cut1, cut2, dbd, lut, slut, rmt;
+  int32_t nref = 0, nsite = 0, ncell = 0;
+  int32_t lut_lo[3] = {0, 0, 0}, lut_dim[3] = {0, 0, 0};
+  double mlambda = 0.0;
+  float dynupz = 0.0f, sgcutoff = 0.0f;
+  float nat = 0.0f;  // total orbit size, float(sum(nat))
+  std::vector<int32_t> h_hkl;
+  // scratch for the CalcLgh3 drop-in
+  DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+
+  DevRefl refl() const {
+    DevRefl R;
+    R.nref = nref; R.g = g.as<float>(); R.hkl = hkl.as<int32_t>(); R.cut1 = cut1.as<float>();
+    R.cut2 = cut2.as<float>(); R.dbd = dbd.as<uint8_t>(); R.sgcutoff = sgcutoff;
+    return R;
+  }
+  DevLut lutv() const {
+    DevLut L;
+    L.lut = lut.as<float2>(); L.slut = slut.as<double2>();
+    for (int i = 0; i < 3; i++) { L.lo[i] = lut_lo[i]; L.dim[i] = lut_dim[i]; }
+    L.ncell = ncell; L.nsite = nsite;
+    return L;
+  }
+};
+
+struct ctem_plan {
+  ctem_ctx* ctx = nullptr;
+  int32_t numk = 0, nz = 0;
+  double dz = 0, thick = 0;
+  DBuf k, kn, lam, ij;
+  // selection
+  bool selected = false;
+  DBuf nns, nnw, soff, woff, sidx, ssg, widx, wsg, err;
+  std::vector<int32_t> h_nns, h_nnw;
+  std::vector<int64_t> h_soff, h_woff;
+  // results
+  DBuf dots;
+  bool ran = false;
+  int32_t keep_dir = -1, keep_n = 0;
+  DBuf keep_lgh;
+  // work
+  DBuf A, W, Psi, sq, dirs, partial;
+  std::vector<int32_t> h_sq;
+  // timing of the last run
+  double t_total = 0, t_gemm = 0, f_gemm = 0;
+  int32_t n_gemm = 0;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+};
+
+// ------------------------------------------------------------------------------------------
+extern "C" const char* ctem_last_error(void) { return g_err.c_str(); }
+
+extern "C" int32_t ctem_ctx_create(int32_t device, ctem_ctx** out) {
+  if (!out) return fail(CTEM_E_ARG, "ctem_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(CTEM_E_HIP, "no HIP device available (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(CTEM_E_ARG, "device %d out of range (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  ctem_ctx* c = new ctem_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(CTEM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  (void)hipEventCreate(&c->ev[0]);
+  (void)hipEventCreate(&c->ev[1]);
+  *out = c;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_ctx_destroy(ctem_ctx* c) {
+  if (!c) return CTEM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (DBuf* b : {&c->g, &c->hkl, &c->cut1, &c->cut2, &c->dbd, &c->lut, &c->slut, &c->rmt, &c->s_A,
+                  &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
+                  &c->s_nns, &c->s_zero, &c->s_off, &c->s_part, &c->s_dots})
+    b->release();
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_device_info(ctem_ctx* c, char* name, int32_t name_len, int32_t* cu_count) {
+  if (!c) return fail(CTEM_E_ARG, "null context");
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, c->device));
+  if (name && name_len > 0) { snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName); }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// per-site structure-matrix table: S_p(d) = sum over the orbit of
+// cmplx(Z^2 DBWF) * cmplx(cos, sin) in float32, summed in double, for every d = g-h of the
+// LUT box (CTEMEBSDmaster.f90:418-449; DBWF = exp(-B |d|^2/4) with |d| from the float32
+// CalcLength, no occupancy factor).
+static void build_slut(const ctem_crystal* c, int ncell, std::vector<double>& out) {
+#pragma clang fp contract(off)
+  out.assign((size_t)2 * ncell * c->nsite, 0.0);
+  const double tpi = 2.0 * kPi;
+  size_t obase = 0;
+  for (int p = 0; p < c->nsite; p++) {
+    const float zf = (float)c->site_z[p];
+    const double znsq = (double)(zf * zf);
+    const int no = c->site_norbit[p];
+    const double* orb = c->orbit + 3 * obase;
+    for (int l = 0; l < c->lut_dim[2]; l++)
+      for (int kk = 0; kk < c->lut_dim[1]; kk++)
+        for (int h = 0; h < c->lut_dim[0]; h++) {
+          const int d[3] = {h + c->lut_lo[0], kk + c->lut_lo[1], l + c->lut_lo[2]};
+          const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+          const float len = std::sqrt(dot_r(df, df, c->rmt));
+          const float len2 = len * len;
+          const double kkl = (double)(0.25f * len2);
+          double sr = 0.0, si = 0.0;
+          for (int a = 0; a < no; a++) {
+            double s = 0.0;
+            for (int t = 0; t < 3; t++) s += (double)d[t] * orb[3 * a + t];
+            const double arg = tpi * s;
+            const double dbwf = std::exp(-((double)c->site_b[p] * kkl));
+            const float pre = (float)(znsq * dbwf);
+            double sd, cd;
+            ::sincos(arg, &sd, &cd);  // one sincos call, as the reference's compiled code makes
+            const float cs = (float)cd, sn = (float)sd;
+            const float re = pre * cs, im = pre * sn;
+            sr += (double)re;
+            si += (double)im;
+          }
+          const size_t cell = (size_t)h + (size_t)c->lut_dim[0] * ((size_t)kk + (size_t)c->lut_dim[1] * l);
+          out[2 * ((size_t)p * ncell + cell)] = sr;
+          out[2 * ((size_t)p * ncell + cell) + 1] = si;
+        }
+    obase += no;
+  }
+}
+
+extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
+#pragma clang fp contract(off)
+  if (!c || !x) return fail(CTEM_E_ARG, "ctem_upload_crystal: null argument");
+  if (x->nref < 1 || !x->hkl || !x->ucg || !x->dbdiff || !x->lut)
+    return fail(CTEM_E_ARG, "ctem_upload_crystal: empty reflection list or LUT");
+  if (x->hkl[0] != 0 || x->hkl[1] != 0 || x->hkl[2] != 0)
+    return fail(CTEM_E_ARG, "ctem_upload_crystal: the first list entry must be 000");
+  if (x->nsite < 0 || x->nsite > 4) return fail(CTEM_E_RANGE, "ctem_upload_crystal: 0..4 sites supported (got %d)", x->nsite);
+  for (int i = 0; i < 3; i++)
+    if (x->lut_dim[i] <= 0) return fail(CTEM_E_ARG, "ctem_upload_crystal: bad LUT extent");
+  HIPCHK(hipSetDevice(c->device));
+  const int n = x->nref;
+  const int64_t ncell64 = (int64_t)x->lut_dim[0] * x->lut_dim[1] * x->lut_dim[2];
+  if (ncell64 > (1ll << 30)) return fail(CTEM_E_RANGE, "LUT too large");
+  const int ncell = (int)ncell64;
+  // every difference of two list entries must fall inside the LUT box
+  int hmax[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) hmax[d] = std::max(hmax[d], std::abs(x->hkl[3 * i + d]));
+  for (int d = 0; d < 3; d++)
+    if (x->lut_lo[d] > -2 * hmax[d] || x->lut_lo[d] + x->lut_dim[d] - 1 < 2 * hmax[d])
+      return fail(CTEM_E_RANGE, "LUT box does not hold all differences g-h along axis %d", d);
+  std::vector<float> g(3 * n), cut1(n), cut2(n);
+  std::vector<uint8_t> dbd(n);
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) g[3 * i + d] = (float)x->hkl[3 * i + d];
+    // lUg = abs(Ucg)*mLambda stored in float32; cut1/cut2 in float32 (gvectors.f90:1029-1032)
+    const float lug = (float)(std::abs(std::complex<double>(x->ucg[2 * i], x->ucg[2 * i + 1])) * x->mlambda);
+    cut1[i] = x->cutoff * lug;
+    cut2[i] = x->weakcutoff * lug;
+    dbd[i] = x->dbdiff[i] ? 1 : 0;
+  }
+  std::vector<float> lutf(2 * (size_t)ncell);
+  for (size_t i = 0; i < 2 * (size_t)ncell; i++) lutf[i] = (float)x->lut[i];
+  std::vector<double> slut;
+  build_slut(x, ncell, slut);
+  int nat = 0;
+  for (int p = 0; p < x->nsite; p++) nat += x->site_norbit[p];
+  int32_t rc;
+  if ((rc = upload(c->g, g.data(), g.size(), c->stream))) return rc;
+  if ((rc = upload(c->hkl, x->hkl, (size_t)3 * n, c->stream))) return rc;
+  if ((rc = upload(c->cut1, cut1.data(), n, c->stream))) return rc;
+  if ((rc = upload(c->cut2, cut2.data(), n, c->stream))) return rc;
+  if ((rc = upload(c->dbd, dbd.data(), n, c->stream))) return rc;
+  if ((rc = upload(c->lut, lutf.data(), lutf.size(), c->stream))) return rc;
+  if ((rc = upload(c->slut, slut.data(), slut.size() ? slut.size() : 0, c->stream))) return rc;
+  if ((rc = upload(c->rmt, x->rmt, 9, c->stream))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nref = n;
+  c->nsite = x->nsite;
+  c->ncell = ncell;
+  for (int i = 0; i < 3; i++) { c->lut_lo[i] = x->lut_lo[i]; c->lut_dim[i] = x->lut_dim[i]; }
+  c->mlambda = x->mlambda;
+  c->dynupz = x->dynupz;
+  c->sgcutoff = x->sgcutoff;
+  c->nat = (float)nat;
+  c->h_hkl.assign(x->hkl, x->hkl + 3 * n);
+  c->have_crystal = true;
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// incident directions (kvectors.f90:421-542, 781-947)
+namespace {
+struct KList {
+  int32_t cap, n;
+  int32_t* ij; double* k; double* kn;
+  const double* dsm; double mlambda; int hemi;
+};
+
+void push_k(KList& L, int i, int j, const double ks_in[3]) {
+  if (L.n < L.cap && L.ij) {
+    double ks[3] = {ks_in[0], ks_in[1], L.hemi < 0 ? -ks_in[2] : ks_in[2]};
+    const double x = std::sqrt(ks[0] * ks[0] + ks[1] * ks[1] + ks[2] * ks[2]);
+    if (x != 0.0) for (int t = 0; t < 3; t++) ks[t] = ks[t] / x;
+    for (int t = 0; t < 3; t++) ks[t] = ks[t] / L.mlambda;
+    for (int a = 0; a < 3; a++) {
+      double s = 0.0;
+      for (int b = 0; b < 3; b++) s += L.dsm[b * 3 + a] * ks[b];
+      L.k[3 * L.n + a] = s;
+    }
+    L.ij[2 * L.n] = i;
+    L.ij[2 * L.n + 1] = j;
+    L.kn[L.n] = 1.0 / L.mlambda;
+  }
+  L.n++;
+}
+
+// The reference's compiled code evaluates sin and cos of one argument through a single
+// sincos call, which differs from separate sin and cos calls in the last bit for some
+// arguments; the same call is used here.
+inline void sincos_ref(double a, double* s, double* c) { ::sincos(a, s, c); }
+
+int sextant(double x, double y) {
+  const double srt = (double)1.732050808f;
+  const double xx = std::fabs(x * srt);
+  if (y >= 0) { if (y >= xx) return 0; return (x > 0.0) ? 1 : 5; }
+  if (std::fabs(y) >= xx) return 3;
+  return (x > 0.0) ? 2 : 4;
+}
+
+void add_k(KList& L, double delta, int i, int j, int usehex) {
+#pragma clang fp contract(off)
+  const double srt = 0.86602540, rtt = 1.7320508076, prea = 0.525037568, preb = 1.050075136,
+               prec = 0.90689968, pred = 2.09439510, ipi = 1.0 / kPi;
+  if (i == 0 && j == 0) return;
+  double x, y, ks[3];
+  if (usehex) {
+    const float jh = (float)j * 0.5f;
+    x = (double)((float)i - jh) * delta;
+    y = (double)j * delta * srt;
+    double XX, YY, xp, yp, sn, cs;
+    switch (sextant(x, y)) {
+      case 0: case 3: sincos_ref(x * prec / y, &sn, &cs); XX = preb * y * cs; YY = preb * y * sn; break;
+      case 1: case 4: xp = y + rtt * x; yp = y * pred / xp; sincos_ref(yp, &sn, &cs); XX = prea * xp * sn; YY = prea * xp * cs; break;
+      default: xp = y - rtt * x; yp = y * pred / xp; sincos_ref(yp, &sn, &cs); XX = prea * xp * sn; YY = -prea * xp * cs; break;
+    }
+    const double q = XX * XX + YY * YY;
+    ks[0] = 0.5 * XX * std::sqrt(4.0 - q);
+    ks[1] = 0.5 * YY * std::sqrt(4.0 - q);
+    ks[2] = 1.0 - 0.5 * q;
+    push_k(L, i - j / 2 + (j % 2) / 2, j, ks);
+  } else {
+    x = i * delta;
+    y = j * delta;
+    double sn, cs;
+    if (std::fabs(x) <= std::fabs(y)) {
+      const double q = 2.0 * y * ipi * std::sqrt(kPi - y * y);
+      sincos_ref(x * kPi * 0.25 / y, &sn, &cs);
+      ks[0] = q * sn; ks[1] = q * cs; ks[2] = 1.0 - 2.0 * y * y * ipi;
+    } else {
+      const double q = 2.0 * x * ipi * std::sqrt(kPi - x * x);
+      sincos_ref(y * kPi * 0.25 / x, &sn, &cs);
+      ks[0] = q * cs; ks[1] = q * sn; ks[2] = 1.0 - 2.0 * x * x * ipi;
+    }
+    push_k(L, i, j, ks);
+  }
+}
+}  // namespace
+
+extern "C" int32_t ctem_kvectors(int32_t npx, int32_t npy, int32_t isym, int32_t usehex,
+                                 int32_t hemisphere, const double dsm[9], double mlambda,
+                                 int32_t cap, int32_t* ij, double* k, double* kn, int32_t* numk) {
+  if (npx < 1 || npy < 1 || !dsm || !numk || !(mlambda > 0) || (hemisphere != 1 && hemisphere != -1))
+    return fail(CTEM_E_ARG, "ctem_kvectors: bad arguments");
+  if ((ij || k || kn) && !(ij && k && kn)) return fail(CTEM_E_ARG, "ctem_kvectors: ij, k, kn must all be given");
+  KList L{ij ? cap : 0, 0, ij, k, kn, dsm, mlambda, hemisphere};
+  const double delta = usehex ? 2.0 * std::sqrt(kPi) / std::pow(3.0, 0.75) / (double)npx
+                              : std::sqrt(kPi * 0.5) / (double)npx;
+  const double head[3] = {0.0, 0.0, 1.0};
+  push_k(L, 0, 0, head);
+  switch (isym) {
+    case 1: for (int j = -npy; j <= npy; j++) for (int i = -npx; i <= npx; i++) add_k(L, delta, i, j, usehex); break;
+    case 2: for (int j = 0; j <= npy; j++) for (int i = -npx; i <= npx; i++) add_k(L, delta, i, j, usehex); break;
+    case 3: case 4: case 10: for (int j = 0; j <= npy; j++) for (int i = 0; i <= npx; i++) add_k(L, delta, i, j, usehex); break;
+    case 5: case 11: for (int i = 0; i <= npx; i++) for (int j = 0; j <= i; j++) add_k(L, delta, i, j, usehex); break;
+    case 6: case 7: for (int j = 0; j <= npy; j++) for (int i = 0; i <= npy; i++) add_k(L, delta, i, j, usehex); break;
+    case 8: for (int j = 0; j <= npy; j++) for (int i = j; i <= npy; i++) add_k(L, delta, i, j, usehex); break;
+    case 9: for (int j = 0; j <= npx; j++) for (int i = 2 * j; i <= npx; i++) add_k(L, delta, i, j, usehex); break;
+    case 12:
+      for (int j = 0; j <= npy / 2; j++)
+        for (int i = 2 * j; i <= npy; i++) { add_k(L, delta, i, j, usehex); add_k(L, delta, i - j, -j, usehex); }
+      break;
+    default: return fail(CTEM_E_ARG, "ctem_kvectors: Laue group %d not in 1..12", isym);
+  }
+  *numk = L.n;
+  if (ij && L.n > cap) return fail(CTEM_E_ARG, "ctem_kvectors: capacity %d < %d directions", cap, L.n);
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_laue_equiv(int32_t ipx, int32_t ipy, int32_t isym, int32_t* iequiv, int32_t* nequiv) {
+  if (!iequiv || !nequiv) return fail(CTEM_E_ARG, "ctem_laue_equiv: null output");
+  int32_t q[24] = {0};
+  const int ne = laue_equiv(ipx, ipy, isym, q);
+  memcpy(iequiv, q, sizeof q);
+  *nequiv = ne;
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// plans
+extern "C" int32_t ctem_plan_create(ctem_ctx* c, int32_t numk, const double* k, const double* kn,
+                                    int32_t nz, const float* lam, double dz, double thick,
+                                    ctem_plan** out) {
+  if (!c || !out) return fail(CTEM_E_ARG, "ctem_plan_create: null argument");
+  *out = nullptr;
+  if (numk < 0 || (numk > 0 && (!k || !kn))) return fail(CTEM_E_ARG, "ctem_plan_create: bad direction list");
+  if (nz < 1 || nz > 4096 || !lam) return fail(CTEM_E_ARG, "ctem_plan_create: need 1..4096 depth weights");
+  if (!(dz > 0) || !(thick > 0)) return fail(CTEM_E_ARG, "ctem_plan_create: dz and thick must be positive");
+  for (int64_t i = 0; i < 3ll * numk; i++)
+    if (!std::isfinite(k[i])) return fail(CTEM_E_ARG, "ctem_plan_create: non-finite wave vector");
+  HIPCHK(hipSetDevice(c->device));
+  ctem_plan* p = new ctem_plan();
+  p->ctx = c;
+  p->numk = numk; p->nz = nz; p->dz = dz; p->thick = thick;
+  std::vector<double> lamd(lam, lam + nz);
+  int32_t rc;
+  if ((rc = upload(p->k, k, (size_t)3 * numk, c->stream)) || (rc = upload(p->kn, kn, (size_t)numk, c->stream)) ||
+      (rc = upload(p->lam, lamd.data(), (size_t)nz, c->stream))) {
+    delete p;
+    return rc;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { delete p; return fail(CTEM_E_HIP, "plan upload failed"); }
+  *out = p;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
+  if (!p) return CTEM_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
+                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->A, &p->W, &p->Psi,
+                  &p->sq, &p->dirs, &p->partial})
+    b->release();
+  for (auto e : p->evpool) (void)hipEventDestroy(e);
+  delete p;
+  return CTEM_OK;
+}
+
+static int32_t check_err_flag(DBuf& err, hipStream_t s, const char* what) {
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, err.p, sizeof h, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h & 1) return fail(CTEM_E_RANGE, "%s: an excitation-error angle left the reproduced cosf range (|g| too large for k)", what);
+  if (h & 2) return fail(CTEM_E_NUMERIC, "%s: non-finite dynamical matrix", what);
+  if (h & 4) return fail(CTEM_E_NUMERIC, "%s: matrix norm needs more squarings than allowed", what);
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw_out) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  ctem_ctx* c = p->ctx;
+  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_select: no crystal uploaded");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int nk = p->numk;
+  int32_t rc;
+  if ((rc = p->nns.ensure(4 * (size_t)std::max(nk, 1))) || (rc = p->nnw.ensure(4 * (size_t)std::max(nk, 1))) ||
+      (rc = p->err.ensure(4)) || (rc = p->soff.ensure(8 * (size_t)std::max(nk, 1))) ||
+      (rc = p->woff.ensure(8 * (size_t)std::max(nk, 1))))
+    return rc;
+  HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
+  SelectOut o{};
+  o.nns = p->nns.as<int32_t>(); o.nnw = p->nnw.as<int32_t>(); o.err = p->err.as<int32_t>();
+  DevRefl R = c->refl();
+  launch_select(s, p->k.as<double>(), nk, R, c->rmt.as<double>(), o, 0);
+  HIPCHK(hipGetLastError());
+  if ((rc = check_err_flag(p->err, s, "beam selection"))) return rc;
+  p->h_nns.resize(nk); p->h_nnw.resize(nk);
+  if (nk) {
+    HIPCHK(hipMemcpyAsync(p->h_nns.data(), p->nns.p, 4 * (size_t)nk, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(p->h_nnw.data(), p->nnw.p, 4 * (size_t)nk, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  p->h_soff.resize(nk); p->h_woff.resize(nk);
+  int64_t ts = 0, tw = 0;
+  for (int i = 0; i < nk; i++) { p->h_soff[i] = ts; p->h_woff[i] = tw; ts += p->h_nns[i]; tw += p->h_nnw[i]; }
+  if ((rc = upload(p->soff, p->h_soff.data(), nk, s)) || (rc = upload(p->woff, p->h_woff.data(), nk, s)) ||
+      (rc = p->sidx.ensure(4 * (size_t)std::max<int64_t>(ts, 1))) || (rc = p->ssg.ensure(4 * (size_t)std::max<int64_t>(ts, 1))) ||
+      (rc = p->widx.ensure(4 * (size_t)std::max<int64_t>(tw, 1))) || (rc = p->wsg.ensure(4 * (size_t)std::max<int64_t>(tw, 1))))
+    return rc;
+  o.soff = p->soff.as<int64_t>(); o.woff = p->woff.as<int64_t>();
+  o.sidx = p->sidx.as<int32_t>(); o.ssg = p->ssg.as<float>();
+  o.widx = p->widx.as<int32_t>(); o.wsg = p->wsg.as<float>();
+  launch_select(s, p->k.as<double>(), nk, R, c->rmt.as<double>(), o, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  p->selected = true;
+  if (nns_out) memcpy(nns_out, p->h_nns.data(), 4 * (size_t)nk);
+  if (nnw_out) memcpy(nnw_out, p->h_nnw.data(), 4 * (size_t)nk);
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_get_beams(ctem_plan* p, int32_t ik, int32_t* strong, float* strongsg,
+                                       int32_t* weak, float* weaksg) {
+  if (!p || !p->selected) return fail(CTEM_E_STATE, "ctem_plan_get_beams: run ctem_plan_select first");
+  if (ik < 0 || ik >= p->numk) return fail(CTEM_E_ARG, "direction %d out of range", ik);
+  hipStream_t s = p->ctx->stream;
+  const size_t ns = p->h_nns[ik], nw = p->h_nnw[ik];
+  if (strong) HIPCHK(hipMemcpyAsync(strong, p->sidx.as<int32_t>() + p->h_soff[ik], 4 * ns, hipMemcpyDeviceToHost, s));
+  if (strongsg) HIPCHK(hipMemcpyAsync(strongsg, p->ssg.as<float>() + p->h_soff[ik], 4 * ns, hipMemcpyDeviceToHost, s));
+  if (weak && nw) HIPCHK(hipMemcpyAsync(weak, p->widx.as<int32_t>() + p->h_woff[ik], 4 * nw, hipMemcpyDeviceToHost, s));
+  if (weaksg && nw) HIPCHK(hipMemcpyAsync(weaksg, p->wsg.as<float>() + p->h_woff[ik], 4 * nw, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// the matrix exponential and depth stage, FP64
+namespace {
+
+struct GemmTimer {
+  ctem_plan* p;  // may be null (no timing)
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  void begin() {
+    if (!p) return;
+    if (p->evused + 2 > p->evpool.size()) {
+      hipEvent_t e1, e2;
+      (void)hipEventCreate(&e1); (void)hipEventCreate(&e2);
+      p->evpool.push_back(e1); p->evpool.push_back(e2);
+    }
+    a = p->evpool[p->evused++]; b = p->evpool[p->evused++];
+    (void)hipEventRecord(a, s);
+  }
+  void end() { if (p) (void)hipEventRecord(b, s); }
+};
+
+constexpr double kTheta64 = 0.9;   // ||A/2^s||_1 bound for the degree-16 Taylor polynomial
+constexpr int kSmax = 48;
+
+// A (nb matrices, npad^2 each) holds 2 pi i dz M' on entry. W holds 5*nb*npad^2 work space.
+// On exit Psi[b][:,k-1] = expm(A_b)^k e_1 for k = 1..nz. h_n[b] are the true sizes
+// (for the algorithmic flop count), h_sq receives the squaring counts.
+int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
+                        double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
+                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq) {
+  const int64_t per = (int64_t)npad * npad;
+  double2* A2 = W;
+  double2* A3 = W + per * nb;
+  double2* A4 = W + 2 * per * nb;
+  double2* H = W + 3 * per * nb;
+  double2* T = W + 4 * per * nb;
+  launch_norm_scale(s, npad, nb, A, kTheta64, kSmax, d_sq, d_err);
+  HIPCHK(hipGetLastError());
+  h_sq.resize(nb);
+  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int smax = 0;
+  for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  double n3 = 0.0;
+  for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
+
+  auto sq_gemm = [&](double2* C, const double2* X, const double2* Y, double betaI,
+                     std::initializer_list<std::pair<const double2*, double>> ms,
+                     const int32_t* active, int step, double flops) -> int32_t {
+    ZGemm g{};
+    g.m = npad; g.n = npad; g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = X; g.sx = per; g.Y = Y; g.sy = per; g.C = C; g.sc = per; g.sm = per;
+    g.alpha = 1.0; g.betaI = betaI;
+    int q = 0;
+    for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
+    g.active = active; g.step = step; g.nb = nb;
+    GemmTimer t{tp, s};
+    t.begin();
+    launch_zgemm(s, g);
+    t.end();
+    HIPCHK(hipGetLastError());
+    if (tp) { tp->f_gemm += flops; tp->n_gemm++; }
+    return CTEM_OK;
+  };
+  int32_t rc;
+  const double f3 = 8.0 * n3;
+  if ((rc = sq_gemm(A2, A, A, 0.0, {}, nullptr, 0, f3))) return rc;
+  if ((rc = sq_gemm(A3, A, A2, 0.0, {}, nullptr, 0, f3))) return rc;
+  if ((rc = sq_gemm(A4, A2, A2, 0.0, {}, nullptr, 0, f3))) return rc;
+  {
+    const double2* M[4] = {A, A2, A3, A4};
+    const double beta[4] = {inv_fact(13), inv_fact(14), inv_fact(15), inv_fact(16)};
+    launch_combo(s, npad, nb, H, inv_fact(12), M, beta);
+    HIPCHK(hipGetLastError());
+  }
+  if ((rc = sq_gemm(T, A4, H, inv_fact(8), {{A, inv_fact(9)}, {A2, inv_fact(10)}, {A3, inv_fact(11)}}, nullptr, 0, f3))) return rc;
+  if ((rc = sq_gemm(H, A4, T, inv_fact(4), {{A, inv_fact(5)}, {A2, inv_fact(6)}, {A3, inv_fact(7)}}, nullptr, 0, f3))) return rc;
+  if ((rc = sq_gemm(T, A4, H, 1.0, {{A, 1.0}, {A2, inv_fact(2)}, {A3, inv_fact(3)}}, nullptr, 0, f3))) return rc;
+  // squarings with per-matrix counts; the result ends in T (buffer 0) after the parity copy
+  double2* buf[2] = {T, H};
+  for (int i = 0; i < smax; i++) {
+    double fl = 0.0;
+    for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
+    if ((rc = sq_gemm(buf[(i + 1) & 1], buf[i & 1], buf[i & 1], 0.0, {}, d_sq, i, fl))) return rc;
+  }
+  launch_parity_copy(s, npad, nb, d_sq, H, T);
+  HIPCHK(hipGetLastError());
+  // depth: Psi[:,0] = E e_1, then doubling with E^(2^j)
+  launch_first_col(s, npad, nb, T, Psi, spsi);
+  HIPCHK(hipGetLastError());
+  double2* cur = T;
+  double2* alt = A2;
+  int m = 1;
+  while (m < nz) {
+    const int cnt = std::min(m, nz - m);
+    ZGemm g{};
+    g.m = npad; g.n = cnt; g.k = npad;
+    g.ldx = npad; g.ldy = npad; g.ldc = npad; g.ldm = npad;
+    g.X = cur; g.sx = per; g.Y = Psi; g.sy = spsi; g.C = Psi + (int64_t)m * npad; g.sc = spsi;
+    g.alpha = 1.0; g.nb = nb;
+    double n2 = 0.0;
+    for (int b = 0; b < nb; b++) n2 += (double)h_n[b] * h_n[b];
+    GemmTimer t{tp, s};
+    t.begin();
+    launch_zgemm(s, g);
+    t.end();
+    HIPCHK(hipGetLastError());
+    if (tp) { tp->f_gemm += 8.0 * n2 * cnt; tp->n_gemm++; }
+    if (m + cnt < nz) {
+      if ((rc = sq_gemm(alt, cur, cur, 0.0, {}, nullptr, 0, f3))) return rc;
+      std::swap(cur, alt);
+    }
+    m += cnt;
+  }
+  return CTEM_OK;
+}
+
+}  // namespace
+
+// bucket sizes: multiples of 32
+static int npad_of(int n) { return std::max(32, (n + 31) / 32 * 32); }
+
+extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
+    return fail(CTEM_E_ARG, "unknown precision %d", precision);
+  ctem_ctx* c = p->ctx;
+  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
+  if (keep_lgh_ik >= p->numk) return fail(CTEM_E_ARG, "keep_lgh_ik out of range");
+  int32_t rc;
+  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int nk = p->numk;
+  p->t_total = p->t_gemm = p->f_gemm = 0.0;
+  p->n_gemm = 0;
+  p->evused = 0;
+  if ((rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * std::max(1, c->nsite)))) return rc;
+  p->keep_dir = keep_lgh_ik;
+  p->keep_n = (keep_lgh_ik >= 0) ? p->h_nns[keep_lgh_ik] : 0;
+  if (p->keep_n && (rc = p->keep_lgh.ensure(16 * (size_t)p->keep_n * p->keep_n))) return rc;
+  if ((rc = p->err.ensure(4))) return rc;
+  HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
+  // buckets by padded size
+  std::map<int, std::vector<int32_t>> buckets;
+  for (int i = 0; i < nk; i++) buckets[npad_of(p->h_nns[i])].push_back(i);
+  std::vector<int32_t> order;
+  for (auto& kv : buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
+  if ((rc = upload(p->dirs, order.data(), order.size(), s))) return rc;
+  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
+             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
+  DevRefl R = c->refl();
+  DevLut L = c->lutv();
+  const double dzt = p->dz / p->thick;
+  hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+  HIPCHK(hipEventRecord(e0, s));
+  // work-memory budget per chunk
+  const size_t budget = (size_t)24 << 30;
+  size_t off = 0;
+  for (auto& kv : buckets) {
+    const int npad = kv.first;
+    const int nbk = (int)kv.second.size();
+    const int64_t per = (int64_t)npad * npad;
+    const int64_t spsi = (int64_t)npad * p->nz;
+    const size_t per_dir = (size_t)(6 * per + spsi) * sizeof(double2) + 64;
+    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
+    for (int c0 = 0; c0 < nbk; c0 += chunk) {
+      const int nb = std::min(chunk, nbk - c0);
+      if ((rc = p->A.ensure((size_t)nb * per * sizeof(double2))) || (rc = p->W.ensure((size_t)5 * nb * per * sizeof(double2))) ||
+          (rc = p->Psi.ensure((size_t)nb * spsi * sizeof(double2))) || (rc = p->sq.ensure(4 * (size_t)nb)) ||
+          (rc = p->partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * std::max(1, c->nsite))))
+        return rc;
+      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
+      std::vector<int> hn(nb);
+      for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
+      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->A.as<double2>());
+      HIPCHK(hipGetLastError());
+      if (precision == CTEM_PREC_FP64) {
+        rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
+                             p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq);
+      } else {
+        rc = fail(CTEM_E_ARG, "mixed precision not built into this library");
+      }
+      if (rc) return rc;
+      launch_lgh_dots(s, B, V, R, L, p->Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
+                      p->partial.as<double>(), p->dots.as<double>(), p->keep_dir, p->keep_lgh.as<double2>());
+      HIPCHK(hipGetLastError());
+    }
+    off += nbk;
+  }
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  p->t_total = ms;
+  for (size_t i = 0; i + 1 < p->evused; i += 2) {
+    float g = 0.0f;
+    HIPCHK(hipEventElapsedTime(&g, p->evpool[i], p->evpool[i + 1]));
+    p->t_gemm += g;
+  }
+  if ((rc = check_err_flag(p->err, s, "ctem_plan_run"))) return rc;
+  p->ran = true;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_get_dots(ctem_plan* p, double* dots) {
+  if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_get_dots: run the plan first");
+  if (!dots) return fail(CTEM_E_ARG, "null output");
+  const size_t n = (size_t)p->numk * p->ctx->nsite;
+  if (n) HIPCHK(hipMemcpy(dots, p->dots.p, 8 * n, hipMemcpyDeviceToHost));
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_get_svals(ctem_plan* p, float* svals) {
+#pragma clang fp contract(off)
+  if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_get_svals: run the plan first");
+  if (!svals) return fail(CTEM_E_ARG, "null output");
+  const size_t n = (size_t)p->numk * p->ctx->nsite;
+  std::vector<double> d(n);
+  if (n) HIPCHK(hipMemcpy(d.data(), p->dots.p, 8 * n, hipMemcpyDeviceToHost));
+  const float nat = p->ctx->nat;
+  for (size_t i = 0; i < n; i++) svals[i] = (float)d[i] / nat;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_get_lgh(ctem_plan* p, double* lgh, int32_t cap, int32_t* n) {
+  if (!p || !p->ran || p->keep_dir < 0) return fail(CTEM_E_STATE, "ctem_plan_get_lgh: no Lgh kept");
+  if (!n) return fail(CTEM_E_ARG, "null output");
+  *n = p->keep_n;
+  if (!lgh) return CTEM_OK;
+  if (cap < p->keep_n * p->keep_n) return fail(CTEM_E_ARG, "capacity too small");
+  HIPCHK(hipMemcpy(lgh, p->keep_lgh.p, 16 * (size_t)p->keep_n * p->keep_n, hipMemcpyDeviceToHost));
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_dynmat(ctem_plan* p, int32_t ik, double* dynmat, int32_t cap, int32_t* n) {
+  if (!p || !n) return fail(CTEM_E_ARG, "null argument");
+  int32_t rc;
+  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
+  if (ik < 0 || ik >= p->numk) return fail(CTEM_E_ARG, "direction out of range");
+  ctem_ctx* c = p->ctx;
+  hipStream_t s = c->stream;
+  const int nn = p->h_nns[ik];
+  *n = nn;
+  if (!dynmat) return CTEM_OK;
+  if (cap < nn * nn) return fail(CTEM_E_ARG, "capacity too small");
+  const int npad = npad_of(nn);
+  if ((rc = c->s_A.ensure(sizeof(double2) * (size_t)npad * npad)) || (rc = c->s_dirs.ensure(4))) return rc;
+  HIPCHK(hipMemcpyAsync(c->s_dirs.p, &ik, 4, hipMemcpyHostToDevice, s));
+  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
+             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
+  Bucket B{npad, 1, c->s_dirs.as<int32_t>()};
+  launch_dynmat(s, B, V, c->refl(), c->lutv(), c->mlambda, c->dynupz, c->s_A.as<double2>());
+  HIPCHK(hipGetLastError());
+  std::vector<double2> h((size_t)npad * npad);
+  HIPCHK(hipMemcpyAsync(h.data(), c->s_A.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int i = 0; i < nn; i++)
+    for (int j = 0; j < nn; j++) {
+      dynmat[2 * (i * nn + j)] = h[(size_t)j * npad + i].x;
+      dynmat[2 * (i * nn + j) + 1] = h[(size_t)j * npad + i].y;
+    }
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_scatter(ctem_plan* p, const int32_t* ij, int32_t npx, int32_t npy,
+                                     int32_t isym, float* sr) {
+  if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_scatter: run the plan first");
+  if (!ij || !sr || npx < 0 || npy < 0) return fail(CTEM_E_ARG, "bad arguments");
+  ctem_ctx* c = p->ctx;
+  hipStream_t s = c->stream;
+  const size_t plane = (size_t)(2 * npx + 1) * (2 * npy + 1);
+  const size_t tot = plane * std::max(1, c->nsite);
+  DBuf d_sr;
+  int32_t rc;
+  if ((rc = upload(p->ij, ij, (size_t)2 * p->numk, s)) || (rc = d_sr.ensure(4 * tot))) return rc;
+  HIPCHK(hipMemcpyAsync(d_sr.p, sr, 4 * tot, hipMemcpyHostToDevice, s));
+  launch_scatter(s, p->numk, p->ij.as<int32_t>(), p->dots.as<double>(), c->nsite, c->nat, npx, npy,
+                 isym, d_sr.as<float>());
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(sr, d_sr.p, 4 * tot, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  d_sr.release();
+  if (e != hipSuccess) return fail(CTEM_E_HIP, "scatter: %s", hipGetErrorString(e));
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_timing(ctem_plan* p, double* total_ms, double* gemm_ms, double* gemm_flops,
+                                    int32_t* gemm_launches) {
+  if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_timing: run the plan first");
+  if (total_ms) *total_ms = p->t_total;
+  if (gemm_ms) *gemm_ms = p->t_gemm;
+  if (gemm_flops) *gemm_flops = p->f_gemm;
+  if (gemm_launches) *gemm_launches = p->n_gemm;
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// CalcLgh3 drop-in (one direction, FP64 on the GPU)
+namespace {
+__global__ void k_scale_dmat(int nn, int npad, const double2* __restrict__ D, double pre,
+                             double2* __restrict__ A) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)npad * npad) return;
+  const int i = (int)(e % npad), j = (int)(e / npad);
+  double2 v = make_double2(0.0, 0.0);
+  if (i < nn && j < nn) {
+    const double2 d = D[(int64_t)j * nn + i];
+    v = make_double2(-pre * d.y, pre * d.x);
+  }
+  A[e] = v;
+}
+}  // namespace
+
+extern "C" int32_t ctem_calclgh3(ctem_ctx* c, const double* dmat, double* lgh, double thick, double kn,
+                                 int32_t nn, int32_t gzero, double depthstep, const float* lambdae,
+                                 int32_t izz) {
+  if (!c || !dmat || !lgh || !lambdae) return fail(CTEM_E_ARG, "ctem_calclgh3: null argument");
+  if (nn < 1 || izz < 1 || !(thick > 0) || !(depthstep > 0) || !(kn > 0))
+    return fail(CTEM_E_ARG, "ctem_calclgh3: bad sizes");
+  if (gzero != 1) return fail(CTEM_E_RANGE, "ctem_calclgh3: only gzero = 1 (the transmitted beam first) is supported");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int npad = npad_of(nn);
+  const int64_t per = (int64_t)npad * npad;
+  const int64_t spsi = (int64_t)npad * izz;
+  int32_t rc;
+  if ((rc = c->s_A.ensure(per * sizeof(double2))) || (rc = c->s_W.ensure(5 * per * sizeof(double2))) ||
+      (rc = c->s_Psi.ensure(spsi * sizeof(double2))) || (rc = c->s_sq.ensure(4)) || (rc = c->s_err.ensure(4)) ||
+      (rc = c->s_lgh.ensure(std::max<size_t>(16 * (size_t)nn * nn, per * sizeof(double2)))) ||
+      (rc = c->s_dirs.ensure(4)) || (rc = c->s_nns.ensure(8)) || (rc = c->s_zero.ensure(4 * (size_t)npad + 16)) ||
+      (rc = c->s_off.ensure(16)) || (rc = c->s_part.ensure(8 * (size_t)(npad / 32) * (npad / 32))) ||
+      (rc = c->s_dots.ensure(8)))
+    return rc;
+  std::vector<double> lamd(lambdae, lambdae + izz);
+  if ((rc = upload(c->s_lam, lamd.data(), lamd.size(), s))) return rc;
+  HIPCHK(hipMemcpyAsync(c->s_lgh.p, dmat, 16 * (size_t)nn * nn, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(c->s_err.p, 0, 4, s));
+  const double pre = 2.0 * kPi * depthstep / (double)(float)(2.0 * kn);
+  hipLaunchKernelGGL(k_scale_dmat, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, s, nn, npad,
+                     c->s_lgh.as<double2>(), pre, c->s_A.as<double2>());
+  HIPCHK(hipGetLastError());
+  std::vector<int> hn{nn};
+  std::vector<int32_t> hsq;
+  rc = expm_depth_fp64(nullptr, s, npad, 1, c->s_A.as<double2>(), c->s_W.as<double2>(), c->s_Psi.as<double2>(),
+                       spsi, izz, c->s_sq.as<int32_t>(), c->s_err.as<int32_t>(), hn, hsq);
+  if (rc) return rc;
+  // Lgh through the contraction kernel with no sites: a one-entry plan view
+  const int32_t zero4[4] = {0, 0, 0, 0};
+  const int32_t nnv[2] = {nn, 0};
+  const int64_t offs[2] = {0, 0};
+  HIPCHK(hipMemsetAsync(c->s_zero.p, 0, 4 * (size_t)npad + 16, s));
+  HIPCHK(hipMemcpyAsync(c->s_dirs.p, zero4, 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->s_nns.p, nnv, 8, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->s_off.p, offs, 16, hipMemcpyHostToDevice, s));
+  BeamView V{c->s_nns.as<int32_t>(), c->s_nns.as<int32_t>() + 1, c->s_off.as<int64_t>(), c->s_off.as<int64_t>() + 1,
+             c->s_zero.as<int32_t>(), nullptr, nullptr, nullptr};
+  DevRefl R{};
+  R.hkl = c->s_zero.as<int32_t>();
+  DevLut L{};
+  L.nsite = 0;
+  L.dim[0] = L.dim[1] = L.dim[2] = 1;
+  Bucket B{npad, 1, c->s_dirs.as<int32_t>()};
+  launch_lgh_dots(s, B, V, R, L, c->s_Psi.as<double2>(), spsi, izz, c->s_lam.as<double>(), depthstep / thick,
+                  c->s_part.as<double>(), c->s_dots.as<double>(), 0, c->s_lgh.as<double2>());
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nn * nn);
+  HIPCHK(hipMemcpyAsync(h.data(), c->s_lgh.p, 16 * (size_t)nn * nn, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if ((rc = check_err_flag(c->s_err, s, "ctem_calclgh3"))) return rc;
+  // row-major from the kernel -> column-major for the Fortran caller
+  for (int i = 0; i < nn; i++)
+    for (int j = 0; j < nn; j++) {
+      lgh[2 * ((size_t)j * nn + i)] = h[2 * ((size_t)i * nn + j)];
+      lgh[2 * ((size_t)j * nn + i) + 1] = h[2 * ((size_t)i * nn + j) + 1];
+    }
+  return CTEM_OK;
+}

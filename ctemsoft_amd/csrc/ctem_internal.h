@@ -1,0 +1,132 @@
+// Internal interfaces between the C-ABI layer (ctem_api.hip) and the kernels
+// (ctem_kernels.hip, ctem_mixed.hip). Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ctem {
+
+// Reflection table on the device, structure of arrays in master-list order.
+struct DevRefl {
+  int32_t nref;
+  const float* g;        // [3*nref] float(hkl)
+  const int32_t* hkl;    // [3*nref]
+  const float* cut1;     // [nref] cutoff * fl32(|Ucg| mLambda)
+  const float* cut2;     // [nref] weakcutoff * fl32(|Ucg| mLambda)
+  const uint8_t* dbd;    // [nref] double-diffraction flags
+  float sgcutoff;
+};
+
+struct DevLut {
+  const float2* lut;     // complex LUT values (float32, exactly the reference's values)
+  const double2* slut;   // [nsite * ncell] per-site structure-matrix table S_p(g-h)
+  int32_t lo[3], dim[3];
+  int32_t ncell;
+  int32_t nsite;
+};
+
+struct SelectOut {       // per-direction beam sets (CSR over directions)
+  int32_t* nns;          // [numk]
+  int32_t* nnw;          // [numk]
+  const int64_t* soff;   // [numk] offsets into sidx/ssg (fill pass)
+  const int64_t* woff;
+  int32_t* sidx; float* ssg;
+  int32_t* widx; float* wsg;
+  int32_t* err;          // set to 1 when an angle leaves the reproduced cosf branch
+};
+
+// Beam selection. fill=0 counts, fill=1 writes the lists.
+void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& R,
+                   const double* rmt9, const SelectOut& out, int fill);
+
+// Per-bucket descriptors for the dense stages. All matrices of a bucket are n_pad x n_pad,
+// column-major, with batch stride n_pad*n_pad; dirs[b] names the plan direction of slot b.
+struct Bucket {
+  int32_t npad;
+  int32_t nb;            // directions in this chunk
+  const int32_t* dirs;   // [nb] plan direction indices (device)
+};
+
+struct BeamView {        // read-only view of the selection lists
+  const int32_t* nns; const int32_t* nnw;
+  const int64_t* soff; const int64_t* woff;
+  const int32_t* sidx; const float* ssg;
+  const int32_t* widx; const float* wsg;
+};
+
+// A = (2 pi i dz / fl32(2 kn)) * DynMat, Bethe-corrected (gvectors.f90:1123-1173), FP64.
+void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                     const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                     double2* A);
+// same matrix, unscaled (DynMat itself), for parity checks
+void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                   const DevLut& L, double mlambda, float dynupz, double2* D);
+
+// ||A||_1 per matrix, squaring count s with ||A||_1 / 2^s <= theta, in-place scaling by 2^-s.
+// sq[b] receives s; err is set on a non-finite norm or s > smax.
+void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, double theta,
+                       int32_t smax, int32_t* sq, int32_t* err);
+
+// Batched complex GEMM C = alpha*X*Y + betaI*I + sum_j beta_j*M_j on FP64 MFMA.
+struct ZGemm {
+  int32_t m, n, k;       // C is m x n, X is m x k, Y is k x n (column-major)
+  int32_t ldx, ldy, ldc, ldm;
+  const double2* X; int64_t sx;
+  const double2* Y; int64_t sy;
+  double2* C; int64_t sc;
+  double alpha;
+  double betaI;
+  const double2* M[4]; double beta[4]; int64_t sm;
+  const int32_t* active; int32_t step;   // skip matrix b when step >= active[b]
+  int32_t nb;
+};
+void launch_zgemm(hipStream_t s, const ZGemm& g);
+
+// elementwise C = betaI*I + sum_j beta_j*M_j (all npad x npad)
+void launch_combo(hipStream_t s, int32_t npad, int32_t nb, double2* C, double betaI,
+                  const double2* const M[4], const double beta[4]);
+// copy matrices whose squaring count is odd from src to dst (so the result sits in dst)
+void launch_parity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
+                        const double2* src, double2* dst);
+// Psi[:,0] = E[:,0] per matrix; Psi has leading dimension npad and nzc columns
+void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
+                      int64_t spsi);
+
+// Lgh = (dz/thick) Psi diag(lam) Psi^H and the per-site contraction with S_p(g-h):
+// partial sums per 32x32 tile, reduced in fixed order into dots[dir*nsite + p].
+void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                     const DevLut& L, const double2* Psi, int64_t spsi, int32_t nz,
+                     const double* lam, double dzt, double* partial, double* dots,
+                     int32_t keep_dir, double2* keep_lgh);
+
+// scatter fl32 svals to the Laue-equivalent pixels of sr[nsite][2npy+1][2npx+1]
+void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,
+                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym, float* sr);
+
+// ---- mixed precision (ctem_mixed.hip) ----
+// Complex GEMM on split-BF16 MFMA with FP32 accumulation for FP32 complex matrices
+// (float2, column-major); same argument meaning as ZGemm.
+struct CGemm {
+  int32_t m, n, k;
+  int32_t ldx, ldy, ldc, ldm;
+  const float2* X; int64_t sx;
+  const float2* Y; int64_t sy;
+  float2* C; int64_t sc;
+  float alpha;
+  float betaI;
+  const float2* M[4]; float beta[4]; int64_t sm;
+  const int32_t* active; int32_t step;
+  int32_t nb;
+  int32_t terms;         // split-BF16 product terms: 3 (hi*hi+hi*lo+lo*hi) or 6 (three-way)
+};
+void launch_cgemm(hipStream_t s, const CGemm& g);
+void launch_z2c(hipStream_t s, int64_t count, const double2* src, float2* dst);
+void launch_c2z(hipStream_t s, int64_t count, const float2* src, double2* dst);
+void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
+                   const float2* const M[4], const float beta[4]);
+void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
+                         const float2* src, float2* dst);
+void launch_cfirst_col(hipStream_t s, int32_t npad, int32_t nb, const float2* E, float2* Psi,
+                       int64_t spsi);
+
+}  // namespace ctem

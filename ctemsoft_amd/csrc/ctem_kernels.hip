@@ -1,0 +1,547 @@
+// HIP kernels of the EBSD master-pattern path for gfx950 (CDNA4, wave64).
+//
+//   k_select       Bethe beam selection and ordering, one workgroup per direction
+//                  (gvectors.f90:988-1120), bit-exact float32 excitation errors
+//   k_assemble     Bethe-corrected DynMat (gvectors.f90:1123-1173): LUT gathers staged in
+//                  LDS, weak-beam sum as a tiled complex product
+//   k_norm_scale   1-norm, squaring count and power-of-two scaling per matrix
+//   k_zgemm        batched complex GEMM on v_mfma_f64_16x16x4_f64 with a fused polynomial
+//                  epilogue (Taylor / Paterson-Stockmeyer steps)
+//   k_lgh_dots     Lgh = (dz/thick) Psi diag(lam) Psi^H and Re sum Lgh (.) S_p (CTEMEBSDmaster
+//                  .f90:456, with Lgh as in :697-720) in fixed-order partial sums
+//   k_scatter      float32 svals to the Laue-equivalent Lambert pixels (:462-479)
+#include <hip/hip_runtime.h>
+
+#include "ctem_internal.h"
+#include "ctem_laue.h"
+#include "ctem_sg.h"
+
+namespace ctem {
+
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------------------
+// beam selection
+__global__ __launch_bounds__(256) void k_select(const double* __restrict__ kvec, int32_t numk,
+                                                DevRefl R, const double* __restrict__ rmt_g,
+                                                SelectOut out, int fill) {
+  const int ik = blockIdx.x;
+  if (ik >= numk) return;
+  __shared__ double rmt[9];
+  __shared__ int wave_s[4], wave_w[4];
+  __shared__ int base_s, base_w;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < 9) rmt[tid] = rmt_g[tid];
+  if (tid == 0) { base_s = 0; base_w = 0; }
+  __syncthreads();
+  const float k[3] = {(float)kvec[3 * ik], (float)kvec[3 * ik + 1], (float)kvec[3 * ik + 2]};
+  const int64_t soff = fill ? out.soff[ik] : 0;
+  const int64_t woff = fill ? out.woff[ik] : 0;
+  bool range_ok = true;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int t0 = 0; t0 < R.nref; t0 += 256) {
+    const int ig = t0 + tid;
+    int cls = 0;
+    float sg = 0.0f;
+    if (ig == 0) {
+      cls = 2;  // the transmitted beam is always strong, with s_g = 0
+    } else if (ig < R.nref) {
+      const float g[3] = {R.g[3 * ig], R.g[3 * ig + 1], R.g[3 * ig + 2]};
+      sg = calcsg(g, k, rmt, &range_ok);
+      const float sgp = fabsf(sg);
+      if (R.dbd[ig]) {
+        if (sgp <= R.sgcutoff) cls = 2;
+      } else if (sgp <= R.cut1[ig]) {
+        cls = (sgp <= R.cut2[ig]) ? 2 : 1;
+      }
+    }
+    const uint64_t bs = __ballot(cls == 2);
+    const uint64_t bw = __ballot(cls == 1);
+    if (lane == 0) { wave_s[wave] = __popcll(bs); wave_w[wave] = __popcll(bw); }
+    __syncthreads();
+    if (fill) {
+      int os = base_s, ow = base_w;
+      for (int w = 0; w < wave; w++) { os += wave_s[w]; ow += wave_w[w]; }
+      if (cls == 2) {
+        const int64_t p = soff + os + __popcll(bs & lt);
+        out.sidx[p] = ig;
+        out.ssg[p] = sg;
+      } else if (cls == 1) {
+        const int64_t p = woff + ow + __popcll(bw & lt);
+        out.widx[p] = ig;
+        out.wsg[p] = sg;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      base_s += wave_s[0] + wave_s[1] + wave_s[2] + wave_s[3];
+      base_w += wave_w[0] + wave_w[1] + wave_w[2] + wave_w[3];
+    }
+    __syncthreads();
+  }
+  if (!fill && tid == 0) {
+    out.nns[ik] = base_s;
+    out.nnw[ik] = base_w;
+  }
+  if (!range_ok) atomicOr(out.err, 1);
+}
+
+void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& R,
+                   const double* rmt9, const SelectOut& out, int fill) {
+  if (numk <= 0) return;
+  hipLaunchKernelGGL(k_select, dim3(numk), dim3(256), 0, s, k, numk, R, rmt9, out, fill);
+}
+
+// ------------------------------------------------------------------------------------------
+// DynMat assembly
+__device__ __forceinline__ int lut_index(const DevLut& L, int h, int k, int l) {
+  return (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
+}
+
+__global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                  const double* __restrict__ kn, double mlambda,
+                                                  float dynupz, double dz, double2* __restrict__ A,
+                                                  int scaled) {
+#pragma clang fp contract(off)
+  const int npad = B.npad, tiles = npad / 32;
+  const int b = blockIdx.x / (tiles * tiles);
+  const int t = blockIdx.x % (tiles * tiles);
+  const int ti = t % tiles, tj = t / tiles;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir], nw = V.nnw[dir];
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const float* ssg = V.ssg + V.soff[dir];
+  const int32_t* widx = V.widx + V.woff[dir];
+  const float* wsg = V.wsg + V.woff[dir];
+  double2* Ab = A + (int64_t)b * npad * npad;
+
+  __shared__ int rh[32][3], ch[32][3], wh[32][3];
+  __shared__ double wsgd[32];
+  __shared__ double2 Ua[32][33], Ub[32][33];
+  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
+  const int gi = ti * 32 + r;
+  if (tid < 32) {
+    const int a = ti * 32 + tid, c = tj * 32 + tid;
+    for (int d = 0; d < 3; d++) {
+      rh[tid][d] = (a < n) ? R.hkl[3 * sidx[a] + d] : 0;
+      ch[tid][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
+    }
+  }
+  __syncthreads();
+  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+  double dsum = 0.0;
+  const bool live = (ti * 32 < n) && (tj * 32 < n);
+  if (live) {
+    for (int w0 = 0; w0 < nw; w0 += 32) {
+      __syncthreads();
+      if (tid < 32) {
+        const int w = w0 + tid;
+        for (int d = 0; d < 3; d++) wh[tid][d] = (w < nw) ? R.hkl[3 * widx[w] + d] : 0;
+        wsgd[tid] = (w < nw) ? (double)wsg[w] : 1.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < 1024; e += 256) {
+        const int rr = e & 31, ww = e >> 5;
+        double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
+        if (w0 + ww < nw) {
+          if (ti * 32 + rr < n) {
+            const float2 u = L.lut[lut_index(L, rh[rr][0] - wh[ww][0], rh[rr][1] - wh[ww][1], rh[rr][2] - wh[ww][2])];
+            va = make_double2(u.x, u.y);
+          }
+          if (tj * 32 + rr < n) {
+            const float2 u = L.lut[lut_index(L, wh[ww][0] - ch[rr][0], wh[ww][1] - ch[rr][1], wh[ww][2] - ch[rr][2])];
+            vb = make_double2(u.x, u.y);
+          }
+        }
+        Ua[rr][ww] = va;  // U(g_r - w)
+        Ub[ww][rr] = vb;  // U(w - h_c)
+      }
+      __syncthreads();
+      const int wn = min(32, nw - w0);
+      for (int w = 0; w < wn; w++) {
+        const double2 a = Ua[r][w];
+        const double inv = 1.0 / wsgd[w];
+        for (int q = 0; q < 4; q++) {
+          const double2 bb = Ub[w][cq + 8 * q];
+          const double pr = a.x * bb.x - a.y * bb.y;
+          const double pi = a.x * bb.y + a.y * bb.x;
+          ar[q] += pr * inv;
+          ai[q] += pi * inv;
+        }
+        const double m = hypot(a.x, a.y);
+        dsum += m * m / wsgd[w];
+      }
+    }
+  }
+  // write the tile
+  const double half = 0.5 * mlambda;
+  const double pre = scaled ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]) : 0.0;
+  for (int q = 0; q < 4; q++) {
+    const int gj = tj * 32 + cq + 8 * q;
+    double vr = 0.0, vi = 0.0;
+    if (gi < n && gj < n) {
+      if (gi != gj) {
+        const int c = cq + 8 * q;
+        const float2 u = L.lut[lut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2])];
+        vr = (double)u.x - half * ar[q];
+        vi = (double)u.y - half * ai[q];
+      } else {
+        const double s = dsum * mlambda / 2.0;
+        vr = 2.0 * (double)ssg[gi] / mlambda - s;
+        vi = (double)dynupz;
+      }
+    }
+    if (gi < npad && gj < npad) {
+      // scaled: A = i * pre * DynMat
+      Ab[(int64_t)gj * npad + gi] = scaled ? make_double2(-pre * vi, pre * vr) : make_double2(vr, vi);
+    }
+  }
+}
+
+void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                     const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                     double2* A) {
+  const int tiles = B.npad / 32;
+  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
+                     mlambda, dynupz, dz, A, 1);
+}
+
+void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                   const DevLut& L, double mlambda, float dynupz, double2* D) {
+  const int tiles = B.npad / 32;
+  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
+                     (const double*)nullptr, mlambda, dynupz, 0.0, D, 0);
+}
+
+// ------------------------------------------------------------------------------------------
+// norm and scaling
+__global__ __launch_bounds__(256) void k_norm_scale(int npad, double2* __restrict__ A,
+                                                    double theta, int smax, int* __restrict__ sq,
+                                                    int* __restrict__ err) {
+  const int b = blockIdx.x;
+  double2* Ab = A + (int64_t)b * npad * npad;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double wmax[4];
+  __shared__ int s_sh;
+  double cmax = 0.0;
+  for (int j = wave; j < npad; j += 4) {
+    double sum = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const double2 v = Ab[(int64_t)j * npad + i];
+      sum += hypot(v.x, v.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    cmax = fmax(cmax, sum);
+    if (!(sum == sum)) cmax = sum;  // keep a NaN
+  }
+  if (lane == 0) wmax[wave] = cmax;
+  __syncthreads();
+  if (tid == 0) {
+    double nrm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    for (int w = 0; w < 4; w++) if (!(wmax[w] == wmax[w])) nrm = wmax[w];
+    int s = 0;
+    if (!isfinite(nrm)) {
+      atomicOr(err, 2);
+    } else {
+      while (nrm > theta && s < smax) { nrm *= 0.5; s++; }
+      if (nrm > theta) atomicOr(err, 4);
+    }
+    s_sh = s;
+    sq[b] = s;
+  }
+  __syncthreads();
+  const double f = ldexp(1.0, -s_sh);
+  if (s_sh > 0) {
+    for (int64_t e = tid; e < (int64_t)npad * npad; e += 256) {
+      double2 v = Ab[e];
+      Ab[e] = make_double2(v.x * f, v.y * f);
+    }
+  }
+}
+
+void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, double theta,
+                       int32_t smax, int32_t* sq, int32_t* err) {
+  hipLaunchKernelGGL(k_norm_scale, dim3(nb), dim3(256), 0, s, npad, A, theta, smax, sq, err);
+}
+
+// ------------------------------------------------------------------------------------------
+// batched complex GEMM, FP64 MFMA. Workgroup tile 64x64, 4 waves of 32x32, K chunks of 16
+// staged through LDS. MFMA 16x16x4 f64 operand layout (lane l): A[l%16][l/16],
+// B[l/16][l%16]; result element e of lane l is C[4*(l/16)+e][l%16].
+#define ZG_LDS 65
+__global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
+  const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
+  const int b = blockIdx.x / (tm * tn);
+  const int t = blockIdx.x % (tm * tn);
+  const int bm = (t % tm) * 64, bn = (t / tm) * 64;
+  if (g.active && g.step >= g.active[b]) return;
+  const double2* X = g.X + (int64_t)b * g.sx;
+  const double2* Y = g.Y + (int64_t)b * g.sy;
+  double2* C = g.C + (int64_t)b * g.sc;
+
+  __shared__ double As_r[16][ZG_LDS], As_i[16][ZG_LDS], Bs_r[16][ZG_LDS], Bs_i[16][ZG_LDS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int l16 = lane & 15, l4 = lane >> 4;
+
+  double4v cr[2][2], ci[2][2];
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) { cr[i][j] = (double4v){0, 0, 0, 0}; ci[i][j] = (double4v){0, 0, 0, 0}; }
+
+  for (int k0 = 0; k0 < g.k; k0 += 16) {
+    // A tile: 64 rows x 16 k, column-major source
+    for (int r = 0; r < 4; r++) {
+      const int mm = tid & 63, kk = (tid >> 6) + 4 * r;
+      const int row = bm + mm, col = k0 + kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (row < g.m && col < g.k) v = X[(int64_t)col * g.ldx + row];
+      As_r[kk][mm] = v.x;
+      As_i[kk][mm] = v.y;
+    }
+    // B tile: 16 k x 64 cols
+    for (int r = 0; r < 4; r++) {
+      const int kk = tid & 15, nn = (tid >> 4) + 16 * r;
+      const int row = k0 + kk, col = bn + nn;
+      double2 v = make_double2(0.0, 0.0);
+      if (row < g.k && col < g.n) v = Y[(int64_t)col * g.ldy + row];
+      Bs_r[kk][nn] = v.x;
+      Bs_i[kk][nn] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      double a_r[2], a_i[2], b_r[2], b_i[2];
+      for (int i = 0; i < 2; i++) {
+        a_r[i] = As_r[ks + l4][wm + 16 * i + l16];
+        a_i[i] = As_i[ks + l4][wm + 16 * i + l16];
+        b_r[i] = Bs_r[ks + l4][wn + 16 * i + l16];
+        b_i[i] = Bs_i[ks + l4][wn + 16 * i + l16];
+      }
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_r[j], cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[i], b_i[j], ci[i][j], 0, 0, 0);
+          cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a_i[i], b_i[j], cr[i][j], 0, 0, 0);
+          ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], ci[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // epilogue
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      for (int e = 0; e < 4; e++) {
+        const int row = bm + wm + 16 * i + 4 * l4 + e;
+        const int col = bn + wn + 16 * j + l16;
+        if (row < g.m && col < g.n) {
+          double vr = g.alpha * cr[i][j][e], vi = g.alpha * ci[i][j][e];
+          if (row == col) vr += g.betaI;
+          for (int q = 0; q < 4; q++)
+            if (g.M[q]) {
+              const double2 mv = g.M[q][(int64_t)b * g.sm + (int64_t)col * g.ldm + row];
+              vr += g.beta[q] * mv.x;
+              vi += g.beta[q] * mv.y;
+            }
+          C[(int64_t)col * g.ldc + row] = make_double2(vr, vi);
+        }
+      }
+}
+
+void launch_zgemm(hipStream_t s, const ZGemm& g) {
+  const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
+  hipLaunchKernelGGL(k_zgemm, dim3(tm * tn * g.nb), dim3(256), 0, s, g);
+}
+
+// ------------------------------------------------------------------------------------------
+// elementwise helpers
+struct ComboArgs { const double2* M[4]; double beta[4]; };
+
+__global__ __launch_bounds__(256) void k_combo(int npad, int nb, double2* __restrict__ C,
+                                               double betaI, ComboArgs a) {
+  const int64_t per = (int64_t)npad * npad;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < per * nb; e += (int64_t)gridDim.x * 256) {
+    const int64_t loc = e % per;
+    const int row = (int)(loc % npad), col = (int)(loc / npad);
+    double vr = (row == col) ? betaI : 0.0, vi = 0.0;
+    for (int q = 0; q < 4; q++)
+      if (a.M[q]) { const double2 v = a.M[q][e]; vr += a.beta[q] * v.x; vi += a.beta[q] * v.y; }
+    C[e] = make_double2(vr, vi);
+  }
+}
+
+void launch_combo(hipStream_t s, int32_t npad, int32_t nb, double2* C, double betaI,
+                  const double2* const M[4], const double beta[4]) {
+  ComboArgs a;
+  for (int q = 0; q < 4; q++) { a.M[q] = M[q]; a.beta[q] = beta[q]; }
+  const int64_t tot = (int64_t)npad * npad * nb;
+  const int grid = (int)((tot + 255) / 256 < 65536 ? (tot + 255) / 256 : 65536);
+  hipLaunchKernelGGL(k_combo, dim3(grid), dim3(256), 0, s, npad, nb, C, betaI, a);
+}
+
+__global__ __launch_bounds__(256) void k_parity_copy(int npad, const int* __restrict__ sq,
+                                                     const double2* __restrict__ src,
+                                                     double2* __restrict__ dst) {
+  const int b = blockIdx.y;
+  if ((sq[b] & 1) == 0) return;
+  const int64_t per = (int64_t)npad * npad;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < per; e += (int64_t)gridDim.x * 256)
+    dst[b * per + e] = src[b * per + e];
+}
+
+void launch_parity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
+                        const double2* src, double2* dst) {
+  const int64_t per = (int64_t)npad * npad;
+  const int gx = (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64);
+  hipLaunchKernelGGL(k_parity_copy, dim3(gx, nb), dim3(256), 0, s, npad, sq, src, dst);
+}
+
+__global__ void k_first_col(int npad, const double2* __restrict__ E, double2* __restrict__ Psi,
+                            int64_t spsi) {
+  const int b = blockIdx.x;
+  for (int i = threadIdx.x; i < npad; i += blockDim.x)
+    Psi[(int64_t)b * spsi + i] = E[(int64_t)b * npad * npad + i];
+}
+
+void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
+                      int64_t spsi) {
+  hipLaunchKernelGGL(k_first_col, dim3(nb), dim3(256), 0, s, npad, E, Psi, spsi);
+}
+
+// ------------------------------------------------------------------------------------------
+// Lgh and the Sgh contraction
+__global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                  const double2* __restrict__ Psi, int64_t spsi,
+                                                  int nz, const double* __restrict__ lam,
+                                                  double dzt, double* __restrict__ partial,
+                                                  int keep_dir, double2* __restrict__ keep_lgh) {
+  const int npad = B.npad, tiles = npad / 32;
+  const int b = blockIdx.x / (tiles * tiles);
+  const int t = blockIdx.x % (tiles * tiles);
+  const int ti = t % tiles, tj = t / tiles;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir];
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const double2* P = Psi + (int64_t)b * spsi;
+  __shared__ double2 Pg[32][33], Ph[32][33];
+  __shared__ double lw[32];
+  __shared__ int rh[32][3], ch[32][3];
+  __shared__ double red[8][4];
+  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
+  const int nsite = L.nsite;
+  double acc_r[4] = {0, 0, 0, 0}, acc_i[4] = {0, 0, 0, 0};
+  const bool live = (ti * 32 < n) && (tj * 32 < n);
+  if (tid < 32) {
+    const int a = ti * 32 + tid, c = tj * 32 + tid;
+    for (int d = 0; d < 3; d++) {
+      rh[tid][d] = (a < n) ? R.hkl[3 * sidx[a] + d] : 0;
+      ch[tid][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
+    }
+  }
+  if (live) {
+    for (int k0 = 0; k0 < nz; k0 += 32) {
+      __syncthreads();
+      for (int e = tid; e < 1024; e += 256) {
+        const int rr = e & 31, kk = e >> 5;
+        const int k = k0 + kk;
+        double2 vg = make_double2(0, 0), vh = make_double2(0, 0);
+        if (k < nz) {
+          vg = P[(int64_t)k * npad + ti * 32 + rr];
+          vh = P[(int64_t)k * npad + tj * 32 + rr];
+        }
+        Pg[rr][kk] = vg;
+        Ph[rr][kk] = vh;
+      }
+      if (tid < 32) lw[tid] = (k0 + tid < nz) ? lam[k0 + tid] : 0.0;
+      __syncthreads();
+      const int kn = min(32, nz - k0);
+      for (int kk = 0; kk < kn; kk++) {
+        const double2 a = Pg[r][kk];
+        const double w = lw[kk];
+        const double awr = w * a.x, awi = w * a.y;
+        for (int q = 0; q < 4; q++) {
+          const double2 h = Ph[cq + 8 * q][kk];
+          // (a w) * conj(h)
+          acc_r[q] += awr * h.x + awi * h.y;
+          acc_i[q] += awi * h.x - awr * h.y;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int gi = ti * 32 + r;
+  double part[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 4; q++) {
+    const int c = cq + 8 * q;
+    const int gj = tj * 32 + c;
+    if (live && gi < n && gj < n) {
+      const double lr = dzt * acc_r[q], li = dzt * acc_i[q];
+      const int cell = lut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]);
+      for (int p = 0; p < nsite && p < 4; p++) {
+        const double2 sv = L.slut[(int64_t)p * L.ncell + cell];
+        part[p] += lr * sv.x - li * sv.y;
+      }
+      if (dir == keep_dir) keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);
+    }
+  }
+  // block reduction in fixed order: wave shuffles, then waves in order
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int p = 0; p < 4; p++) {
+    double v = part[p];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane == 0) red[wave][p] = v;
+  }
+  __syncthreads();
+  if (tid < nsite && tid < 4) {
+    const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+    partial[((int64_t)b * tiles * tiles + t) * nsite + tid] = v;
+  }
+}
+
+__global__ void k_reduce_dots(int nb, int ntile, int nsite, const int* __restrict__ dirs,
+                              const double* __restrict__ partial, double* __restrict__ dots) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  for (int p = 0; p < nsite; p++) {
+    double s = 0.0;
+    for (int t = 0; t < ntile; t++) s += partial[((int64_t)b * ntile + t) * nsite + p];
+    dots[(int64_t)dirs[b] * nsite + p] = s;
+  }
+}
+
+void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                     const DevLut& L, const double2* Psi, int64_t spsi, int32_t nz,
+                     const double* lam, double dzt, double* partial, double* dots,
+                     int32_t keep_dir, double2* keep_lgh) {
+  const int tiles = B.npad / 32;
+  hipLaunchKernelGGL(k_lgh_dots, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
+                     spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
+  hipLaunchKernelGGL(k_reduce_dots, dim3((B.nb + 127) / 128), dim3(128), 0, s, B.nb,
+                     tiles * tiles, L.nsite, B.dirs, partial, dots);
+}
+
+// ------------------------------------------------------------------------------------------
+// Lambert scatter
+__global__ void k_scatter(int numk, const int32_t* __restrict__ ij, const double* __restrict__ dots,
+                          int nsite, float nat, int npx, int npy, int isym, float* __restrict__ sr) {
+  const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ik >= numk) return;
+  int32_t q[24];
+  const int ne = laue_equiv(ij[2 * ik], ij[2 * ik + 1], isym, q);
+  const int64_t plane = (int64_t)(2 * npx + 1) * (2 * npy + 1);
+  for (int p = 0; p < nsite; p++) {
+    const float v = __fdiv_rn((float)dots[(int64_t)ik * nsite + p], nat);
+    for (int e = 0; e < ne; e++) {
+      const int i = q[2 * e], j = q[2 * e + 1];
+      if (i < -npx || i > npx || j < -npy || j > npy) continue;
+      sr[p * plane + (int64_t)(j + npy) * (2 * npx + 1) + (i + npx)] = v;
+    }
+  }
+}
+
+void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,
+                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym, float* sr) {
+  hipLaunchKernelGGL(k_scatter, dim3((numk + 127) / 128), dim3(128), 0, s, numk, ij, dots, nsite,
+                     nat, npx, npy, isym, sr);
+}
+
+}  // namespace ctem

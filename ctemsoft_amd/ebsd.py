@@ -1,0 +1,283 @@
+"""Python host mirror of the EBSD master-pattern path, over the C-ABI of
+include/ctem_ebsd.h (ctypes). It plays the role of CTEMEBSDmaster's ComputeMasterPattern
+(src_programs/CTEMEBSDmaster.f90:99-587) around the GPU beam loop:
+
+    ctx = Context()                      # one GPU
+    ctx.upload_crystal(setup)            # per-energy setup (LUT, reflection list, sites)
+    ij, k, kn = kvectors(...)            # Calckvectors 'RoscaLambert'
+    plan = ctx.plan(k, kn, lam, dz, thick)
+    plan.run("fp64")                     # beam selection, DynMat, Lgh, Sgh contraction
+    svals = plan.svals()                 # float32, as the reference stores them
+    sr = plan.scatter(ij, npx, npy, isym)
+
+Errors from the library raise CtemError with the library's message; there is no CPU path:
+if the HIP library or the GPU is missing the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libctem_ebsd.so")
+
+OK = 0
+PREC = {"fp64": 0, "mixed": 1}
+
+EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_device_info",
+           "ctem_upload_crystal", "ctem_kvectors", "ctem_laue_equiv", "ctem_plan_create",
+           "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
+           "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
+           "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3"]
+
+
+class CtemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ctem error {code}: {msg}")
+        self.code = code
+
+
+class _Crystal(C.Structure):
+    _fields_ = [("nref", C.c_int32), ("hkl", C.c_void_p), ("ucg", C.c_void_p),
+                ("dbdiff", C.c_void_p), ("lut_lo", C.c_int32 * 3), ("lut_dim", C.c_int32 * 3),
+                ("lut", C.c_void_p), ("mlambda", C.c_double), ("dynupz", C.c_float),
+                ("weakcutoff", C.c_float), ("cutoff", C.c_float), ("sgcutoff", C.c_float),
+                ("rmt", C.c_double * 9), ("nsite", C.c_int32), ("site_z", C.c_void_p),
+                ("site_b", C.c_void_p), ("site_norbit", C.c_void_p), ("orbit", C.c_void_p)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with `python -m ctemsoft_amd.build`")
+        L = C.CDLL(LIBPATH)
+        P, I32, D = C.c_void_p, C.c_int32, C.c_double
+        L.ctem_last_error.restype = C.c_char_p
+        sig = {
+            "ctem_ctx_create": [I32, P], "ctem_ctx_destroy": [P],
+            "ctem_device_info": [P, C.c_char_p, I32, P], "ctem_upload_crystal": [P, P],
+            "ctem_kvectors": [I32, I32, I32, I32, I32, P, D, I32, P, P, P, P],
+            "ctem_laue_equiv": [I32, I32, I32, P, P],
+            "ctem_plan_create": [P, I32, P, P, I32, P, D, D, P], "ctem_plan_destroy": [P],
+            "ctem_plan_select": [P, P, P], "ctem_plan_get_beams": [P, I32, P, P, P, P],
+            "ctem_plan_run": [P, I32, I32], "ctem_plan_get_dots": [P, P],
+            "ctem_plan_get_svals": [P, P], "ctem_plan_get_lgh": [P, P, I32, P],
+            "ctem_plan_dynmat": [P, I32, P, I32, P], "ctem_plan_scatter": [P, P, I32, I32, I32, P],
+            "ctem_plan_timing": [P, P, P, P, P],
+            "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = I32
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != OK:
+        raise CtemError(rc, lib().ctem_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def kvectors(npx, npy, isym, usehex, dsm, mlambda, hemisphere=1):
+    """Incident directions of Calckvectors 'RoscaLambert' (host computation in the library)."""
+    L = lib()
+    dsm = np.ascontiguousarray(dsm, np.float64).ravel()
+    n = C.c_int32()
+    _check(L.ctem_kvectors(npx, npy, isym, int(usehex), hemisphere, _p(dsm), mlambda, 0, None,
+                           None, None, C.byref(n)))
+    cap = n.value
+    ij = np.zeros(2 * cap, np.int32); k = np.zeros(3 * cap); kn = np.zeros(cap)
+    _check(L.ctem_kvectors(npx, npy, isym, int(usehex), hemisphere, _p(dsm), mlambda, cap, _p(ij),
+                           _p(k), _p(kn), C.byref(n)))
+    return ij.reshape(-1, 2), k.reshape(-1, 3), kn
+
+
+def laue_equiv(ipx, ipy, isym):
+    q = np.zeros(24, np.int32); n = C.c_int32()
+    _check(lib().ctem_laue_equiv(ipx, ipy, isym, _p(q), C.byref(n)))
+    return q[:2 * n.value].reshape(-1, 2)
+
+
+class Setup:
+    """Per-energy crystal setup arrays in the layout of ctem_crystal."""
+
+    def __init__(self, hkl, ucg, dbdiff_ref, lut, lut_lo, mlambda, dynupz, weakcutoff, cutoff,
+                 sgcutoff, rmt, sites, dsm=None, isym=None, usehex=0):
+        self.hkl = np.ascontiguousarray(hkl, np.int32).reshape(-1, 3)
+        u = np.asarray(ucg, np.complex128)
+        self.ucg = np.ascontiguousarray(np.stack([u.real, u.imag], -1).ravel())
+        self.dbdiff = np.ascontiguousarray(dbdiff_ref, np.uint8)
+        lutc = np.asarray(lut, np.complex128)  # indexed [l][k][h]
+        self.lut_dim = np.array(lutc.shape[::-1], np.int32)
+        self.lut_lo = np.asarray(lut_lo, np.int32)
+        self.lut = np.ascontiguousarray(np.stack([lutc.real, lutc.imag], -1).ravel())
+        self.mlambda, self.dynupz = float(mlambda), float(dynupz)
+        self.weakcutoff, self.cutoff, self.sgcutoff = float(weakcutoff), float(cutoff), float(sgcutoff)
+        self.rmt = np.ascontiguousarray(rmt, np.float64).ravel()
+        self.sites = sites
+        self.site_z = np.array([s["Z"] for s in sites], np.int32)
+        self.site_b = np.array([s["B"] for s in sites], np.float32)
+        self.site_norbit = np.array([len(s["orbit"]) for s in sites], np.int32)
+        self.orbit = np.ascontiguousarray(np.concatenate([np.asarray(s["orbit"], np.float64).ravel()
+                                                          for s in sites]))
+        self.nat = int(self.site_norbit.sum())
+        self.dsm = None if dsm is None else np.ascontiguousarray(dsm, np.float64).ravel()
+        self.isym, self.usehex = isym, usehex
+
+    @classmethod
+    def from_fixture(cls, fx):
+        lut = np.asarray(fx["lut"])
+        lo = np.asarray(fx["lut_lo"])
+        hkl = np.asarray(fx["hkl"], np.int32)
+        h, k, l = (hkl - lo).T
+        dbd = np.asarray(fx["dbdiff"])[l, k, h]
+        sites = [dict(Z=int(fx[f"site{i}_Z"]), B=float(fx[f"site{i}_pos"][4]),
+                      orbit=np.asarray(fx[f"site{i}_orbit"])) for i in range(int(fx["nsite"]))]
+        wc, c, sgc = (float(v) for v in fx["cutoffs"])
+        return cls(hkl, fx["ucg"], dbd, lut, lo, float(fx["mLambda"]), float(fx["DynUpz"]), wc, c,
+                   sgc, fx["rmt"], sites, dsm=fx["dsm"], isym=int(fx["isym"]), usehex=int(fx["usehex"]))
+
+    def struct(self):
+        s = _Crystal()
+        s.nref = len(self.hkl)
+        s.hkl, s.ucg, s.dbdiff, s.lut = _p(self.hkl), _p(self.ucg), _p(self.dbdiff), _p(self.lut)
+        for i in range(3):
+            s.lut_lo[i] = int(self.lut_lo[i]); s.lut_dim[i] = int(self.lut_dim[i])
+        s.mlambda, s.dynupz = self.mlambda, self.dynupz
+        s.weakcutoff, s.cutoff, s.sgcutoff = self.weakcutoff, self.cutoff, self.sgcutoff
+        for i in range(9):
+            s.rmt[i] = float(self.rmt[i])
+        s.nsite = len(self.sites)
+        s.site_z, s.site_b = _p(self.site_z), _p(self.site_b)
+        s.site_norbit, s.orbit = _p(self.site_norbit), _p(self.orbit)
+        return s
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().ctem_ctx_create(device, C.byref(self._h)))
+        self.setup = None
+
+    def close(self):
+        if self._h:
+            lib().ctem_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        buf = C.create_string_buffer(256); cu = C.c_int32()
+        _check(lib().ctem_device_info(self._h, buf, 256, C.byref(cu)))
+        return buf.value.decode(), cu.value
+
+    def upload_crystal(self, setup: Setup):
+        st = setup.struct()
+        _check(lib().ctem_upload_crystal(self._h, C.byref(st)))
+        self.setup = setup
+
+    def plan(self, k, kn, lam, dz, thick):
+        return Plan(self, k, kn, lam, dz, thick)
+
+    def calclgh3(self, dmat, thick, kn, depthstep, lam):
+        """Drop-in for CalcLgh3 (CTEMEBSDmaster.f90:610-722): DynMat -> Lgh, FP64 on the GPU."""
+        d = np.asfortranarray(dmat, np.complex128)
+        n = d.shape[0]
+        out = np.zeros((n, n), np.complex128, order="F")
+        lam = np.ascontiguousarray(lam, np.float32)
+        _check(lib().ctem_calclgh3(self._h, d.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+                                   float(thick), float(kn), n, 1, float(depthstep), _p(lam), len(lam)))
+        return np.array(out)
+
+
+class Plan:
+    def __init__(self, ctx: Context, k, kn, lam, dz, thick):
+        self.ctx = ctx
+        self.k = np.ascontiguousarray(k, np.float64).reshape(-1, 3)
+        self.kn = np.ascontiguousarray(kn, np.float64)
+        self.lam = np.ascontiguousarray(lam, np.float32)
+        self.numk = len(self.k)
+        self.nsite = len(ctx.setup.sites) if ctx.setup is not None else 0
+        self._h = C.c_void_p()
+        _check(lib().ctem_plan_create(ctx._h, self.numk, _p(self.k), _p(self.kn), len(self.lam),
+                                      _p(self.lam), float(dz), float(thick), C.byref(self._h)))
+        self.nns = self.nnw = None
+
+    def close(self):
+        if self._h:
+            lib().ctem_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def select(self):
+        ns = np.zeros(self.numk, np.int32); nw = np.zeros(self.numk, np.int32)
+        _check(lib().ctem_plan_select(self._h, _p(ns), _p(nw)))
+        self.nns, self.nnw = ns, nw
+        return ns, nw
+
+    def beams(self, ik):
+        ns, nw = int(self.nns[ik]), int(self.nnw[ik])
+        s = np.zeros(ns, np.int32); ssg = np.zeros(ns, np.float32)
+        w = np.zeros(max(nw, 1), np.int32); wsg = np.zeros(max(nw, 1), np.float32)
+        _check(lib().ctem_plan_get_beams(self._h, ik, _p(s), _p(ssg), _p(w), _p(wsg)))
+        return s, ssg, w[:nw], wsg[:nw]
+
+    def run(self, precision="fp64", keep_lgh=-1):
+        _check(lib().ctem_plan_run(self._h, PREC[precision], int(keep_lgh)))
+
+    def dots(self):
+        d = np.zeros(self.numk * self.nsite)
+        _check(lib().ctem_plan_get_dots(self._h, _p(d)))
+        return d.reshape(self.numk, self.nsite)
+
+    def svals(self):
+        s = np.zeros(self.numk * self.nsite, np.float32)
+        _check(lib().ctem_plan_get_svals(self._h, _p(s)))
+        return s.reshape(self.numk, self.nsite)
+
+    def lgh(self):
+        n = C.c_int32()
+        _check(lib().ctem_plan_get_lgh(self._h, None, 0, C.byref(n)))
+        out = np.zeros(2 * n.value * n.value)
+        _check(lib().ctem_plan_get_lgh(self._h, _p(out), n.value * n.value, C.byref(n)))
+        return out.view(np.complex128).reshape(n.value, n.value)
+
+    def dynmat(self, ik):
+        n = C.c_int32()
+        _check(lib().ctem_plan_dynmat(self._h, ik, None, 0, C.byref(n)))
+        out = np.zeros(2 * n.value * n.value)
+        _check(lib().ctem_plan_dynmat(self._h, ik, _p(out), n.value * n.value, C.byref(n)))
+        return out.view(np.complex128).reshape(n.value, n.value)
+
+    def scatter(self, ij, npx, npy, isym, sr=None):
+        if sr is None:
+            sr = np.zeros((self.nsite, 2 * npy + 1, 2 * npx + 1), np.float32)
+        ij = np.ascontiguousarray(ij, np.int32)
+        _check(lib().ctem_plan_scatter(self._h, _p(ij), npx, npy, isym, _p(sr)))
+        return sr
+
+    def timing(self):
+        t, g, f, n = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+        _check(lib().ctem_plan_timing(self._h, C.byref(t), C.byref(g), C.byref(f), C.byref(n)))
+        return dict(total_ms=t.value, gemm_ms=g.value, gemm_flops=f.value, gemm_launches=n.value)

@@ -1,0 +1,158 @@
+/* ctem_ebsd.h -- C-ABI of the MI355X-native EBSD master-pattern Bloch-wave path.
+ *
+ * This library replaces the per-beam-direction loop of CTEMsoft's CTEMEBSDmaster
+ * (src_programs/CTEMEBSDmaster.f90:406-484): Bethe beam selection and ordering, assembly
+ * of the dynamical matrix from the Ucg look-up table, the depth-integrated Lgh intensity
+ * matrix, the Sgh contraction and the Lambert-grid scatter. The caller (a Fortran host
+ * through ISO_C_BINDING, C, or Python through ctypes) keeps namelist input, the per-energy
+ * setup and the master-file output.
+ *
+ * Conventions
+ *   - Every function returns int32_t status: CTEM_OK (0) or a negative CTEM_E* code; it never
+ *     aborts. ctem_last_error() gives a message for the calling thread.
+ *   - Plain pointers and sizes only. Host buffers are caller-owned; device buffers are owned
+ *     by the context or plan that allocated them. Work is ordered on the context's HIP
+ *     stream and every call that returns host data synchronises it.
+ *   - Complex numbers are interleaved (re, im) doubles. 3x3 matrices are row-major
+ *     (m[3*i+j] = M(i,j)). The potential look-up table is in the reference's Fortran order
+ *     LUT(h,k,l), h fastest, with lower bounds lut_lo[3] and extents lut_dim[3]
+ *     (gvectors.f90:1316-1317).
+ *   - One host thread per context.
+ */
+#ifndef CTEM_EBSD_H
+#define CTEM_EBSD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define CTEM_API __attribute__((visibility("default")))
+#else
+#define CTEM_API
+#endif
+
+#define CTEM_OK 0
+#define CTEM_E_ARG -1       /* invalid argument */
+#define CTEM_E_HIP -2       /* HIP runtime error (no device, out of memory, launch failure) */
+#define CTEM_E_STATE -3     /* call out of order (e.g. no crystal uploaded) */
+#define CTEM_E_RANGE -4     /* input outside what the kernels support */
+#define CTEM_E_NUMERIC -5   /* non-finite values or norm overflow in the matrix exponential */
+
+/* precision of the Lgh stage */
+#define CTEM_PREC_FP64 0    /* complex FP64 throughout (FP64 MFMA) */
+#define CTEM_PREC_MIXED 1   /* split-BF16 MFMA products with FP32 accumulation; FP64 norm,
+                               squaring-count choice and final contraction */
+
+typedef struct ctem_ctx ctem_ctx;
+typedef struct ctem_plan ctem_plan;
+
+/* Per-energy crystal setup, as the reference host computes it (CTEMEBSDmaster.f90:358-396):
+ * CalcWaveLength (diffraction.f90:229-302), Compute_ReflectionList 'ALL'
+ * (gvectors.f90:1262-1444), Set_Bethe_Parameters (gvectors.f90:1196-1234), CalcOrbit
+ * (symmetry.f90:614-676). */
+typedef struct ctem_crystal {
+  int32_t nref;               /* master reflection list length (DynNbeamsLinked); entry 0 is 000 */
+  const int32_t* hkl;         /* [3*nref] Miller indices in list order */
+  const double* ucg;          /* [2*nref] Ucg per list entry (float32 values widened) */
+  const uint8_t* dbdiff;      /* [nref] double-diffraction flag of each list entry */
+  int32_t lut_lo[3];          /* lower bounds of LUT (= -2*imh, -2*imk, -2*iml) */
+  int32_t lut_dim[3];         /* extents of LUT */
+  const double* lut;          /* [2*prod(lut_dim)] complex LUT, Fortran order */
+  double mlambda;             /* electron wavelength [nm] (mLambda) */
+  float dynupz;               /* DynUpz = Vpmod of 000 [V] */
+  float weakcutoff, cutoff, sgcutoff; /* &BetheList */
+  double rmt[9];              /* reciprocal metric tensor */
+  int32_t nsite;              /* atoms in the asymmetric unit (numset) */
+  const int32_t* site_z;      /* [nsite] atomic numbers */
+  const float* site_b;        /* [nsite] Debye-Waller B [nm^2] */
+  const int32_t* site_norbit; /* [nsite] orbit sizes */
+  const double* orbit;        /* [3*sum(norbit)] orbit positions, site by site */
+} ctem_crystal;
+
+/* ----- context ----------------------------------------------------------------------- */
+CTEM_API int32_t ctem_ctx_create(int32_t device, ctem_ctx** out);
+CTEM_API int32_t ctem_ctx_destroy(ctem_ctx* ctx);
+CTEM_API const char* ctem_last_error(void);
+/* device name and compute-unit count, for logs */
+CTEM_API int32_t ctem_device_info(ctem_ctx* ctx, char* name, int32_t name_len, int32_t* cu_count);
+
+/* Upload the per-energy crystal setup; replaces the previous one. Precomputes the per-site
+ * structure-matrix table S_p(g-h) of CTEMEBSDmaster.f90:418-449 over the LUT box. */
+CTEM_API int32_t ctem_upload_crystal(ctem_ctx* ctx, const ctem_crystal* c);
+
+/* ----- incident directions ------------------------------------------------------------ */
+/* RoscaLambert incident directions of Calckvectors (kvectors.f90:421-542, AddkVector
+ * :839-947) for Laue group isym (1..12) on the square (usehex=0) or hexagonal grid.
+ * hemisphere = +1 samples the northern hemisphere as the reference does; -1 returns the
+ * same Lambert pixels mirrored to the southern hemisphere (k_z -> -k_z before the
+ * transformation to crystal coordinates). Returns the count in *numk; pass ij/k/kn = NULL to
+ * query the count only. dsm is the direct structure matrix. */
+CTEM_API int32_t ctem_kvectors(int32_t npx, int32_t npy, int32_t isym, int32_t usehex, int32_t hemisphere,
+                      const double dsm[9], double mlambda, int32_t cap, int32_t* ij, double* k,
+                      double* kn, int32_t* numk);
+
+/* Laue-symmetry equivalent Lambert pixels (Apply2DLaueSymmetry, Lambert.f90:1942-2046);
+ * iequiv holds up to 12 (i,j) pairs. */
+CTEM_API int32_t ctem_laue_equiv(int32_t ipx, int32_t ipy, int32_t isym, int32_t* iequiv, int32_t* nequiv);
+
+/* ----- batched master-pattern plan ---------------------------------------------------- */
+/* A plan holds numk incident directions resident in device memory plus the depth profile:
+ * lam[nz] float32 weights for depths z_k = k*dz, k = 1..nz, the reference's integer
+ * thickness `thick` (CTEMEBSDmaster.f90:142,306) and kn = 1/lambda per direction. */
+CTEM_API int32_t ctem_plan_create(ctem_ctx* ctx, int32_t numk, const double* k /*3*numk*/,
+                         const double* kn /*numk*/, int32_t nz, const float* lam, double dz,
+                         double thick, ctem_plan** out);
+CTEM_API int32_t ctem_plan_destroy(ctem_plan* plan);
+
+/* Beam selection on the GPU (gvectors.f90:988-1120): per direction the strong and weak
+ * reflections in list order with their float32 excitation errors, bit-exact with the
+ * reference. Optionally copies the counts to the host. */
+CTEM_API int32_t ctem_plan_select(ctem_plan* plan, int32_t* nns /*numk or NULL*/, int32_t* nnw /*numk or NULL*/);
+/* Copy one direction's beam set to the host (after ctem_plan_select). */
+CTEM_API int32_t ctem_plan_get_beams(ctem_plan* plan, int32_t ik, int32_t* strong, float* strongsg,
+                            int32_t* weak, float* weaksg);
+
+/* Run the whole path for every direction of the plan: selection (if not done), DynMat
+ * assembly, Lgh and the Sgh contraction, in the given precision. Results stay on the
+ * device until fetched. keep_lgh_ik (or -1) names one direction whose Lgh is kept for
+ * ctem_plan_get_lgh. */
+CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_lgh_ik);
+
+/* Re sum_gh Lgh(g,h) Sgh(g,h,p) per direction and site, in double, before the reference's
+ * float32 store (CTEMEBSDmaster.f90:456): dots[ik*nsite + p]. */
+CTEM_API int32_t ctem_plan_get_dots(ctem_plan* plan, double* dots);
+/* The reference's float32 svals: fl32(fl32(dot)/fl32(sum of orbit sizes)) (:456-459). */
+CTEM_API int32_t ctem_plan_get_svals(ctem_plan* plan, float* svals);
+/* Lgh (n x n complex, row-major) of the direction named in ctem_plan_run; *n on return. */
+CTEM_API int32_t ctem_plan_get_lgh(ctem_plan* plan, double* lgh, int32_t cap, int32_t* n);
+/* DynMat of one direction assembled on the GPU (row-major n x n), for parity tests. */
+CTEM_API int32_t ctem_plan_dynmat(ctem_plan* plan, int32_t ik, double* dynmat, int32_t cap, int32_t* n);
+
+/* Scatter svals into a Lambert master array sr[nsite][2*npy+1][2*npx+1] (float32), applying
+ * the Laue-symmetry equivalents per direction (CTEMEBSDmaster.f90:462-479). ij holds the
+ * (i,j) of every direction of the plan. sr is a host buffer. */
+CTEM_API int32_t ctem_plan_scatter(ctem_plan* plan, const int32_t* ij, int32_t npx, int32_t npy,
+                          int32_t isym, float* sr);
+
+/* Timing of the last ctem_plan_run, measured with HIP events on the context's stream:
+ * total milliseconds and milliseconds in the dominant (matrix-exponential GEMM) kernels,
+ * with the algorithmic flop count of those kernels. */
+CTEM_API int32_t ctem_plan_timing(ctem_plan* plan, double* total_ms, double* gemm_ms, double* gemm_flops,
+                         int32_t* gemm_launches);
+
+/* ----- single-direction drop-in for CalcLgh3 ------------------------------------------ */
+/* Same arguments and meaning as the reference's external subroutine CalcLgh3
+ * (CTEMEBSDmaster.f90:610-630): DMat and Lgh are nn x nn complex, column-major (Fortran),
+ * lambdaE[izz] float32. Computed on the GPU of ctx in FP64. The reference's unset `kin` and
+ * unused `debug` are accepted and ignored. */
+CTEM_API int32_t ctem_calclgh3(ctem_ctx* ctx, const double* dmat, double* lgh, double thick, double kn,
+                      int32_t nn, int32_t gzero, double depthstep, const float* lambdae,
+                      int32_t izz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
