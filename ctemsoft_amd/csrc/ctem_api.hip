@@ -87,6 +87,7 @@ struct ctem_ctx {
   // scratch for the CalcLgh3 drop-in
   DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
   hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<ctem_plan*> plans;  // live plans, destroyed with the context
 
   DevRefl refl() const {
     DevRefl R;
@@ -154,6 +155,7 @@ extern "C" int32_t ctem_ctx_destroy(ctem_ctx* c) {
   if (!c) return CTEM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  while (!c->plans.empty()) ctem_plan_destroy(c->plans.back());
   for (DBuf* b : {&c->g, &c->hkl, &c->cut1, &c->cut2, &c->dbd, &c->lut, &c->slut, &c->rmt, &c->s_A,
                   &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
                   &c->s_nns, &c->s_zero, &c->s_off, &c->s_part, &c->s_dots})
@@ -421,6 +423,7 @@ extern "C" int32_t ctem_plan_create(ctem_ctx* c, int32_t numk, const double* k, 
     return rc;
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { delete p; return fail(CTEM_E_HIP, "plan upload failed"); }
+  c->plans.push_back(p);
   *out = p;
   return CTEM_OK;
 }
@@ -434,6 +437,8 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
                   &p->sq, &p->dirs, &p->partial})
     b->release();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
+  auto& v = p->ctx->plans;
+  v.erase(std::remove(v.begin(), v.end(), p), v.end());
   delete p;
   return CTEM_OK;
 }

@@ -267,7 +267,8 @@ void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, doub
 // ------------------------------------------------------------------------------------------
 // batched complex GEMM, FP64 MFMA. Workgroup tile 64x64, 4 waves of 32x32, K chunks of 16
 // staged through LDS. MFMA 16x16x4 f64 operand layout (lane l): A[l%16][l/16],
-// B[l/16][l%16]; result element e of lane l is C[4*(l/16)+e][l%16].
+// B[l/16][l%16]; result element e of lane l is C[l/16 + 4*e][l%16] (rows interleaved by 4,
+// unlike the f32/bf16 16x16 variants whose lane holds four consecutive rows).
 #define ZG_LDS 65
 __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
       for (int e = 0; e < 4; e++) {
-        const int row = bm + wm + 16 * i + 4 * l4 + e;
+        const int row = bm + wm + 16 * i + l4 + 4 * e;
         const int col = bn + wn + 16 * j + l16;
         if (row < g.m && col < g.n) {
           double vr = g.alpha * cr[i][j][e], vi = g.alpha * ci[i][j][e];

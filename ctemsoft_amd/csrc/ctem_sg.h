@@ -23,14 +23,17 @@
 
 #if defined(__HIPCC__)
 #define CTEM_HD __host__ __device__
-#define CTEM_FSQRT(x) __fsqrt_rn(x)
+#else
+#define CTEM_HD
+#endif
+// correctly rounded float32 division on both sides (the device builtin rounds correctly)
+#if defined(__HIP_DEVICE_COMPILE__)
 #define CTEM_FDIV(a, b) __fdiv_rn((a), (b))
 #else
 #include <math.h>
-#define CTEM_HD
-#define CTEM_FSQRT(x) sqrtf(x)
 #define CTEM_FDIV(a, b) ((a) / (b))
 #endif
+#define CTEM_FSQRT(x) ::ctem::sqrtf_cr(x)
 
 #pragma clang fp contract(off)
 
@@ -38,6 +41,24 @@ namespace ctem {
 
 CTEM_HD inline uint32_t f2u(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
 CTEM_HD inline float u2f(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+
+// Correctly rounded float32 square root. The device's single-precision square root is
+// faithful but not correctly rounded; this takes the double-precision root, rounds it to
+// float and then decides between the float and its two neighbours with exact double
+// arithmetic: the midpoints between adjacent floats have at most 25 significant bits, so
+// their squares are exact in double, and a float argument can never equal such a square.
+CTEM_HD inline float sqrtf_cr(float x) {
+#pragma clang fp contract(off)
+  if (!(x > 0.0f) || x == __builtin_inff()) return (x == 0.0f || x == __builtin_inff()) ? x : __builtin_nanf("");
+  const double d = (double)x;
+  float s = (float)__builtin_sqrt(d);
+  const float up = u2f(f2u(s) + 1u), dn = u2f(f2u(s) - 1u);
+  const double mhi = ((double)s + (double)up) * 0.5;
+  const double mlo = ((double)s + (double)dn) * 0.5;
+  if (d > mhi * mhi) s = up;
+  else if (d < mlo * mlo) s = dn;
+  return s;
+}
 
 // fdlibm acosf
 CTEM_HD inline float acosf_ref(float x) {

