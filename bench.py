@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark: EBSD master-pattern directions/s on one MI355X.
+
+One step = the whole master pattern of the workload: every incident direction of both
+Lambert hemispheres through beam selection, DynMat assembly, the depth-integrated Lgh and
+the Sgh contraction, with the inputs (per-energy setup and wave vectors) resident in HBM
+before the timed region. `value` is directions per second of the mixed-precision form; the
+FP64 form is timed in the same run and reported beside it with the ratio and the error of
+the mixed form against it.
+
+    python bench.py [--steps K] [--warmup W] [--workload fe_c1w] [--precision both|fp64|mixed]
+
+The per-energy setup (Ucg look-up table, reflection list, sites) is the reference's own, as
+its host computes it for bcc Fe at 20 kV (tests/golden/<workload>.npz); the wave vectors,
+depth profile and Bethe parameters are generated here.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+PEAK = {"fp64": 78.6e12, "mixed": 2.5e15}  # MI355X dense FP64 MFMA / BF16 MFMA
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_setup(name):
+    from ctemsoft_amd import ebsd, workloads
+    wl = workloads.get(name)
+    path = os.path.join(REPO, "tests", "golden", f"{name}.npz")
+    fx = np.load(path)
+    st = ebsd.Setup.from_fixture(fx)
+    return wl, st, fx
+
+
+def directions(wl, st):
+    from ctemsoft_amd import ebsd
+    ij, k, kn = ebsd.kvectors(wl.npx, wl.npx, st.isym, st.usehex, st.dsm, st.mlambda, 1)
+    if wl.hemispheres == 2:
+        ij2, k2, kn2 = ebsd.kvectors(wl.npx, wl.npx, st.isym, st.usehex, st.dsm, st.mlambda, -1)
+        ij = np.concatenate([ij, ij2]); k = np.concatenate([k, k2]); kn = np.concatenate([kn, kn2])
+    return ij, k, kn
+
+
+def cpu_baseline(wl, lam, k_count, seconds=20.0):
+    """Reference beam loop (compiled from the reference sources, MKL sequential) on one
+    host core over a seeded sample of the workload's northern directions."""
+    from oracle import refrun
+    if not refrun.have_reference():
+        return None
+    rng = np.random.default_rng(wl.seed)
+    with tempfile.TemporaryDirectory(prefix="ctem_cpu_") as work:
+        # calibrate on 4 directions, then size the sample to ~`seconds`
+        ks = np.sort(rng.choice(k_count, 4, replace=False))
+        t4 = max(refrun.time_reference(work, wl, lam, ks), 1e-3)
+        n = int(max(8, min(512, 4 * seconds / t4)))
+        ks = np.sort(rng.choice(k_count, n, replace=False))
+        t = refrun.time_reference(work, wl, lam, ks)
+    return dict(value=n / t, unit="directions/s", cores=1, kind="reference",
+                sample=f"{n} seeded directions of {wl.name} (northern hemisphere), Compute_DynMat + "
+                       f"CalcLgh3 per direction, amdflang -O2 + MKL 2021.4 sequential, {t:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="fe_c1w")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); the directions of a "
+              "master pattern would shard across GPUs without exchange, which this build does "
+              "not run", file=sys.stderr)
+        sys.exit(2)
+    import torch  # noqa: F401  (HIP runtime present)
+    from ctemsoft_amd import ebsd
+
+    wl, st, fx = load_setup(a.workload)
+    ij, k, kn = directions(wl, st)
+    lam = wl.depth_weights()
+    thick = float(int(wl.nz * wl.dz))
+    ctx = ebsd.Context(0)
+    name, cus = ctx.device_info()
+    log(f"device {name}, {cus} CUs; workload {wl.name}: {len(k)} directions, nz={wl.nz}")
+    ctx.upload_crystal(st)
+    plan = ctx.plan(k, kn, lam, wl.dz, thick)
+    ns, nw = plan.select()
+    log(f"strong beams {ns.min()}..{ns.max()} (mean {ns.mean():.1f}); weak {nw.min()}..{nw.max()} (mean {nw.mean():.1f})")
+    precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
+    res = {}
+    for prec in precs:
+        try:
+            for w in range(a.warmup):
+                plan.run(prec)
+                log(f"{prec}: warmup {w + 1}/{a.warmup} done")
+        except ebsd.CtemError as e:
+            log(f"{prec}: not available ({e})")
+            continue
+        times, gemm = [], []
+        for s in range(a.steps):
+            t0 = time.perf_counter()
+            plan.run(prec)
+            times.append(time.perf_counter() - t0)
+            gemm.append(plan.timing())
+            log(f"{prec}: step {s + 1}/{a.steps} {times[-1] * 1e3:.1f} ms")
+        dots = plan.dots().copy()
+        ms = float(np.mean(times) * 1e3)
+        g = gemm[-1]
+        res[prec] = dict(ms=ms, dps=len(k) / (ms * 1e-3), gemm=g, dots=dots,
+                         gemm_tflops=g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12 if g["gemm_ms"] > 0 else 0.0)
+        log(f"{prec}: {ms:.1f} ms/step, {res[prec]['dps']:.0f} directions/s, GEMM {g['gemm_ms']:.1f} ms "
+            f"({res[prec]['gemm_tflops']:.2f} TFLOP/s over {g['gemm_launches']} launches)")
+    head = "mixed" if "mixed" in res else "fp64"
+    r = res[head]
+    err = None
+    if "mixed" in res and "fp64" in res:
+        d64, dmx = res["fp64"]["dots"], res["mixed"]["dots"]
+        err = float(np.max(np.abs(dmx - d64) / np.abs(d64)))
+    cpu = None if a.no_cpu else cpu_baseline(wl, lam, len(k) // wl.hemispheres)
+    gflops = r["gemm"]["gemm_flops"]
+    line = {
+        "metric": "master-pattern directions/s, 1 GPU: mixed vs FP64 at <=1e-4 rel err",
+        "value": round(r["dps"], 2),
+        "unit": "directions/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(r["ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16-split/f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic wave vectors and depth profile; reference per-energy setup",
+        "config": {"workload": f"{wl.name}: bcc Fe Im-3m a=0.28665 nm, 20 kV, dmin 0.05 nm, "
+                               f"Bethe {wl.weakcutoff:g}/{wl.cutoff:g}/{wl.sgcutoff:g}, npx {wl.npx}, "
+                               f"{wl.hemispheres} hemispheres, {wl.nz} x {wl.dz} nm depth steps",
+                   "directions": int(len(k)), "strong_beams_mean": round(float(ns.mean()), 1),
+                   "weak_beams_mean": round(float(nw.mean()), 1), "precision": head},
+        "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
+        "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
+        "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,
+        "mixed_max_rel_err_vs_fp64": err,
+        "roofline": {"bound": "mfma", "achieved": round(r["gemm_tflops"], 3),
+                     "peak": PEAK[head] / 1e12, "unit": "TFLOP/s",
+                     "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4), "traffic": None,
+                     "kernel": "k_zgemm" if head == "fp64" else "k_cgemm",
+                     "flops_per_step": gflops},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

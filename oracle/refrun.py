@@ -1,0 +1,220 @@
+"""Run the compiled reference (oracle/_ref/, built by `make -C oracle ref`) on inputs written
+here: the .xtal crystal file, the Monte Carlo depth file, the namelists, and the
+instrumentation driver's stream output.
+
+TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py to write fixtures and by
+bench.py's CPU-baseline leg to time the reference's own beam loop. Never used by the
+product.
+
+File layouts written and read here follow the reference:
+  .xtal       files.f90:271-277 (two Fortran sequential records)
+  MC file     CTEMMC.f90:264-276 / CTEMEBSDmaster.f90:190-235
+  master file CTEMEBSDmaster.f90:540-568
+"""
+from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+REF_BIN = os.path.join(HERE, "_ref")
+MAXPASYM = 250
+
+
+# ----------------------------------------------------------------------------------------
+# Fortran sequential unformatted records (4-byte little-endian markers)
+def _rec(buf: bytes) -> bytes:
+    n = struct.pack("<i", len(buf))
+    return n + buf + n
+
+
+def _chars(s: str, n: int) -> bytes:
+    return s.encode().ljust(n, b" ")[:n]
+
+
+def write_xtal(path: str, xtal: dict) -> None:
+    """.xtal writer (files.f90:271-277): R1 = system, a,b,c,alpha,beta,gamma;
+    R2 = ATOM_type(250), ATOM_pos(250,5) f32 column-major, ntype, SGnum, SGset."""
+    r1 = struct.pack("<i6d", xtal["system"], *xtal["abc"], *xtal["angles"])
+    types = np.zeros(MAXPASYM, np.int32)
+    pos = np.zeros((MAXPASYM, 5), np.float32)
+    for i, s in enumerate(xtal["sites"]):
+        types[i] = s["Z"]
+        pos[i, :3] = s["xyz"]
+        pos[i, 3] = s["occ"]
+        pos[i, 4] = s["B"]
+    r2 = types.tobytes() + pos.T.copy().tobytes() + struct.pack(
+        "<3i", len(xtal["sites"]), xtal["sg"], xtal.get("sgset", 1))
+    with open(path, "wb") as f:
+        f.write(_rec(r1) + _rec(r2))
+
+
+def write_mcfile(path: str, xtalname: str, ekev: float, dz: float, counts, tail) -> None:
+    Z = len(counts)
+    nzb = Z + 2
+    acc_z = np.zeros(nzb, np.int32)
+    acc_z[:Z] = counts
+    acc_z[Z] = tail
+    num_el = int(np.sum(counts))
+    recs = [
+        _rec(_chars("CTEMMC.f90", 132)),
+        _rec(_chars("2.0/2013", 8)),
+        _rec(_chars(xtalname, 132)),
+        _rec(struct.pack("<4iqi", 1, nzb, 1, 1, num_el, 1)),
+        _rec(struct.pack("<5d", ekev, ekev, 1.0, dz * nzb, dz)),
+        _rec(struct.pack("<2d", 70.0, 0.0)),
+        _rec(_chars("full", 4)),
+        _rec(np.array([num_el], np.int32).tobytes()),
+        _rec(acc_z.tobytes()),
+    ]
+    with open(path, "wb") as f:
+        f.write(b"".join(recs))
+
+
+def read_master(path: str) -> dict:
+    with open(path, "rb") as f:
+        data = f.read()
+    recs, off = [], 0
+    while off < len(data):
+        (n,) = struct.unpack_from("<i", data, off)
+        recs.append(data[off + 4: off + 4 + n])
+        off += 8 + n
+    npx, npy, nE, nset = struct.unpack("<4i", recs[4])
+    sr = np.frombuffer(recs[8], np.float32).reshape((nset, nE, 2 * npy + 1, 2 * npx + 1))
+    return dict(npx=npx, npy=npy, nE=nE, nset=nset,
+                ekevs=np.frombuffer(recs[5], np.float32).copy(),
+                atom_types=np.frombuffer(recs[6], np.int32).copy(),
+                grid=recs[7].decode(), sr=sr.copy())
+
+
+# ----------------------------------------------------------------------------------------
+class Stream:
+    def __init__(self, buf: bytes):
+        self.b, self.o = buf, 0
+
+    def get(self, dtype, count):
+        dt = np.dtype(dtype)
+        a = np.frombuffer(self.b, dt, count, self.o)
+        self.o += dt.itemsize * count
+        return a.copy()
+
+
+def parse_dump(path: str, dolgh: int) -> dict:
+    with open(path, "rb") as f:
+        s = Stream(f.read())
+    magic, isym, usehex, numk, nref, nsel, nsite, sgnum = s.get("<i4", 8)
+    assert magic == 20260
+    dims = s.get("<i4", 6)
+    consts = s.get("<f8", 9)
+    mats = s.get("<f8", 36).reshape(4, 3, 3)  # Fortran column-major 3x3 each
+    mats = np.transpose(mats, (0, 2, 1))
+    nlut = int(dims[0] * dims[1] * dims[2])
+    lut = s.get("<c16", nlut).reshape(dims[2], dims[1], dims[0])  # [l][k][h]
+    dbd = s.get("<i4", nlut).reshape(dims[2], dims[1], dims[0])
+    hkl = np.zeros((nref, 3), np.int32)
+    ucg = np.zeros(nref, np.complex128)
+    for i in range(nref):
+        hkl[i] = s.get("<i4", 3)
+        ucg[i] = s.get("<c16", 1)[0]
+    sites = []
+    for _ in range(nsite):
+        Z = int(s.get("<i4", 1)[0])
+        pos = s.get("<f4", 5)
+        n = int(s.get("<i4", 1)[0])
+        orb = s.get("<f8", 3 * n).reshape(3, n).T.copy()
+        sites.append(dict(Z=Z, pos=pos, orbit=orb))
+    kij = np.zeros((numk, 2), np.int32)
+    kvec = np.zeros((numk, 3), np.float64)
+    kn = np.zeros(numk, np.float64)
+    for i in range(numk):
+        kij[i] = s.get("<i4", 2)
+        kvec[i] = s.get("<f8", 3)
+        kn[i] = s.get("<f8", 1)[0]
+    hkl_index = {tuple(h): i for i, h in enumerate(hkl)}
+    per = []
+    for _ in range(nsel):
+        ik, nns, nnw = s.get("<i4", 3)
+        strong = s.get("<i4", nns) - 1
+        ssg = s.get("<f4", nns)
+        whkl = s.get("<i4", 3 * nnw).reshape(nnw, 3)
+        wsg = s.get("<f4", nnw)
+        weak = np.array([hkl_index[tuple(h)] for h in whkl], np.int32)
+        d = dict(ik=int(ik) - 1, strong=strong, strongsg=ssg, weak=weak, weaksg=wsg)
+        if dolgh >= 1:
+            d["dynmat"] = s.get("<c16", nns * nns).reshape(nns, nns).T.copy()
+        if dolgh >= 2:
+            d["lgh"] = s.get("<c16", nns * nns).reshape(nns, nns).T.copy()
+        per.append(d)
+    assert s.o == len(s.b), "trailing bytes in dump"
+    return dict(isym=int(isym), usehex=int(usehex), numk=int(numk), nref=int(nref),
+                sgnum=int(sgnum), lut_dims=dims, mLambda=consts[0], mRelcor=consts[1],
+                mPsihat=consts[2], mSigma=consts[3], DynUpz=consts[4], weakcutoff=consts[5],
+                cutoff=consts[6], sgcutoff=consts[7], vol=consts[8], dmt=mats[0], rmt=mats[1],
+                dsm=mats[2], rsm=mats[3], lut=lut, dbdiff=dbd, hkl=hkl, ucg=ucg, sites=sites,
+                kij=kij, kvec=kvec, kn=kn, per=per)
+
+
+def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: int,
+               ekev: float | None = None) -> dict:
+    xt = os.path.join(work, "crystal.xtal")
+    write_xtal(xt, wl.xtal)
+    with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
+        f.write(f" &BetheList\n weakcutoff = {wl.weakcutoff!r},\n cutoff = {wl.cutoff!r},\n"
+                f" sgcutoff = {wl.sgcutoff!r},\n /\n")
+    lam.astype(np.float32).tofile(os.path.join(work, "lam.bin"))
+    ksf = "undefined"
+    if ksel is not None:
+        ksf = "ksel.bin"
+        ks = np.asarray(ksel, np.int32) + 1
+        with open(os.path.join(work, ksf), "wb") as f:
+            f.write(struct.pack("<i", len(ks)) + ks.tobytes())
+    ek = wl.ekev if ekev is None else ekev
+    dz = wl.dz_for(ekev)
+    thick = float(int(len(lam) * dz))  # integer thickness (CTEMEBSDmaster.f90:142,306)
+    with open(os.path.join(work, "refdrv.nml"), "w") as f:
+        f.write(f" &refdrv\n xtalname = 'crystal.xtal',\n ekev = {ek!r},\n dmin = {wl.dmin!r},\n"
+                f" npx = {wl.npx},\n izz = {len(lam)},\n depthstep = {dz!r},\n"
+                f" thick = {thick!r},\n lamfile = 'lam.bin',\n kselfile = '{ksf}',\n"
+                f" outname = 'dump.bin',\n dolgh = {dolgh},\n /\n")
+    env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && {REF_BIN}/ctemref_driver > driver.log 2>&1"],
+                   cwd=work, check=True, env=env)
+    d = parse_dump(os.path.join(work, "dump.bin"), dolgh)
+    d["thick"] = thick
+    return d
+
+
+def run_program(work: str, wl: workloads.Workload, counts, tail) -> dict:
+    write_xtal(os.path.join(work, "crystal.xtal"), wl.xtal)
+    write_mcfile(os.path.join(work, "mc.data"), "crystal.xtal", wl.ekev, wl.dz, counts, tail)
+    with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
+        f.write(f" &BetheList\n weakcutoff = {wl.weakcutoff!r},\n cutoff = {wl.cutoff!r},\n"
+                f" sgcutoff = {wl.sgcutoff!r},\n /\n")
+    with open(os.path.join(work, "CTEMEBSDmaster.nml"), "w") as f:
+        f.write(f" &EBSDmastervars\n dmin = {wl.dmin!r},\n npx = {wl.npx},\n"
+                f" outname = 'master.data',\n energyfile = 'mc.data',\n Esel = -1,\n /\n")
+    env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && {REF_BIN}/CTEMEBSDmaster CTEMEBSDmaster.nml "
+                    "> program.log 2>&1"], cwd=work, check=True, env=env)
+    return read_master(os.path.join(work, "master.data"))
+
+
+def have_reference() -> bool:
+    return os.path.exists(os.path.join(REF_BIN, "ctemref_driver"))
+
+
+def time_reference(work: str, wl, lam: np.ndarray, ksel) -> float:
+    """Wall time of the reference beam loop (Compute_DynMat + CalcLgh3 per direction) over
+    the directions ksel, single-threaded (MKL sequential), minus the time of the same run on
+    one direction (which carries the per-energy setup and file output)."""
+    t0 = time.perf_counter()
+    run_driver(work, wl, lam, list(ksel), 2)
+    t1 = time.perf_counter()
+    run_driver(work, wl, lam, [int(ksel[0])], 2)
+    t2 = time.perf_counter()
+    return (t1 - t0) - (t2 - t1)
