@@ -120,11 +120,12 @@ struct ctem_plan {
   int32_t keep_dir = -1, keep_n = 0;
   DBuf keep_lgh;
   // work
-  DBuf A, W, Psi, sq, dirs, partial;
+  DBuf A, W, Psi, Psi32, sq, dirs, partial;
   std::vector<int32_t> h_sq;
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
   int32_t n_gemm = 0;
+  int32_t terms_taylor = 6, terms_square = 6;  // split-BF16 product terms of the mixed form
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
 };
@@ -434,7 +435,7 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
   (void)hipStreamSynchronize(p->ctx->stream);
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
                   &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->A, &p->W, &p->Psi,
-                  &p->sq, &p->dirs, &p->partial})
+                  &p->Psi32, &p->sq, &p->dirs, &p->partial})
     b->release();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
   auto& v = p->ctx->plans;
@@ -629,6 +630,108 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
   return CTEM_OK;
 }
 
+// Mixed precision: the same scaling-and-squaring with the exponential kept as E = I + F, so
+// that every product acts on the small part F and its rounding error scales with |F|^2:
+//   F_0 = B + B^2/2 + B^3 (I/6 + B/24 + B^2/120 + B^3/720),  B = A / 2^s, ||B||_1 <= 1/4
+//   F_{i+1} = F_i^2 + 2 F_i            (E_{i+1} = E_i^2)
+// The depth powers use the same form, psi_{m+j} = psi_j + F_m psi_j with F_m = E^m - I.
+// Products run on split-BF16 MFMA with FP32 accumulation (ctem_mixed.hip); the norm and the
+// squaring counts come from the FP64 matrix, the final Lgh/Sgh contraction runs in FP64.
+constexpr double kThetaMixed = 0.25;
+
+int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A64, float2* W32,
+                         float2* Psi32, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
+                         int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
+                         int terms_taylor, int terms_square) {
+  const int64_t per = (int64_t)npad * npad;
+  float2* B = W32;
+  float2* G2 = W32 + per * nb;
+  float2* G3 = W32 + 2 * per * nb;
+  float2* P = W32 + 3 * per * nb;
+  float2* F = W32 + 4 * per * nb;
+  float2* H = W32 + 5 * per * nb;
+  launch_norm_scale(s, npad, nb, A64, kThetaMixed, kSmax, d_sq, d_err);
+  HIPCHK(hipGetLastError());
+  h_sq.resize(nb);
+  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int smax = 0;
+  for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  double n3 = 0.0, n2 = 0.0;
+  for (int b = 0; b < nb; b++) { n3 += (double)h_n[b] * h_n[b] * h_n[b]; n2 += (double)h_n[b] * h_n[b]; }
+  launch_z2c(s, per * nb, A64, B);
+  HIPCHK(hipGetLastError());
+  auto cg = [&](float2* C, const float2* X, const float2* Y, float betaI,
+                std::initializer_list<std::pair<const float2*, float>> ms, const int32_t* active,
+                int step, double flops, int terms, int ncols, int64_t sy, int64_t sc, int64_t sm) -> int32_t {
+    CGemm g{};
+    g.m = npad; g.n = ncols; g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = X; g.sx = per; g.Y = Y; g.sy = sy; g.C = C; g.sc = sc; g.sm = sm;
+    g.alpha = 1.0f; g.betaI = betaI;
+    int q = 0;
+    for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
+    g.active = active; g.step = step; g.nb = nb; g.terms = terms;
+    GemmTimer t{tp, s};
+    t.begin();
+    launch_cgemm(s, g);
+    t.end();
+    HIPCHK(hipGetLastError());
+    if (tp) { tp->f_gemm += flops; tp->n_gemm++; }
+    return CTEM_OK;
+  };
+  int32_t rc;
+  const double f3 = 8.0 * n3;
+  if ((rc = cg(G2, B, B, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  if ((rc = cg(G3, B, G2, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  {
+    const float2* M[4] = {B, G2, G3, nullptr};
+    const float beta[4] = {(float)inv_fact(4), (float)inv_fact(5), (float)inv_fact(6), 0.f};
+    launch_ccombo(s, npad, nb, P, (float)inv_fact(3), M, beta);
+    HIPCHK(hipGetLastError());
+  }
+  if ((rc = cg(F, G3, P, 0.f, {{B, 1.f}, {G2, 0.5f}}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  float2* buf[2] = {F, H};
+  for (int i = 0; i < smax; i++) {
+    double fl = 0.0;
+    for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
+    float2* X = buf[i & 1];
+    if ((rc = cg(buf[(i + 1) & 1], X, X, 0.f, {{X, 2.f}}, d_sq, i, fl, terms_square, npad, per, per, per))) return rc;
+  }
+  launch_cparity_copy(s, npad, nb, d_sq, H, F);
+  HIPCHK(hipGetLastError());
+  launch_cfirst_col(s, npad, nb, F, Psi32, spsi);
+  HIPCHK(hipGetLastError());
+  float2* cur = F;
+  float2* alt = G2;
+  int m = 1;
+  while (m < nz) {
+    const int cnt = std::min(m, nz - m);
+    {
+      CGemm g{};
+      g.m = npad; g.n = cnt; g.k = npad;
+      g.ldx = g.ldy = g.ldc = g.ldm = npad;
+      g.X = cur; g.sx = per; g.Y = Psi32; g.sy = spsi; g.C = Psi32 + (int64_t)m * npad; g.sc = spsi;
+      g.M[0] = Psi32; g.beta[0] = 1.f; g.sm = spsi;
+      g.alpha = 1.f; g.nb = nb; g.terms = terms_square;
+      GemmTimer t{tp, s};
+      t.begin();
+      launch_cgemm(s, g);
+      t.end();
+      HIPCHK(hipGetLastError());
+      if (tp) { tp->f_gemm += 8.0 * n2 * cnt; tp->n_gemm++; }
+    }
+    if (m + cnt < nz) {
+      if ((rc = cg(alt, cur, cur, 0.f, {{cur, 2.f}}, nullptr, 0, f3, terms_square, npad, per, per, per))) return rc;
+      std::swap(cur, alt);
+    }
+    m += cnt;
+  }
+  launch_c2z(s, spsi * nb, Psi32, Psi64);
+  HIPCHK(hipGetLastError());
+  return CTEM_OK;
+}
+
 }  // namespace
 
 // bucket sizes: multiples of 32
@@ -693,7 +796,10 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
                              p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq);
       } else {
-        rc = fail(CTEM_E_ARG, "mixed precision not built into this library");
+        if ((rc = p->Psi32.ensure((size_t)nb * spsi * sizeof(float2)))) return rc;
+        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(), p->Psi32.as<float2>(),
+                              p->Psi.as<double2>(), spsi, p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn,
+                              p->h_sq, p->terms_taylor, p->terms_square);
       }
       if (rc) return rc;
       launch_lgh_dots(s, B, V, R, L, p->Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
@@ -714,6 +820,15 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   }
   if ((rc = check_err_flag(p->err, s, "ctem_plan_run"))) return rc;
   p->ran = true;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_set_mixed_terms(ctem_plan* p, int32_t taylor_terms, int32_t square_terms) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if ((taylor_terms != 3 && taylor_terms != 6) || (square_terms != 3 && square_terms != 6))
+    return fail(CTEM_E_ARG, "split-BF16 terms must be 3 or 6");
+  p->terms_taylor = taylor_terms;
+  p->terms_square = square_terms;
   return CTEM_OK;
 }
 

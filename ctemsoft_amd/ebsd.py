@@ -31,7 +31,7 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_upload_crystal", "ctem_kvectors", "ctem_laue_equiv", "ctem_plan_create",
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
-           "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3"]
+           "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms"]
 
 
 class CtemError(RuntimeError):
@@ -71,7 +71,7 @@ def lib():
             "ctem_plan_run": [P, I32, I32], "ctem_plan_get_dots": [P, P],
             "ctem_plan_get_svals": [P, P], "ctem_plan_get_lgh": [P, P, I32, P],
             "ctem_plan_dynmat": [P, I32, P, I32, P], "ctem_plan_scatter": [P, P, I32, I32, I32, P],
-            "ctem_plan_timing": [P, P, P, P, P],
+            "ctem_plan_timing": [P, P, P, P, P], "ctem_plan_set_mixed_terms": [P, I32, I32],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
         }
         for name, args in sig.items():
@@ -248,6 +248,9 @@ class Plan:
         w = np.zeros(max(nw, 1), np.int32); wsg = np.zeros(max(nw, 1), np.float32)
         _check(lib().ctem_plan_get_beams(self._h, ik, _p(s), _p(ssg), _p(w), _p(wsg)))
         return s, ssg, w[:nw], wsg[:nw]
+
+    def set_mixed_terms(self, taylor=6, square=6):
+        _check(lib().ctem_plan_set_mixed_terms(self._h, int(taylor), int(square)))
 
     def run(self, precision="fp64", keep_lgh=-1):
         _check(lib().ctem_plan_run(self._h, PREC[precision], int(keep_lgh)))
