@@ -74,6 +74,7 @@ double inv_fact(int j) { double f = 1.0; for (int i = 2; i <= j; i++) f *= i; re
 
 struct ctem_ctx {
   int device = 0;
+  int cu_count = 256;
   hipStream_t stream = nullptr;
   bool have_crystal = false;
   // crystal on the device
@@ -144,6 +145,11 @@ extern "C" int32_t ctem_ctx_create(int32_t device, ctem_ctx** out) {
   HIPCHK(hipSetDevice(device));
   ctem_ctx* c = new ctem_ctx();
   c->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->cu_count = prop.multiProcessorCount;
+  }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete c; return fail(CTEM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   (void)hipEventCreate(&c->ev[0]);
@@ -541,7 +547,7 @@ constexpr int kSmax = 48;
 // (for the algorithmic flop count), h_sq receives the squaring counts.
 int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                         double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
-                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq) {
+                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
   double2* A2 = W;
   double2* A3 = W + per * nb;
@@ -600,33 +606,9 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
   }
   launch_parity_copy(s, npad, nb, d_sq, H, T);
   HIPCHK(hipGetLastError());
-  // depth: Psi[:,0] = E e_1, then doubling with E^(2^j)
-  launch_first_col(s, npad, nb, T, Psi, spsi);
+  // depth: psi_k = E^k e_1 by the cache-resident recursion
+  launch_depth(s, npad, nb, nz, T, 0, 0, Psi, spsi, depth_grid);
   HIPCHK(hipGetLastError());
-  double2* cur = T;
-  double2* alt = A2;
-  int m = 1;
-  while (m < nz) {
-    const int cnt = std::min(m, nz - m);
-    ZGemm g{};
-    g.m = npad; g.n = cnt; g.k = npad;
-    g.ldx = npad; g.ldy = npad; g.ldc = npad; g.ldm = npad;
-    g.X = cur; g.sx = per; g.Y = Psi; g.sy = spsi; g.C = Psi + (int64_t)m * npad; g.sc = spsi;
-    g.alpha = 1.0; g.nb = nb;
-    double n2 = 0.0;
-    for (int b = 0; b < nb; b++) n2 += (double)h_n[b] * h_n[b];
-    GemmTimer t{tp, s};
-    t.begin();
-    launch_zgemm(s, g);
-    t.end();
-    HIPCHK(hipGetLastError());
-    if (tp) { tp->f_gemm += 8.0 * n2 * cnt; tp->n_gemm++; }
-    if (m + cnt < nz) {
-      if ((rc = sq_gemm(alt, cur, cur, 0.0, {}, nullptr, 0, f3))) return rc;
-      std::swap(cur, alt);
-    }
-    m += cnt;
-  }
   return CTEM_OK;
 }
 
@@ -640,9 +622,9 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 constexpr double kThetaMixed = 0.25;
 
 int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A64, float2* W32,
-                         float2* Psi32, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
+                         double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
                          int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
-                         int terms_taylor, int terms_square) {
+                         int terms_taylor, int terms_square, int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
   float2* B = W32;
   float2* G2 = W32 + per * nb;
@@ -657,8 +639,8 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   HIPCHK(hipStreamSynchronize(s));
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
-  double n3 = 0.0, n2 = 0.0;
-  for (int b = 0; b < nb; b++) { n3 += (double)h_n[b] * h_n[b] * h_n[b]; n2 += (double)h_n[b] * h_n[b]; }
+  double n3 = 0.0;
+  for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
   launch_z2c(s, per * nb, A64, B);
   HIPCHK(hipGetLastError());
   auto cg = [&](float2* C, const float2* X, const float2* Y, float betaI,
@@ -700,34 +682,8 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   }
   launch_cparity_copy(s, npad, nb, d_sq, H, F);
   HIPCHK(hipGetLastError());
-  launch_cfirst_col(s, npad, nb, F, Psi32, spsi);
-  HIPCHK(hipGetLastError());
-  float2* cur = F;
-  float2* alt = G2;
-  int m = 1;
-  while (m < nz) {
-    const int cnt = std::min(m, nz - m);
-    {
-      CGemm g{};
-      g.m = npad; g.n = cnt; g.k = npad;
-      g.ldx = g.ldy = g.ldc = g.ldm = npad;
-      g.X = cur; g.sx = per; g.Y = Psi32; g.sy = spsi; g.C = Psi32 + (int64_t)m * npad; g.sc = spsi;
-      g.M[0] = Psi32; g.beta[0] = 1.f; g.sm = spsi;
-      g.alpha = 1.f; g.nb = nb; g.terms = terms_square;
-      GemmTimer t{tp, s};
-      t.begin();
-      launch_cgemm(s, g);
-      t.end();
-      HIPCHK(hipGetLastError());
-      if (tp) { tp->f_gemm += 8.0 * n2 * cnt; tp->n_gemm++; }
-    }
-    if (m + cnt < nz) {
-      if ((rc = cg(alt, cur, cur, 0.f, {{cur, 2.f}}, nullptr, 0, f3, terms_square, npad, per, per, per))) return rc;
-      std::swap(cur, alt);
-    }
-    m += cnt;
-  }
-  launch_c2z(s, spsi * nb, Psi32, Psi64);
+  // depth on E = I + F with FP64 accumulation
+  launch_depth(s, npad, nb, nz, F, 1, 1, Psi64, spsi, depth_grid);
   HIPCHK(hipGetLastError());
   return CTEM_OK;
 }
@@ -794,12 +750,11 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       HIPCHK(hipGetLastError());
       if (precision == CTEM_PREC_FP64) {
         rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
-                             p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq);
+                             p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
-        if ((rc = p->Psi32.ensure((size_t)nb * spsi * sizeof(float2)))) return rc;
-        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(), p->Psi32.as<float2>(),
+        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(),
                               p->Psi.as<double2>(), spsi, p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn,
-                              p->h_sq, p->terms_taylor, p->terms_square);
+                              p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
       }
       if (rc) return rc;
       launch_lgh_dots(s, B, V, R, L, p->Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
@@ -972,7 +927,7 @@ extern "C" int32_t ctem_calclgh3(ctem_ctx* c, const double* dmat, double* lgh, d
   std::vector<int> hn{nn};
   std::vector<int32_t> hsq;
   rc = expm_depth_fp64(nullptr, s, npad, 1, c->s_A.as<double2>(), c->s_W.as<double2>(), c->s_Psi.as<double2>(),
-                       spsi, izz, c->s_sq.as<int32_t>(), c->s_err.as<int32_t>(), hn, hsq);
+                       spsi, izz, c->s_sq.as<int32_t>(), c->s_err.as<int32_t>(), hn, hsq, c->cu_count);
   if (rc) return rc;
   // Lgh through the contraction kernel with no sites: a one-entry plan view
   const int32_t zero4[4] = {0, 0, 0, 0};

@@ -92,6 +92,12 @@ void launch_parity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* 
 void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
                       int64_t spsi);
 
+// depth recursion on the GPU's last-level caches: Psi[b][:,k] = E_b^(k+1) e_1, FP64
+// accumulation; E is double2 (fp32 = 0) or float2 (fp32 = 1) holding E, or E - I when
+// add_identity is set. grid = number of persistent workgroups (normally the CU count).
+void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const void* E, int fp32,
+                  int add_identity, double2* Psi, int64_t spsi, int32_t grid);
+
 // Lgh = (dz/thick) Psi diag(lam) Psi^H and the per-site contraction with S_p(g-h):
 // partial sums per 32x32 tile, reduced in fixed order into dots[dir*nsite + p].
 void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,

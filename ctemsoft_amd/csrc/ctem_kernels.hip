@@ -12,6 +12,8 @@
 //   k_scatter      float32 svals to the Laue-equivalent Lambert pixels (:462-479)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ctem_internal.h"
 #include "ctem_laue.h"
 #include "ctem_sg.h"
@@ -131,6 +133,8 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
   double dsum = 0.0;
   const bool live = (ti * 32 < n) && (tj * 32 < n);
+  // the thread whose row meets one of its columns on the diagonal keeps the |U|^2/s sum
+  const bool diag_owner = (ti == tj) && (((r - cq) & 7) == 0);
   if (live) {
     for (int w0 = 0; w0 < nw; w0 += 32) {
       __syncthreads();
@@ -168,8 +172,10 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
           ar[q] += pr * inv;
           ai[q] += pi * inv;
         }
-        const double m = hypot(a.x, a.y);
-        dsum += m * m / wsgd[w];
+        if (diag_owner) {
+          const double m = hypot(a.x, a.y);
+          dsum += m * m / wsgd[w];
+        }
       }
     }
   }
@@ -284,6 +290,8 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int l16 = lane & 15, l4 = lane >> 4;
+  // waves whose 32x32 sub-tile lies wholly in the padding only help to stage operands
+  const bool wave_live = (bm + wm < g.m) && (bn + wn < g.n);
 
   double4v cr[2][2], ci[2][2];
   for (int i = 0; i < 2; i++)
@@ -311,6 +319,7 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 16; ks += 4) {
+      if (!wave_live) break;
       double a_r[2], a_i[2], b_r[2], b_i[2];
       for (int i = 0; i < 2; i++) {
         a_r[i] = As_r[ks + l4][wm + 16 * i + l16];
@@ -328,6 +337,7 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
     }
     __syncthreads();
   }
+  if (!wave_live) return;
   // epilogue
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
@@ -406,6 +416,80 @@ __global__ void k_first_col(int npad, const double2* __restrict__ E, double2* __
 void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
                       int64_t spsi) {
   hipLaunchKernelGGL(k_first_col, dim3(nb), dim3(256), 0, s, npad, E, Psi, spsi);
+}
+
+// ------------------------------------------------------------------------------------------
+// depth recursion psi_{k+1} = E psi_k, k = 1..nz-1, psi_1 = E e_1 (CTEMEBSDmaster.f90:697-720
+// restated as a propagation). Persistent: one 1024-thread workgroup per CU walks the
+// directions of the chunk, so the live set of E matrices (one per CU) stays in the
+// last-level caches while it is read nz-1 times. Rows are split over 256 threads and the
+// columns in four quarters whose partial sums meet in LDS; accumulation is in FP64 for both
+// storage types. With add_identity the stored matrix is F = E - I (mixed form).
+template <typename CS>
+__device__ __forceinline__ double2 ld2(const CS* p) { const CS v = *p; return make_double2((double)v.x, (double)v.y); }
+
+template <typename CS>
+__global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const CS* __restrict__ E,
+                                                int add_identity, double2* __restrict__ Psi, int64_t spsi) {
+  extern __shared__ double2 sm[];
+  double2* x = sm;              // [npad]
+  double2* part = sm + npad;    // [4][npad]
+  const int tid = threadIdx.x, q = tid >> 8, r0 = tid & 255;
+  const int64_t per = (int64_t)npad * npad;
+  const int cq = (npad + 3) / 4;
+  const int c0 = q * cq, c1 = min(npad, c0 + cq);
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const CS* Eb = E + (int64_t)b * per;
+    double2* P = Psi + (int64_t)b * spsi;
+    __syncthreads();
+    for (int i = tid; i < npad; i += 1024) {
+      double2 v = ld2(Eb + i);
+      if (add_identity && i == 0) v.x += 1.0;
+      x[i] = v;
+      P[i] = v;
+    }
+    __syncthreads();
+    for (int k = 1; k < nz; k++) {
+      for (int r = r0; r < npad; r += 256) {
+        double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+        int c = c0;
+        for (; c + 1 < c1; c += 2) {
+          const double2 e0 = ld2(Eb + (int64_t)c * npad + r), e1 = ld2(Eb + (int64_t)(c + 1) * npad + r);
+          const double2 x0 = x[c], x1 = x[c + 1];
+          ar += e0.x * x0.x - e0.y * x0.y; ai += e0.x * x0.y + e0.y * x0.x;
+          br += e1.x * x1.x - e1.y * x1.y; bi += e1.x * x1.y + e1.y * x1.x;
+        }
+        if (c < c1) {
+          const double2 e0 = ld2(Eb + (int64_t)c * npad + r), x0 = x[c];
+          ar += e0.x * x0.x - e0.y * x0.y; ai += e0.x * x0.y + e0.y * x0.x;
+        }
+        part[q * npad + r] = make_double2(ar + br, ai + bi);
+      }
+      __syncthreads();
+      for (int r = tid; r < npad; r += 1024) {
+        double2 y = part[r];
+        for (int qq = 1; qq < 4; qq++) { const double2 t = part[qq * npad + r]; y.x += t.x; y.y += t.y; }
+        if (add_identity) { y.x += x[r].x; y.y += x[r].y; }
+        P[(int64_t)k * npad + r] = y;
+        part[r] = y;  // staged; copied into x after the barrier
+      }
+      __syncthreads();
+      for (int r = tid; r < npad; r += 1024) x[r] = part[r];
+      __syncthreads();
+    }
+  }
+}
+
+void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const void* E, int fp32,
+                  int add_identity, double2* Psi, int64_t spsi, int32_t grid) {
+  const size_t lds = (size_t)5 * npad * sizeof(double2);
+  const int g = std::min(nb, grid);
+  if (fp32)
+    hipLaunchKernelGGL(k_depth<float2>, dim3(g), dim3(1024), lds, s, npad, nb, nz, (const float2*)E,
+                       add_identity, Psi, spsi);
+  else
+    hipLaunchKernelGGL(k_depth<double2>, dim3(g), dim3(1024), lds, s, npad, nb, nz, (const double2*)E,
+                       add_identity, Psi, spsi);
 }
 
 // ------------------------------------------------------------------------------------------
