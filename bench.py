@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--workload", default="fe_c1w")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--terms", default="3,6", help="split-BF16 terms of the mixed form: taylor,square")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); the directions of a "
@@ -99,6 +100,8 @@ def main():
     ctx.upload_crystal(st)
     plan = ctx.plan(k, kn, lam, wl.dz, thick)
     ns, nw = plan.select()
+    tt, ts = (int(v) for v in a.terms.split(","))
+    plan.set_mixed_terms(tt, ts)
     log(f"strong beams {ns.min()}..{ns.max()} (mean {ns.mean():.1f}); weak {nw.min()}..{nw.max()} (mean {nw.mean():.1f})")
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     res = {}
@@ -145,7 +148,8 @@ def main():
                                f"Bethe {wl.weakcutoff:g}/{wl.cutoff:g}/{wl.sgcutoff:g}, npx {wl.npx}, "
                                f"{wl.hemispheres} hemispheres, {wl.nz} x {wl.dz} nm depth steps",
                    "directions": int(len(k)), "strong_beams_mean": round(float(ns.mean()), 1),
-                   "weak_beams_mean": round(float(nw.mean()), 1), "precision": head},
+                   "weak_beams_mean": round(float(nw.mean()), 1), "precision": head,
+                   "mixed_terms": a.terms},
         "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
         "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,

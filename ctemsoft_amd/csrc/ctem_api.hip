@@ -126,7 +126,7 @@ struct ctem_plan {
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
   int32_t n_gemm = 0;
-  int32_t terms_taylor = 6, terms_square = 6;  // split-BF16 product terms of the mixed form
+  int32_t terms_taylor = 3, terms_square = 6;  // split-BF16 product terms of the mixed form
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
 };

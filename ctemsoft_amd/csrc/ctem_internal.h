@@ -6,6 +6,16 @@
 
 namespace ctem {
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X (XCD = blockIdx % 8),
+// each with its own L2. This maps the hardware block index to a logical index such that
+// consecutive logical indices (the tiles of one matrix) run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_logical_block(int bid, int nblk) {
+  constexpr int X = 8;
+  const int per = nblk / X, rem = nblk % X;
+  const int xcd = bid % X, idx = bid / X;
+  return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
 // Reflection table on the device, structure of arrays in master-list order.
 struct DevRefl {
   int32_t nref;

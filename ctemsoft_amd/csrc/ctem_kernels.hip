@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   double2* Ab = A + (int64_t)b * npad * npad;
 
   __shared__ int rh[32][3], ch[32][3], wh[32][3];
-  __shared__ double wsgd[32];
+  __shared__ double wsgd[32], winv[32];
   __shared__ double2 Ua[32][33], Ub[32][33];
   const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
   const int gi = ti * 32 + r;
@@ -142,6 +142,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
         const int w = w0 + tid;
         for (int d = 0; d < 3; d++) wh[tid][d] = (w < nw) ? R.hkl[3 * widx[w] + d] : 0;
         wsgd[tid] = (w < nw) ? (double)wsg[w] : 1.0;
+        winv[tid] = 1.0 / wsgd[tid];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
       }
       __syncthreads();
       for (int e = tid; e < 1024; e += 256) {
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
       const int wn = min(32, nw - w0);
       for (int w = 0; w < wn; w++) {
         const double2 a = Ua[r][w];
-        const double inv = 1.0 / wsgd[w];
+        const double inv = winv[w];
         for (int q = 0; q < 4; q++) {
           const double2 bb = Ub[w][cq + 8 * q];
           const double pr = a.x * bb.x - a.y * bb.y;
@@ -278,8 +279,9 @@ void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, doub
 #define ZG_LDS 65
 __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
-  const int b = blockIdx.x / (tm * tn);
-  const int t = blockIdx.x % (tm * tn);
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int b = lb / (tm * tn);
+  const int t = lb % (tm * tn);
   const int bm = (t % tm) * 64, bn = (t / tm) * 64;
   if (g.active && g.step >= g.active[b]) return;
   const double2* X = g.X + (int64_t)b * g.sx;
@@ -500,8 +502,9 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
                                                   double dzt, double* __restrict__ partial,
                                                   int keep_dir, double2* __restrict__ keep_lgh) {
   const int npad = B.npad, tiles = npad / 32;
-  const int b = blockIdx.x / (tiles * tiles);
-  const int t = blockIdx.x % (tiles * tiles);
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int b = lb / (tiles * tiles);
+  const int t = lb % (tiles * tiles);
   const int ti = t % tiles, tj = t / tiles;
   const int dir = B.dirs[b];
   const int n = V.nns[dir];

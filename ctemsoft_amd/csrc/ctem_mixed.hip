@@ -37,8 +37,9 @@ template <int TERMS>
 __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   constexpr int NP = (TERMS == 6) ? 3 : 2;  // bf16 planes per real component
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
-  const int b = blockIdx.x / (tm * tn);
-  const int t = blockIdx.x % (tm * tn);
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int b = lb / (tm * tn);
+  const int t = lb % (tm * tn);
   const int bm = (t % tm) * 64, bn = (t / tm) * 64;
   if (g.active && g.step >= g.active[b]) return;
   const float2* X = g.X + (int64_t)b * g.sx;

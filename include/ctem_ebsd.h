@@ -122,7 +122,7 @@ CTEM_API int32_t ctem_plan_get_beams(ctem_plan* plan, int32_t ik, int32_t* stron
 CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_lgh_ik);
 
 /* Split-BF16 product terms of the mixed form (3: ~2^-16, 6: ~FP32 accuracy) for the
- * Taylor products and for the squarings/depth products; default 6 and 6. */
+ * Taylor products (on the scaled matrix, |B| <= 1/4) and for the squarings; default 3 and 6. */
 CTEM_API int32_t ctem_plan_set_mixed_terms(ctem_plan* plan, int32_t taylor_terms, int32_t square_terms);
 
 /* Re sum_gh Lgh(g,h) Sgh(g,h,p) per direction and site, in double, before the reference's
