@@ -34,6 +34,25 @@ def flags() -> list[str]:
             "-fvisibility=hidden", f"-I{os.path.join(REPO, 'include')}"]
 
 
+def build_fortran(verbose: bool = True) -> str | None:
+    """ISO_C_BINDING interface module and the Fortran boundary check program, linked to the
+    HIP library (amdflang from the ROCm image)."""
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    out = os.path.join(fdir, "_build")
+    os.makedirs(out, exist_ok=True)
+    exe = os.path.join(HERE, "ctem_lgh_check")
+    cmd = [fc, "-O2", f"-J{out}", os.path.join(fdir, "ctem_ebsd_iface.f90"),
+           os.path.join(fdir, "ctem_lgh_check.f90"), f"-L{HERE}", "-lctem_ebsd",
+           "-Wl,-rpath,$ORIGIN", "-o", exe]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=out)
+    return exe
+
+
 def build(verbose: bool = True) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
@@ -54,6 +73,7 @@ def build(verbose: bool = True) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 
