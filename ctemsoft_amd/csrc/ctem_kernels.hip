@@ -299,29 +299,37 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++) { cr[i][j] = (double4v){0, 0, 0, 0}; ci[i][j] = (double4v){0, 0, 0, 0}; }
 
-  for (int k0 = 0; k0 < g.k; k0 += 16) {
-    // A tile: 64 rows x 16 k, column-major source
+  // register-prefetch pipeline: the global loads of chunk k0+16 are in flight while the
+  // MFMAs of chunk k0 run from LDS
+  double2 ra[4], rb[4];
+  auto gload = [&](int k0) {
     for (int r = 0; r < 4; r++) {
       const int mm = tid & 63, kk = (tid >> 6) + 4 * r;
       const int row = bm + mm, col = k0 + kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (row < g.m && col < g.k) v = X[(int64_t)col * g.ldx + row];
-      As_r[kk][mm] = v.x;
-      As_i[kk][mm] = v.y;
+      ra[r] = (row < g.m && col < g.k) ? X[(int64_t)col * g.ldx + row] : make_double2(0.0, 0.0);
     }
-    // B tile: 16 k x 64 cols
     for (int r = 0; r < 4; r++) {
       const int kk = tid & 15, nn = (tid >> 4) + 16 * r;
       const int row = k0 + kk, col = bn + nn;
-      double2 v = make_double2(0.0, 0.0);
-      if (row < g.k && col < g.n) v = Y[(int64_t)col * g.ldy + row];
-      Bs_r[kk][nn] = v.x;
-      Bs_i[kk][nn] = v.y;
+      rb[r] = (row < g.k && col < g.n) ? Y[(int64_t)col * g.ldy + row] : make_double2(0.0, 0.0);
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < g.k; k0 += 16) {
+    __syncthreads();
+    for (int r = 0; r < 4; r++) {
+      const int mm = tid & 63, kk = (tid >> 6) + 4 * r;
+      As_r[kk][mm] = ra[r].x;
+      As_i[kk][mm] = ra[r].y;
+      const int kb = tid & 15, nn = (tid >> 4) + 16 * r;
+      Bs_r[kb][nn] = rb[r].x;
+      Bs_i[kb][nn] = rb[r].y;
     }
     __syncthreads();
+    if (k0 + 16 < g.k) gload(k0 + 16);
+    if (!wave_live) continue;
 #pragma unroll
     for (int ks = 0; ks < 16; ks += 4) {
-      if (!wave_live) break;
       double a_r[2], a_i[2], b_r[2], b_i[2];
       for (int i = 0; i < 2; i++) {
         a_r[i] = As_r[ks + l4][wm + 16 * i + l16];
@@ -337,7 +345,6 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemm g) {
           ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[i], b_r[j], ci[i][j], 0, 0, 0);
         }
     }
-    __syncthreads();
   }
   if (!wave_live) return;
   // epilogue

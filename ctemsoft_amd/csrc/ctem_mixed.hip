@@ -62,44 +62,39 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
       ci[i][j] = (f4v){0, 0, 0, 0};
     }
 
+  // register-prefetch pipeline: thread t stages row/col (t & 63) and k = 8*(t >> 6) .. +7 of
+  // both tiles (16 lanes hit 16 distinct 16-byte bank groups in LDS); the global loads of the
+  // next chunk are in flight during the MFMAs of the current one
+  const int srow = tid & 63, skq = (tid >> 6) * 8;
+  float2 ra[8], rb[8];
+  auto gload = [&](int k0) {
+    for (int j = 0; j < 8; j++) {
+      const int kk = k0 + skq + j;
+      ra[j] = (bm + srow < g.m && kk < g.k) ? X[(int64_t)kk * g.ldx + bm + srow] : make_float2(0.f, 0.f);
+      rb[j] = (bn + srow < g.n && kk < g.k) ? Y[(int64_t)(bn + srow) * g.ldy + kk] : make_float2(0.f, 0.f);
+    }
+  };
+  auto stage = [&](const float2* v, __bf16 (*P)[NP][64][CG_KS]) {
+    __attribute__((aligned(16))) __bf16 pr[NP][8], pi[NP][8];
+    for (int j = 0; j < 8; j++) {
+      __bf16 a, bb, c;
+      split3(v[j].x, a, bb, c);
+      pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
+      split3(v[j].y, a, bb, c);
+      pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
+    }
+    for (int p = 0; p < NP; p++) {
+      *(bf8v*)&P[0][p][srow][skq] = *(bf8v*)pr[p];
+      *(bf8v*)&P[1][p][srow][skq] = *(bf8v*)pi[p];
+    }
+  };
+  gload(0);
   for (int k0 = 0; k0 < g.k; k0 += 32) {
-    {  // A tile: rows bm..bm+63, k k0..k0+31; thread -> (row, 8 consecutive k)
-      const int row = tid & 63, kq = (tid >> 6) * 8;
-      __attribute__((aligned(16))) __bf16 pr[NP][8], pi[NP][8];
-      for (int j = 0; j < 8; j++) {
-        const int kk = k0 + kq + j;
-        float2 v = make_float2(0.f, 0.f);
-        if (bm + row < g.m && kk < g.k) v = X[(int64_t)kk * g.ldx + bm + row];
-        __bf16 a, bb, c;
-        split3(v.x, a, bb, c);
-        pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
-        split3(v.y, a, bb, c);
-        pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
-      }
-      for (int p = 0; p < NP; p++) {
-        *(bf8v*)&As[0][p][row][kq] = *(bf8v*)pr[p];
-        *(bf8v*)&As[1][p][row][kq] = *(bf8v*)pi[p];
-      }
-    }
-    {  // B tile: k k0..k0+31, cols bn..bn+63; thread -> (col, 8 consecutive k)
-      const int col = tid >> 2, kq = (tid & 3) * 8;
-      __attribute__((aligned(16))) __bf16 pr[NP][8], pi[NP][8];
-      for (int j = 0; j < 8; j++) {
-        const int kk = k0 + kq + j;
-        float2 v = make_float2(0.f, 0.f);
-        if (bn + col < g.n && kk < g.k) v = Y[(int64_t)(bn + col) * g.ldy + kk];
-        __bf16 a, bb, c;
-        split3(v.x, a, bb, c);
-        pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
-        split3(v.y, a, bb, c);
-        pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
-      }
-      for (int p = 0; p < NP; p++) {
-        *(bf8v*)&Bs[0][p][col][kq] = *(bf8v*)pr[p];
-        *(bf8v*)&Bs[1][p][col][kq] = *(bf8v*)pi[p];
-      }
-    }
     __syncthreads();
+    stage(ra, As);
+    stage(rb, Bs);
+    __syncthreads();
+    if (k0 + 32 < g.k) gload(k0 + 32);
     if (wave_live) {
       bf8v a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
       for (int i = 0; i < 2; i++)
@@ -125,7 +120,6 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
 #undef MM
         }
     }
-    __syncthreads();
   }
   if (!wave_live) return;
   for (int i = 0; i < 2; i++)
