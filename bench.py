@@ -53,6 +53,19 @@ def directions(wl, st):
     return ij, k, kn
 
 
+def measured_traffic(workload, head):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel), when one exists for this
+    workload and precision."""
+    path = os.path.join(REPO, "profiles", f"{workload}_{head}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_k_cgemm6:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(wl, lam, k_count, seconds=20.0):
     """Reference beam loop (compiled from the reference sources, MKL sequential) on one
     host core over a seeded sample of the workload's northern directions."""
@@ -156,7 +169,8 @@ def main():
         "mixed_max_rel_err_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": round(r["gemm_tflops"], 3),
                      "peak": PEAK[head] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4), "traffic": None,
+                     "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),
+                     "traffic": measured_traffic(wl.name, head),
                      "kernel": "k_zgemm" if head == "fp64" else "k_cgemm",
                      "flops_per_step": gflops},
         "cpu_baseline": cpu,
