@@ -40,15 +40,20 @@ def load_setup(name):
     wl = workloads.get(name)
     path = os.path.join(REPO, "tests", "golden", f"{name}.npz")
     fx = np.load(path)
-    st = ebsd.Setup.from_fixture(fx)
+    # a reference-sized LUT does not hold every g-h of a hexagonal list (the reference reads
+    # outside its array there); use the fixture's full-box LUT when there is one
+    base = os.path.join(REPO, "tests", "golden", f"{name[:-1] if name.endswith('w') else name}.npz")
+    full = np.load(base) if os.path.exists(base) else None
+    st = ebsd.Setup.from_fixture(fx, full if full is not None and "lut_full" in full.files else None)
     return wl, st, fx
 
 
 def directions(wl, st):
     from ctemsoft_amd import ebsd
-    ij, k, kn = ebsd.kvectors(wl.npx, wl.npx, st.isym, st.usehex, st.dsm, st.mlambda, 1)
+    npx, npy = ebsd.lambert_dims(wl.npx, st.usehex)
+    ij, k, kn = ebsd.kvectors(npx, npy, st.isym, st.usehex, st.dsm, st.mlambda, 1)
     if wl.hemispheres == 2:
-        ij2, k2, kn2 = ebsd.kvectors(wl.npx, wl.npx, st.isym, st.usehex, st.dsm, st.mlambda, -1)
+        ij2, k2, kn2 = ebsd.kvectors(npx, npy, st.isym, st.usehex, st.dsm, st.mlambda, -1)
         ij = np.concatenate([ij, ij2]); k = np.concatenate([k, k2]); kn = np.concatenate([kn, kn2])
     return ij, k, kn
 
@@ -93,7 +98,8 @@ def main():
     ap.add_argument("--workload", default="fe_c1w")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--terms", default="3,6", help="split-BF16 terms of the mixed form: taylor,square")
+    ap.add_argument("--terms", default="16,16",
+                    help="split schemes of the mixed form, taylor,square: 3 = BF16x3, 6 = BF16x6, 16 = FP16x3")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); the directions of a "
@@ -142,10 +148,19 @@ def main():
             f"({res[prec]['gemm_tflops']:.2f} TFLOP/s over {g['gemm_launches']} launches)")
     head = "mixed" if "mixed" in res else "fp64"
     r = res[head]
-    err = None
+    err, nonfinite = None, None
     if "mixed" in res and "fp64" in res:
         d64, dmx = res["fp64"]["dots"], res["mixed"]["dots"]
-        err = float(np.max(np.abs(dmx - d64) / np.abs(d64)))
+        # a direction whose Bethe-corrected matrix grows without bound (a weak beam with an
+        # excitation error near zero) overflows in FP64 as in the reference; it is compared
+        # only for being non-finite in both forms
+        fin = np.isfinite(d64).all(1)
+        nonfinite = int((~fin).sum())
+        if nonfinite:
+            log(f"{nonfinite} directions non-finite in FP64: {np.argwhere(~fin)[:8, 0].tolist()}")
+        err = float(np.max(np.abs(dmx[fin] - d64[fin]) / np.abs(d64[fin]))) if fin.any() else None
+        if np.isfinite(dmx[~fin]).all(1).any():
+            err = float("inf")
     cpu = None if a.no_cpu else cpu_baseline(wl, lam, len(k) // wl.hemispheres)
     gflops = r["gemm"]["gemm_flops"]
     line = {
@@ -155,11 +170,10 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(r["ms"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16-split/f32+f64" if head == "mixed" else "f64",
+        "dtype": (("fp16" if a.terms == "16,16" else "bf16/fp16") + "-split MFMA, f32 accumulate, f64 depth+contraction")
+                 if head == "mixed" else "f64",
         "data": "synthetic wave vectors and depth profile; reference per-energy setup",
-        "config": {"workload": f"{wl.name}: bcc Fe Im-3m a=0.28665 nm, 20 kV, dmin 0.05 nm, "
-                               f"Bethe {wl.weakcutoff:g}/{wl.cutoff:g}/{wl.sgcutoff:g}, npx {wl.npx}, "
-                               f"{wl.hemispheres} hemispheres, {wl.nz} x {wl.dz} nm depth steps",
+        "config": {"workload": f"{wl.name}: {wl.describe()}",
                    "directions": int(len(k)), "strong_beams_mean": round(float(ns.mean()), 1),
                    "weak_beams_mean": round(float(nw.mean()), 1), "precision": head,
                    "mixed_terms": a.terms},
@@ -167,6 +181,7 @@ def main():
         "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,
         "mixed_max_rel_err_vs_fp64": err,
+        "nonfinite_directions": nonfinite if err is not None else None,
         "roofline": {"bound": "mfma", "achieved": round(r["gemm_tflops"], 3),
                      "peak": PEAK[head] / 1e12, "unit": "TFLOP/s",
                      "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),

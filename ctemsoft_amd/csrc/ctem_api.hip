@@ -81,6 +81,8 @@ struct ctem_ctx {
   DBuf g, hkl, cut1, cut2, dbd, lut, slut, rmt;
   int32_t nref = 0, nsite = 0, ncell = 0;
   int32_t lut_lo[3] = {0, 0, 0}, lut_dim[3] = {0, 0, 0};
+  int32_t slut_lo[3] = {0, 0, 0}, slut_dim[3] = {0, 0, 0};
+  int64_t sncell = 0;
   double mlambda = 0.0;
   float dynupz = 0.0f, sgcutoff = 0.0f;
   float nat = 0.0f;  // total orbit size, float(sum(nat))
@@ -96,11 +98,14 @@ struct ctem_ctx {
     R.cut2 = cut2.as<float>(); R.dbd = dbd.as<uint8_t>(); R.sgcutoff = sgcutoff;
     return R;
   }
-  DevLut lutv() const {
+  DevLut lutv(int32_t* err) const {
     DevLut L;
     L.lut = lut.as<float2>(); L.slut = slut.as<double2>();
-    for (int i = 0; i < 3; i++) { L.lo[i] = lut_lo[i]; L.dim[i] = lut_dim[i]; }
-    L.ncell = ncell; L.nsite = nsite;
+    for (int i = 0; i < 3; i++) {
+      L.lo[i] = lut_lo[i]; L.dim[i] = lut_dim[i];
+      L.slo[i] = slut_lo[i]; L.sdim[i] = slut_dim[i];
+    }
+    L.ncell = ncell; L.sncell = sncell; L.nsite = nsite; L.err = err;
     return L;
   }
 };
@@ -121,12 +126,12 @@ struct ctem_plan {
   int32_t keep_dir = -1, keep_n = 0;
   DBuf keep_lgh;
   // work
-  DBuf A, W, Psi, Psi32, sq, dirs, partial;
+  DBuf A, W, Psi, Psi32, sq, dirs, partial, wmax;
   std::vector<int32_t> h_sq;
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
   int32_t n_gemm = 0;
-  int32_t terms_taylor = 3, terms_square = 6;  // split-BF16 product terms of the mixed form
+  int32_t terms_taylor = CTEM_SPLIT_FP16X3, terms_square = CTEM_SPLIT_FP16X3;  // mixed-form split schemes
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
 };
@@ -187,8 +192,11 @@ extern "C" int32_t ctem_device_info(ctem_ctx* c, char* name, int32_t name_len, i
 // cmplx(Z^2 DBWF) * cmplx(cos, sin) in float32, summed in double, for every d = g-h of the
 // LUT box (CTEMEBSDmaster.f90:418-449; DBWF = exp(-B |d|^2/4) with |d| from the float32
 // CalcLength, no occupancy factor).
-static void build_slut(const ctem_crystal* c, int ncell, std::vector<double>& out) {
+// S_p(d) for every d in the box lo..lo+dim-1, site after site, column-major (h fastest)
+static void build_slut(const ctem_crystal* c, const int32_t lo[3], const int32_t dim[3],
+                       std::vector<double>& out) {
 #pragma clang fp contract(off)
+  const size_t ncell = (size_t)dim[0] * dim[1] * dim[2];
   out.assign((size_t)2 * ncell * c->nsite, 0.0);
   const double tpi = 2.0 * kPi;
   size_t obase = 0;
@@ -197,10 +205,10 @@ static void build_slut(const ctem_crystal* c, int ncell, std::vector<double>& ou
     const double znsq = (double)(zf * zf);
     const int no = c->site_norbit[p];
     const double* orb = c->orbit + 3 * obase;
-    for (int l = 0; l < c->lut_dim[2]; l++)
-      for (int kk = 0; kk < c->lut_dim[1]; kk++)
-        for (int h = 0; h < c->lut_dim[0]; h++) {
-          const int d[3] = {h + c->lut_lo[0], kk + c->lut_lo[1], l + c->lut_lo[2]};
+    for (int l = 0; l < dim[2]; l++)
+      for (int kk = 0; kk < dim[1]; kk++)
+        for (int h = 0; h < dim[0]; h++) {
+          const int d[3] = {h + lo[0], kk + lo[1], l + lo[2]};
           const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
           const float len = std::sqrt(dot_r(df, df, c->rmt));
           const float len2 = len * len;
@@ -219,7 +227,7 @@ static void build_slut(const ctem_crystal* c, int ncell, std::vector<double>& ou
             sr += (double)re;
             si += (double)im;
           }
-          const size_t cell = (size_t)h + (size_t)c->lut_dim[0] * ((size_t)kk + (size_t)c->lut_dim[1] * l);
+          const size_t cell = (size_t)h + (size_t)dim[0] * ((size_t)kk + (size_t)dim[1] * l);
           out[2 * ((size_t)p * ncell + cell)] = sr;
           out[2 * ((size_t)p * ncell + cell) + 1] = si;
         }
@@ -242,13 +250,17 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   const int64_t ncell64 = (int64_t)x->lut_dim[0] * x->lut_dim[1] * x->lut_dim[2];
   if (ncell64 > (1ll << 30)) return fail(CTEM_E_RANGE, "LUT too large");
   const int ncell = (int)ncell64;
-  // every difference of two list entries must fall inside the LUT box
+  // The S_p(g-h) table covers every difference of two list entries. The LUT box is the
+  // caller's: the reference's list can hold entries beyond half of it (hexagonal axes), and
+  // the assembly kernel reads the LUT at the reference's column-major offsets, checked
+  // against the allocation (CTEM_E_RANGE when a lookup leaves it).
   int hmax[3] = {0, 0, 0};
   for (int i = 0; i < n; i++)
     for (int d = 0; d < 3; d++) hmax[d] = std::max(hmax[d], std::abs(x->hkl[3 * i + d]));
-  for (int d = 0; d < 3; d++)
-    if (x->lut_lo[d] > -2 * hmax[d] || x->lut_lo[d] + x->lut_dim[d] - 1 < 2 * hmax[d])
-      return fail(CTEM_E_RANGE, "LUT box does not hold all differences g-h along axis %d", d);
+  int32_t slo[3], sdim[3];
+  for (int d = 0; d < 3; d++) { slo[d] = -2 * hmax[d]; sdim[d] = 4 * hmax[d] + 1; }
+  const int64_t sncell = (int64_t)sdim[0] * sdim[1] * sdim[2];
+  if (sncell * std::max(1, x->nsite) > (1ll << 31)) return fail(CTEM_E_RANGE, "reflection list spans too large a box");
   std::vector<float> g(3 * n), cut1(n), cut2(n);
   std::vector<uint8_t> dbd(n);
   for (int i = 0; i < n; i++) {
@@ -262,7 +274,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   std::vector<float> lutf(2 * (size_t)ncell);
   for (size_t i = 0; i < 2 * (size_t)ncell; i++) lutf[i] = (float)x->lut[i];
   std::vector<double> slut;
-  build_slut(x, ncell, slut);
+  build_slut(x, slo, sdim, slut);
   int nat = 0;
   for (int p = 0; p < x->nsite; p++) nat += x->site_norbit[p];
   int32_t rc;
@@ -278,7 +290,11 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   c->nref = n;
   c->nsite = x->nsite;
   c->ncell = ncell;
-  for (int i = 0; i < 3; i++) { c->lut_lo[i] = x->lut_lo[i]; c->lut_dim[i] = x->lut_dim[i]; }
+  for (int i = 0; i < 3; i++) {
+    c->lut_lo[i] = x->lut_lo[i]; c->lut_dim[i] = x->lut_dim[i];
+    c->slut_lo[i] = slo[i]; c->slut_dim[i] = sdim[i];
+  }
+  c->sncell = sncell;
   c->mlambda = x->mlambda;
   c->dynupz = x->dynupz;
   c->sgcutoff = x->sgcutoff;
@@ -441,7 +457,7 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
   (void)hipStreamSynchronize(p->ctx->stream);
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
                   &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->A, &p->W, &p->Psi,
-                  &p->Psi32, &p->sq, &p->dirs, &p->partial})
+                  &p->Psi32, &p->sq, &p->dirs, &p->partial, &p->wmax})
     b->release();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
   auto& v = p->ctx->plans;
@@ -457,6 +473,12 @@ static int32_t check_err_flag(DBuf& err, hipStream_t s, const char* what) {
   if (h & 1) return fail(CTEM_E_RANGE, "%s: an excitation-error angle left the reproduced cosf range (|g| too large for k)", what);
   if (h & 2) return fail(CTEM_E_NUMERIC, "%s: non-finite dynamical matrix", what);
   if (h & 4) return fail(CTEM_E_NUMERIC, "%s: matrix norm needs more squarings than allowed", what);
+  if (h & 16)
+    return fail(CTEM_E_NUMERIC, "%s: a matrix entry left the split-FP16 range (|x| >= 16); "
+                "use CTEM_SPLIT_BF16X6 for this crystal", what);
+  if (h & 8)
+    return fail(CTEM_E_RANGE, "%s: a beam pair's g-h lies outside the LUT allocation (the reference's "
+                "Compute_DynMat reads outside its LUT array there); pass a larger LUT box", what);
   return CTEM_OK;
 }
 
@@ -622,7 +644,7 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 constexpr double kThetaMixed = 0.25;
 
 int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A64, float2* W32,
-                         double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
+                         float* wmax, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
                          int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
                          int terms_taylor, int terms_square, int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
@@ -632,6 +654,14 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   float2* P = W32 + 3 * per * nb;
   float2* F = W32 + 4 * per * nb;
   float2* H = W32 + 5 * per * nb;
+  // per-matrix max |entry| of each buffer (split-FP16 operand scaling)
+  float* mB = wmax;
+  float* mG2 = wmax + nb;
+  float* mG3 = wmax + 2 * nb;
+  float* mP = wmax + 3 * nb;
+  float* mF = wmax + 4 * nb;
+  float* mH = wmax + 5 * nb;
+  HIPCHK(hipMemsetAsync(wmax, 0, 6 * sizeof(float) * (size_t)nb, s));
   launch_norm_scale(s, npad, nb, A64, kThetaMixed, kSmax, d_sq, d_err);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
@@ -641,11 +671,12 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
-  launch_z2c(s, per * nb, A64, B);
+  launch_z2c(s, npad, nb, A64, B, mB);
   HIPCHK(hipGetLastError());
-  auto cg = [&](float2* C, const float2* X, const float2* Y, float betaI,
-                std::initializer_list<std::pair<const float2*, float>> ms, const int32_t* active,
-                int step, double flops, int terms, int ncols, int64_t sy, int64_t sc, int64_t sm) -> int32_t {
+  auto cg = [&](float2* C, float* cmax, const float2* X, const float* xmax, const float2* Y,
+                const float* ymax, float betaI, std::initializer_list<std::pair<const float2*, float>> ms,
+                const int32_t* active, int step, double flops, int terms, int ncols, int64_t sy,
+                int64_t sc, int64_t sm) -> int32_t {
     CGemm g{};
     g.m = npad; g.n = ncols; g.k = npad;
     g.ldx = g.ldy = g.ldc = g.ldm = npad;
@@ -653,7 +684,8 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
     g.alpha = 1.0f; g.betaI = betaI;
     int q = 0;
     for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
-    g.active = active; g.step = step; g.nb = nb; g.terms = terms;
+    g.active = active; g.step = step; g.nb = nb; g.terms = terms; g.err = d_err;
+    g.xmax = xmax; g.ymax = ymax; g.cmax = cmax;
     GemmTimer t{tp, s};
     t.begin();
     launch_cgemm(s, g);
@@ -664,21 +696,28 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   };
   int32_t rc;
   const double f3 = 8.0 * n3;
-  if ((rc = cg(G2, B, B, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
-  if ((rc = cg(G3, B, G2, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  if ((rc = cg(G2, mG2, B, mB, B, mB, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  if ((rc = cg(G3, mG3, B, mB, G2, mG2, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
   {
     const float2* M[4] = {B, G2, G3, nullptr};
     const float beta[4] = {(float)inv_fact(4), (float)inv_fact(5), (float)inv_fact(6), 0.f};
-    launch_ccombo(s, npad, nb, P, (float)inv_fact(3), M, beta);
+    launch_ccombo(s, npad, nb, P, (float)inv_fact(3), M, beta, mP);
     HIPCHK(hipGetLastError());
   }
-  if ((rc = cg(F, G3, P, 0.f, {{B, 1.f}, {G2, 0.5f}}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  if ((rc = cg(F, mF, G3, mG3, P, mP, 0.f, {{B, 1.f}, {G2, 0.5f}}, nullptr, 0, f3, terms_taylor, npad, per, per, per)))
+    return rc;
   float2* buf[2] = {F, H};
+  float* bmax[2] = {mF, mH};
   for (int i = 0; i < smax; i++) {
     double fl = 0.0;
     for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
     float2* X = buf[i & 1];
-    if ((rc = cg(buf[(i + 1) & 1], X, X, 0.f, {{X, 2.f}}, d_sq, i, fl, terms_square, npad, per, per, per))) return rc;
+    float* out_max = bmax[(i + 1) & 1];
+    // the output buffer's maxima were those of two steps back
+    HIPCHK(hipMemsetAsync(out_max, 0, sizeof(float) * (size_t)nb, s));
+    if ((rc = cg(buf[(i + 1) & 1], out_max, X, bmax[i & 1], X, bmax[i & 1], 0.f, {{X, 2.f}}, d_sq, i, fl,
+                 terms_square, npad, per, per, per)))
+      return rc;
   }
   launch_cparity_copy(s, npad, nb, d_sq, H, F);
   HIPCHK(hipGetLastError());
@@ -723,7 +762,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
              p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
   DevRefl R = c->refl();
-  DevLut L = c->lutv();
+  DevLut L = c->lutv(p->err.as<int32_t>());
   const double dzt = p->dz / p->thick;
   hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
   HIPCHK(hipEventRecord(e0, s));
@@ -752,7 +791,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
                              p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
-        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(),
+        if ((rc = p->wmax.ensure(6 * sizeof(float) * (size_t)nb))) return rc;
+        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(), p->wmax.as<float>(),
                               p->Psi.as<double2>(), spsi, p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn,
                               p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
       }
@@ -780,8 +820,9 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
 
 extern "C" int32_t ctem_plan_set_mixed_terms(ctem_plan* p, int32_t taylor_terms, int32_t square_terms) {
   if (!p) return fail(CTEM_E_ARG, "null plan");
-  if ((taylor_terms != 3 && taylor_terms != 6) || (square_terms != 3 && square_terms != 6))
-    return fail(CTEM_E_ARG, "split-BF16 terms must be 3 or 6");
+  auto ok = [](int32_t t) { return t == CTEM_SPLIT_BF16X3 || t == CTEM_SPLIT_BF16X6 || t == CTEM_SPLIT_FP16X3; };
+  if (!ok(taylor_terms) || !ok(square_terms))
+    return fail(CTEM_E_ARG, "split scheme must be CTEM_SPLIT_BF16X3 (3), CTEM_SPLIT_BF16X6 (6) or CTEM_SPLIT_FP16X3 (16)");
   p->terms_taylor = taylor_terms;
   p->terms_square = square_terms;
   return CTEM_OK;
@@ -829,13 +870,17 @@ extern "C" int32_t ctem_plan_dynmat(ctem_plan* p, int32_t ik, double* dynmat, in
   if (!dynmat) return CTEM_OK;
   if (cap < nn * nn) return fail(CTEM_E_ARG, "capacity too small");
   const int npad = npad_of(nn);
-  if ((rc = c->s_A.ensure(sizeof(double2) * (size_t)npad * npad)) || (rc = c->s_dirs.ensure(4))) return rc;
+  if ((rc = c->s_A.ensure(sizeof(double2) * (size_t)npad * npad)) || (rc = c->s_dirs.ensure(4)) ||
+      (rc = c->s_err.ensure(4)))
+    return rc;
   HIPCHK(hipMemcpyAsync(c->s_dirs.p, &ik, 4, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(c->s_err.p, 0, 4, s));
   BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
              p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
   Bucket B{npad, 1, c->s_dirs.as<int32_t>()};
-  launch_dynmat(s, B, V, c->refl(), c->lutv(), c->mlambda, c->dynupz, c->s_A.as<double2>());
+  launch_dynmat(s, B, V, c->refl(), c->lutv(c->s_err.as<int32_t>()), c->mlambda, c->dynupz, c->s_A.as<double2>());
   HIPCHK(hipGetLastError());
+  if ((rc = check_err_flag(c->s_err, s, "ctem_plan_dynmat"))) return rc;
   std::vector<double2> h((size_t)npad * npad);
   HIPCHK(hipMemcpyAsync(h.data(), c->s_A.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -29,9 +29,12 @@ struct DevRefl {
 
 struct DevLut {
   const float2* lut;     // complex LUT values (float32, exactly the reference's values)
-  const double2* slut;   // [nsite * ncell] per-site structure-matrix table S_p(g-h)
-  int32_t lo[3], dim[3];
+  int32_t lo[3], dim[3]; // Fortran bounds of the LUT box
   int32_t ncell;
+  int32_t* err;          // bit 8: a lookup fell outside the LUT allocation
+  const double2* slut;   // [nsite * sncell] per-site structure-matrix table S_p(g-h)
+  int32_t slo[3], sdim[3];  // its box, which holds every difference of two list entries
+  int64_t sncell;
   int32_t nsite;
 };
 
@@ -120,7 +123,8 @@ void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double
                     int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym, float* sr);
 
 // ---- mixed precision (ctem_mixed.hip) ----
-// Complex GEMM on split-BF16 MFMA with FP32 accumulation for FP32 complex matrices
+// Complex GEMM on split 16-bit MFMA (BF16 or FP16 planes) with FP32 accumulation for FP32
+// complex matrices
 // (float2, column-major); same argument meaning as ZGemm.
 struct CGemm {
   int32_t m, n, k;
@@ -133,16 +137,20 @@ struct CGemm {
   const float2* M[4]; float beta[4]; int64_t sm;
   const int32_t* active; int32_t step;
   int32_t nb;
-  int32_t terms;         // split-BF16 product terms: 3 (hi*hi+hi*lo+lo*hi) or 6 (three-way)
+  int32_t terms;         // split scheme: CTEM_SPLIT_BF16X3, CTEM_SPLIT_BF16X6 or CTEM_SPLIT_FP16X3
+  int32_t* err;          // bit 16: a finite scaled entry outside the FP16 range
+  const float* xmax;     // [nb] max |entry| of X and Y per matrix (split-FP16 scaling)
+  const float* ymax;
+  float* cmax;           // [nb] receives max |entry| of C per matrix (zeroed by the caller), or null
 };
 void launch_cgemm(hipStream_t s, const CGemm& g);
-void launch_z2c(hipStream_t s, int64_t count, const double2* src, float2* dst);
-void launch_c2z(hipStream_t s, int64_t count, const float2* src, double2* dst);
+// FP64 -> FP32 copy of nb npad x npad matrices; dmax[b] (zeroed by the caller) receives the
+// max |entry| of matrix b
+void launch_z2c(hipStream_t s, int32_t npad, int32_t nb, const double2* src, float2* dst, float* dmax);
+// C = betaI*I + sum_j beta_j*M_j per matrix, with the max |entry| into cmax[b]
 void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
-                   const float2* const M[4], const float beta[4]);
+                   const float2* const M[4], const float beta[4], float* cmax);
 void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                          const float2* src, float2* dst);
-void launch_cfirst_col(hipStream_t s, int32_t npad, int32_t nb, const float2* E, float2* Psi,
-                       int64_t spsi);
 
 }  // namespace ctem

@@ -96,8 +96,24 @@ void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& 
 
 // ------------------------------------------------------------------------------------------
 // DynMat assembly
-__device__ __forceinline__ int lut_index(const DevLut& L, int h, int k, int l) {
-  return (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
+// U(g-h) as Compute_DynMat reads it: LUT(h,k,l) at the column-major offset of the Fortran
+// bounds, without a per-axis bounds check (gvectors.f90:1133-1160). When a beam set spans
+// more than the LUT box along an axis (hexagonal axes, where list entries reach |h+k| up to
+// twice the box), an offset that still lies inside the array reads the aliased entry, as the
+// reference does; an offset outside the array (which the reference reads from neighbouring
+// heap memory) returns 0 and raises error bit 8.
+__device__ __forceinline__ float2 lut_get(const DevLut& L, int h, int k, int l) {
+  const int idx = (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
+  if ((unsigned)idx >= (unsigned)L.ncell) {
+    atomicOr(L.err, 8);
+    return make_float2(0.0f, 0.0f);
+  }
+  return L.lut[idx];
+}
+
+// S_p(g-h) table index; the table box holds every difference of two list entries
+__device__ __forceinline__ int64_t slut_index(const DevLut& L, int h, int k, int l) {
+  return (int64_t)(h - L.slo[0]) + (int64_t)L.sdim[0] * ((k - L.slo[1]) + (int64_t)L.sdim[1] * (l - L.slo[2]));
 }
 
 __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
@@ -150,11 +166,11 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
         double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
         if (w0 + ww < nw) {
           if (ti * 32 + rr < n) {
-            const float2 u = L.lut[lut_index(L, rh[rr][0] - wh[ww][0], rh[rr][1] - wh[ww][1], rh[rr][2] - wh[ww][2])];
+            const float2 u = lut_get(L, rh[rr][0] - wh[ww][0], rh[rr][1] - wh[ww][1], rh[rr][2] - wh[ww][2]);
             va = make_double2(u.x, u.y);
           }
           if (tj * 32 + rr < n) {
-            const float2 u = L.lut[lut_index(L, wh[ww][0] - ch[rr][0], wh[ww][1] - ch[rr][1], wh[ww][2] - ch[rr][2])];
+            const float2 u = lut_get(L, wh[ww][0] - ch[rr][0], wh[ww][1] - ch[rr][1], wh[ww][2] - ch[rr][2]);
             vb = make_double2(u.x, u.y);
           }
         }
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
     if (gi < n && gj < n) {
       if (gi != gj) {
         const int c = cq + 8 * q;
-        const float2 u = L.lut[lut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2])];
+        const float2 u = lut_get(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]);
         vr = (double)u.x - half * ar[q];
         vi = (double)u.y - half * ai[q];
       } else {
@@ -570,9 +586,9 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
     const int gj = tj * 32 + c;
     if (live && gi < n && gj < n) {
       const double lr = dzt * acc_r[q], li = dzt * acc_i[q];
-      const int cell = lut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]);
+      const int64_t cell = nsite ? slut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]) : 0;
       for (int p = 0; p < nsite && p < 4; p++) {
-        const double2 sv = L.slut[(int64_t)p * L.ncell + cell];
+        const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
         part[p] += lr * sv.x - li * sv.y;
       }
       if (dir == keep_dir) keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);

@@ -1,11 +1,18 @@
-// Mixed-precision kernels: complex GEMM on split-BF16 MFMA (v_mfma_f32_16x16x32_bf16,
-// gfx950) with FP32 accumulation, for FP32 complex matrices.
+// Mixed-precision kernels: complex GEMM on split 16-bit MFMA (v_mfma_f32_16x16x32_bf16 and
+// v_mfma_f32_16x16x32_f16, gfx950) with FP32 accumulation, for FP32 complex matrices.
 //
-// Each FP32 operand x is split into bfloat16 terms x = x0 + x1 + x2 + O(2^-24 |x|)
+// Split-BF16: each FP32 operand x is split into bfloat16 terms x = x0 + x1 + x2 + O(2^-24 |x|)
 // (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)). A real product is then
-//   terms = 3:  x0*y0 + x0*y1 + x1*y0                       (~2^-16 relative)
-//   terms = 6:  the above + x0*y2 + x1*y1 + x2*y0           (~FP32 accuracy)
-// each term one MFMA with FP32 accumulation. Complex products use four real products
+//   scheme 3:  x0*y0 + x0*y1 + x1*y0                       (~2^-16 relative)
+//   scheme 6:  the above + x0*y2 + x1*y1 + x2*y0           (~FP32 accuracy)
+// Split-FP16 (scheme 16): each operand matrix is scaled by 2^e with e = 14 - ceil(log2 max|x|)
+// (max over the matrix, tracked by the kernel that wrote it), so that every scaled entry lies
+// below 2^14 < 65504; then x 2^e = x0 + x1 + O(2^-22 |x| 2^e) with x0 = fp16(x 2^e),
+// x1 = fp16(x 2^e - x0). The product x0*y0 + x0*y1 + x1*y0 has ~2^-21 error relative to
+// max|x| max|y|, 32 times better than split-BF16 scheme 3 for the same three MFMAs, because
+// FP16 carries 11 significand bits to BF16's 8; the scaling keeps residuals of entries down
+// to ~2^-24 max|x| in the normal FP16 range. Non-finite entries pass through as in FP32.
+// Each term is one MFMA with FP32 accumulation. Complex products use four real products
 // (re*re, im*im, re*im, im*re) into three accumulators (re+, re-, im).
 //
 // Tile: 64x64 per workgroup of 4 waves (32x32 per wave = 2x2 MFMA blocks), K chunks of 32
@@ -16,26 +23,62 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
+#include "ctem_ebsd.h"
 #include "ctem_internal.h"
 
 namespace ctem {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+template <bool F16> struct Split;
+template <> struct Split<false> {
+  typedef __bf16 E;
+  typedef bf8v V;
+  __device__ static f4v mma(V a, V b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <> struct Split<true> {
+  typedef _Float16 E;
+  typedef h8v V;
+  __device__ static f4v mma(V a, V b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
 
 #define CG_KS 40  // LDS row stride in bf16 elements (32 used, padded against bank conflicts)
 
-__device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c) {
-  a = (__bf16)x;
+template <typename E>
+__device__ __forceinline__ void split3(float x, E& a, E& b, E& c) {
+  a = (E)x;
   const float r1 = x - (float)a;
-  b = (__bf16)r1;
+  b = (E)r1;
   const float r2 = r1 - (float)b;
-  c = (__bf16)r2;
+  c = (E)r2;
 }
 
-template <int TERMS>
+// NaN-propagating max of non-negative values
+__device__ __forceinline__ float nmax(float a, float b) { return (a != a) ? a : ((b != b) ? b : fmaxf(a, b)); }
+
+// per-matrix max |entry|: wave reduction, one atomic per wave (non-negative floats order as
+// their bit patterns, a NaN above every finite value)
+__device__ __forceinline__ void wave_max_to(float v, float* slot) {
+  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax((unsigned int*)slot, __float_as_uint(v));
+  if ((threadIdx.x & 63) == 0 && v != v) atomicMax((unsigned int*)slot, 0x7fc00000u);
+}
+
+// scaling exponent of the split-FP16 operands: max |x| 2^e lies in [2^13, 2^14)
+__device__ __forceinline__ int f16_scale_exp(float m) {
+  if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0;
+  int q;
+  frexpf(m, &q);
+  return 14 - q;
+}
+
+template <int TERMS, bool F16>
 __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
-  constexpr int NP = (TERMS == 6) ? 3 : 2;  // bf16 planes per real component
+  typedef typename Split<F16>::E E;
+  typedef typename Split<F16>::V V;
+  constexpr int NP = (TERMS == 6) ? 3 : 2;  // 16-bit planes per real component
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int b = lb / (tm * tn);
@@ -45,10 +88,15 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   const float2* X = g.X + (int64_t)b * g.sx;
   const float2* Y = g.Y + (int64_t)b * g.sy;
   float2* C = g.C + (int64_t)b * g.sc;
+  int ex = 0, ey = 0;
+  if constexpr (F16) {
+    ex = f16_scale_exp(g.xmax[b]);
+    ey = f16_scale_exp(g.ymax[b]);
+  }
 
   // planes: [component re/im][term][row or col][k]
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][NP][64][CG_KS];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NP][64][CG_KS];
+  __shared__ __attribute__((aligned(16))) E As[2][NP][64][CG_KS];
+  __shared__ __attribute__((aligned(16))) E Bs[2][NP][64][CG_KS];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int l16 = lane & 15, l4 = lane >> 4;
@@ -74,38 +122,47 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
       rb[j] = (bn + srow < g.n && kk < g.k) ? Y[(int64_t)(bn + srow) * g.ldy + kk] : make_float2(0.f, 0.f);
     }
   };
-  auto stage = [&](const float2* v, __bf16 (*P)[NP][64][CG_KS]) {
-    __attribute__((aligned(16))) __bf16 pr[NP][8], pi[NP][8];
+  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], int e) {
+    __attribute__((aligned(16))) E pr[NP][8], pi[NP][8];
+    bool bad = false;
     for (int j = 0; j < 8; j++) {
-      __bf16 a, bb, c;
-      split3(v[j].x, a, bb, c);
+      E a, bb, c;
+      float xr = v[j].x, xi = v[j].y;
+      if constexpr (F16) {
+        xr = ldexpf(xr, e);
+        xi = ldexpf(xi, e);
+        // finite entries above the range mean a producer did not track its maximum
+        bad |= (fabsf(xr) >= 65504.0f && fabsf(xr) <= 3.4e38f) || (fabsf(xi) >= 65504.0f && fabsf(xi) <= 3.4e38f);
+      }
+      split3(xr, a, bb, c);
       pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
-      split3(v[j].y, a, bb, c);
+      split3(xi, a, bb, c);
       pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
     }
+    if (bad && g.err) atomicOr(g.err, 16);
     for (int p = 0; p < NP; p++) {
-      *(bf8v*)&P[0][p][srow][skq] = *(bf8v*)pr[p];
-      *(bf8v*)&P[1][p][srow][skq] = *(bf8v*)pi[p];
+      *(V*)&P[0][p][srow][skq] = *(V*)pr[p];
+      *(V*)&P[1][p][srow][skq] = *(V*)pi[p];
     }
   };
   gload(0);
   for (int k0 = 0; k0 < g.k; k0 += 32) {
     __syncthreads();
-    stage(ra, As);
-    stage(rb, Bs);
+    stage(ra, As, ex);
+    stage(rb, Bs, ey);
     __syncthreads();
     if (k0 + 32 < g.k) gload(k0 + 32);
     if (wave_live) {
-      bf8v a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
+      V a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
       for (int i = 0; i < 2; i++)
         for (int c = 0; c < 2; c++)
           for (int p = 0; p < NP; p++) {
-            a[i][c][p] = *(const bf8v*)&As[c][p][wm + 16 * i + l16][8 * l4];
-            bq[i][c][p] = *(const bf8v*)&Bs[c][p][wn + 16 * i + l16][8 * l4];
+            a[i][c][p] = *(const V*)&As[c][p][wm + 16 * i + l16][8 * l4];
+            bq[i][c][p] = *(const V*)&Bs[c][p][wn + 16 * i + l16][8 * l4];
           }
       for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) {
-#define MM(acc, ca, pa, cb, pb) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][ca][pa], bq[j][cb][pb], acc, 0, 0, 0)
+#define MM(acc, ca, pa, cb, pb) acc = Split<F16>::mma(a[i][ca][pa], bq[j][cb][pb], acc)
           // smallest terms first
           if constexpr (TERMS == 6) {
             MM(crp[i][j], 0, 2, 0, 0); MM(crp[i][j], 0, 1, 0, 1); MM(crp[i][j], 0, 0, 0, 2);
@@ -122,14 +179,17 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
     }
   }
   if (!wave_live) return;
+  // undo the operand scaling (a power of two)
+  const float alpha = ldexpf(g.alpha, -(ex + ey));
+  float vmax = 0.0f;
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
       for (int e = 0; e < 4; e++) {
         const int row = bm + wm + 16 * i + 4 * l4 + e;
         const int col = bn + wn + 16 * j + l16;
         if (row < g.m && col < g.n) {
-          float vr = g.alpha * (crp[i][j][e] - crn[i][j][e]);
-          float vi = g.alpha * ci[i][j][e];
+          float vr = alpha * (crp[i][j][e] - crn[i][j][e]);
+          float vi = alpha * ci[i][j][e];
           if (row == col) vr += g.betaI;
           for (int q = 0; q < 4; q++)
             if (g.M[q]) {
@@ -138,52 +198,62 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
               vi += g.beta[q] * mv.y;
             }
           C[(int64_t)col * g.ldc + row] = make_float2(vr, vi);
+          vmax = nmax(vmax, nmax(fabsf(vr), fabsf(vi)));
         }
       }
+  if (g.cmax) wave_max_to(vmax, g.cmax + b);
 }
 
 void launch_cgemm(hipStream_t s, const CGemm& g) {
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
-  if (g.terms == 6)
-    hipLaunchKernelGGL(k_cgemm<6>, dim3(tm * tn * g.nb), dim3(256), 0, s, g);
+  if (g.terms == CTEM_SPLIT_FP16X3)
+    hipLaunchKernelGGL((k_cgemm<3, true>), dim3(tm * tn * g.nb), dim3(256), 0, s, g);
+  else if (g.terms == CTEM_SPLIT_BF16X6)
+    hipLaunchKernelGGL((k_cgemm<6, false>), dim3(tm * tn * g.nb), dim3(256), 0, s, g);
   else
-    hipLaunchKernelGGL(k_cgemm<3>, dim3(tm * tn * g.nb), dim3(256), 0, s, g);
+    hipLaunchKernelGGL((k_cgemm<3, false>), dim3(tm * tn * g.nb), dim3(256), 0, s, g);
 }
 
 // ------------------------------------------------------------------------------------------
-__global__ void k_z2c(int64_t n, const double2* __restrict__ a, float2* __restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = make_float2((float)a[i].x, (float)a[i].y);
+// FP64 -> FP32 matrices, one grid row per matrix, with the per-matrix max |entry|
+__global__ void k_z2c(int64_t per, const double2* __restrict__ a, float2* __restrict__ c,
+                      float* __restrict__ cmax) {
+  const int64_t base = (int64_t)blockIdx.y * per;
+  float vmax = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = make_float2((float)a[base + i].x, (float)a[base + i].y);
+    c[base + i] = v;
+    vmax = nmax(vmax, nmax(fabsf(v.x), fabsf(v.y)));
+  }
+  wave_max_to(vmax, cmax + blockIdx.y);
 }
-__global__ void k_c2z(int64_t n, const float2* __restrict__ a, double2* __restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = make_double2(a[i].x, a[i].y);
-}
-static int grid_for(int64_t n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
-void launch_z2c(hipStream_t s, int64_t count, const double2* src, float2* dst) {
-  if (count > 0) hipLaunchKernelGGL(k_z2c, dim3(grid_for(count)), dim3(256), 0, s, count, src, dst);
-}
-void launch_c2z(hipStream_t s, int64_t count, const float2* src, double2* dst) {
-  if (count > 0) hipLaunchKernelGGL(k_c2z, dim3(grid_for(count)), dim3(256), 0, s, count, src, dst);
+static int slot_grid(int64_t per) { return (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64); }
+void launch_z2c(hipStream_t s, int32_t npad, int32_t nb, const double2* src, float2* dst, float* dmax) {
+  const int64_t per = (int64_t)npad * npad;
+  if (nb > 0) hipLaunchKernelGGL(k_z2c, dim3(slot_grid(per), nb), dim3(256), 0, s, per, src, dst, dmax);
 }
 
 struct CComboArgs { const float2* M[4]; float beta[4]; };
-__global__ void k_ccombo(int npad, int nb, float2* __restrict__ C, float betaI, CComboArgs a) {
+__global__ void k_ccombo(int npad, float2* __restrict__ C, float betaI, CComboArgs a, float* __restrict__ cmax) {
   const int64_t per = (int64_t)npad * npad;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < per * nb; e += (int64_t)gridDim.x * 256) {
-    const int64_t loc = e % per;
+  const int64_t base = (int64_t)blockIdx.y * per;
+  float vmax = 0.0f;
+  for (int64_t loc = (int64_t)blockIdx.x * 256 + threadIdx.x; loc < per; loc += (int64_t)gridDim.x * 256) {
     const int row = (int)(loc % npad), col = (int)(loc / npad);
     float vr = (row == col) ? betaI : 0.f, vi = 0.f;
     for (int q = 0; q < 4; q++)
-      if (a.M[q]) { const float2 v = a.M[q][e]; vr += a.beta[q] * v.x; vi += a.beta[q] * v.y; }
-    C[e] = make_float2(vr, vi);
+      if (a.M[q]) { const float2 v = a.M[q][base + loc]; vr += a.beta[q] * v.x; vi += a.beta[q] * v.y; }
+    C[base + loc] = make_float2(vr, vi);
+    vmax = nmax(vmax, nmax(fabsf(vr), fabsf(vi)));
   }
+  wave_max_to(vmax, cmax + blockIdx.y);
 }
 void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
-                   const float2* const M[4], const float beta[4]) {
+                   const float2* const M[4], const float beta[4], float* cmax) {
   CComboArgs a;
   for (int q = 0; q < 4; q++) { a.M[q] = M[q]; a.beta[q] = beta[q]; }
-  hipLaunchKernelGGL(k_ccombo, dim3(grid_for((int64_t)npad * npad * nb)), dim3(256), 0, s, npad, nb, C, betaI, a);
+  const int64_t per = (int64_t)npad * npad;
+  hipLaunchKernelGGL(k_ccombo, dim3(slot_grid(per), nb), dim3(256), 0, s, npad, C, betaI, a, cmax);
 }
 
 __global__ void k_cparity_copy(int npad, const int* __restrict__ sq, const float2* __restrict__ src,
@@ -199,20 +269,6 @@ void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t*
   const int64_t per = (int64_t)npad * npad;
   const int gx = (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64);
   hipLaunchKernelGGL(k_cparity_copy, dim3(gx, nb), dim3(256), 0, s, npad, sq, src, dst);
-}
-
-// Psi[:,0] = (I + F)[:,0]: first column of F plus e_1
-__global__ void k_cfirst_col(int npad, const float2* __restrict__ F, float2* __restrict__ Psi, int64_t spsi) {
-  const int b = blockIdx.x;
-  for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-    float2 v = F[(int64_t)b * npad * npad + i];
-    if (i == 0) v.x += 1.0f;
-    Psi[(int64_t)b * spsi + i] = v;
-  }
-}
-void launch_cfirst_col(hipStream_t s, int32_t npad, int32_t nb, const float2* E, float2* Psi,
-                       int64_t spsi) {
-  hipLaunchKernelGGL(k_cfirst_col, dim3(nb), dim3(256), 0, s, npad, E, Psi, spsi);
 }
 
 }  // namespace ctem

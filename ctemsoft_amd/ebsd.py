@@ -91,6 +91,16 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def lambert_dims(npx, usehex):
+    """(npx, npy) of the Lambert sampling grid as the reference program sets them up
+    (CTEMEBSDmaster.f90:270-271, 388-400): npy = npx for the square grid; for the hexagonal
+    grid npy = nint(2 npx / sqrt(3)) evaluated in float32."""
+    if usehex:
+        v = np.float32(2.0) * np.float32(npx) / np.sqrt(np.float32(3.0))
+        return int(npx), int(np.floor(float(v) + 0.5))
+    return int(npx), int(npx)
+
+
 def kvectors(npx, npy, isym, usehex, dsm, mlambda, hemisphere=1):
     """Incident directions of Calckvectors 'RoscaLambert' (host computation in the library)."""
     L = lib()
@@ -138,12 +148,20 @@ class Setup:
         self.isym, self.usehex = isym, usehex
 
     @classmethod
-    def from_fixture(cls, fx):
+    def from_fixture(cls, fx, lut_from=None):
+        """Setup arrays from a fixture (.npz). With lut_from (a fixture of the same crystal and
+        energy that holds `lut_full`), the LUT is the one over the box that holds every
+        difference of two list entries instead of the reference-sized one."""
         lut = np.asarray(fx["lut"])
         lo = np.asarray(fx["lut_lo"])
         hkl = np.asarray(fx["hkl"], np.int32)
         h, k, l = (hkl - lo).T
         dbd = np.asarray(fx["dbdiff"])[l, k, h]
+        if lut_from is not None:
+            if not np.array_equal(np.asarray(lut_from["hkl"]), np.asarray(fx["hkl"])):
+                raise ValueError("lut_from holds another reflection list")
+            lut = np.asarray(lut_from["lut_full"])
+            lo = np.asarray(lut_from["lut_full_lo"])
         sites = [dict(Z=int(fx[f"site{i}_Z"]), B=float(fx[f"site{i}_pos"][4]),
                       orbit=np.asarray(fx[f"site{i}_orbit"])) for i in range(int(fx["nsite"]))]
         wc, c, sgc = (float(v) for v in fx["cutoffs"])
@@ -249,7 +267,8 @@ class Plan:
         _check(lib().ctem_plan_get_beams(self._h, ik, _p(s), _p(ssg), _p(w), _p(wsg)))
         return s, ssg, w[:nw], wsg[:nw]
 
-    def set_mixed_terms(self, taylor=6, square=6):
+    def set_mixed_terms(self, taylor=16, square=16):
+        """Split schemes of the mixed form: 3 = BF16x3, 6 = BF16x6, 16 = FP16x3 (default)."""
         _check(lib().ctem_plan_set_mixed_terms(self._h, int(taylor), int(square)))
 
     def run(self, precision="fp64", keep_lgh=-1):

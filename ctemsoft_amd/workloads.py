@@ -76,6 +76,21 @@ class Workload:
         w = w / w.sum()
         return w.astype(np.float32)
 
+    def describe(self) -> str:
+        """One line naming the crystal, energy, grid, Bethe cutoffs and depth sampling."""
+        x = self.xtal
+        sg = {225: "Fm-3m", 229: "Im-3m", 167: "R-3c"}.get(x["sg"], f"SG {x['sg']}")
+        cell = (f"a={x['abc'][0]:g} nm" if x["system"] == 1 else
+                f"a={x['abc'][0]:g} c={x['abc'][2]:g} nm (hexagonal axes)")
+        sites = "+".join(f"Z{st['Z']}" for st in x["sites"])
+        energy = (f"{self.energies[0]:g}..{self.energies[-1]:g} kV ({len(self.energies)} bins)"
+                  if self.energies else f"{self.ekev:g} kV")
+        depth = (f"{self.nz} depth bins of E/10 nm" if self.profile == "gamma" else
+                 f"{self.nz} x {self.dz:g} nm depth steps")
+        return (f"{sg} {cell}, sites {sites}, {energy}, dmin {self.dmin:g} nm, Bethe "
+                f"{self.weakcutoff:g}/{self.cutoff:g}/{self.sgcutoff:g}, npx {self.npx}, "
+                f"{self.hemispheres} hemisphere{'s' if self.hemispheres > 1 else ''}, {depth}")
+
     def dz_for(self, ekev: float | None = None) -> float:
         if self.profile == "gamma":
             e = self.ekev if ekev is None else ekev

@@ -43,8 +43,14 @@ extern "C" {
 
 /* precision of the Lgh stage */
 #define CTEM_PREC_FP64 0    /* complex FP64 throughout (FP64 MFMA) */
-#define CTEM_PREC_MIXED 1   /* split-BF16 MFMA products with FP32 accumulation; FP64 norm,
-                               squaring-count choice and final contraction */
+#define CTEM_PREC_MIXED 1   /* split 16-bit MFMA products with FP32 accumulation; FP64 norm,
+                               squaring-count choice, depth accumulation and final contraction */
+
+/* split schemes of the mixed form's complex products (ctem_plan_set_mixed_terms) */
+#define CTEM_SPLIT_BF16X3 3   /* two BF16 planes, 3 MFMA products: ~2^-16 relative */
+#define CTEM_SPLIT_BF16X6 6   /* three BF16 planes, 6 MFMA products: ~FP32 accuracy */
+#define CTEM_SPLIT_FP16X3 16  /* two FP16 planes of x*2^12, 3 MFMA products: ~2^-21 relative;
+                                 entries must stay below 16 in magnitude (CTEM_E_NUMERIC) */
 
 typedef struct ctem_ctx ctem_ctx;
 typedef struct ctem_plan ctem_plan;
@@ -121,8 +127,8 @@ CTEM_API int32_t ctem_plan_get_beams(ctem_plan* plan, int32_t ik, int32_t* stron
  * ctem_plan_get_lgh. */
 CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_lgh_ik);
 
-/* Split-BF16 product terms of the mixed form (3: ~2^-16, 6: ~FP32 accuracy) for the
- * Taylor products (on the scaled matrix, |B| <= 1/4) and for the squarings; default 3 and 6. */
+/* Split schemes (CTEM_SPLIT_*) of the mixed form for the Taylor products (on the scaled
+ * matrix, |B| <= 1/4) and for the squarings; default CTEM_SPLIT_FP16X3 for both. */
 CTEM_API int32_t ctem_plan_set_mixed_terms(ctem_plan* plan, int32_t taylor_terms, int32_t square_terms);
 
 /* Re sum_gh Lgh(g,h) Sgh(g,h,p) per direction and site, in double, before the reference's

@@ -108,9 +108,14 @@ int orc_select_beams(int nref, const int32_t* hkl, const double* ucg, const uint
   return 0;
 }
 
+/* LUT(h,k,l) at the column-major offset of its Fortran bounds, with no per-axis check, as the
+ * compiled reference addresses it (gvectors.f90:1133-1160): for a hexagonal list whose beams
+ * span more than the LUT box, an offset inside the array reads the aliased entry. An offset
+ * outside the array (where the reference reads neighbouring heap memory) gives NaN. */
 static inline double complex lut_at(const double* lut, const int32_t lo[3], const int32_t dim[3],
                                     int h, int k, int l) {
-  size_t i = (size_t)(h - lo[0]) + (size_t)dim[0] * ((size_t)(k - lo[1]) + (size_t)dim[1] * (size_t)(l - lo[2]));
+  const long i = (long)(h - lo[0]) + (long)dim[0] * ((long)(k - lo[1]) + (long)dim[1] * (long)(l - lo[2]));
+  if (i < 0 || i >= (long)dim[0] * dim[1] * dim[2]) return NAN;
   return lut[2 * i] + I * lut[2 * i + 1];
 }
 

@@ -38,14 +38,19 @@ real(kind=sgl),allocatable :: lam(:)
 integer(kind=irg),allocatable :: ksel(:), kdone(:)
 complex(kind=dbl),allocatable :: Lgh(:,:)
 real(kind=dbl) :: ctmp(192,3)
+integer(kind=irg) :: gbox(3), hm(3), gg(3), ix, iy, iz, lutfull
+complex(kind=dbl),allocatable :: lutx(:,:,:)
+real(kind=sgl) :: gtmp(3)
 integer, parameter :: ou = 77, iu = 78
 type(kvectorlist),pointer :: kp
 
 ! dolgh: 0 = beam sets only, 1 = beam sets + DynMat, 2 = beam sets + DynMat + Lgh
-namelist /refdrv/ xtalname, ekev, dmin, npx, izz, depthstep, thick, lamfile, kselfile, outname, dolgh
+! lutfull: 1 = also dump CalcUcg over the box that holds every difference of two list entries
+namelist /refdrv/ xtalname, ekev, dmin, npx, izz, depthstep, thick, lamfile, kselfile, outname, dolgh, &
+                  lutfull
 
 xtalname = 'undefined'; ekev = 20.0; dmin = 0.1; npx = 50; izz = 1; depthstep = 1.D0; thick = 1.D0
-lamfile = 'undefined'; kselfile = 'undefined'; outname = 'refdump.bin'; dolgh = 0
+lamfile = 'undefined'; kselfile = 'undefined'; outname = 'refdump.bin'; dolgh = 0; lutfull = 0
 
 open(unit=iu, file='refdrv.nml', status='old', DELIM='APOSTROPHE')
 read(unit=iu, nml=refdrv)
@@ -83,6 +88,24 @@ call Set_Bethe_Parameters()
 call CalcWaveLength(dble(ekev*1000.0), 3)
 if (associated(reflist)) call Delete_gvectorlist()
 nullify(reflist)
+! Compute_ReflectionList (gvectors.f90:1333-1337) flags visited family members in the module
+! array refdone, sized to the box -imh:imh x -imk:imk x -iml:iml. With hexagonal axes a family
+! member of an in-box g can have |h+k| up to 2*imh, so those flag writes land outside the
+! array: they alias other entries or, at the box ends, corrupt the heap (corundum, SG 167,
+! crashes). The routine allocates refdone with stat= and keeps an existing allocation, so the
+! driver allocates it first with the bounds of the LUT box (-2*imh:2*imh, ...). For cubic
+! crystals every family member of an in-box g stays in the box and nothing changes.
+gbox(1:3) = 1
+do i = 1, 3
+  do
+    gbox(i) = gbox(i) + 1
+    gtmp = 0.0
+    gtmp(i) = float(gbox(i))
+    if (1.0/CalcLength(gtmp, 'r') .lt. dmin) exit
+  end do
+end do
+if (allocated(refdone)) deallocate(refdone)
+allocate(refdone(-2*gbox(1):2*gbox(1), -2*gbox(2):2*gbox(2), -2*gbox(3):2*gbox(3)))
 call Compute_ReflectionList(dmin, (/0,0,1/), (/1,0,0/), (/0,1,0/), 'ALL', .FALSE., 0)
 nref = DynNbeamsLinked
 
@@ -134,6 +157,34 @@ do ik = 1, numk
   write(ou) kp%i, kp%j, kp%k, kp%kn
   kp => kp%next
 end do
+
+! optional: the potential coefficients over the box -2*hm:2*hm (hm = largest |index| of the list
+! along each axis), CalcUcg (diffraction.f90:515-626) for every allowed g, 0 elsewhere, as the
+! reference fills its own LUT (gvectors.f90:1342-1427) over the smaller box
+if (lutfull.eq.1) then
+  hm = 0
+  rltmpa => reflist%next
+  do i = 1, nref
+    hm = max(hm, abs(rltmpa%hkl))
+    rltmpa => rltmpa%next
+  end do
+  allocate(lutx(-2*hm(1):2*hm(1), -2*hm(2):2*hm(2), -2*hm(3):2*hm(3)))
+  lutx = dcmplx(0.D0, 0.D0)
+  do iz = -2*hm(3), 2*hm(3)
+    do iy = -2*hm(2), 2*hm(2)
+      do ix = -2*hm(1), 2*hm(1)
+        gg = (/ ix, iy, iz /)
+        if (IsGAllowed(gg)) then
+          call CalcUcg(gg)
+          lutx(ix,iy,iz) = rlp%Ucg
+        end if
+      end do
+    end do
+  end do
+  write(ou) hm
+  write(ou) lutx
+  deallocate(lutx)
+end if
 
 ! per selected direction (beam loop body, CTEMEBSDmaster.f90:410-453)
 allocate(kdone(numk))

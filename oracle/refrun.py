@@ -104,7 +104,7 @@ class Stream:
         return a.copy()
 
 
-def parse_dump(path: str, dolgh: int) -> dict:
+def parse_dump(path: str, dolgh: int, lutfull: bool = False) -> dict:
     with open(path, "rb") as f:
         s = Stream(f.read())
     magic, isym, usehex, numk, nref, nsel, nsite, sgnum = s.get("<i4", 8)
@@ -135,6 +135,12 @@ def parse_dump(path: str, dolgh: int) -> dict:
         kij[i] = s.get("<i4", 2)
         kvec[i] = s.get("<f8", 3)
         kn[i] = s.get("<f8", 1)[0]
+    full = {}
+    if lutfull:
+        hm = s.get("<i4", 3)
+        fd = 4 * hm + 1
+        full = dict(lut_full=s.get("<c16", int(np.prod(fd))).reshape(fd[2], fd[1], fd[0]),
+                    lut_full_lo=(-2 * hm).astype(np.int32))
     hkl_index = {tuple(h): i for i, h in enumerate(hkl)}
     per = []
     for _ in range(nsel):
@@ -156,11 +162,11 @@ def parse_dump(path: str, dolgh: int) -> dict:
                 mPsihat=consts[2], mSigma=consts[3], DynUpz=consts[4], weakcutoff=consts[5],
                 cutoff=consts[6], sgcutoff=consts[7], vol=consts[8], dmt=mats[0], rmt=mats[1],
                 dsm=mats[2], rsm=mats[3], lut=lut, dbdiff=dbd, hkl=hkl, ucg=ucg, sites=sites,
-                kij=kij, kvec=kvec, kn=kn, per=per)
+                kij=kij, kvec=kvec, kn=kn, per=per, **full)
 
 
 def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: int,
-               ekev: float | None = None) -> dict:
+               ekev: float | None = None, lutfull: bool = False) -> dict:
     xt = os.path.join(work, "crystal.xtal")
     write_xtal(xt, wl.xtal)
     with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
@@ -180,11 +186,11 @@ def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: 
         f.write(f" &refdrv\n xtalname = 'crystal.xtal',\n ekev = {ek!r},\n dmin = {wl.dmin!r},\n"
                 f" npx = {wl.npx},\n izz = {len(lam)},\n depthstep = {dz!r},\n"
                 f" thick = {thick!r},\n lamfile = 'lam.bin',\n kselfile = '{ksf}',\n"
-                f" outname = 'dump.bin',\n dolgh = {dolgh},\n /\n")
+                f" outname = 'dump.bin',\n dolgh = {dolgh},\n lutfull = {int(lutfull)},\n /\n")
     env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
     subprocess.run(["bash", "-c", f"ulimit -s unlimited && {REF_BIN}/ctemref_driver > driver.log 2>&1"],
                    cwd=work, check=True, env=env)
-    d = parse_dump(os.path.join(work, "dump.bin"), dolgh)
+    d = parse_dump(os.path.join(work, "dump.bin"), dolgh, lutfull)
     d["thick"] = thick
     return d
 

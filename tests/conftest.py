@@ -23,6 +23,12 @@ def load_golden(name):
     return np.load(path)
 
 
+def lambert_dims(name, fx):
+    """(npx, npy) that the reference passes to Calckvectors for the fixture's workload."""
+    from ctemsoft_amd import ebsd, workloads
+    return ebsd.lambert_dims(workloads.get(name).npx, int(fx["usehex"]))
+
+
 @pytest.fixture(scope="session")
 def oracle_mod():
     from oracle import oracle

@@ -64,11 +64,12 @@ def _beams_csr(per) -> dict:
 
 
 def golden_workload(name: str, nsel_dyn: int, nsel_beams: int | None, full_program: bool,
-                    ekev: float | None = None, tag: str | None = None) -> str:
+                    ekev: float | None = None, tag: str | None = None,
+                    corners: bool = True, lutfull: bool = False) -> str:
     wl = workloads.get(name)
     lam, counts, tail = wl.depth_weights_for_program(ekev)
     with tempfile.TemporaryDirectory(prefix="ctemgold_") as work:
-        setup = run_driver(work, wl, lam, [0], 0, ekev)
+        setup = run_driver(work, wl, lam, [0], 0, ekev, lutfull=lutfull)
         numk = setup["numk"]
         rng = np.random.default_rng(wl.seed)
         # beam sets: all directions, or a seeded subset
@@ -77,12 +78,29 @@ def golden_workload(name: str, nsel_dyn: int, nsel_beams: int | None, full_progr
         else:
             kb = np.sort(rng.choice(numk, nsel_beams, replace=False))
         beams = run_driver(work, wl, lam, kb, 0, ekev)
-        # DynMat + Lgh on a small seeded subset that always holds the zone axis and the
-        # wedge corners
-        special = {0, numk - 1}
+        # DynMat + Lgh on a small seeded subset that always holds the zone axis and (unless
+        # the matrices are large) the last direction of the list
+        special = {0, numk - 1} if corners else {0}
         kd = np.sort(np.array(sorted(special | set(rng.choice(numk, nsel_dyn, replace=False).tolist()))))
         dyn = run_driver(work, wl, lam, kd, 2, ekev)
         out = _setup_arrays(setup)
+        if lutfull:
+            # CalcUcg over the box that holds every difference of two list entries (lattice-
+            # allowed g only). Inside the reference's own LUT box the two agree except where
+            # the reference also stored family members that the centring rule forbids
+            # (|Ucg| at round-off level); the reference's values are kept there.
+            lf = setup["lut_full"].copy()
+            lo, flo = setup["lut_dims"][3:6], setup["lut_full_lo"]
+            d = setup["lut"].shape
+            o = lo - flo
+            inner = lf[o[2]:o[2] + d[0], o[1]:o[1] + d[1], o[0]:o[0] + d[2]]
+            diff = inner != setup["lut"]
+            assert np.all(inner[diff] == 0)
+            assert np.abs(setup["lut"][diff]).max() < 1e-6 * np.abs(setup["lut"]).max()
+            inner[...] = setup["lut"]
+            out["lut_full"] = lf.astype(np.complex64)
+            assert np.array_equal(out["lut_full"].astype(np.complex128), lf)
+            out["lut_full_lo"] = flo
         out["lam"] = lam
         out["dz"] = wl.dz_for(ekev)
         out["ekev"] = wl.ekev if ekev is None else ekev
@@ -110,8 +128,11 @@ def main():
             print(golden_workload(w, nsel_dyn=5, nsel_beams=None, full_program=True))
         elif w in ("fe_c1", "fe_c1w"):
             print(golden_workload(w, nsel_dyn=3, nsel_beams=256, full_program=False))
-        elif w in ("al2o3_c2", "al2o3_c2w"):
-            print(golden_workload(w, nsel_dyn=1, nsel_beams=64, full_program=False))
+        elif w == "al2o3_c2":
+            print(golden_workload(w, nsel_dyn=5, nsel_beams=256, full_program=False, lutfull=True))
+        elif w == "al2o3_c2w":
+            print(golden_workload(w, nsel_dyn=1, nsel_beams=64, full_program=False,
+                                  corners=False))
         elif w in ("fe_c3", "fe_c3w"):
             for e in (10.0, 15.0, 20.0):
                 print(golden_workload(w, nsel_dyn=1, nsel_beams=64, full_program=False,

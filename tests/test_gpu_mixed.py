@@ -20,11 +20,13 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("name,nsample", [("fe_c1w", 1024), ("ni_c0w", None), ("fe_c1", 2048), ("ni_c0", None)])
+@pytest.mark.parametrize("name,nsample", [("fe_c1w", 1024), ("ni_c0w", None), ("fe_c1", 2048), ("ni_c0", None),
+                                          ("al2o3_c2w", 512), ("al2o3_c2", 2048)])
 def test_mixed_matches_fp64(ctx, name, nsample):
     from ctemsoft_amd import ebsd
     fx = load_golden(name)
-    st = ebsd.Setup.from_fixture(fx)
+    # corundum: the full-box LUT (a reference-sized one does not hold every g-h of its list)
+    st = ebsd.Setup.from_fixture(fx, load_golden("al2o3_c2") if name.startswith("al2o3") else None)
     ctx.upload_crystal(st)
     nk = len(fx["kvec"])
     ks = np.arange(nk) if nsample is None else np.sort(np.random.default_rng(5).choice(nk, nsample, replace=False))
@@ -37,4 +39,33 @@ def test_mixed_matches_fp64(ctx, name, nsample):
     print(f"{name}: {len(ks)} directions, mixed vs fp64 max rel {rel.max():.2e}, mean {rel.mean():.2e}", flush=True)
     sys.stdout.flush()
     assert rel.max() <= TOL
+    plan.close()
+
+
+@pytest.mark.parametrize("taylor,square", [(3, 6), (6, 6), (3, 16), (16, 6)])
+def test_split_schemes_match_fp64(ctx, taylor, square):
+    """Every split scheme (3 = BF16x3, 6 = BF16x6, 16 = FP16x3) within the bar on a seeded
+    sample of the Fe workload."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden("fe_c1w")
+    ctx.upload_crystal(ebsd.Setup.from_fixture(fx))
+    ks = np.sort(np.random.default_rng(11).choice(len(fx["kvec"]), 512, replace=False))
+    plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.run("fp64")
+    d64 = plan.dots().copy()
+    plan.set_mixed_terms(taylor, square)
+    plan.run("mixed")
+    rel = np.abs(plan.dots() - d64) / np.abs(d64)
+    print(f"schemes {taylor}/{square}: max rel {rel.max():.2e}", flush=True)
+    assert rel.max() <= TOL
+    plan.close()
+
+
+def test_split_scheme_rejects_unknown(ctx):
+    from ctemsoft_amd import ebsd
+    fx = load_golden("ni_c0")
+    ctx.upload_crystal(ebsd.Setup.from_fixture(fx))
+    plan = ctx.plan(fx["kvec"][:4], fx["kn"][:4], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    with pytest.raises(ebsd.CtemError):
+        plan.set_mixed_terms(4, 6)
     plan.close()

@@ -14,7 +14,7 @@ from conftest import load_golden
 
 pytestmark = pytest.mark.gpu
 
-FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"]
+FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w", "al2o3_c2", "al2o3_c2w"]
 
 
 def _say(*a):
@@ -126,6 +126,49 @@ def test_svals_against_oracle_random_directions(ctx, oracle_mod):
     for j, k in enumerate(ks):
         ref, _, _ = oracle_mod.master_direction(cr, fx["kvec"][k], fx["kn"][k], fx["lam"], float(fx["dz"]), float(fx["thick"]))
         assert abs(d[j] - ref[0]) <= 1e-11 * abs(ref[0]), (int(k), d[j], ref[0])
+    plan.close()
+
+
+def test_svals_two_sites_against_oracle(ctx, oracle_mod):
+    """Corundum (R-3c, hexagonal axes): two sites (Al orbit 12, O orbit 18), one svals value per
+    site, random directions of the hexagonal Lambert grid against the oracle."""
+    fx = load_golden("al2o3_c2")
+    rng = np.random.default_rng(7)
+    ks = rng.choice(len(fx["kvec"]), 8, replace=False)
+    plan, _ = _plan_for(ctx, fx, ks)
+    plan.run("fp64")
+    d = plan.dots()
+    sv = plan.svals()
+    assert d.shape == (len(ks), 2) and sv.shape == (len(ks), 2)
+    cr = oracle_mod.Crystal(fx)
+    nat = sum(len(fx[f"site{p}_orbit"]) for p in range(int(fx["nsite"])))
+    assert nat == 30
+    for j, k in enumerate(ks):
+        ref, _, _ = oracle_mod.master_direction(cr, fx["kvec"][k], fx["kn"][k], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+        for p in range(2):
+            assert abs(d[j, p] - ref[p]) <= 1e-11 * abs(ref[p]), (int(k), p, d[j, p], ref[p])
+            svr = oracle_mod.svals_f32(ref[p], nat)
+            # a float32 rounding flip needs the double sums within ~1e-14 of a boundary
+            assert abs(int(sv[j, p].view(np.int32)) - int(svr.view(np.int32))) <= 1
+    plan.close()
+
+
+def test_full_box_lut_repeats_reference_inside_its_box(ctx):
+    """Corundum with the LUT over the box that holds every g-h of the list (CalcUcg of the
+    reference's own modules): directions whose beam pairs stay inside the reference's LUT box
+    give the reference's DynMat."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden("al2o3_c2")
+    st = ebsd.Setup.from_fixture(fx, fx)
+    assert st.lut_dim.tolist() == [73, 73, 105]
+    ctx.upload_crystal(st)
+    ks = fx["dyn_ik"]
+    plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.select()
+    for j in range(len(ks)):
+        ref = fx[f"dyn{j}_dynmat"]
+        err = np.abs(plan.dynmat(j) - ref).max() / np.abs(ref).max()
+        assert err <= 1e-15, (int(ks[j]), err)
     plan.close()
 
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import REPO, load_golden
+from conftest import REPO, lambert_dims, load_golden
 
 
 def _declared():
@@ -42,12 +42,12 @@ def test_no_gpu_error_is_clean():
     assert e.value.code == -2
 
 
-@pytest.mark.parametrize("name", ["ni_c0", "fe_c1"])
+@pytest.mark.parametrize("name", ["ni_c0", "fe_c1", "al2o3_c2"])
 def test_kvectors_bit_exact(name):
     from ctemsoft_amd import ebsd
     fx = load_golden(name)
-    npx = int(fx["kij"][:, 0].max())
-    ij, k, kn = ebsd.kvectors(npx, npx, int(fx["isym"]), int(fx["usehex"]), fx["dsm"], float(fx["mLambda"]))
+    npx, npy = lambert_dims(name, fx)
+    ij, k, kn = ebsd.kvectors(npx, npy, int(fx["isym"]), int(fx["usehex"]), fx["dsm"], float(fx["mLambda"]))
     assert np.array_equal(ij, fx["kij"])
     assert np.array_equal(k, fx["kvec"])
     assert np.array_equal(kn, fx["kn"])

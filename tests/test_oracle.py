@@ -10,9 +10,9 @@ import numpy as np
 import pytest
 import scipy.linalg
 
-from conftest import load_golden
+from conftest import lambert_dims, load_golden
 
-FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"]
+FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w", "al2o3_c2", "al2o3_c2w"]
 
 
 def _split(fx, j, ns, nw):
@@ -58,8 +58,8 @@ def test_dynmat_and_lgh_match_reference(name, oracle_mod):
 @pytest.mark.parametrize("name", FIXTURES)
 def test_kvectors_bit_exact(name, oracle_mod):
     fx = load_golden(name)
-    npx = int(fx["kij"][:, 0].max())
-    ij, k, kn = oracle_mod.kvectors(npx, npx, int(fx["isym"]), int(fx["usehex"]), fx["dsm"],
+    npx, npy = lambert_dims(name, fx)
+    ij, k, kn = oracle_mod.kvectors(npx, npy, int(fx["isym"]), int(fx["usehex"]), fx["dsm"],
                                     float(fx["mLambda"]))
     assert np.array_equal(ij, fx["kij"])
     assert np.array_equal(k, fx["kvec"])
@@ -199,3 +199,19 @@ def test_calcsg_ewald_identity(oracle_mod):
         cosa = kpg @ kk / (np.linalg.norm(kpg) * np.linalg.norm(kk))
         ref = (kk @ kk - kpg @ kpg) / (2 * np.linalg.norm(kpg) * cosa)
         assert abs(s - ref) <= 2e-6 * max(1.0, abs(ref))
+
+
+def test_corundum_beam_sets_reach_beyond_lut_box(oracle_mod):
+    """The corundum list holds family members with |h+k| up to twice the LUT box, so beam pairs
+    of one direction have g-h outside the box; the bit-exact DynMat comparison above then
+    exercises the reference's column-major (aliased) LUT reads."""
+    fx = load_golden("al2o3_c2w")
+    cr = oracle_mod.Crystal(fx)
+    h = fx["hkl"].astype(np.int64)
+    lo = fx["lut_lo"].astype(np.int64)
+    hi = lo + np.array(fx["lut"].shape[::-1]) - 1
+    st, ssg, wk, wsg = oracle_mod.select_beams(cr, fx["kvec"][fx["dyn_ik"][0]])
+    g = h[np.concatenate([st, wk]) - 1]
+    d = g[:len(st), None, :] - g[None, :, :]
+    outside = ((d < lo) | (d > hi)).any(-1)
+    assert outside.sum() > 100
