@@ -116,6 +116,11 @@ __device__ __forceinline__ int64_t slut_index(const DevLut& L, int h, int k, int
   return (int64_t)(h - L.slo[0]) + (int64_t)L.sdim[0] * ((k - L.slo[1]) + (int64_t)L.sdim[1] * (l - L.slo[2]));
 }
 
+// One workgroup per 32x32 tile of one matrix. Per chunk of 32 weak beams the LUT values
+// U(g_r - w) and U(w - h_c) / s_w are gathered into LDS; the weak-beam sum
+// sum_w U(g-w) U(w-h) / s_w is a 32x32x32 complex product on v_mfma_f64_16x16x4_f64, each
+// wave one 16x16 quadrant (four real MFMAs per complex step). The diagonal sum
+// sum_w |U(g-w)|^2 / s_w (gvectors.f90:1165) is kept per row by the first 32 threads.
 __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double* __restrict__ kn, double mlambda,
                                                   float dynupz, double dz, double2* __restrict__ A,
@@ -134,10 +139,11 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   double2* Ab = A + (int64_t)b * npad * npad;
 
   __shared__ int rh[32][3], ch[32][3], wh[32][3];
-  __shared__ double wsgd[32], winv[32];
+  __shared__ double wsgd[32], winv[32], dsum_s[32];
   __shared__ double2 Ua[32][33], Ub[32][33];
-  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
-  const int gi = ti * 32 + r;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
   if (tid < 32) {
     const int a = ti * 32 + tid, c = tj * 32 + tid;
     for (int d = 0; d < 3; d++) {
@@ -146,11 +152,10 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
     }
   }
   __syncthreads();
-  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+  double4v accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
   double dsum = 0.0;
   const bool live = (ti * 32 < n) && (tj * 32 < n);
-  // the thread whose row meets one of its columns on the diagonal keeps the |U|^2/s sum
-  const bool diag_owner = (ti == tj) && (((r - cq) & 7) == 0);
+  const bool diag_tile = (ti == tj);
   if (live) {
     for (int w0 = 0; w0 < nw; w0 += 32) {
       __syncthreads();
@@ -171,46 +176,48 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
           }
           if (tj * 32 + rr < n) {
             const float2 u = lut_get(L, wh[ww][0] - ch[rr][0], wh[ww][1] - ch[rr][1], wh[ww][2] - ch[rr][2]);
-            vb = make_double2(u.x, u.y);
+            vb = make_double2((double)u.x * winv[ww], (double)u.y * winv[ww]);
           }
         }
         Ua[rr][ww] = va;  // U(g_r - w)
-        Ub[ww][rr] = vb;  // U(w - h_c)
+        Ub[ww][rr] = vb;  // U(w - h_c) / s_w
       }
       __syncthreads();
       const int wn = min(32, nw - w0);
-      for (int w = 0; w < wn; w++) {
-        const double2 a = Ua[r][w];
-        const double inv = winv[w];
-        for (int q = 0; q < 4; q++) {
-          const double2 bb = Ub[w][cq + 8 * q];
-          const double pr = a.x * bb.x - a.y * bb.y;
-          const double pi = a.x * bb.y + a.y * bb.x;
-          ar[q] += pr * inv;
-          ai[q] += pi * inv;
-        }
-        if (diag_owner) {
-          const double m = hypot(a.x, a.y);
+      // entries past nw are zero, so the product may run over whole steps of 4
+      for (int k4 = 0; k4 < wn; k4 += 4) {
+        const double2 a = Ua[wr + l16][k4 + l4];
+        const double2 bq = Ub[k4 + l4][wc + l16];
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.x, accr, 0, 0, 0);
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bq.y, accr, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.y, acci, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bq.x, acci, 0, 0, 0);
+      }
+      if (diag_tile && tid < 32) {
+        for (int w = 0; w < wn; w++) {
+          const double m = hypot(Ua[tid][w].x, Ua[tid][w].y);
           dsum += m * m / wsgd[w];
         }
       }
     }
   }
-  // write the tile
+  if (diag_tile && tid < 32) dsum_s[tid] = dsum;
+  __syncthreads();
+  // write the tile: lane l holds rows wr + l/16 + 4e, column wc + l%16
   const double half = 0.5 * mlambda;
   const double pre = scaled ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]) : 0.0;
-  for (int q = 0; q < 4; q++) {
-    const int gj = tj * 32 + cq + 8 * q;
+  for (int e = 0; e < 4; e++) {
+    const int r = wr + l4 + 4 * e, c = wc + l16;
+    const int gi = ti * 32 + r, gj = tj * 32 + c;
     double vr = 0.0, vi = 0.0;
     if (gi < n && gj < n) {
       if (gi != gj) {
-        const int c = cq + 8 * q;
         const float2 u = lut_get(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]);
-        vr = (double)u.x - half * ar[q];
-        vi = (double)u.y - half * ai[q];
+        vr = (double)u.x - half * accr[e];
+        vi = (double)u.y - half * acci[e];
       } else {
-        const double s = dsum * mlambda / 2.0;
-        vr = 2.0 * (double)ssg[gi] / mlambda - s;
+        const double sd = dsum_s[r] * mlambda / 2.0;
+        vr = 2.0 * (double)ssg[gi] / mlambda - sd;
         vi = (double)dynupz;
       }
     }

@@ -110,19 +110,32 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
       ci[i][j] = (f4v){0, 0, 0, 0};
     }
 
-  // register-prefetch pipeline: thread t stages row/col (t & 63) and k = 8*(t >> 6) .. +7 of
-  // both tiles (16 lanes hit 16 distinct 16-byte bank groups in LDS); the global loads of the
-  // next chunk are in flight during the MFMAs of the current one
-  const int srow = tid & 63, skq = (tid >> 6) * 8;
+  // register-prefetch pipeline; the global loads of the next chunk are in flight during the
+  // MFMAs of the current one. X (column-major, rows contiguous): thread t loads row t & 63 at
+  // k = 8*(t >> 6) .. +7, each load coalesced over the 64 rows of a wave. Y (column-major, k
+  // contiguous): thread t loads column t >> 2 at k = 8*(t & 3) .. +7 as four 16-byte loads,
+  // four lanes covering one column's 256 contiguous bytes.
+  const int arow = tid & 63, akq = (tid >> 6) * 8;
+  const int bcol = tid >> 2, bkq = (tid & 3) * 8;
   float2 ra[8], rb[8];
   auto gload = [&](int k0) {
     for (int j = 0; j < 8; j++) {
-      const int kk = k0 + skq + j;
-      ra[j] = (bm + srow < g.m && kk < g.k) ? X[(int64_t)kk * g.ldx + bm + srow] : make_float2(0.f, 0.f);
-      rb[j] = (bn + srow < g.n && kk < g.k) ? Y[(int64_t)(bn + srow) * g.ldy + kk] : make_float2(0.f, 0.f);
+      const int kk = k0 + akq + j;
+      ra[j] = (bm + arow < g.m && kk < g.k) ? X[(int64_t)kk * g.ldx + bm + arow] : make_float2(0.f, 0.f);
+    }
+    const float2* yc = Y + (int64_t)(bn + bcol) * g.ldy + k0 + bkq;
+    if (bn + bcol < g.n && k0 + bkq + 8 <= g.k && ((g.ldy & 1) == 0)) {
+      for (int j = 0; j < 4; j++) {
+        const float4 v = *(const float4*)(yc + 2 * j);
+        rb[2 * j] = make_float2(v.x, v.y);
+        rb[2 * j + 1] = make_float2(v.z, v.w);
+      }
+    } else {
+      for (int j = 0; j < 8; j++)
+        rb[j] = (bn + bcol < g.n && k0 + bkq + j < g.k) ? yc[j] : make_float2(0.f, 0.f);
     }
   };
-  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], int e) {
+  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], int e, int row, int kq) {
     __attribute__((aligned(16))) E pr[NP][8], pi[NP][8];
     bool bad = false;
     for (int j = 0; j < 8; j++) {
@@ -141,15 +154,15 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
     }
     if (bad && g.err) atomicOr(g.err, 16);
     for (int p = 0; p < NP; p++) {
-      *(V*)&P[0][p][srow][skq] = *(V*)pr[p];
-      *(V*)&P[1][p][srow][skq] = *(V*)pi[p];
+      *(V*)&P[0][p][row][kq] = *(V*)pr[p];
+      *(V*)&P[1][p][row][kq] = *(V*)pi[p];
     }
   };
   gload(0);
   for (int k0 = 0; k0 < g.k; k0 += 32) {
     __syncthreads();
-    stage(ra, As, ex);
-    stage(rb, Bs, ey);
+    stage(ra, As, ex, arow, akq);
+    stage(rb, Bs, ey, bcol, bkq);
     __syncthreads();
     if (k0 + 32 < g.k) gload(k0 + 32);
     if (wave_live) {
