@@ -138,7 +138,7 @@ struct CGemm {
   const int32_t* active; int32_t step;
   int32_t nb;
   int32_t terms;         // split scheme: CTEM_SPLIT_BF16X3, CTEM_SPLIT_BF16X6 or CTEM_SPLIT_FP16X3
-  int32_t* err;          // bit 16: a finite scaled entry outside the FP16 range
+  int32_t* err;          // reserved for kernel-side error bits
   const float* xmax;     // [nb] max |entry| of X and Y per matrix (split-FP16 scaling)
   const float* ymax;
   float* cmax;           // [nb] receives max |entry| of C per matrix (zeroed by the caller), or null

@@ -12,6 +12,7 @@
 // max|x| max|y|, 32 times better than split-BF16 scheme 3 for the same three MFMAs, because
 // FP16 carries 11 significand bits to BF16's 8; the scaling keeps residuals of entries down
 // to ~2^-24 max|x| in the normal FP16 range. Non-finite entries pass through as in FP32.
+// All launches have k a multiple of 8 (matrices padded to multiples of 32).
 // Each term is one MFMA with FP32 accumulation. Complex products use four real products
 // (re*re, im*im, re*im, im*re) into three accumulators (re+, re-, im).
 //
@@ -21,6 +22,7 @@
 // (lane l): A[l%16][8*(l/16)+t], B[8*(l/16)+t][l%16], t = 0..7; result element e of lane l is
 // C[4*(l/16)+e][l%16].
 #include <hip/hip_bf16.h>
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "ctem_ebsd.h"
@@ -117,80 +119,95 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   // four lanes covering one column's 256 contiguous bytes.
   const int arow = tid & 63, akq = (tid >> 6) * 8;
   const int bcol = tid >> 2, bkq = (tid & 3) * 8;
+  // loads are branch-free: indices are clamped into the matrix and out-of-range values zeroed
+  const bool arow_ok = bm + arow < g.m, bcol_ok = bn + bcol < g.n;
+  const float2* xr0 = X + min(bm + arow, g.m - 1);
+  const float2* yc0 = Y + (int64_t)min(bn + bcol, g.n - 1) * g.ldy;
+  const bool yvec = (g.k % 8 == 0) && (g.ldy % 2 == 0);  // 16-byte loads of Y columns
   float2 ra[8], rb[8];
   auto gload = [&](int k0) {
     for (int j = 0; j < 8; j++) {
       const int kk = k0 + akq + j;
-      ra[j] = (bm + arow < g.m && kk < g.k) ? X[(int64_t)kk * g.ldx + bm + arow] : make_float2(0.f, 0.f);
+      const float2 v = xr0[(int64_t)min(kk, g.k - 1) * g.ldx];
+      ra[j] = (arow_ok && kk < g.k) ? v : make_float2(0.f, 0.f);
     }
-    const float2* yc = Y + (int64_t)(bn + bcol) * g.ldy + k0 + bkq;
-    if (bn + bcol < g.n && k0 + bkq + 8 <= g.k && ((g.ldy & 1) == 0)) {
+    if (yvec) {
+      const bool ok = bcol_ok && (k0 + bkq < g.k);
+      const float4* yc = (const float4*)(yc0 + min(k0 + bkq, g.k - 8));
       for (int j = 0; j < 4; j++) {
-        const float4 v = *(const float4*)(yc + 2 * j);
-        rb[2 * j] = make_float2(v.x, v.y);
-        rb[2 * j + 1] = make_float2(v.z, v.w);
+        const float4 v = yc[j];
+        rb[2 * j] = ok ? make_float2(v.x, v.y) : make_float2(0.f, 0.f);
+        rb[2 * j + 1] = ok ? make_float2(v.z, v.w) : make_float2(0.f, 0.f);
       }
     } else {
-      for (int j = 0; j < 8; j++)
-        rb[j] = (bn + bcol < g.n && k0 + bkq + j < g.k) ? yc[j] : make_float2(0.f, 0.f);
+      for (int j = 0; j < 8; j++) {
+        const int kk = k0 + bkq + j;
+        const float2 v = yc0[min(kk, g.k - 1)];
+        rb[j] = (bcol_ok && kk < g.k) ? v : make_float2(0.f, 0.f);
+      }
     }
   };
-  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], int e, int row, int kq) {
+  const float sx = ldexpf(1.0f, ex), sy = ldexpf(1.0f, ey);  // exact powers of two
+  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], float sc, int row, int kq) {
     __attribute__((aligned(16))) E pr[NP][8], pi[NP][8];
-    bool bad = false;
     for (int j = 0; j < 8; j++) {
       E a, bb, c;
       float xr = v[j].x, xi = v[j].y;
       if constexpr (F16) {
-        xr = ldexpf(xr, e);
-        xi = ldexpf(xi, e);
-        // finite entries above the range mean a producer did not track its maximum
-        bad |= (fabsf(xr) >= 65504.0f && fabsf(xr) <= 3.4e38f) || (fabsf(xi) >= 65504.0f && fabsf(xi) <= 3.4e38f);
+        xr *= sc;
+        xi *= sc;
       }
       split3(xr, a, bb, c);
       pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
       split3(xi, a, bb, c);
       pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
     }
-    if (bad && g.err) atomicOr(g.err, 16);
     for (int p = 0; p < NP; p++) {
       *(V*)&P[0][p][row][kq] = *(V*)pr[p];
       *(V*)&P[1][p][row][kq] = *(V*)pi[p];
     }
   };
   gload(0);
+  // A wave whose 32x32 quadrant lies past the matrix edge only stages its share of the
+  // tiles; the choice is made once, outside the loop, so that the live copy of the loop keeps
+  // its accumulators in place across iterations.
+  auto kloop = [&](auto live) {
   for (int k0 = 0; k0 < g.k; k0 += 32) {
     __syncthreads();
-    stage(ra, As, ex, arow, akq);
-    stage(rb, Bs, ey, bcol, bkq);
+    stage(ra, As, sx, arow, akq);
+    stage(rb, Bs, sy, bcol, bkq);
     __syncthreads();
     if (k0 + 32 < g.k) gload(k0 + 32);
-    if (wave_live) {
-      V a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
-      for (int i = 0; i < 2; i++)
-        for (int c = 0; c < 2; c++)
-          for (int p = 0; p < NP; p++) {
-            a[i][c][p] = *(const V*)&As[c][p][wm + 16 * i + l16][8 * l4];
-            bq[i][c][p] = *(const V*)&Bs[c][p][wn + 16 * i + l16][8 * l4];
-          }
-      for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) {
-#define MM(acc, ca, pa, cb, pb) acc = Split<F16>::mma(a[i][ca][pa], bq[j][cb][pb], acc)
-          // smallest terms first
-          if constexpr (TERMS == 6) {
-            MM(crp[i][j], 0, 2, 0, 0); MM(crp[i][j], 0, 1, 0, 1); MM(crp[i][j], 0, 0, 0, 2);
-            MM(crn[i][j], 1, 2, 1, 0); MM(crn[i][j], 1, 1, 1, 1); MM(crn[i][j], 1, 0, 1, 2);
-            MM(ci[i][j], 0, 2, 1, 0); MM(ci[i][j], 0, 1, 1, 1); MM(ci[i][j], 0, 0, 1, 2);
-            MM(ci[i][j], 1, 2, 0, 0); MM(ci[i][j], 1, 1, 0, 1); MM(ci[i][j], 1, 0, 0, 2);
-          }
-          MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
-          MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
-          MM(ci[i][j], 0, 1, 1, 0); MM(ci[i][j], 0, 0, 1, 1); MM(ci[i][j], 0, 0, 1, 0);
-          MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
-#undef MM
+    if constexpr (!decltype(live)::value) continue;
+    V a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
+    for (int i = 0; i < 2; i++)
+      for (int c = 0; c < 2; c++)
+        for (int p = 0; p < NP; p++) {
+          a[i][c][p] = *(const V*)&As[c][p][wm + 16 * i + l16][8 * l4];
+          bq[i][c][p] = *(const V*)&Bs[c][p][wn + 16 * i + l16][8 * l4];
         }
-    }
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+#define MM(acc, ca, pa, cb, pb) acc = Split<F16>::mma(a[i][ca][pa], bq[j][cb][pb], acc)
+        // smallest terms first
+        if constexpr (TERMS == 6) {
+          MM(crp[i][j], 0, 2, 0, 0); MM(crp[i][j], 0, 1, 0, 1); MM(crp[i][j], 0, 0, 0, 2);
+          MM(crn[i][j], 1, 2, 1, 0); MM(crn[i][j], 1, 1, 1, 1); MM(crn[i][j], 1, 0, 1, 2);
+          MM(ci[i][j], 0, 2, 1, 0); MM(ci[i][j], 0, 1, 1, 1); MM(ci[i][j], 0, 0, 1, 2);
+          MM(ci[i][j], 1, 2, 0, 0); MM(ci[i][j], 1, 1, 0, 1); MM(ci[i][j], 1, 0, 0, 2);
+        }
+        MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
+        MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
+        MM(ci[i][j], 0, 1, 1, 0); MM(ci[i][j], 0, 0, 1, 1); MM(ci[i][j], 0, 0, 1, 0);
+        MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
+#undef MM
+      }
   }
+  };
+  if (wave_live)
+    kloop(std::true_type{});
+  else
+    kloop(std::false_type{});
   if (!wave_live) return;
   // undo the operand scaling (a power of two)
   const float alpha = ldexpf(g.alpha, -(ex + ey));
