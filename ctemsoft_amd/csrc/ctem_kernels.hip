@@ -512,10 +512,88 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
   }
 }
 
+// Depth recursion with the whole matrix held on chip (FP32 storage, npad <= 224): E is read
+// from memory once per direction instead of once per depth step. The 1024 threads form a
+// 32 x 32 grid; thread (tr, tc) keeps the RT x RT block of rows tr*RT.. and columns tc*RT..
+// (RT = npad/32) in registers, or for npad = 224 five of its seven rows in registers and two
+// in LDS. Per step each thread forms its RT partial row sums in FP32 (RT terms each), the 32
+// threads of a row group (half a wave) reduce them in FP64 with lane shuffles, and the new
+// vector psi_{k+1} is kept in FP64 in LDS (double-buffered) and written out.
+template <int NPAD>
+__global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const float2* __restrict__ E,
+                                                       int add_identity, double2* __restrict__ Psi,
+                                                       int64_t spsi) {
+  constexpr int RT = NPAD / 32;
+  constexpr int LR = (RT == 7) ? 2 : 0;  // rows of the block kept in LDS
+  constexpr int RR = RT - LR;            // rows kept in registers
+  __shared__ double2 xs[2][NPAD];
+  __shared__ float2 Elds[LR > 0 ? LR : 1][RT][1024];
+  const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
+  const int64_t per = (int64_t)NPAD * NPAD;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const float2* Eb = E + (int64_t)b * per;
+    double2* P = Psi + (int64_t)b * spsi;
+    float2 er[RR][RT];
+    __syncthreads();  // the previous direction is done with xs and Elds
+    for (int i = 0; i < RT; i++)
+      for (int j = 0; j < RT; j++) {
+        const float2 v = Eb[(int64_t)(tc * RT + j) * NPAD + tr * RT + i];
+        if (i < RR) er[i < RR ? i : 0][j] = v;
+        else if constexpr (LR > 0) Elds[i - RR][j][tid] = v;
+      }
+    for (int i = tid; i < NPAD; i += 1024) {
+      double2 v = make_double2((double)Eb[i].x, (double)Eb[i].y);
+      if (add_identity && i == 0) v.x += 1.0;
+      xs[0][i] = v;
+      P[i] = v;
+    }
+    __syncthreads();
+    for (int k = 1; k < nz; k++) {
+      const int cur = (k - 1) & 1;
+      float2 xf[RT];
+      for (int j = 0; j < RT; j++) {
+        const double2 v = xs[cur][tc * RT + j];
+        xf[j] = make_float2((float)v.x, (float)v.y);
+      }
+      for (int i = 0; i < RT; i++) {
+        float ar = 0.f, ai = 0.f;
+        for (int j = 0; j < RT; j++) {
+          float2 e;
+          if (i < RR) e = er[i < RR ? i : 0][j];
+          else e = Elds[LR > 0 ? i - RR : 0][j][tid];
+          ar = fmaf(e.x, xf[j].x, ar); ar = fmaf(-e.y, xf[j].y, ar);
+          ai = fmaf(e.x, xf[j].y, ai); ai = fmaf(e.y, xf[j].x, ai);
+        }
+        double sr = ar, si = ai;
+        for (int o = 16; o > 0; o >>= 1) {
+          sr += __shfl_xor(sr, o, 32);
+          si += __shfl_xor(si, o, 32);
+        }
+        if (tc == i) {
+          const int r = tr * RT + i;
+          if (add_identity) { sr += xs[cur][r].x; si += xs[cur][r].y; }
+          const double2 y = make_double2(sr, si);
+          xs[cur ^ 1][r] = y;
+          P[(int64_t)k * NPAD + r] = y;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const void* E, int fp32,
                   int add_identity, double2* Psi, int64_t spsi, int32_t grid) {
   const size_t lds = (size_t)5 * npad * sizeof(double2);
   const int g = std::min(nb, grid);
+  if (fp32 && npad <= 224 && npad % 32 == 0) {
+    const float2* Ef = (const float2*)E;
+    switch (npad) {
+#define DEPTH_CASE(N) case N: hipLaunchKernelGGL(k_depth_onchip<N>, dim3(g), dim3(1024), 0, s, nb, nz, Ef, add_identity, Psi, spsi); return;
+      DEPTH_CASE(32) DEPTH_CASE(64) DEPTH_CASE(96) DEPTH_CASE(128) DEPTH_CASE(160) DEPTH_CASE(192) DEPTH_CASE(224)
+#undef DEPTH_CASE
+    }
+  }
   if (fp32)
     hipLaunchKernelGGL(k_depth<float2>, dim3(g), dim3(1024), lds, s, npad, nb, nz, (const float2*)E,
                        add_identity, Psi, spsi);
