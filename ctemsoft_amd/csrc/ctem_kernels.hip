@@ -517,8 +517,19 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
 // 32 x 32 grid; thread (tr, tc) keeps the RT x RT block of rows tr*RT.. and columns tc*RT..
 // (RT = npad/32) in registers, or for npad = 224 five of its seven rows in registers and two
 // in LDS. Per step each thread forms its RT partial row sums in FP32 (RT terms each), the 32
-// threads of a row group (half a wave) reduce them in FP64 with lane shuffles, and the new
-// vector psi_{k+1} is kept in FP64 in LDS (double-buffered) and written out.
+// threads of a row group (half a wave) add them with DPP lane moves, and the new vector
+// psi_{k+1} = E psi_k is formed and kept in FP64 in LDS (double-buffered) and written out.
+// sum over the 32 lanes of each half wave, result in every lane: quad_perm [1,0,3,2] and
+// [2,3,0,1], row_ror 4 and 8 (DPP within a row of 16), then one exchange across rows
+__device__ __forceinline__ float half_wave_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+  v += __shfl_xor(v, 16, 32);
+  return v;
+}
+
 template <int NPAD>
 __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const float2* __restrict__ E,
                                                        int add_identity, double2* __restrict__ Psi,
@@ -564,11 +575,9 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
           ar = fmaf(e.x, xf[j].x, ar); ar = fmaf(-e.y, xf[j].y, ar);
           ai = fmaf(e.x, xf[j].y, ai); ai = fmaf(e.y, xf[j].x, ai);
         }
-        double sr = ar, si = ai;
-        for (int o = 16; o > 0; o >>= 1) {
-          sr += __shfl_xor(sr, o, 32);
-          si += __shfl_xor(si, o, 32);
-        }
+        // the 32 partial sums of row r meet in FP32 (each holds RT terms); the recursion
+        // itself is carried in FP64
+        double sr = half_wave_sum(ar), si = half_wave_sum(ai);
         if (tc == i) {
           const int r = tr * RT + i;
           if (add_identity) { sr += xs[cur][r].x; si += xs[cur][r].y; }
@@ -622,9 +631,11 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   __shared__ double lw[32];
   __shared__ int rh[32][3], ch[32][3];
   __shared__ double red[8][4];
-  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+  const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
   const int nsite = L.nsite;
-  double acc_r[4] = {0, 0, 0, 0}, acc_i[4] = {0, 0, 0, 0};
+  double4v accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
   const bool live = (ti * 32 < n) && (tj * 32 < n);
   if (tid < 32) {
     const int a = ti * 32 + tid, c = tj * 32 + tid;
@@ -649,28 +660,29 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       }
       if (tid < 32) lw[tid] = (k0 + tid < nz) ? lam[k0 + tid] : 0.0;
       __syncthreads();
+      // sum_k (lam_k psi_k(g)) conj(psi_k(h)) on v_mfma_f64_16x16x4_f64, each wave one 16x16
+      // quadrant; entries past nz are zero
       const int kn = min(32, nz - k0);
-      for (int kk = 0; kk < kn; kk++) {
-        const double2 a = Pg[r][kk];
-        const double w = lw[kk];
-        const double awr = w * a.x, awi = w * a.y;
-        for (int q = 0; q < 4; q++) {
-          const double2 h = Ph[cq + 8 * q][kk];
-          // (a w) * conj(h)
-          acc_r[q] += awr * h.x + awi * h.y;
-          acc_i[q] += awi * h.x - awr * h.y;
-        }
+      for (int k4 = 0; k4 < kn; k4 += 4) {
+        const double2 a = Pg[wr + l16][k4 + l4];
+        const double w = lw[k4 + l4];
+        const double ar = w * a.x, ai = w * a.y;
+        const double2 h = Ph[wc + l16][k4 + l4];
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, h.x, accr, 0, 0, 0);
+        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.y, accr, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.x, acci, 0, 0, 0);
+        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, h.y, acci, 0, 0, 0);
       }
     }
   }
   __syncthreads();
-  const int gi = ti * 32 + r;
   double part[4] = {0, 0, 0, 0};
+  // lane l holds rows wr + l/16 + 4e, column wc + l%16 of the tile
   for (int q = 0; q < 4; q++) {
-    const int c = cq + 8 * q;
-    const int gj = tj * 32 + c;
+    const int r = wr + l4 + 4 * q, c = wc + l16;
+    const int gi = ti * 32 + r, gj = tj * 32 + c;
     if (live && gi < n && gj < n) {
-      const double lr = dzt * acc_r[q], li = dzt * acc_i[q];
+      const double lr = dzt * accr[q], li = dzt * acci[q];
       const int64_t cell = nsite ? slut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]) : 0;
       for (int p = 0; p < nsite && p < 4; p++) {
         const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
@@ -680,7 +692,6 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
     }
   }
   // block reduction in fixed order: wave shuffles, then waves in order
-  const int lane = tid & 63, wave = tid >> 6;
   for (int p = 0; p < 4; p++) {
     double v = part[p];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
