@@ -643,7 +643,7 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 // squaring counts come from the FP64 matrix, the final Lgh/Sgh contraction runs in FP64.
 constexpr double kThetaMixed = 0.25;
 
-int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A64, float2* W32,
+int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32,
                          float* wmax, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
                          int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
                          int terms_taylor, int terms_square, int depth_grid) {
@@ -662,7 +662,8 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   float* mF = wmax + 4 * nb;
   float* mH = wmax + 5 * nb;
   HIPCHK(hipMemsetAsync(wmax, 0, 6 * sizeof(float) * (size_t)nb, s));
-  launch_norm_scale(s, npad, nb, A64, kThetaMixed, kSmax, d_sq, d_err);
+  // B (the first buffer) holds A in FP32 as assembled; scale it by 2^-s in place
+  launch_norm_scale32(s, npad, nb, B, kThetaMixed, kSmax, d_sq, d_err, mB);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
@@ -671,8 +672,6 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, double2
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
-  launch_z2c(s, npad, nb, A64, B, mB);
-  HIPCHK(hipGetLastError());
   auto cg = [&](float2* C, float* cmax, const float2* X, const float* xmax, const float2* Y,
                 const float* ymax, float betaI, std::initializer_list<std::pair<const float2*, float>> ms,
                 const int32_t* active, int step, double flops, int terms, int ncols, int64_t sy,
@@ -785,14 +784,17 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb);
       for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
-      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->A.as<double2>());
+      if (precision == CTEM_PREC_FP64)
+        launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->A.as<double2>());
+      else
+        launch_assemble32(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->W.as<float2>());
       HIPCHK(hipGetLastError());
       if (precision == CTEM_PREC_FP64) {
         rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
                              p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
         if ((rc = p->wmax.ensure(6 * sizeof(float) * (size_t)nb))) return rc;
-        rc = expm_depth_mixed(p, s, npad, nb, p->A.as<double2>(), p->W.as<float2>(), p->wmax.as<float>(),
+        rc = expm_depth_mixed(p, s, npad, nb, p->W.as<float2>(), p->wmax.as<float>(),
                               p->Psi.as<double2>(), spsi, p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn,
                               p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
       }

@@ -71,6 +71,10 @@ struct BeamView {        // read-only view of the selection lists
 void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                      double2* A);
+// the same scaled matrix rounded to FP32 (mixed form)
+void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                       const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                       float2* A32);
 // same matrix, unscaled (DynMat itself), for parity checks
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D);
@@ -144,9 +148,10 @@ struct CGemm {
   float* cmax;           // [nb] receives max |entry| of C per matrix (zeroed by the caller), or null
 };
 void launch_cgemm(hipStream_t s, const CGemm& g);
-// FP64 -> FP32 copy of nb npad x npad matrices; dmax[b] (zeroed by the caller) receives the
-// max |entry| of matrix b
-void launch_z2c(hipStream_t s, int32_t npad, int32_t nb, const double2* src, float2* dst, float* dmax);
+// FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
+// 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
+void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,
+                         int32_t smax, int32_t* sq, int32_t* err, float* amax);
 // C = betaI*I + sum_j beta_j*M_j per matrix, with the max |entry| into cmax[b]
 void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
                    const float2* const M[4], const float beta[4], float* cmax);

@@ -117,14 +117,15 @@ __device__ __forceinline__ int64_t slut_index(const DevLut& L, int h, int k, int
 }
 
 // One workgroup per 32x32 tile of one matrix. Per chunk of 32 weak beams the LUT values
-// U(g_r - w) and U(w - h_c) / s_w are gathered into LDS; the weak-beam sum
+// U(g_r - w) and U(w - h_c) are gathered (into registers one chunk ahead, then LDS); the
+// weak-beam sum
 // sum_w U(g-w) U(w-h) / s_w is a 32x32x32 complex product on v_mfma_f64_16x16x4_f64, each
 // wave one 16x16 quadrant (four real MFMAs per complex step). The diagonal sum
 // sum_w |U(g-w)|^2 / s_w (gvectors.f90:1165) is kept per row by the first 32 threads.
 __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double* __restrict__ kn, double mlambda,
                                                   float dynupz, double dz, double2* __restrict__ A,
-                                                  int scaled) {
+                                                  float2* __restrict__ A32, int scaled) {
 #pragma clang fp contract(off)
   const int npad = B.npad, tiles = npad / 32;
   const int b = blockIdx.x / (tiles * tiles);
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   const float* wsg = V.wsg + V.woff[dir];
   double2* Ab = A + (int64_t)b * npad * npad;
 
-  __shared__ int rh[32][3], ch[32][3], wh[32][3];
+  __shared__ int rh[32][3], ch[32][3];
   __shared__ double wsgd[32], winv[32], dsum_s[32];
   __shared__ double2 Ua[32][33], Ub[32][33];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -156,38 +157,49 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   double dsum = 0.0;
   const bool live = (ti * 32 < n) && (tj * 32 < n);
   const bool diag_tile = (ti == tj);
+  // register prefetch of the next chunk: thread t gathers rows/columns t & 31 for the weak
+  // beams (t >> 5) + 8m, m = 0..3, while the MFMAs of the current chunk run
+  const int prr = tid & 31, pww = tid >> 5;
+  const bool row_ok = ti * 32 + prr < n, col_ok = tj * 32 + prr < n;
+  float2 pa[4], pb[4];
+  float psg[4];
+  auto prefetch = [&](int w0) {
+    for (int m = 0; m < 4; m++) {
+      const int w = w0 + pww + 8 * m;
+      pa[m] = make_float2(0.f, 0.f);
+      pb[m] = make_float2(0.f, 0.f);
+      psg[m] = 1.0f;
+      if (w < nw) {
+        const int32_t* hw = R.hkl + 3 * widx[w];
+        const int h0 = hw[0], h1 = hw[1], h2 = hw[2];
+        psg[m] = wsg[w];
+        if (row_ok) pa[m] = lut_get(L, rh[prr][0] - h0, rh[prr][1] - h1, rh[prr][2] - h2);
+        if (col_ok) pb[m] = lut_get(L, h0 - ch[prr][0], h1 - ch[prr][1], h2 - ch[prr][2]);
+      }
+    }
+  };
   if (live) {
+    prefetch(0);
     for (int w0 = 0; w0 < nw; w0 += 32) {
       __syncthreads();
-      if (tid < 32) {
-        const int w = w0 + tid;
-        for (int d = 0; d < 3; d++) wh[tid][d] = (w < nw) ? R.hkl[3 * widx[w] + d] : 0;
-        wsgd[tid] = (w < nw) ? (double)wsg[w] : 1.0;
-        winv[tid] = 1.0 / wsgd[tid];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
-      }
-      __syncthreads();
-      for (int e = tid; e < 1024; e += 256) {
-        const int rr = e & 31, ww = e >> 5;
-        double2 va = make_double2(0.0, 0.0), vb = make_double2(0.0, 0.0);
-        if (w0 + ww < nw) {
-          if (ti * 32 + rr < n) {
-            const float2 u = lut_get(L, rh[rr][0] - wh[ww][0], rh[rr][1] - wh[ww][1], rh[rr][2] - wh[ww][2]);
-            va = make_double2(u.x, u.y);
-          }
-          if (tj * 32 + rr < n) {
-            const float2 u = lut_get(L, wh[ww][0] - ch[rr][0], wh[ww][1] - ch[rr][1], wh[ww][2] - ch[rr][2]);
-            vb = make_double2((double)u.x * winv[ww], (double)u.y * winv[ww]);
-          }
+      for (int m = 0; m < 4; m++) {
+        const int ww = pww + 8 * m;
+        Ua[prr][ww] = make_double2(pa[m].x, pa[m].y);  // U(g_r - w)
+        Ub[ww][prr] = make_double2(pb[m].x, pb[m].y);  // U(w - h_c)
+        if (prr == 0) {
+          wsgd[ww] = (double)psg[m];
+          winv[ww] = 1.0 / (double)psg[m];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
         }
-        Ua[rr][ww] = va;  // U(g_r - w)
-        Ub[ww][rr] = vb;  // U(w - h_c) / s_w
       }
       __syncthreads();
+      if (w0 + 32 < nw) prefetch(w0 + 32);
       const int wn = min(32, nw - w0);
       // entries past nw are zero, so the product may run over whole steps of 4
       for (int k4 = 0; k4 < wn; k4 += 4) {
         const double2 a = Ua[wr + l16][k4 + l4];
-        const double2 bq = Ub[k4 + l4][wc + l16];
+        const double2 u = Ub[k4 + l4][wc + l16];
+        const double iv = winv[k4 + l4];
+        const double2 bq = make_double2(u.x * iv, u.y * iv);  // U(w - h) / s_w
         accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.x, accr, 0, 0, 0);
         accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bq.y, accr, 0, 0, 0);
         acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.y, acci, 0, 0, 0);
@@ -222,8 +234,12 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
       }
     }
     if (gi < npad && gj < npad) {
-      // scaled: A = i * pre * DynMat
-      Ab[(int64_t)gj * npad + gi] = scaled ? make_double2(-pre * vi, pre * vr) : make_double2(vr, vi);
+      // scaled: A = i * pre * DynMat, in FP64 or (A32) rounded to FP32
+      const double2 v = scaled ? make_double2(-pre * vi, pre * vr) : make_double2(vr, vi);
+      if (A32)
+        A32[(int64_t)b * npad * npad + (int64_t)gj * npad + gi] = make_float2((float)v.x, (float)v.y);
+      else
+        Ab[(int64_t)gj * npad + gi] = v;
     }
   }
 }
@@ -233,14 +249,22 @@ void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      double2* A) {
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
-                     mlambda, dynupz, dz, A, 1);
+                     mlambda, dynupz, dz, A, (float2*)nullptr, 1);
+}
+
+void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                       const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                       float2* A32) {
+  const int tiles = B.npad / 32;
+  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
+                     mlambda, dynupz, dz, (double2*)nullptr, A32, 1);
 }
 
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D) {
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
-                     (const double*)nullptr, mlambda, dynupz, 0.0, D, 0);
+                     (const double*)nullptr, mlambda, dynupz, 0.0, D, (float2*)nullptr, 0);
 }
 
 // ------------------------------------------------------------------------------------------

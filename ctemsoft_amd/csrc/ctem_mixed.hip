@@ -245,22 +245,59 @@ void launch_cgemm(hipStream_t s, const CGemm& g) {
 }
 
 // ------------------------------------------------------------------------------------------
-// FP64 -> FP32 matrices, one grid row per matrix, with the per-matrix max |entry|
-__global__ void k_z2c(int64_t per, const double2* __restrict__ a, float2* __restrict__ c,
-                      float* __restrict__ cmax) {
-  const int64_t base = (int64_t)blockIdx.y * per;
-  float vmax = 0.0f;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per; i += (int64_t)gridDim.x * blockDim.x) {
-    const float2 v = make_float2((float)a[base + i].x, (float)a[base + i].y);
-    c[base + i] = v;
-    vmax = nmax(vmax, nmax(fabsf(v.x), fabsf(v.y)));
-  }
-  wave_max_to(vmax, cmax + blockIdx.y);
-}
 static int slot_grid(int64_t per) { return (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64); }
-void launch_z2c(hipStream_t s, int32_t npad, int32_t nb, const double2* src, float2* dst, float* dmax) {
-  const int64_t per = (int64_t)npad * npad;
-  if (nb > 0) hipLaunchKernelGGL(k_z2c, dim3(slot_grid(per), nb), dim3(256), 0, s, per, src, dst, dmax);
+
+// 1-norm (max column sum of |a|, accumulated in FP64), squaring count s with
+// ||A||_1 / 2^s <= theta, in-place scaling by 2^-s and the max |entry| afterwards; one
+// workgroup per matrix
+__global__ __launch_bounds__(256) void k_norm_scale32(int npad, float2* __restrict__ A, double theta,
+                                                      int smax, int* __restrict__ sq, int* __restrict__ err,
+                                                      float* __restrict__ amax) {
+  const int b = blockIdx.x;
+  float2* Ab = A + (int64_t)b * npad * npad;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double wmax[4];
+  __shared__ int s_sh;
+  double cmax = 0.0;
+  for (int j = wave; j < npad; j += 4) {
+    double sum = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const float2 v = Ab[(int64_t)j * npad + i];
+      sum += (double)hypotf(v.x, v.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    cmax = fmax(cmax, sum);
+    if (!(sum == sum)) cmax = sum;  // keep a NaN
+  }
+  if (lane == 0) wmax[wave] = cmax;
+  __syncthreads();
+  if (tid == 0) {
+    double nrm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    for (int w = 0; w < 4; w++) if (!(wmax[w] == wmax[w])) nrm = wmax[w];
+    int sc = 0;
+    if (!isfinite(nrm)) {
+      atomicOr(err, 2);
+    } else {
+      while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
+      if (nrm > theta) atomicOr(err, 4);
+    }
+    s_sh = sc;
+    sq[b] = sc;
+  }
+  __syncthreads();
+  const float f = ldexpf(1.0f, -s_sh);
+  float vmax = 0.0f;
+  for (int64_t e = tid; e < (int64_t)npad * npad; e += 256) {
+    const float2 v = Ab[e];
+    const float2 w = make_float2(v.x * f, v.y * f);
+    Ab[e] = w;
+    vmax = nmax(vmax, nmax(fabsf(w.x), fabsf(w.y)));
+  }
+  wave_max_to(vmax, amax + b);
+}
+void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,
+                         int32_t smax, int32_t* sq, int32_t* err, float* amax) {
+  hipLaunchKernelGGL(k_norm_scale32, dim3(nb), dim3(256), 0, s, npad, A, theta, smax, sq, err, amax);
 }
 
 struct CComboArgs { const float2* M[4]; float beta[4]; };
