@@ -1,0 +1,408 @@
+// Reverse-mode kernels of the FP64 master-pattern path: the gradient of a real function
+// f(dots) of the per-direction, per-site intensities dots_p = Re sum_gh Lgh(g,h) S_p(g,h)
+// with respect to the complex LUT entries U(g) and the per-site Debye-Waller factors B_p.
+//
+// Forward (per direction, beam set fixed):
+//   D(U)   Bethe-corrected DynMat (gvectors.f90:1123-1173)
+//   A      = alpha D,  alpha = 2 pi i dz / fl32(2 kn)
+//   E      = expm(A),  psi_1 = E e_1,  psi_{k+1} = E psi_k
+//   Lgh    = c sum_k lam_k psi_k psi_k^H,  c = dz / thick
+//   dots_p = Re sum_gh Lgh(g,h) S_p(g-h)
+// Complex adjoints x_bar are taken as df/dRe(x) + i df/dIm(x). With g_p = df/ddots_p:
+//   Lgh_bar = sum_p g_p conj(S_p),  M = c (Lgh_bar + Lgh_bar^H)
+//   phi_Z = lam_Z M psi_Z,  phi_k = lam_k M psi_k + E^H phi_{k+1}
+//   E_bar = sum_k phi_k psi_{k-1}^H  (psi_0 = e_1)
+//   A_bar = L_exp(A^H, E_bar)  (the Frechet derivative of expm at A^H in the direction E_bar,
+//           the upper-right block of expm([[A^H, E_bar], [0, A^H]]))
+//   D_bar = conj(alpha) A_bar
+//   U_bar(g-h) += D_bar(g,h)                                       (g != h)
+//   U_bar(g-w) += -lam/2 sum_{h != g} D_bar(g,h) conj(U(w-h)) / s_w  (Bethe, g != h)
+//   U_bar(w-h) += -lam/2 sum_{g != h} conj(U(g-w)) D_bar(g,h) / s_w
+//   U_bar(g-w) += -lam/s_w Re(D_bar(g,g)) U(g-w)                    (diagonal |U|^2 term)
+//   B_bar_p    = g_p Re sum_gh Lgh(g,h) dS_p/dB_p (g-h)
+// LUT cells are addressed as the forward assembly reads them (column-major offsets of the
+// Fortran bounds, checked against the allocation).
+#include <hip/hip_runtime.h>
+
+#include "ctem_internal.h"
+
+namespace ctem {
+
+__device__ __forceinline__ int lut_cell(const DevLut& L, int h, int k, int l) {
+  const int idx = (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
+  return ((unsigned)idx < (unsigned)L.ncell) ? idx : -1;
+}
+
+__device__ __forceinline__ double2 lut_val(const DevLut& L, int idx) {
+  if (idx < 0) return make_double2(0.0, 0.0);
+  const float2 u = L.lut[idx];
+  return make_double2((double)u.x, (double)u.y);
+}
+
+__device__ __forceinline__ void lut_add(double* glut, int idx, double re, double im) {
+  if (idx < 0) return;
+  atomicAdd(glut + 2 * idx, re);
+  atomicAdd(glut + 2 * idx + 1, im);
+}
+
+__device__ __forceinline__ int64_t s_index(const DevLut& L, int h, int k, int l) {
+  return (int64_t)(h - L.slo[0]) + (int64_t)L.sdim[0] * ((k - L.slo[1]) + (int64_t)L.sdim[1] * (l - L.slo[2]));
+}
+
+// M = c sum_p g_p (conj(S_p(g-h)) + S_p(h-g)) on the n x n block, zero in the padding
+__global__ void k_adj_m(Bucket B, BeamView V, DevRefl R, DevLut L, const double* __restrict__ gdots,
+                        double c, double2* __restrict__ M) {
+  const int npad = B.npad, b = blockIdx.y;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir];
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const int64_t per = (int64_t)npad * npad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % npad), j = (int)(e / npad);
+    double2 v = make_double2(0.0, 0.0);
+    if (i < n && j < n) {
+      const int32_t* gi = R.hkl + 3 * sidx[i];
+      const int32_t* gj = R.hkl + 3 * sidx[j];
+      const int64_t cp = s_index(L, gi[0] - gj[0], gi[1] - gj[1], gi[2] - gj[2]);
+      const int64_t cm = s_index(L, gj[0] - gi[0], gj[1] - gi[1], gj[2] - gi[2]);
+      for (int p = 0; p < L.nsite; p++) {
+        const double gp = gdots[(int64_t)dir * L.nsite + p];
+        const double2 sp = L.slut[(int64_t)p * L.sncell + cp];
+        const double2 sm = L.slut[(int64_t)p * L.sncell + cm];
+        v.x += c * gp * (sp.x + sm.x);
+        v.y += c * gp * (-sp.y + sm.y);
+      }
+    }
+    M[(int64_t)b * per + e] = v;
+  }
+}
+
+// Hm (nz x npad, column-major): row k holds psi_k^H, psi_0 = e_1
+__global__ void k_adj_hm(int npad, int nz, const double2* __restrict__ Psi, int64_t spsi,
+                         double2* __restrict__ Hm) {
+  const int b = blockIdx.y;
+  const int64_t tot = (int64_t)npad * nz;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e % nz), j = (int)(e / nz);
+    double2 v;
+    if (k == 0) v = make_double2(j == 0 ? 1.0 : 0.0, 0.0);
+    else {
+      const double2 p = Psi[(int64_t)b * spsi + (int64_t)(k - 1) * npad + j];
+      v = make_double2(p.x, -p.y);
+    }
+    Hm[(int64_t)b * tot + e] = v;
+  }
+}
+
+// backward depth recursion: Phi[:,k-1] = phi_k, phi_Z = lam_Z Q_Z, phi_k = lam_k Q_k + E^H phi_{k+1}
+// (Q = M Psi). One workgroup per matrix; row r of E^H is column r of E (contiguous).
+__global__ __launch_bounds__(256) void k_adj_depth(int npad, int nz, const double2* __restrict__ E,
+                                                   const double2* __restrict__ Q, const double* __restrict__ lam,
+                                                   double2* __restrict__ Phi) {
+  extern __shared__ double2 ph[];  // [npad]
+  const int b = blockIdx.x;
+  const double2* Eb = E + (int64_t)b * npad * npad;
+  const double2* Qb = Q + (int64_t)b * npad * nz;
+  double2* Pb = Phi + (int64_t)b * npad * nz;
+  for (int r = threadIdx.x; r < npad; r += blockDim.x) {
+    const double2 q = Qb[(int64_t)(nz - 1) * npad + r];
+    const double2 v = make_double2(lam[nz - 1] * q.x, lam[nz - 1] * q.y);
+    ph[r] = v;
+    Pb[(int64_t)(nz - 1) * npad + r] = v;
+  }
+  __syncthreads();
+  for (int k = nz - 2; k >= 0; k--) {
+    double2 y[2];
+    int cnt = 0;
+    for (int r = threadIdx.x; r < npad; r += blockDim.x, cnt++) {
+      double sr = 0.0, si = 0.0;
+      const double2* col = Eb + (int64_t)r * npad;
+      for (int c = 0; c < npad; c++) {
+        const double2 e = col[c], x = ph[c];
+        // conj(e) * x
+        sr += e.x * x.x + e.y * x.y;
+        si += e.x * x.y - e.y * x.x;
+      }
+      const double2 q = Qb[(int64_t)k * npad + r];
+      y[cnt] = make_double2(lam[k] * q.x + sr, lam[k] * q.y + si);
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int r = threadIdx.x; r < npad; r += blockDim.x, cnt++) {
+      ph[r] = y[cnt];
+      Pb[(int64_t)k * npad + r] = y[cnt];
+    }
+    __syncthreads();
+  }
+}
+
+// sigma_b = 2^(floor(log2 ||A||_1) - ceil(log2 ||Ebar||_1)) (0 when Ebar = 0), then
+// Mblk = [[A^H, sigma Ebar], [0, A^H]] (2 npad x 2 npad, column-major)
+__global__ __launch_bounds__(256) void k_adj_block(int npad, const double2* __restrict__ A,
+                                                   const double2* __restrict__ Eb, double* __restrict__ sigma,
+                                                   double2* __restrict__ Mblk) {
+  const int b = blockIdx.x;
+  const int64_t per = (int64_t)npad * npad;
+  const double2* Ab = A + b * per;
+  const double2* Gb = Eb + b * per;
+  double2* Mb = Mblk + 4 * b * per;
+  __shared__ double red[2][4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double na = 0.0, ng = 0.0;
+  for (int j = wave; j < npad; j += 4) {
+    double sa = 0.0, sg = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const double2 a = Ab[(int64_t)j * npad + i], g = Gb[(int64_t)j * npad + i];
+      sa += hypot(a.x, a.y);
+      sg += hypot(g.x, g.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) { sa += __shfl_xor(sa, o); sg += __shfl_xor(sg, o); }
+    na = fmax(na, sa);
+    ng = fmax(ng, sg);
+  }
+  if (lane == 0) { red[0][wave] = na; red[1][wave] = ng; }
+  __syncthreads();
+  na = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+  ng = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+  double sg = 0.0;
+  if (ng > 0.0 && isfinite(ng)) {
+    int ea, eg;
+    frexp(na > 0.0 ? na : 1.0, &ea);
+    frexp(ng, &eg);
+    sg = ldexp(1.0, (ea - 1) - eg);
+  }
+  if (tid == 0) sigma[b] = sg;
+  const int n2 = 2 * npad;
+  for (int64_t e = tid; e < (int64_t)n2 * n2; e += 256) {
+    const int i = (int)(e % n2), j = (int)(e / n2);
+    double2 v = make_double2(0.0, 0.0);
+    if (i < npad && j < npad) {
+      const double2 a = Ab[(int64_t)i * npad + j];  // A^H(i,j) = conj(A(j,i))
+      v = make_double2(a.x, -a.y);
+    } else if (i >= npad && j >= npad) {
+      const double2 a = Ab[(int64_t)(i - npad) * npad + (j - npad)];
+      v = make_double2(a.x, -a.y);
+    } else if (i < npad && j >= npad) {
+      const double2 g = Gb[(int64_t)(j - npad) * npad + i];
+      v = make_double2(sg * g.x, sg * g.y);
+    }
+    Mb[e] = v;
+  }
+}
+
+// D_bar = conj(alpha) A_bar / sigma from the upper-right block of expm(Mblk); conj(alpha) =
+// -i pre with pre = 2 pi dz / fl32(2 kn); zero outside the n x n block
+__global__ void k_adj_extract(Bucket B, BeamView V, const double* __restrict__ kn, double dz,
+                              const double2* __restrict__ T2, const double* __restrict__ sigma,
+                              double2* __restrict__ Dbar) {
+  const int npad = B.npad, b = blockIdx.y;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir];
+  const int64_t per = (int64_t)npad * npad;
+  const int n2 = 2 * npad;
+  const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
+  const double sg = sigma[b];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % npad), j = (int)(e / npad);
+    double2 v = make_double2(0.0, 0.0);
+    if (i < n && j < n && sg > 0.0) {
+      const double2 a = T2[(int64_t)b * 4 * per + (int64_t)(j + npad) * n2 + i];
+      // (-i pre) (a.x + i a.y) / sigma
+      v = make_double2(pre * a.y / sg, -pre * a.x / sg);
+    }
+    Dbar[b * per + e] = v;
+  }
+}
+
+// LUT gradient, terms indexed by (g, w): one workgroup per (direction, 32-row tile of g,
+// 32-chunk of weak beams). X(g,w) = sum_{h != g} D_bar(g,h) conj(U(w-h)) by chunks of h.
+__global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                    double mlambda, const double2* __restrict__ Dbar,
+                                                    double* __restrict__ glut, int wchunks) {
+  const int npad = B.npad, tiles = npad / 32;
+  const int b = blockIdx.y;
+  const int ti = blockIdx.x / wchunks, wc = blockIdx.x % wchunks;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir], nw = V.nnw[dir];
+  const int w0 = wc * 32;
+  if (ti >= tiles || ti * 32 >= n || w0 >= nw) return;
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const int32_t* widx = V.widx + V.woff[dir];
+  const float* wsg = V.wsg + V.woff[dir];
+  const double2* Db = Dbar + (int64_t)b * npad * npad;
+  __shared__ int gh[32][3], wh[32][3], hh[32][3];
+  __shared__ double2 Dt[32][33], Uc[32][33];
+  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
+  if (tid < 32) {
+    const int g = ti * 32 + tid, w = w0 + tid;
+    for (int d = 0; d < 3; d++) {
+      gh[tid][d] = (g < n) ? R.hkl[3 * sidx[g] + d] : 0;
+      wh[tid][d] = (w < nw) ? R.hkl[3 * widx[w] + d] : 0;
+    }
+  }
+  double xr[4] = {0, 0, 0, 0}, xi[4] = {0, 0, 0, 0};
+  for (int h0 = 0; h0 < n; h0 += 32) {
+    __syncthreads();
+    if (tid < 32) {
+      const int h = h0 + tid;
+      for (int d = 0; d < 3; d++) hh[tid][d] = (h < n) ? R.hkl[3 * sidx[h] + d] : 0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 256) {
+      const int a = e & 31, c = e >> 5;
+      // Dt[g][h] without the diagonal; Uc[h][w] = conj(U(w - h))
+      const int g = ti * 32 + a, h = h0 + c;
+      double2 dv = make_double2(0.0, 0.0);
+      if (g < n && h < n && g != h) dv = Db[(int64_t)h * npad + g];
+      Dt[a][c] = dv;
+      const int w = w0 + a, hq = h0 + c;
+      double2 uv = make_double2(0.0, 0.0);
+      if (w < nw && hq < n) {
+        const double2 u = lut_val(L, lut_cell(L, wh[a][0] - hh[c][0], wh[a][1] - hh[c][1], wh[a][2] - hh[c][2]));
+        uv = make_double2(u.x, -u.y);
+      }
+      Uc[c][a] = uv;
+    }
+    __syncthreads();
+    for (int c = 0; c < 32; c++) {
+      const double2 d = Dt[r][c];
+      for (int q = 0; q < 4; q++) {
+        const double2 u = Uc[c][cq + 8 * q];
+        xr[q] += d.x * u.x - d.y * u.y;
+        xi[q] += d.x * u.y + d.y * u.x;
+      }
+    }
+  }
+  const int g = ti * 32 + r;
+  if (g >= n) return;
+  const double2 dgg = Db[(int64_t)g * npad + g];
+  for (int q = 0; q < 4; q++) {
+    const int wl = cq + 8 * q, w = w0 + wl;
+    if (w >= nw) continue;
+    const int idx = lut_cell(L, gh[r][0] - wh[wl][0], gh[r][1] - wh[wl][1], gh[r][2] - wh[wl][2]);
+    const double sw = (double)wsg[w];
+    const double f = -0.5 * mlambda / sw;
+    const double2 u = lut_val(L, idx);
+    const double fd = -mlambda / sw * dgg.x;  // diagonal |U(g-w)|^2 / s_w term
+    lut_add(glut, idx, f * xr[q] + fd * u.x, f * xi[q] + fd * u.y);
+  }
+}
+
+// LUT gradient, terms indexed by (w, h): Y(w,h) = sum_{g != h} conj(U(g-w)) D_bar(g,h), and
+// the direct term D_bar(g,h) into U(g-h) (done by the first weak chunk).
+__global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                    double mlambda, const double2* __restrict__ Dbar,
+                                                    double* __restrict__ glut, int wchunks) {
+  const int npad = B.npad, tiles = npad / 32;
+  const int b = blockIdx.y;
+  const int tj = blockIdx.x / (wchunks + 1), wc = blockIdx.x % (wchunks + 1) - 1;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir], nw = V.nnw[dir];
+  if (tj >= tiles || tj * 32 >= n) return;
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const int32_t* widx = V.widx + V.woff[dir];
+  const float* wsg = V.wsg + V.woff[dir];
+  const double2* Db = Dbar + (int64_t)b * npad * npad;
+  const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
+  if (wc < 0) {
+    // direct term: column tile tj, all rows g != h
+    const int h = tj * 32 + r;
+    if (h >= n) return;
+    const int32_t* hv = R.hkl + 3 * sidx[h];
+    for (int g = cq; g < n; g += 8) {
+      if (g == h) continue;
+      const int32_t* gv = R.hkl + 3 * sidx[g];
+      const double2 d = Db[(int64_t)h * npad + g];
+      lut_add(glut, lut_cell(L, gv[0] - hv[0], gv[1] - hv[1], gv[2] - hv[2]), d.x, d.y);
+    }
+    return;
+  }
+  const int w0 = wc * 32;
+  if (w0 >= nw) return;
+  __shared__ int hh[32][3], wh[32][3], gg[32][3];
+  __shared__ double2 Dt[32][33], Uc[32][33];
+  if (tid < 32) {
+    const int h = tj * 32 + tid, w = w0 + tid;
+    for (int d = 0; d < 3; d++) {
+      hh[tid][d] = (h < n) ? R.hkl[3 * sidx[h] + d] : 0;
+      wh[tid][d] = (w < nw) ? R.hkl[3 * widx[w] + d] : 0;
+    }
+  }
+  double yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
+  for (int g0 = 0; g0 < n; g0 += 32) {
+    __syncthreads();
+    if (tid < 32) {
+      const int g = g0 + tid;
+      for (int d = 0; d < 3; d++) gg[tid][d] = (g < n) ? R.hkl[3 * sidx[g] + d] : 0;
+    }
+    __syncthreads();
+    for (int e = tid; e < 1024; e += 256) {
+      const int a = e & 31, c = e >> 5;
+      // Dt[g][h] (rows g0.., columns of tile tj) without the diagonal; Uc[g][w] = conj(U(g - w))
+      const int g = g0 + a, h = tj * 32 + c;
+      double2 dv = make_double2(0.0, 0.0);
+      if (g < n && h < n && g != h) dv = Db[(int64_t)h * npad + g];
+      Dt[a][c] = dv;
+      const int w = w0 + c;
+      double2 uv = make_double2(0.0, 0.0);
+      if (g < n && w < nw) {
+        const double2 u = lut_val(L, lut_cell(L, gg[a][0] - wh[c][0], gg[a][1] - wh[c][1], gg[a][2] - wh[c][2]));
+        uv = make_double2(u.x, -u.y);
+      }
+      Uc[a][c] = uv;
+    }
+    __syncthreads();
+    // thread (r = h, cq): Y(w, h) for w = cq + 8q
+    for (int a = 0; a < 32; a++) {
+      const double2 d = Dt[a][r];
+      for (int q = 0; q < 4; q++) {
+        const double2 u = Uc[a][cq + 8 * q];
+        yr[q] += u.x * d.x - u.y * d.y;
+        yi[q] += u.x * d.y + u.y * d.x;
+      }
+    }
+  }
+  const int h = tj * 32 + r;
+  if (h >= n) return;
+  for (int q = 0; q < 4; q++) {
+    const int wl = cq + 8 * q, w = w0 + wl;
+    if (w >= nw) continue;
+    const int idx = lut_cell(L, wh[wl][0] - hh[r][0], wh[wl][1] - hh[r][1], wh[wl][2] - hh[r][2]);
+    const double f = -0.5 * mlambda / (double)wsg[w];
+    lut_add(glut, idx, f * yr[q], f * yi[q]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+static int grid_rows(int64_t per) { return (int)((per + 255) / 256 < 256 ? (per + 255) / 256 : 256); }
+
+void launch_adj_m(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                  const double* gdots, double c, double2* M) {
+  const int64_t per = (int64_t)B.npad * B.npad;
+  hipLaunchKernelGGL(k_adj_m, dim3(grid_rows(per), B.nb), dim3(256), 0, s, B, V, R, L, gdots, c, M);
+}
+void launch_adj_hm(hipStream_t s, int npad, int nb, int nz, const double2* Psi, int64_t spsi, double2* Hm) {
+  hipLaunchKernelGGL(k_adj_hm, dim3(grid_rows((int64_t)npad * nz), nb), dim3(256), 0, s, npad, nz, Psi, spsi, Hm);
+}
+void launch_adj_depth(hipStream_t s, int npad, int nb, int nz, const double2* E, const double2* Q,
+                      const double* lam, double2* Phi) {
+  hipLaunchKernelGGL(k_adj_depth, dim3(nb), dim3(256), npad * sizeof(double2), s, npad, nz, E, Q, lam, Phi);
+}
+void launch_adj_block(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar,
+                      double* sigma, double2* Mblk) {
+  hipLaunchKernelGGL(k_adj_block, dim3(nb), dim3(256), 0, s, npad, A, Ebar, sigma, Mblk);
+}
+void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
+                        const double2* T2, const double* sigma, double2* Dbar) {
+  const int64_t per = (int64_t)B.npad * B.npad;
+  hipLaunchKernelGGL(k_adj_extract, dim3(grid_rows(per), B.nb), dim3(256), 0, s, B, V, kn, dz, T2, sigma, Dbar);
+}
+void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                    double mlambda, const double2* Dbar, double* glut, int max_nw) {
+  const int tiles = B.npad / 32;
+  const int wchunks = std::max(1, (max_nw + 31) / 32);
+  hipLaunchKernelGGL(k_adj_lut_gw, dim3(tiles * wchunks, B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut, wchunks);
+  hipLaunchKernelGGL(k_adj_lut_wh, dim3(tiles * (wchunks + 1), B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut, wchunks);
+}
+
+}  // namespace ctem

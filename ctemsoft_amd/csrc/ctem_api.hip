@@ -78,7 +78,7 @@ struct ctem_ctx {
   hipStream_t stream = nullptr;
   bool have_crystal = false;
   // crystal on the device
-  DBuf g, hkl, cut1, cut2, dbd, lut, slut, rmt;
+  DBuf g, hkl, cut1, cut2, dbd, lut, slut, dslut, rmt;
   int32_t nref = 0, nsite = 0, ncell = 0;
   int32_t lut_lo[3] = {0, 0, 0}, lut_dim[3] = {0, 0, 0};
   int32_t slut_lo[3] = {0, 0, 0}, slut_dim[3] = {0, 0, 0};
@@ -127,6 +127,8 @@ struct ctem_plan {
   DBuf keep_lgh;
   // work
   DBuf A, W, Psi, Psi32, sq, dirs, partial, wmax;
+  // adjoint work
+  DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
   std::vector<int32_t> h_sq;
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
@@ -168,7 +170,7 @@ extern "C" int32_t ctem_ctx_destroy(ctem_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   while (!c->plans.empty()) ctem_plan_destroy(c->plans.back());
-  for (DBuf* b : {&c->g, &c->hkl, &c->cut1, &c->cut2, &c->dbd, &c->lut, &c->slut, &c->rmt, &c->s_A,
+  for (DBuf* b : {&c->g, &c->hkl, &c->cut1, &c->cut2, &c->dbd, &c->lut, &c->slut, &c->dslut, &c->rmt, &c->s_A,
                   &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
                   &c->s_nns, &c->s_zero, &c->s_off, &c->s_part, &c->s_dots})
     b->release();
@@ -193,11 +195,14 @@ extern "C" int32_t ctem_device_info(ctem_ctx* c, char* name, int32_t name_len, i
 // LUT box (CTEMEBSDmaster.f90:418-449; DBWF = exp(-B |d|^2/4) with |d| from the float32
 // CalcLength, no occupancy factor).
 // S_p(d) for every d in the box lo..lo+dim-1, site after site, column-major (h fastest)
+// dout receives dS_p/dB_p = sum over the orbit of -|d|^2/4 Z^2 DBWF (cos, sin), in double
+// (the derivative of the smooth function the float32 terms round).
 static void build_slut(const ctem_crystal* c, const int32_t lo[3], const int32_t dim[3],
-                       std::vector<double>& out) {
+                       std::vector<double>& out, std::vector<double>& dout) {
 #pragma clang fp contract(off)
   const size_t ncell = (size_t)dim[0] * dim[1] * dim[2];
   out.assign((size_t)2 * ncell * c->nsite, 0.0);
+  dout.assign((size_t)2 * ncell * c->nsite, 0.0);
   const double tpi = 2.0 * kPi;
   size_t obase = 0;
   for (int p = 0; p < c->nsite; p++) {
@@ -213,7 +218,7 @@ static void build_slut(const ctem_crystal* c, const int32_t lo[3], const int32_t
           const float len = std::sqrt(dot_r(df, df, c->rmt));
           const float len2 = len * len;
           const double kkl = (double)(0.25f * len2);
-          double sr = 0.0, si = 0.0;
+          double sr = 0.0, si = 0.0, dr = 0.0, di = 0.0;
           for (int a = 0; a < no; a++) {
             double s = 0.0;
             for (int t = 0; t < 3; t++) s += (double)d[t] * orb[3 * a + t];
@@ -226,10 +231,14 @@ static void build_slut(const ctem_crystal* c, const int32_t lo[3], const int32_t
             const float re = pre * cs, im = pre * sn;
             sr += (double)re;
             si += (double)im;
+            dr += -kkl * znsq * dbwf * cd;
+            di += -kkl * znsq * dbwf * sd;
           }
           const size_t cell = (size_t)h + (size_t)dim[0] * ((size_t)kk + (size_t)dim[1] * l);
           out[2 * ((size_t)p * ncell + cell)] = sr;
           out[2 * ((size_t)p * ncell + cell) + 1] = si;
+          dout[2 * ((size_t)p * ncell + cell)] = dr;
+          dout[2 * ((size_t)p * ncell + cell) + 1] = di;
         }
     obase += no;
   }
@@ -273,8 +282,8 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   }
   std::vector<float> lutf(2 * (size_t)ncell);
   for (size_t i = 0; i < 2 * (size_t)ncell; i++) lutf[i] = (float)x->lut[i];
-  std::vector<double> slut;
-  build_slut(x, slo, sdim, slut);
+  std::vector<double> slut, dslut;
+  build_slut(x, slo, sdim, slut, dslut);
   int nat = 0;
   for (int p = 0; p < x->nsite; p++) nat += x->site_norbit[p];
   int32_t rc;
@@ -285,6 +294,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   if ((rc = upload(c->dbd, dbd.data(), n, c->stream))) return rc;
   if ((rc = upload(c->lut, lutf.data(), lutf.size(), c->stream))) return rc;
   if ((rc = upload(c->slut, slut.data(), slut.size() ? slut.size() : 0, c->stream))) return rc;
+  if ((rc = upload(c->dslut, dslut.data(), dslut.size() ? dslut.size() : 0, c->stream))) return rc;
   if ((rc = upload(c->rmt, x->rmt, 9, c->stream))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->nref = n;
@@ -457,7 +467,8 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
   (void)hipStreamSynchronize(p->ctx->stream);
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
                   &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->A, &p->W, &p->Psi,
-                  &p->Psi32, &p->sq, &p->dirs, &p->partial, &p->wmax})
+                  &p->Psi32, &p->sq, &p->dirs, &p->partial, &p->wmax, &p->g_dots, &p->g_lut, &p->g_dots2,
+                  &p->aM, &p->aQ, &p->aPhi, &p->aHm, &p->aEbar, &p->aBlk, &p->aW2, &p->aDbar, &p->aSig})
     b->release();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
   auto& v = p->ctx->plans;
@@ -564,23 +575,26 @@ struct GemmTimer {
 constexpr double kTheta64 = 0.9;   // ||A/2^s||_1 bound for the degree-16 Taylor polynomial
 constexpr int kSmax = 48;
 
-// A (nb matrices, npad^2 each) holds 2 pi i dz M' on entry. W holds 5*nb*npad^2 work space.
-// On exit Psi[b][:,k-1] = expm(A_b)^k e_1 for k = 1..nz. h_n[b] are the true sizes
-// (for the algorithmic flop count), h_sq receives the squaring counts.
-int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
-                        double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
-                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
+// A (nb matrices, npad^2 each) holds 2 pi i dz M' on entry and is scaled by 2^-s in place.
+// W holds 5*nb*npad^2 work space; on exit E_b = expm(A_b) is in the fifth slice of W (the
+// returned pointer). h_n[b] are the true sizes (for the algorithmic flop count), h_sq
+// receives the squaring counts.
+double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
+                   int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n,
+                   std::vector<int32_t>& h_sq, int32_t& rc_out) {
+  rc_out = CTEM_OK;
   const int64_t per = (int64_t)npad * npad;
   double2* A2 = W;
   double2* A3 = W + per * nb;
   double2* A4 = W + 2 * per * nb;
   double2* H = W + 3 * per * nb;
   double2* T = W + 4 * per * nb;
+#define EXPM_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { rc_out = fail(CTEM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); return nullptr; } } while (0)
   launch_norm_scale(s, npad, nb, A, kTheta64, kSmax, d_sq, d_err);
-  HIPCHK(hipGetLastError());
+  EXPM_CHK(hipGetLastError());
   h_sq.resize(nb);
-  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  EXPM_CHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  EXPM_CHK(hipStreamSynchronize(s));
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
   double n3 = 0.0;
@@ -607,29 +621,39 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
   };
   int32_t rc;
   const double f3 = 8.0 * n3;
-  if ((rc = sq_gemm(A2, A, A, 0.0, {}, nullptr, 0, f3))) return rc;
-  if ((rc = sq_gemm(A3, A, A2, 0.0, {}, nullptr, 0, f3))) return rc;
-  if ((rc = sq_gemm(A4, A2, A2, 0.0, {}, nullptr, 0, f3))) return rc;
+  if ((rc = sq_gemm(A2, A, A, 0.0, {}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
+  if ((rc = sq_gemm(A3, A, A2, 0.0, {}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
+  if ((rc = sq_gemm(A4, A2, A2, 0.0, {}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
   {
     const double2* M[4] = {A, A2, A3, A4};
     const double beta[4] = {inv_fact(13), inv_fact(14), inv_fact(15), inv_fact(16)};
     launch_combo(s, npad, nb, H, inv_fact(12), M, beta);
-    HIPCHK(hipGetLastError());
+    EXPM_CHK(hipGetLastError());
   }
-  if ((rc = sq_gemm(T, A4, H, inv_fact(8), {{A, inv_fact(9)}, {A2, inv_fact(10)}, {A3, inv_fact(11)}}, nullptr, 0, f3))) return rc;
-  if ((rc = sq_gemm(H, A4, T, inv_fact(4), {{A, inv_fact(5)}, {A2, inv_fact(6)}, {A3, inv_fact(7)}}, nullptr, 0, f3))) return rc;
-  if ((rc = sq_gemm(T, A4, H, 1.0, {{A, 1.0}, {A2, inv_fact(2)}, {A3, inv_fact(3)}}, nullptr, 0, f3))) return rc;
-  // squarings with per-matrix counts; the result ends in T (buffer 0) after the parity copy
+  if ((rc = sq_gemm(T, A4, H, inv_fact(8), {{A, inv_fact(9)}, {A2, inv_fact(10)}, {A3, inv_fact(11)}}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
+  if ((rc = sq_gemm(H, A4, T, inv_fact(4), {{A, inv_fact(5)}, {A2, inv_fact(6)}, {A3, inv_fact(7)}}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
+  if ((rc = sq_gemm(T, A4, H, 1.0, {{A, 1.0}, {A2, inv_fact(2)}, {A3, inv_fact(3)}}, nullptr, 0, f3))) { rc_out = rc; return nullptr; }
+  // squarings with per-matrix counts; the result ends in T after the parity copy
   double2* buf[2] = {T, H};
   for (int i = 0; i < smax; i++) {
     double fl = 0.0;
     for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
-    if ((rc = sq_gemm(buf[(i + 1) & 1], buf[i & 1], buf[i & 1], 0.0, {}, d_sq, i, fl))) return rc;
+    if ((rc = sq_gemm(buf[(i + 1) & 1], buf[i & 1], buf[i & 1], 0.0, {}, d_sq, i, fl))) { rc_out = rc; return nullptr; }
   }
   launch_parity_copy(s, npad, nb, d_sq, H, T);
-  HIPCHK(hipGetLastError());
-  // depth: psi_k = E^k e_1 by the cache-resident recursion
-  launch_depth(s, npad, nb, nz, T, 0, 0, Psi, spsi, depth_grid);
+  EXPM_CHK(hipGetLastError());
+#undef EXPM_CHK
+  return T;
+}
+
+// FP64 form: expm, then psi_k = E^k e_1 by the depth recursion
+int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
+                        double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
+                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
+  int32_t rc;
+  double2* E = expm_fp64(tp, s, npad, nb, A, W, d_sq, d_err, h_n, h_sq, rc);
+  if (!E) return rc;
+  launch_depth(s, npad, nb, nz, E, 0, 0, Psi, spsi, depth_grid);
   HIPCHK(hipGetLastError());
   return CTEM_OK;
 }
@@ -857,6 +881,139 @@ extern "C" int32_t ctem_plan_get_lgh(ctem_plan* p, double* lgh, int32_t cap, int
   if (!lgh) return CTEM_OK;
   if (cap < p->keep_n * p->keep_n) return fail(CTEM_E_ARG, "capacity too small");
   HIPCHK(hipMemcpy(lgh, p->keep_lgh.p, 16 * (size_t)p->keep_n * p->keep_n, hipMemcpyDeviceToHost));
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// reverse mode (FP64): gradient of f(dots) with respect to the LUT and the site B factors
+extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* glut, double* gb) {
+  if (!p || !gdots || !glut) return fail(CTEM_E_ARG, "ctem_plan_adjoint: null argument");
+  ctem_ctx* c = p->ctx;
+  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_adjoint: no crystal uploaded");
+  if (c->nsite > 0 && !gb) return fail(CTEM_E_ARG, "ctem_plan_adjoint: gb is NULL");
+  int32_t rc;
+  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int nk = p->numk, ns = std::max(1, c->nsite), nz = p->nz;
+  for (int i = 0; i < nk; i++)
+    if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "ctem_plan_adjoint: at most 512 strong beams per direction");
+  if ((rc = upload(p->g_dots, gdots, (size_t)nk * c->nsite, s)) || (rc = p->g_lut.ensure(16 * (size_t)c->ncell)) ||
+      (rc = p->g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
+      (rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * ns)))
+    return rc;
+  HIPCHK(hipMemsetAsync(p->g_lut.p, 0, 16 * (size_t)c->ncell, s));
+  HIPCHK(hipMemsetAsync(p->g_dots2.p, 0, 8 * (size_t)nk * ns, s));
+  HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
+  std::map<int, std::vector<int32_t>> buckets;
+  for (int i = 0; i < nk; i++) buckets[npad_of(p->h_nns[i])].push_back(i);
+  std::vector<int32_t> order;
+  for (auto& kv : buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
+  if ((rc = upload(p->dirs, order.data(), order.size(), s))) return rc;
+  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
+             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
+  DevRefl R = c->refl();
+  DevLut L = c->lutv(p->err.as<int32_t>());
+  DevLut LdB = L;
+  LdB.slut = c->dslut.as<double2>();
+  const double dzt = p->dz / p->thick;
+  const size_t budget = (size_t)16 << 30;
+  size_t off = 0;
+  std::vector<int32_t> h_sq, h_sq2;
+  for (auto& kv : buckets) {
+    const int npad = kv.first;
+    const int nbk = (int)kv.second.size();
+    const int64_t per = (int64_t)npad * npad;
+    const int64_t spsi = (int64_t)npad * nz;
+    const size_t per_dir = (size_t)(33 * per + 4 * spsi) * sizeof(double2) + 256;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
+    for (int c0 = 0; c0 < nbk; c0 += chunk) {
+      const int nb = std::min(chunk, nbk - c0);
+      if ((rc = p->A.ensure((size_t)nb * per * 16)) || (rc = p->W.ensure((size_t)5 * nb * per * 16)) ||
+          (rc = p->Psi.ensure((size_t)nb * spsi * 16)) || (rc = p->sq.ensure(4 * (size_t)nb)) ||
+          (rc = p->partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
+          (rc = p->aM.ensure((size_t)nb * per * 16)) || (rc = p->aQ.ensure((size_t)nb * spsi * 16)) ||
+          (rc = p->aPhi.ensure((size_t)nb * spsi * 16)) || (rc = p->aHm.ensure((size_t)nb * spsi * 16)) ||
+          (rc = p->aEbar.ensure((size_t)nb * per * 16)) || (rc = p->aBlk.ensure((size_t)4 * nb * per * 16)) ||
+          (rc = p->aW2.ensure((size_t)20 * nb * per * 16)) || (rc = p->aDbar.ensure((size_t)nb * per * 16)) ||
+          (rc = p->aSig.ensure(8 * (size_t)nb)))
+        return rc;
+      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
+      std::vector<int> hn(nb), hn2(nb);
+      int max_nw = 0;
+      for (int b = 0; b < nb; b++) {
+        const int d = kv.second[c0 + b];
+        hn[b] = p->h_nns[d];
+        hn2[b] = 2 * p->h_nns[d];
+        max_nw = std::max(max_nw, p->h_nnw[d]);
+      }
+      double2* A = p->A.as<double2>();
+      // forward: E and psi_1..psi_Z
+      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, A);
+      HIPCHK(hipGetLastError());
+      double2* E = expm_fp64(nullptr, s, npad, nb, A, p->W.as<double2>(), p->sq.as<int32_t>(),
+                             p->err.as<int32_t>(), hn, h_sq, rc);
+      if (!E) return rc;
+      launch_depth(s, npad, nb, nz, E, 0, 0, p->Psi.as<double2>(), spsi, c->cu_count);
+      HIPCHK(hipGetLastError());
+      // B: d dots_p / d B_p = Re sum Lgh dS_p/dB_p
+      launch_lgh_dots(s, B, V, R, LdB, p->Psi.as<double2>(), spsi, nz, p->lam.as<double>(), dzt,
+                      p->partial.as<double>(), p->g_dots2.as<double>(), -1, nullptr);
+      HIPCHK(hipGetLastError());
+      // M = c (Lbar + Lbar^H), Q = M Psi
+      launch_adj_m(s, B, V, R, L, p->g_dots.as<double>(), dzt, p->aM.as<double2>());
+      HIPCHK(hipGetLastError());
+      {
+        ZGemm g{};
+        g.m = npad; g.n = nz; g.k = npad;
+        g.ldx = npad; g.ldy = npad; g.ldc = npad; g.ldm = npad;
+        g.X = p->aM.as<double2>(); g.sx = per;
+        g.Y = p->Psi.as<double2>(); g.sy = spsi;
+        g.C = p->aQ.as<double2>(); g.sc = spsi;
+        g.alpha = 1.0; g.nb = nb;
+        launch_zgemm(s, g);
+        HIPCHK(hipGetLastError());
+      }
+      // phi_k backward, Ebar = Phi [psi_0 .. psi_{Z-1}]^H
+      launch_adj_depth(s, npad, nb, nz, E, p->aQ.as<double2>(), p->lam.as<double>(), p->aPhi.as<double2>());
+      launch_adj_hm(s, npad, nb, nz, p->Psi.as<double2>(), spsi, p->aHm.as<double2>());
+      HIPCHK(hipGetLastError());
+      {
+        ZGemm g{};
+        g.m = npad; g.n = npad; g.k = nz;
+        g.ldx = npad; g.ldy = nz; g.ldc = npad; g.ldm = npad;
+        g.X = p->aPhi.as<double2>(); g.sx = spsi;
+        g.Y = p->aHm.as<double2>(); g.sy = spsi;
+        g.C = p->aEbar.as<double2>(); g.sc = per;
+        g.alpha = 1.0; g.nb = nb;
+        launch_zgemm(s, g);
+        HIPCHK(hipGetLastError());
+      }
+      // A_bar = L_exp(A^H, Ebar) from expm([[A^H, sigma Ebar], [0, A^H]]); A rebuilt unscaled
+      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, A);
+      launch_adj_block(s, npad, nb, A, p->aEbar.as<double2>(), p->aSig.as<double>(), p->aBlk.as<double2>());
+      HIPCHK(hipGetLastError());
+      double2* T2 = expm_fp64(nullptr, s, 2 * npad, nb, p->aBlk.as<double2>(), p->aW2.as<double2>(),
+                              p->sq.as<int32_t>(), p->err.as<int32_t>(), hn2, h_sq2, rc);
+      if (!T2) return rc;
+      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2, p->aSig.as<double>(), p->aDbar.as<double2>());
+      HIPCHK(hipGetLastError());
+      // LUT entries
+      launch_adj_lut(s, B, V, R, L, c->mlambda, p->aDbar.as<double2>(), p->g_lut.as<double>(), max_nw);
+      HIPCHK(hipGetLastError());
+    }
+    off += nbk;
+  }
+  std::vector<double> d2((size_t)nk * ns);
+  HIPCHK(hipMemcpyAsync(glut, p->g_lut.p, 16 * (size_t)c->ncell, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(d2.data(), p->g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if ((rc = check_err_flag(p->err, s, "ctem_plan_adjoint"))) return rc;
+  for (int q = 0; q < c->nsite; q++) {
+    double acc = 0.0;
+    for (int i = 0; i < nk; i++) acc += gdots[(size_t)i * c->nsite + q] * d2[(size_t)i * c->nsite + q];
+    gb[q] = acc;
+  }
   return CTEM_OK;
 }
 

@@ -158,4 +158,17 @@ void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float bet
 void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                          const float2* src, float2* dst);
 
+// ---- reverse mode (ctem_adjoint.hip) ----
+void launch_adj_m(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                  const double* gdots, double c, double2* M);
+void launch_adj_hm(hipStream_t s, int npad, int nb, int nz, const double2* Psi, int64_t spsi, double2* Hm);
+void launch_adj_depth(hipStream_t s, int npad, int nb, int nz, const double2* E, const double2* Q,
+                      const double* lam, double2* Phi);
+void launch_adj_block(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar,
+                      double* sigma, double2* Mblk);
+void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
+                        const double2* T2, const double* sigma, double2* Dbar);
+void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                    double mlambda, const double2* Dbar, double* glut, int max_nw);
+
 }  // namespace ctem

@@ -31,7 +31,8 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_upload_crystal", "ctem_kvectors", "ctem_laue_equiv", "ctem_plan_create",
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
-           "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms"]
+           "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
+           "ctem_plan_adjoint"]
 
 
 class CtemError(RuntimeError):
@@ -73,6 +74,7 @@ def lib():
             "ctem_plan_dynmat": [P, I32, P, I32, P], "ctem_plan_scatter": [P, P, I32, I32, I32, P],
             "ctem_plan_timing": [P, P, P, P, P], "ctem_plan_set_mixed_terms": [P, I32, I32],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
+            "ctem_plan_adjoint": [P, P, P, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -278,6 +280,18 @@ class Plan:
         d = np.zeros(self.numk * self.nsite)
         _check(lib().ctem_plan_get_dots(self._h, _p(d)))
         return d.reshape(self.numk, self.nsite)
+
+    def adjoint(self, gdots):
+        """FP64 reverse mode: gdots[ik, p] = df/ddots -> (df/dU as a complex array shaped like
+        the uploaded LUT, [l][k][h], with df/dRe U + i df/dIm U; df/dB per site)."""
+        st = self.ctx.setup
+        g = np.ascontiguousarray(gdots, np.float64).reshape(self.numk, self.nsite)
+        ncell = int(np.prod(st.lut_dim))
+        glut = np.zeros(2 * ncell)
+        gb = np.zeros(max(1, self.nsite))
+        _check(lib().ctem_plan_adjoint(self._h, _p(g), _p(glut), _p(gb)))
+        d = st.lut_dim
+        return glut.view(np.complex128).reshape(d[2], d[1], d[0]), gb[:self.nsite].copy()
 
     def svals(self):
         s = np.zeros(self.numk * self.nsite, np.float32)

@@ -141,6 +141,15 @@ CTEM_API int32_t ctem_plan_get_lgh(ctem_plan* plan, double* lgh, int32_t cap, in
 /* DynMat of one direction assembled on the GPU (row-major n x n), for parity tests. */
 CTEM_API int32_t ctem_plan_dynmat(ctem_plan* plan, int32_t ik, double* dynmat, int32_t cap, int32_t* n);
 
+/* Reverse mode, FP64: given gdots[ik*nsite + p] = df/ddots for a real function f of the plan's
+ * intensities (ctem_plan_get_dots), returns df/dU for every LUT entry (glut[2*cell] real part,
+ * glut[2*cell+1] imaginary part, cells in the Fortran order of the uploaded LUT) and df/dB_p
+ * per site (gb[p], B in nm^2). The beam sets are held fixed; the forward matrices are
+ * recomputed, not stored. The reference has no derivatives; this entry serves the fitting of
+ * Ucg and Debye-Waller factors to a measured master pattern (the mixed forward is the
+ * intended inner loop, this gradient the outer step). Up to 512 strong beams per direction. */
+CTEM_API int32_t ctem_plan_adjoint(ctem_plan* plan, const double* gdots, double* glut, double* gb);
+
 /* Scatter svals into a Lambert master array sr[nsite][2*npy+1][2*npx+1] (float32), applying
  * the Laue-symmetry equivalents per direction (CTEMEBSDmaster.f90:462-479). ij holds the
  * (i,j) of every direction of the plan. sr is a host buffer. */
