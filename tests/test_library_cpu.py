@@ -77,3 +77,31 @@ def test_hex_kvectors_match_oracle(oracle_mod):
         b = oracle_mod.kvectors(20, 23, isym, 1, dsm, 0.0098)
         for x, y in zip(a, b):
             assert np.array_equal(x, y), isym
+
+
+def test_pixel_owner_covers_the_master_array():
+    """Every pixel of the square Lambert array is written by some direction of the m-3m wedge,
+    and the owner is the last direction whose Laue equivalents include it."""
+    from ctemsoft_amd import ebsd, fit
+    fx = load_golden("ni_c0")
+    ij, _, _ = ebsd.kvectors(12, 12, 11, 0, fx["dsm"], float(fx["mLambda"]))
+    own = fit.pixel_owner(ij, 12, 12, 11)
+    assert (own >= 0).all()
+    i, j = ij[own[5 * 25 + 7]]
+    q = ebsd.laue_equiv(int(i), int(j), 11)
+    assert any((a + 12) + 25 * (b + 12) == 5 * 25 + 7 for a, b in q)
+
+
+def test_target_setup_shares_factors_between_g_and_minus_g():
+    from ctemsoft_amd import ebsd, fit
+    fx = load_golden("ni_c0")
+    st = ebsd.Setup.from_fixture(fx)
+    tt = fit.perturbed_target_setup(st, 20264)
+    d = st.lut_dim
+    a = st.lut.view(np.complex128).reshape(d[2], d[1], d[0])
+    b = tt.lut.view(np.complex128).reshape(d[2], d[1], d[0])
+    m = np.abs(a) > 0
+    f = np.where(m, b / np.where(m, a, 1), 0)
+    assert np.allclose(f[m], f[::-1, ::-1, ::-1][m], rtol=1e-6)
+    assert 0.005 < np.std(np.abs(f[m])) < 0.02
+    assert np.allclose(tt.site_b, st.site_b * np.float32(1.05))
