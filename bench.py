@@ -48,6 +48,25 @@ def load_setup(name):
     return wl, st, fx
 
 
+def load_jobs(name):
+    """One job per energy bin: (label, setup, lam, dz, thick, fixture-or-None). Single-energy
+    workloads read their fixture; energy series read the per-bin setups archive."""
+    from ctemsoft_amd import ebsd, workloads
+    wl = workloads.get(name)
+    if wl.energies:
+        arc = np.load(os.path.join(REPO, "tests", "golden", f"{name}_setups.npz"))
+        jobs = []
+        for i in range(int(arc["nenergy"])):
+            view = {k[len(f"e{i}_"):]: arc[k] for k in arc.files if k.startswith(f"e{i}_")}
+            st = ebsd.Setup.from_fixture(view)
+            e = float(view["ekev"])
+            dz = wl.dz_for(e)
+            jobs.append((f"{e:g} kV", st, wl.depth_weights(e), dz, float(int(wl.nz * dz)), None))
+        return wl, jobs
+    wl, st, fx = load_setup(name)
+    return wl, [(f"{wl.ekev:g} kV", st, wl.depth_weights(), wl.dz, float(int(wl.nz * wl.dz)), fx)]
+
+
 def directions(wl, st):
     from ctemsoft_amd import ebsd
     npx, npy = ebsd.lambert_dims(wl.npx, st.usehex)
@@ -71,7 +90,7 @@ def measured_traffic(workload, head):
     return None
 
 
-def cpu_baseline(wl, lam, k_count, seconds=20.0):
+def cpu_baseline(wl, lam, k_count, seconds=20.0, ekev=None):
     """Reference beam loop (compiled from the reference sources, MKL sequential) on one
     host core over a seeded sample of the workload's northern directions."""
     from oracle import refrun
@@ -81,12 +100,13 @@ def cpu_baseline(wl, lam, k_count, seconds=20.0):
     with tempfile.TemporaryDirectory(prefix="ctem_cpu_") as work:
         # calibrate on 4 directions, then size the sample to ~`seconds`
         ks = np.sort(rng.choice(k_count, 4, replace=False))
-        t4 = max(refrun.time_reference(work, wl, lam, ks), 1e-3)
+        t4 = max(refrun.time_reference(work, wl, lam, ks, ekev), 1e-3)
         n = int(max(8, min(512, 4 * seconds / t4)))
         ks = np.sort(rng.choice(k_count, n, replace=False))
-        t = refrun.time_reference(work, wl, lam, ks)
+        t = refrun.time_reference(work, wl, lam, ks, ekev)
     return dict(value=n / t, unit="directions/s", cores=1, kind="reference",
-                sample=f"{n} seeded directions of {wl.name} (northern hemisphere), Compute_DynMat + "
+                sample=f"{n} seeded directions of {wl.name}"
+                       f"{'' if ekev is None else f' at {ekev:g} kV'} (northern hemisphere), Compute_DynMat + "
                        f"CalcLgh3 per direction, amdflang -O2 + MKL 2021.4 sequential, {t:.1f} s")
 
 
@@ -109,40 +129,50 @@ def main():
     import torch  # noqa: F401  (HIP runtime present)
     from ctemsoft_amd import ebsd
 
-    wl, st, fx = load_setup(a.workload)
-    ij, k, kn = directions(wl, st)
-    lam = wl.depth_weights()
-    thick = float(int(wl.nz * wl.dz))
-    ctx = ebsd.Context(0)
-    name, cus = ctx.device_info()
-    log(f"device {name}, {cus} CUs; workload {wl.name}: {len(k)} directions, nz={wl.nz}")
-    ctx.upload_crystal(st)
-    plan = ctx.plan(k, kn, lam, wl.dz, thick)
-    ns, nw = plan.select()
+    wl, jobs = load_jobs(a.workload)
     tt, ts = (int(v) for v in a.terms.split(","))
-    plan.set_mixed_terms(tt, ts)
+    # one plan per energy bin, each holding its bin's crystal, all resident before timing; the
+    # plans share the context's work space
+    ctx = ebsd.Context(0)
+    plans, ndir = [], 0
+    ns_all, nw_all = [], []
+    for label, st, lam, dz, thick, _ in jobs:
+        ij, k, kn = directions(wl, st)
+        ctx.upload_crystal(st)
+        plan = ctx.plan(k, kn, lam, dz, thick)
+        ns, nw = plan.select()
+        plan.set_mixed_terms(tt, ts)
+        ns_all.append(ns); nw_all.append(nw)
+        plans.append(plan)
+        ndir += len(k)
+    ns, nw = np.concatenate(ns_all), np.concatenate(nw_all)
+    name, cus = ctx.device_info()
+    log(f"device {name}, {cus} CUs; workload {wl.name}: {len(jobs)} energy bin(s), {ndir} directions, nz={wl.nz}")
     log(f"strong beams {ns.min()}..{ns.max()} (mean {ns.mean():.1f}); weak {nw.min()}..{nw.max()} (mean {nw.mean():.1f})")
     precs = ["fp64", "mixed"] if a.precision == "both" else [a.precision]
     res = {}
     for prec in precs:
         try:
             for w in range(a.warmup):
-                plan.run(prec)
+                for plan in plans:
+                    plan.run(prec)
                 log(f"{prec}: warmup {w + 1}/{a.warmup} done")
         except ebsd.CtemError as e:
             log(f"{prec}: not available ({e})")
             continue
-        times, gemm = [], []
-        for s in range(a.steps):
+        times = []
+        for s_ in range(a.steps):
             t0 = time.perf_counter()
-            plan.run(prec)
+            for plan in plans:
+                plan.run(prec)
             times.append(time.perf_counter() - t0)
-            gemm.append(plan.timing())
-            log(f"{prec}: step {s + 1}/{a.steps} {times[-1] * 1e3:.1f} ms")
-        dots = plan.dots().copy()
+            log(f"{prec}: step {s_ + 1}/{a.steps} {times[-1] * 1e3:.1f} ms")
+        tims = [plan.timing() for plan in plans]
+        g = dict(gemm_ms=sum(t["gemm_ms"] for t in tims), gemm_flops=sum(t["gemm_flops"] for t in tims),
+                 gemm_launches=sum(t["gemm_launches"] for t in tims))
+        dots = np.concatenate([plan.dots().copy() for plan in plans])
         ms = float(np.mean(times) * 1e3)
-        g = gemm[-1]
-        res[prec] = dict(ms=ms, dps=len(k) / (ms * 1e-3), gemm=g, dots=dots,
+        res[prec] = dict(ms=ms, dps=ndir / (ms * 1e-3), gemm=g, dots=dots,
                          gemm_tflops=g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12 if g["gemm_ms"] > 0 else 0.0)
         log(f"{prec}: {ms:.1f} ms/step, {res[prec]['dps']:.0f} directions/s, GEMM {g['gemm_ms']:.1f} ms "
             f"({res[prec]['gemm_tflops']:.2f} TFLOP/s over {g['gemm_launches']} launches)")
@@ -161,7 +191,8 @@ def main():
         err = float(np.max(np.abs(dmx[fin] - d64[fin]) / np.abs(d64[fin]))) if fin.any() else None
         if np.isfinite(dmx[~fin]).all(1).any():
             err = float("inf")
-    cpu = None if a.no_cpu else cpu_baseline(wl, lam, len(k) // wl.hemispheres)
+    cpu = None if a.no_cpu else cpu_baseline(wl, jobs[-1][2], len(plans[-1].k) // wl.hemispheres,
+                                              ekev=None if not wl.energies else float(jobs[-1][0].split()[0]))
     gflops = r["gemm"]["gemm_flops"]
     line = {
         "metric": "master-pattern directions/s, 1 GPU: mixed vs FP64 at <=1e-4 rel err",
@@ -174,7 +205,7 @@ def main():
                  if head == "mixed" else "f64",
         "data": "synthetic wave vectors and depth profile; reference per-energy setup",
         "config": {"workload": f"{wl.name}: {wl.describe()}",
-                   "directions": int(len(k)), "strong_beams_mean": round(float(ns.mean()), 1),
+                   "directions": int(ndir), "strong_beams_mean": round(float(ns.mean()), 1),
                    "weak_beams_mean": round(float(nw.mean()), 1), "precision": head,
                    "mixed_terms": a.terms},
         "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
@@ -191,7 +222,8 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    plan.close()
+    for plan in plans:
+        plan.close()
     ctx.close()
 
 

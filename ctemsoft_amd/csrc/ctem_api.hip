@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -72,12 +73,10 @@ double inv_fact(int j) { double f = 1.0; for (int i = 2; i <= j; i++) f *= i; re
 
 }  // namespace
 
-struct ctem_ctx {
+// One uploaded crystal (one energy bin). A plan keeps the crystal that was current when it
+// was created, so that several bins can be resident at once under one context.
+struct Crystal {
   int device = 0;
-  int cu_count = 256;
-  hipStream_t stream = nullptr;
-  bool have_crystal = false;
-  // crystal on the device
   DBuf g, hkl, cut1, cut2, dbd, lut, slut, dslut, rmt;
   int32_t nref = 0, nsite = 0, ncell = 0;
   int32_t lut_lo[3] = {0, 0, 0}, lut_dim[3] = {0, 0, 0};
@@ -87,11 +86,11 @@ struct ctem_ctx {
   float dynupz = 0.0f, sgcutoff = 0.0f;
   float nat = 0.0f;  // total orbit size, float(sum(nat))
   std::vector<int32_t> h_hkl;
-  // scratch for the CalcLgh3 drop-in
-  DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  std::vector<ctem_plan*> plans;  // live plans, destroyed with the context
 
+  ~Crystal() {
+    (void)hipSetDevice(device);
+    for (DBuf* b : {&g, &hkl, &cut1, &cut2, &dbd, &lut, &slut, &dslut, &rmt}) b->release();
+  }
   DevRefl refl() const {
     DevRefl R;
     R.nref = nref; R.g = g.as<float>(); R.hkl = hkl.as<int32_t>(); R.cut1 = cut1.as<float>();
@@ -110,6 +109,30 @@ struct ctem_ctx {
   }
 };
 
+// Work space of ctem_plan_run and ctem_plan_adjoint, shared by the plans of a context (calls
+// on one context are sequential); it grows to the largest chunk run and is kept.
+struct Work {
+  DBuf A, W, Psi, sq, dirs, partial, wmax;
+  DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
+  void release() {
+    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
+                    &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig})
+      b->release();
+  }
+};
+
+struct ctem_ctx {
+  int device = 0;
+  int cu_count = 256;
+  hipStream_t stream = nullptr;
+  std::shared_ptr<Crystal> cr;  // the crystal that plans created next will use
+  Work w;
+  // scratch for the CalcLgh3 drop-in
+  DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<ctem_plan*> plans;  // live plans, destroyed with the context
+};
+
 struct ctem_plan {
   ctem_ctx* ctx = nullptr;
   int32_t numk = 0, nz = 0;
@@ -125,10 +148,7 @@ struct ctem_plan {
   bool ran = false;
   int32_t keep_dir = -1, keep_n = 0;
   DBuf keep_lgh;
-  // work
-  DBuf A, W, Psi, Psi32, sq, dirs, partial, wmax;
-  // adjoint work
-  DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
+  std::shared_ptr<Crystal> cr;  // the crystal of this plan (current at creation)
   std::vector<int32_t> h_sq;
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
@@ -170,8 +190,9 @@ extern "C" int32_t ctem_ctx_destroy(ctem_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   while (!c->plans.empty()) ctem_plan_destroy(c->plans.back());
-  for (DBuf* b : {&c->g, &c->hkl, &c->cut1, &c->cut2, &c->dbd, &c->lut, &c->slut, &c->dslut, &c->rmt, &c->s_A,
-                  &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
+  c->cr.reset();
+  c->w.release();
+  for (DBuf* b : {&c->s_A, &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
                   &c->s_nns, &c->s_zero, &c->s_off, &c->s_part, &c->s_dots})
     b->release();
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
@@ -287,30 +308,32 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   int nat = 0;
   for (int p = 0; p < x->nsite; p++) nat += x->site_norbit[p];
   int32_t rc;
-  if ((rc = upload(c->g, g.data(), g.size(), c->stream))) return rc;
-  if ((rc = upload(c->hkl, x->hkl, (size_t)3 * n, c->stream))) return rc;
-  if ((rc = upload(c->cut1, cut1.data(), n, c->stream))) return rc;
-  if ((rc = upload(c->cut2, cut2.data(), n, c->stream))) return rc;
-  if ((rc = upload(c->dbd, dbd.data(), n, c->stream))) return rc;
-  if ((rc = upload(c->lut, lutf.data(), lutf.size(), c->stream))) return rc;
-  if ((rc = upload(c->slut, slut.data(), slut.size() ? slut.size() : 0, c->stream))) return rc;
-  if ((rc = upload(c->dslut, dslut.data(), dslut.size() ? dslut.size() : 0, c->stream))) return rc;
-  if ((rc = upload(c->rmt, x->rmt, 9, c->stream))) return rc;
+  auto cr = std::make_shared<Crystal>();
+  cr->device = c->device;
+  if ((rc = upload(cr->g, g.data(), g.size(), c->stream))) return rc;
+  if ((rc = upload(cr->hkl, x->hkl, (size_t)3 * n, c->stream))) return rc;
+  if ((rc = upload(cr->cut1, cut1.data(), n, c->stream))) return rc;
+  if ((rc = upload(cr->cut2, cut2.data(), n, c->stream))) return rc;
+  if ((rc = upload(cr->dbd, dbd.data(), n, c->stream))) return rc;
+  if ((rc = upload(cr->lut, lutf.data(), lutf.size(), c->stream))) return rc;
+  if ((rc = upload(cr->slut, slut.data(), slut.size() ? slut.size() : 0, c->stream))) return rc;
+  if ((rc = upload(cr->dslut, dslut.data(), dslut.size() ? dslut.size() : 0, c->stream))) return rc;
+  if ((rc = upload(cr->rmt, x->rmt, 9, c->stream))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->nref = n;
-  c->nsite = x->nsite;
-  c->ncell = ncell;
+  cr->nref = n;
+  cr->nsite = x->nsite;
+  cr->ncell = ncell;
   for (int i = 0; i < 3; i++) {
-    c->lut_lo[i] = x->lut_lo[i]; c->lut_dim[i] = x->lut_dim[i];
-    c->slut_lo[i] = slo[i]; c->slut_dim[i] = sdim[i];
+    cr->lut_lo[i] = x->lut_lo[i]; cr->lut_dim[i] = x->lut_dim[i];
+    cr->slut_lo[i] = slo[i]; cr->slut_dim[i] = sdim[i];
   }
-  c->sncell = sncell;
-  c->mlambda = x->mlambda;
-  c->dynupz = x->dynupz;
-  c->sgcutoff = x->sgcutoff;
-  c->nat = (float)nat;
-  c->h_hkl.assign(x->hkl, x->hkl + 3 * n);
-  c->have_crystal = true;
+  cr->sncell = sncell;
+  cr->mlambda = x->mlambda;
+  cr->dynupz = x->dynupz;
+  cr->sgcutoff = x->sgcutoff;
+  cr->nat = (float)nat;
+  cr->h_hkl.assign(x->hkl, x->hkl + 3 * n);
+  c->cr = cr;
   return CTEM_OK;
 }
 
@@ -447,6 +470,7 @@ extern "C" int32_t ctem_plan_create(ctem_ctx* c, int32_t numk, const double* k, 
   HIPCHK(hipSetDevice(c->device));
   ctem_plan* p = new ctem_plan();
   p->ctx = c;
+  p->cr = c->cr;
   p->numk = numk; p->nz = nz; p->dz = dz; p->thick = thick;
   std::vector<double> lamd(lam, lam + nz);
   int32_t rc;
@@ -466,10 +490,9 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
-                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->A, &p->W, &p->Psi,
-                  &p->Psi32, &p->sq, &p->dirs, &p->partial, &p->wmax, &p->g_dots, &p->g_lut, &p->g_dots2,
-                  &p->aM, &p->aQ, &p->aPhi, &p->aHm, &p->aEbar, &p->aBlk, &p->aW2, &p->aDbar, &p->aSig})
+                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh})
     b->release();
+  p->cr.reset();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
   auto& v = p->ctx->plans;
   v.erase(std::remove(v.begin(), v.end(), p), v.end());
@@ -496,7 +519,8 @@ static int32_t check_err_flag(DBuf& err, hipStream_t s, const char* what) {
 extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw_out) {
   if (!p) return fail(CTEM_E_ARG, "null plan");
   ctem_ctx* c = p->ctx;
-  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_select: no crystal uploaded");
+  if (!p->cr) p->cr = c->cr;
+  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_select: no crystal uploaded");
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   const int nk = p->numk;
@@ -508,8 +532,8 @@ extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw
   HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
   SelectOut o{};
   o.nns = p->nns.as<int32_t>(); o.nnw = p->nnw.as<int32_t>(); o.err = p->err.as<int32_t>();
-  DevRefl R = c->refl();
-  launch_select(s, p->k.as<double>(), nk, R, c->rmt.as<double>(), o, 0);
+  DevRefl R = p->cr->refl();
+  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), o, 0);
   HIPCHK(hipGetLastError());
   if ((rc = check_err_flag(p->err, s, "beam selection"))) return rc;
   p->h_nns.resize(nk); p->h_nnw.resize(nk);
@@ -528,7 +552,7 @@ extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw
   o.soff = p->soff.as<int64_t>(); o.woff = p->woff.as<int64_t>();
   o.sidx = p->sidx.as<int32_t>(); o.ssg = p->ssg.as<float>();
   o.widx = p->widx.as<int32_t>(); o.wsg = p->wsg.as<float>();
-  launch_select(s, p->k.as<double>(), nk, R, c->rmt.as<double>(), o, 1);
+  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), o, 1);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   p->selected = true;
@@ -760,7 +784,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
     return fail(CTEM_E_ARG, "unknown precision %d", precision);
   ctem_ctx* c = p->ctx;
-  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
+  if (!p->cr) p->cr = c->cr;
+  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
   if (keep_lgh_ik >= p->numk) return fail(CTEM_E_ARG, "keep_lgh_ik out of range");
   int32_t rc;
   if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
@@ -770,7 +795,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   p->t_total = p->t_gemm = p->f_gemm = 0.0;
   p->n_gemm = 0;
   p->evused = 0;
-  if ((rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * std::max(1, c->nsite)))) return rc;
+  if ((rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * std::max(1, p->cr->nsite)))) return rc;
   p->keep_dir = keep_lgh_ik;
   p->keep_n = (keep_lgh_ik >= 0) ? p->h_nns[keep_lgh_ik] : 0;
   if (p->keep_n && (rc = p->keep_lgh.ensure(16 * (size_t)p->keep_n * p->keep_n))) return rc;
@@ -781,11 +806,11 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   for (int i = 0; i < nk; i++) buckets[npad_of(p->h_nns[i])].push_back(i);
   std::vector<int32_t> order;
   for (auto& kv : buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
-  if ((rc = upload(p->dirs, order.data(), order.size(), s))) return rc;
+  if ((rc = upload(c->w.dirs, order.data(), order.size(), s))) return rc;
   BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
              p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
-  DevRefl R = c->refl();
-  DevLut L = c->lutv(p->err.as<int32_t>());
+  DevRefl R = p->cr->refl();
+  DevLut L = p->cr->lutv(p->err.as<int32_t>());
   const double dzt = p->dz / p->thick;
   hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
   HIPCHK(hipEventRecord(e0, s));
@@ -801,30 +826,30 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
     int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
     for (int c0 = 0; c0 < nbk; c0 += chunk) {
       const int nb = std::min(chunk, nbk - c0);
-      if ((rc = p->A.ensure((size_t)nb * per * sizeof(double2))) || (rc = p->W.ensure((size_t)5 * nb * per * sizeof(double2))) ||
-          (rc = p->Psi.ensure((size_t)nb * spsi * sizeof(double2))) || (rc = p->sq.ensure(4 * (size_t)nb)) ||
-          (rc = p->partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * std::max(1, c->nsite))))
+      if ((rc = c->w.A.ensure((size_t)nb * per * sizeof(double2))) || (rc = c->w.W.ensure((size_t)5 * nb * per * sizeof(double2))) ||
+          (rc = c->w.Psi.ensure((size_t)nb * spsi * sizeof(double2))) || (rc = c->w.sq.ensure(4 * (size_t)nb)) ||
+          (rc = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * std::max(1, p->cr->nsite))))
         return rc;
-      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
+      Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb);
       for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
       if (precision == CTEM_PREC_FP64)
-        launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->A.as<double2>());
+        launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.A.as<double2>());
       else
-        launch_assemble32(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, p->W.as<float2>());
+        launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
       HIPCHK(hipGetLastError());
       if (precision == CTEM_PREC_FP64) {
-        rc = expm_depth_fp64(p, s, npad, nb, p->A.as<double2>(), p->W.as<double2>(), p->Psi.as<double2>(), spsi,
-                             p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+        rc = expm_depth_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
+                             p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
-        if ((rc = p->wmax.ensure(6 * sizeof(float) * (size_t)nb))) return rc;
-        rc = expm_depth_mixed(p, s, npad, nb, p->W.as<float2>(), p->wmax.as<float>(),
-                              p->Psi.as<double2>(), spsi, p->nz, p->sq.as<int32_t>(), p->err.as<int32_t>(), hn,
+        if ((rc = c->w.wmax.ensure(6 * sizeof(float) * (size_t)nb))) return rc;
+        rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
+                              c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
                               p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
       }
       if (rc) return rc;
-      launch_lgh_dots(s, B, V, R, L, p->Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
-                      p->partial.as<double>(), p->dots.as<double>(), p->keep_dir, p->keep_lgh.as<double2>());
+      launch_lgh_dots(s, B, V, R, L, c->w.Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
+                      c->w.partial.as<double>(), p->dots.as<double>(), p->keep_dir, p->keep_lgh.as<double2>());
       HIPCHK(hipGetLastError());
     }
     off += nbk;
@@ -857,7 +882,7 @@ extern "C" int32_t ctem_plan_set_mixed_terms(ctem_plan* p, int32_t taylor_terms,
 extern "C" int32_t ctem_plan_get_dots(ctem_plan* p, double* dots) {
   if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_get_dots: run the plan first");
   if (!dots) return fail(CTEM_E_ARG, "null output");
-  const size_t n = (size_t)p->numk * p->ctx->nsite;
+  const size_t n = (size_t)p->numk * p->cr->nsite;
   if (n) HIPCHK(hipMemcpy(dots, p->dots.p, 8 * n, hipMemcpyDeviceToHost));
   return CTEM_OK;
 }
@@ -866,10 +891,10 @@ extern "C" int32_t ctem_plan_get_svals(ctem_plan* p, float* svals) {
 #pragma clang fp contract(off)
   if (!p || !p->ran) return fail(CTEM_E_STATE, "ctem_plan_get_svals: run the plan first");
   if (!svals) return fail(CTEM_E_ARG, "null output");
-  const size_t n = (size_t)p->numk * p->ctx->nsite;
+  const size_t n = (size_t)p->numk * p->cr->nsite;
   std::vector<double> d(n);
   if (n) HIPCHK(hipMemcpy(d.data(), p->dots.p, 8 * n, hipMemcpyDeviceToHost));
-  const float nat = p->ctx->nat;
+  const float nat = p->cr->nat;
   for (size_t i = 0; i < n; i++) svals[i] = (float)d[i] / nat;
   return CTEM_OK;
 }
@@ -889,33 +914,34 @@ extern "C" int32_t ctem_plan_get_lgh(ctem_plan* p, double* lgh, int32_t cap, int
 extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* glut, double* gb) {
   if (!p || !gdots || !glut) return fail(CTEM_E_ARG, "ctem_plan_adjoint: null argument");
   ctem_ctx* c = p->ctx;
-  if (!c->have_crystal) return fail(CTEM_E_STATE, "ctem_plan_adjoint: no crystal uploaded");
-  if (c->nsite > 0 && !gb) return fail(CTEM_E_ARG, "ctem_plan_adjoint: gb is NULL");
+  if (!p->cr) p->cr = c->cr;
+  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_adjoint: no crystal uploaded");
+  if (p->cr->nsite > 0 && !gb) return fail(CTEM_E_ARG, "ctem_plan_adjoint: gb is NULL");
   int32_t rc;
   if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  const int nk = p->numk, ns = std::max(1, c->nsite), nz = p->nz;
+  const int nk = p->numk, ns = std::max(1, p->cr->nsite), nz = p->nz;
   for (int i = 0; i < nk; i++)
     if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "ctem_plan_adjoint: at most 512 strong beams per direction");
-  if ((rc = upload(p->g_dots, gdots, (size_t)nk * c->nsite, s)) || (rc = p->g_lut.ensure(16 * (size_t)c->ncell)) ||
-      (rc = p->g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
+  if ((rc = upload(c->w.g_dots, gdots, (size_t)nk * p->cr->nsite, s)) || (rc = c->w.g_lut.ensure(16 * (size_t)p->cr->ncell)) ||
+      (rc = c->w.g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
       (rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * ns)))
     return rc;
-  HIPCHK(hipMemsetAsync(p->g_lut.p, 0, 16 * (size_t)c->ncell, s));
-  HIPCHK(hipMemsetAsync(p->g_dots2.p, 0, 8 * (size_t)nk * ns, s));
+  HIPCHK(hipMemsetAsync(c->w.g_lut.p, 0, 16 * (size_t)p->cr->ncell, s));
+  HIPCHK(hipMemsetAsync(c->w.g_dots2.p, 0, 8 * (size_t)nk * ns, s));
   HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
   std::map<int, std::vector<int32_t>> buckets;
   for (int i = 0; i < nk; i++) buckets[npad_of(p->h_nns[i])].push_back(i);
   std::vector<int32_t> order;
   for (auto& kv : buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
-  if ((rc = upload(p->dirs, order.data(), order.size(), s))) return rc;
+  if ((rc = upload(c->w.dirs, order.data(), order.size(), s))) return rc;
   BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
              p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
-  DevRefl R = c->refl();
-  DevLut L = c->lutv(p->err.as<int32_t>());
+  DevRefl R = p->cr->refl();
+  DevLut L = p->cr->lutv(p->err.as<int32_t>());
   DevLut LdB = L;
-  LdB.slut = c->dslut.as<double2>();
+  LdB.slut = p->cr->dslut.as<double2>();
   const double dzt = p->dz / p->thick;
   const size_t budget = (size_t)16 << 30;
   size_t off = 0;
@@ -929,16 +955,16 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
     for (int c0 = 0; c0 < nbk; c0 += chunk) {
       const int nb = std::min(chunk, nbk - c0);
-      if ((rc = p->A.ensure((size_t)nb * per * 16)) || (rc = p->W.ensure((size_t)5 * nb * per * 16)) ||
-          (rc = p->Psi.ensure((size_t)nb * spsi * 16)) || (rc = p->sq.ensure(4 * (size_t)nb)) ||
-          (rc = p->partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
-          (rc = p->aM.ensure((size_t)nb * per * 16)) || (rc = p->aQ.ensure((size_t)nb * spsi * 16)) ||
-          (rc = p->aPhi.ensure((size_t)nb * spsi * 16)) || (rc = p->aHm.ensure((size_t)nb * spsi * 16)) ||
-          (rc = p->aEbar.ensure((size_t)nb * per * 16)) || (rc = p->aBlk.ensure((size_t)4 * nb * per * 16)) ||
-          (rc = p->aW2.ensure((size_t)20 * nb * per * 16)) || (rc = p->aDbar.ensure((size_t)nb * per * 16)) ||
-          (rc = p->aSig.ensure(8 * (size_t)nb)))
+      if ((rc = c->w.A.ensure((size_t)nb * per * 16)) || (rc = c->w.W.ensure((size_t)5 * nb * per * 16)) ||
+          (rc = c->w.Psi.ensure((size_t)nb * spsi * 16)) || (rc = c->w.sq.ensure(4 * (size_t)nb)) ||
+          (rc = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
+          (rc = c->w.aM.ensure((size_t)nb * per * 16)) || (rc = c->w.aQ.ensure((size_t)nb * spsi * 16)) ||
+          (rc = c->w.aPhi.ensure((size_t)nb * spsi * 16)) || (rc = c->w.aHm.ensure((size_t)nb * spsi * 16)) ||
+          (rc = c->w.aEbar.ensure((size_t)nb * per * 16)) || (rc = c->w.aBlk.ensure((size_t)4 * nb * per * 16)) ||
+          (rc = c->w.aW2.ensure((size_t)20 * nb * per * 16)) || (rc = c->w.aDbar.ensure((size_t)nb * per * 16)) ||
+          (rc = c->w.aSig.ensure(8 * (size_t)nb)))
         return rc;
-      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
+      Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb), hn2(nb);
       int max_nw = 0;
       for (int b = 0; b < nb; b++) {
@@ -947,71 +973,71 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
         hn2[b] = 2 * p->h_nns[d];
         max_nw = std::max(max_nw, p->h_nnw[d]);
       }
-      double2* A = p->A.as<double2>();
+      double2* A = c->w.A.as<double2>();
       // forward: E and psi_1..psi_Z
-      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, A);
+      launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, A);
       HIPCHK(hipGetLastError());
-      double2* E = expm_fp64(nullptr, s, npad, nb, A, p->W.as<double2>(), p->sq.as<int32_t>(),
+      double2* E = expm_fp64(nullptr, s, npad, nb, A, c->w.W.as<double2>(), c->w.sq.as<int32_t>(),
                              p->err.as<int32_t>(), hn, h_sq, rc);
       if (!E) return rc;
-      launch_depth(s, npad, nb, nz, E, 0, 0, p->Psi.as<double2>(), spsi, c->cu_count);
+      launch_depth(s, npad, nb, nz, E, 0, 0, c->w.Psi.as<double2>(), spsi, c->cu_count);
       HIPCHK(hipGetLastError());
       // B: d dots_p / d B_p = Re sum Lgh dS_p/dB_p
-      launch_lgh_dots(s, B, V, R, LdB, p->Psi.as<double2>(), spsi, nz, p->lam.as<double>(), dzt,
-                      p->partial.as<double>(), p->g_dots2.as<double>(), -1, nullptr);
+      launch_lgh_dots(s, B, V, R, LdB, c->w.Psi.as<double2>(), spsi, nz, p->lam.as<double>(), dzt,
+                      c->w.partial.as<double>(), c->w.g_dots2.as<double>(), -1, nullptr);
       HIPCHK(hipGetLastError());
       // M = c (Lbar + Lbar^H), Q = M Psi
-      launch_adj_m(s, B, V, R, L, p->g_dots.as<double>(), dzt, p->aM.as<double2>());
+      launch_adj_m(s, B, V, R, L, c->w.g_dots.as<double>(), dzt, c->w.aM.as<double2>());
       HIPCHK(hipGetLastError());
       {
         ZGemm g{};
         g.m = npad; g.n = nz; g.k = npad;
         g.ldx = npad; g.ldy = npad; g.ldc = npad; g.ldm = npad;
-        g.X = p->aM.as<double2>(); g.sx = per;
-        g.Y = p->Psi.as<double2>(); g.sy = spsi;
-        g.C = p->aQ.as<double2>(); g.sc = spsi;
+        g.X = c->w.aM.as<double2>(); g.sx = per;
+        g.Y = c->w.Psi.as<double2>(); g.sy = spsi;
+        g.C = c->w.aQ.as<double2>(); g.sc = spsi;
         g.alpha = 1.0; g.nb = nb;
         launch_zgemm(s, g);
         HIPCHK(hipGetLastError());
       }
       // phi_k backward, Ebar = Phi [psi_0 .. psi_{Z-1}]^H
-      launch_adj_depth(s, npad, nb, nz, E, p->aQ.as<double2>(), p->lam.as<double>(), p->aPhi.as<double2>());
-      launch_adj_hm(s, npad, nb, nz, p->Psi.as<double2>(), spsi, p->aHm.as<double2>());
+      launch_adj_depth(s, npad, nb, nz, E, c->w.aQ.as<double2>(), p->lam.as<double>(), c->w.aPhi.as<double2>());
+      launch_adj_hm(s, npad, nb, nz, c->w.Psi.as<double2>(), spsi, c->w.aHm.as<double2>());
       HIPCHK(hipGetLastError());
       {
         ZGemm g{};
         g.m = npad; g.n = npad; g.k = nz;
         g.ldx = npad; g.ldy = nz; g.ldc = npad; g.ldm = npad;
-        g.X = p->aPhi.as<double2>(); g.sx = spsi;
-        g.Y = p->aHm.as<double2>(); g.sy = spsi;
-        g.C = p->aEbar.as<double2>(); g.sc = per;
+        g.X = c->w.aPhi.as<double2>(); g.sx = spsi;
+        g.Y = c->w.aHm.as<double2>(); g.sy = spsi;
+        g.C = c->w.aEbar.as<double2>(); g.sc = per;
         g.alpha = 1.0; g.nb = nb;
         launch_zgemm(s, g);
         HIPCHK(hipGetLastError());
       }
       // A_bar = L_exp(A^H, Ebar) from expm([[A^H, sigma Ebar], [0, A^H]]); A rebuilt unscaled
-      launch_assemble(s, B, V, R, L, p->kn.as<double>(), c->mlambda, c->dynupz, p->dz, A);
-      launch_adj_block(s, npad, nb, A, p->aEbar.as<double2>(), p->aSig.as<double>(), p->aBlk.as<double2>());
+      launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, A);
+      launch_adj_block(s, npad, nb, A, c->w.aEbar.as<double2>(), c->w.aSig.as<double>(), c->w.aBlk.as<double2>());
       HIPCHK(hipGetLastError());
-      double2* T2 = expm_fp64(nullptr, s, 2 * npad, nb, p->aBlk.as<double2>(), p->aW2.as<double2>(),
-                              p->sq.as<int32_t>(), p->err.as<int32_t>(), hn2, h_sq2, rc);
+      double2* T2 = expm_fp64(nullptr, s, 2 * npad, nb, c->w.aBlk.as<double2>(), c->w.aW2.as<double2>(),
+                              c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn2, h_sq2, rc);
       if (!T2) return rc;
-      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2, p->aSig.as<double>(), p->aDbar.as<double2>());
+      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2, c->w.aSig.as<double>(), c->w.aDbar.as<double2>());
       HIPCHK(hipGetLastError());
       // LUT entries
-      launch_adj_lut(s, B, V, R, L, c->mlambda, p->aDbar.as<double2>(), p->g_lut.as<double>(), max_nw);
+      launch_adj_lut(s, B, V, R, L, p->cr->mlambda, c->w.aDbar.as<double2>(), c->w.g_lut.as<double>(), max_nw);
       HIPCHK(hipGetLastError());
     }
     off += nbk;
   }
   std::vector<double> d2((size_t)nk * ns);
-  HIPCHK(hipMemcpyAsync(glut, p->g_lut.p, 16 * (size_t)c->ncell, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(d2.data(), p->g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(glut, c->w.g_lut.p, 16 * (size_t)p->cr->ncell, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(d2.data(), c->w.g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if ((rc = check_err_flag(p->err, s, "ctem_plan_adjoint"))) return rc;
-  for (int q = 0; q < c->nsite; q++) {
+  for (int q = 0; q < p->cr->nsite; q++) {
     double acc = 0.0;
-    for (int i = 0; i < nk; i++) acc += gdots[(size_t)i * c->nsite + q] * d2[(size_t)i * c->nsite + q];
+    for (int i = 0; i < nk; i++) acc += gdots[(size_t)i * p->cr->nsite + q] * d2[(size_t)i * p->cr->nsite + q];
     gb[q] = acc;
   }
   return CTEM_OK;
@@ -1037,7 +1063,7 @@ extern "C" int32_t ctem_plan_dynmat(ctem_plan* p, int32_t ik, double* dynmat, in
   BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
              p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
   Bucket B{npad, 1, c->s_dirs.as<int32_t>()};
-  launch_dynmat(s, B, V, c->refl(), c->lutv(c->s_err.as<int32_t>()), c->mlambda, c->dynupz, c->s_A.as<double2>());
+  launch_dynmat(s, B, V, p->cr->refl(), p->cr->lutv(c->s_err.as<int32_t>()), p->cr->mlambda, p->cr->dynupz, c->s_A.as<double2>());
   HIPCHK(hipGetLastError());
   if ((rc = check_err_flag(c->s_err, s, "ctem_plan_dynmat"))) return rc;
   std::vector<double2> h((size_t)npad * npad);
@@ -1058,12 +1084,12 @@ extern "C" int32_t ctem_plan_scatter(ctem_plan* p, const int32_t* ij, int32_t np
   ctem_ctx* c = p->ctx;
   hipStream_t s = c->stream;
   const size_t plane = (size_t)(2 * npx + 1) * (2 * npy + 1);
-  const size_t tot = plane * std::max(1, c->nsite);
+  const size_t tot = plane * std::max(1, p->cr->nsite);
   DBuf d_sr;
   int32_t rc;
   if ((rc = upload(p->ij, ij, (size_t)2 * p->numk, s)) || (rc = d_sr.ensure(4 * tot))) return rc;
   HIPCHK(hipMemcpyAsync(d_sr.p, sr, 4 * tot, hipMemcpyHostToDevice, s));
-  launch_scatter(s, p->numk, p->ij.as<int32_t>(), p->dots.as<double>(), c->nsite, c->nat, npx, npy,
+  launch_scatter(s, p->numk, p->ij.as<int32_t>(), p->dots.as<double>(), p->cr->nsite, p->cr->nat, npx, npy,
                  isym, d_sr.as<float>());
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(sr, d_sr.p, 4 * tot, hipMemcpyDeviceToHost, s);

@@ -239,6 +239,8 @@ class Plan:
         self.kn = np.ascontiguousarray(kn, np.float64)
         self.lam = np.ascontiguousarray(lam, np.float32)
         self.numk = len(self.k)
+        # the crystal current at creation (the library keeps it with the plan)
+        self.setup = ctx.setup
         self.nsite = len(ctx.setup.sites) if ctx.setup is not None else 0
         self._h = C.c_void_p()
         _check(lib().ctem_plan_create(ctx._h, self.numk, _p(self.k), _p(self.kn), len(self.lam),
@@ -284,7 +286,7 @@ class Plan:
     def adjoint(self, gdots):
         """FP64 reverse mode: gdots[ik, p] = df/ddots -> (df/dU as a complex array shaped like
         the uploaded LUT, [l][k][h], with df/dRe U + i df/dIm U; df/dB per site)."""
-        st = self.ctx.setup
+        st = self.setup
         g = np.ascontiguousarray(gdots, np.float64).reshape(self.numk, self.nsite)
         ncell = int(np.prod(st.lut_dim))
         glut = np.zeros(2 * ncell)

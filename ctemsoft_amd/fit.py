@@ -45,7 +45,7 @@ def master_loss_grad(plan, ij, npx, npy, isym, target, owner=None, precision="fp
         owner = pixel_owner(ij, npx, npy, isym)
     nsite = plan.nsite
     r = (sr.astype(np.float64) - np.asarray(target, np.float64)).reshape(nsite, -1)
-    nat = float(np.float32(plan.ctx.setup.nat))
+    nat = float(np.float32(plan.setup.nat))
     gsv = np.zeros((plan.numk, nsite))
     m = owner >= 0
     for p in range(nsite):

@@ -107,6 +107,10 @@ class Workload:
         S = int(counts.sum())
         tail = int(S * 0.05 / 0.95 * 0.99)
         cum = np.cumsum(counts)
+        if not cum[-2] < 0.95 * (S + tail):
+            # a profile whose last bin holds almost nothing (gamma): the largest tail that
+            # still keeps the cut on the last bin
+            tail = S // 19 - 1
         assert cum[-2] < 0.95 * (S + tail) <= cum[-1]
         lam = (counts.astype(np.float32).astype(np.float64) / float(S)).astype(np.float32)
         return lam, counts, tail

@@ -85,8 +85,12 @@ CTEM_API const char* ctem_last_error(void);
 /* device name and compute-unit count, for logs */
 CTEM_API int32_t ctem_device_info(ctem_ctx* ctx, char* name, int32_t name_len, int32_t* cu_count);
 
-/* Upload the per-energy crystal setup; replaces the previous one. Precomputes the per-site
- * structure-matrix table S_p(g-h) of CTEMEBSDmaster.f90:418-449 over the LUT box. */
+/* Upload the per-energy crystal setup (the state Compute_ReflectionList and CalcOrbit leave
+ * for the beam loop, CTEMEBSDmaster.f90:381-401 and :427). It becomes the context's current
+ * crystal: plans created afterwards use it, and each plan keeps its crystal until it is
+ * destroyed, so the bins of an energy series can be resident together (one upload and one
+ * plan per bin). Precomputes the per-site structure-matrix table S_p(g-h) of
+ * CTEMEBSDmaster.f90:418-449 and its derivative in B over the box of all g-h. */
 CTEM_API int32_t ctem_upload_crystal(ctem_ctx* ctx, const ctem_crystal* c);
 
 /* ----- incident directions ------------------------------------------------------------ */

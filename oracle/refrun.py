@@ -214,13 +214,13 @@ def have_reference() -> bool:
     return os.path.exists(os.path.join(REF_BIN, "ctemref_driver"))
 
 
-def time_reference(work: str, wl, lam: np.ndarray, ksel) -> float:
+def time_reference(work: str, wl, lam: np.ndarray, ksel, ekev: float | None = None) -> float:
     """Wall time of the reference beam loop (Compute_DynMat + CalcLgh3 per direction) over
     the directions ksel, single-threaded (MKL sequential), minus the time of the same run on
     one direction (which carries the per-energy setup and file output)."""
     t0 = time.perf_counter()
-    run_driver(work, wl, lam, list(ksel), 2)
+    run_driver(work, wl, lam, list(ksel), 2, ekev)
     t1 = time.perf_counter()
-    run_driver(work, wl, lam, [int(ksel[0])], 2)
+    run_driver(work, wl, lam, [int(ksel[0])], 2, ekev)
     t2 = time.perf_counter()
     return (t1 - t0) - (t2 - t1)

@@ -26,7 +26,8 @@ def load_golden(name):
 def lambert_dims(name, fx):
     """(npx, npy) that the reference passes to Calckvectors for the fixture's workload."""
     from ctemsoft_amd import ebsd, workloads
-    return ebsd.lambert_dims(workloads.get(name).npx, int(fx["usehex"]))
+    base = name.split("_")[0] + "_" + name.split("_")[1]  # drop an energy tag (fe_c3w_10.0kV)
+    return ebsd.lambert_dims(workloads.get(base).npx, int(fx["usehex"]))
 
 
 @pytest.fixture(scope="session")

@@ -119,6 +119,28 @@ def golden_workload(name: str, nsel_dyn: int, nsel_beams: int | None, full_progr
     return fn
 
 
+def setup_series(name: str) -> str:
+    """Per-energy setup (LUT, reflection list, sites, wavelength) of every energy bin of an
+    energy-series workload, from the reference's own modules; one archive, keys e<i>_<field>."""
+    wl = workloads.get(name)
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="ctemser_") as work:
+        for i, e in enumerate(wl.energies):
+            lam, _, _ = wl.depth_weights_for_program(e)
+            d = run_driver(work, wl, lam, [0], 0, e)
+            for k, v in _setup_arrays(d).items():
+                if k in ("kij", "kvec", "kn"):
+                    continue  # the library computes the directions
+                out[f"e{i}_{k}"] = v
+            out[f"e{i}_ekev"] = e
+            out[f"e{i}_lam"] = lam
+            out[f"e{i}_dz"] = wl.dz_for(e)
+    out["nenergy"] = len(wl.energies)
+    fn = os.path.join(HERE, f"{name}_setups.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -135,8 +157,10 @@ def main():
                                   corners=False))
         elif w in ("fe_c3", "fe_c3w"):
             for e in (10.0, 15.0, 20.0):
-                print(golden_workload(w, nsel_dyn=1, nsel_beams=64, full_program=False,
-                                      ekev=e, tag=f"{w}_{e:.1f}kV"))
+                print(golden_workload(w, nsel_dyn=0, nsel_beams=64, full_program=False,
+                                      ekev=e, tag=f"{w}_{e:.1f}kV", corners=False))
+            if w == "fe_c3w":
+                print(setup_series(w))
         else:
             raise SystemExit(f"unknown workload {w}")
 

@@ -21,7 +21,8 @@ def ctx():
 
 
 @pytest.mark.parametrize("name,nsample", [("fe_c1w", 1024), ("ni_c0w", None), ("fe_c1", 2048), ("ni_c0", None),
-                                          ("al2o3_c2w", 512), ("al2o3_c2", 2048)])
+                                          ("al2o3_c2w", 512), ("al2o3_c2", 2048), ("fe_c3w_10.0kV", 512),
+                                          ("fe_c3w_20.0kV", 512)])
 def test_mixed_matches_fp64(ctx, name, nsample):
     from ctemsoft_amd import ebsd
     fx = load_golden(name)

@@ -12,7 +12,8 @@ import scipy.linalg
 
 from conftest import lambert_dims, load_golden
 
-FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w", "al2o3_c2", "al2o3_c2w"]
+FIXTURES = ["ni_c0", "ni_c0w", "fe_c1", "fe_c1w", "al2o3_c2", "al2o3_c2w",
+            "fe_c3w_10.0kV", "fe_c3w_15.0kV", "fe_c3w_20.0kV"]
 
 
 def _split(fx, j, ns, nw):
