@@ -85,7 +85,7 @@ def measured_traffic(workload, head):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_k_cgemm6:"):
+        if line.startswith("traffic_bytes_dominant:"):
             return float(line.split(":")[1])
     return None
 
