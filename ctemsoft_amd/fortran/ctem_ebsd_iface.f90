@@ -14,6 +14,7 @@ implicit none
 integer(c_int32_t), parameter :: CTEM_OK = 0, CTEM_E_ARG = -1, CTEM_E_HIP = -2, &
                                  CTEM_E_STATE = -3, CTEM_E_RANGE = -4, CTEM_E_NUMERIC = -5
 integer(c_int32_t), parameter :: CTEM_PREC_FP64 = 0, CTEM_PREC_MIXED = 1
+integer(c_int32_t), parameter :: CTEM_SPLIT_BF16X3 = 3, CTEM_SPLIT_BF16X6 = 6, CTEM_SPLIT_FP16X3 = 16
 
 ! per-energy crystal setup (struct ctem_crystal)
 type, bind(C) :: ctem_crystal
@@ -125,6 +126,24 @@ interface
     integer(c_int32_t), value :: npx, npy, isym
     real(c_float), intent(inout) :: sr(*)
     integer(c_int32_t)        :: rc
+  end function
+
+  function ctem_plan_set_mixed_terms(plan, taylor_terms, square_terms) &
+           bind(C, name='ctem_plan_set_mixed_terms') result(rc)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value        :: plan
+    integer(c_int32_t), value :: taylor_terms, square_terms
+    integer(c_int32_t)        :: rc
+  end function
+
+  ! reverse mode: gdots(p, ik) = df/ddots -> glut = df/dLUT (complex, LUT order), gb(p) = df/dB_p
+  function ctem_plan_adjoint(plan, gdots, glut, gb) bind(C, name='ctem_plan_adjoint') result(rc)
+    import :: c_int32_t, c_double, c_double_complex, c_ptr
+    type(c_ptr), value :: plan
+    real(c_double), intent(in) :: gdots(*)
+    complex(c_double_complex), intent(out) :: glut(*)
+    real(c_double), intent(out) :: gb(*)
+    integer(c_int32_t) :: rc
   end function
 
   ! same argument list and meaning as the reference's external CalcLgh3
