@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "ctem_internal.h"
 #include "ctem_laue.h"
@@ -21,6 +22,7 @@
 namespace ctem {
 
 typedef double double4v __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------------------------
 // beam selection
@@ -120,8 +122,10 @@ __device__ __forceinline__ int64_t slut_index(const DevLut& L, int h, int k, int
 // U(g_r - w) and U(w - h_c) are gathered (into registers one chunk ahead, then LDS); the
 // weak-beam sum
 // sum_w U(g-w) U(w-h) / s_w is a 32x32x32 complex product on v_mfma_f64_16x16x4_f64, each
-// wave one 16x16 quadrant (four real MFMAs per complex step). The diagonal sum
-// sum_w |U(g-w)|^2 / s_w (gvectors.f90:1165) is kept per row by the first 32 threads.
+// wave one 16x16 quadrant (four real MFMAs per complex step); the mixed form (F32) runs the
+// same product on v_mfma_f32_16x16x4_f32 with FP32 accumulation. The diagonal sum
+// sum_w |U(g-w)|^2 / s_w (gvectors.f90:1165) is kept per row, in FP64, by the first 32 threads.
+template <bool F32>
 __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double* __restrict__ kn, double mlambda,
                                                   float dynupz, double dz, double2* __restrict__ A,
@@ -141,7 +145,9 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
 
   __shared__ int rh[32][3], ch[32][3];
   __shared__ double wsgd[32], winv[32], dsum_s[32];
-  __shared__ double2 Ua[32][33], Ub[32][33];
+  typedef typename std::conditional<F32, float2, double2>::type CT;  // staged LUT values
+  typedef typename std::conditional<F32, f32x4, double4v>::type AccT;
+  __shared__ CT Ua[32][33], Ub[32][33];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int l16 = lane & 15, l4 = lane >> 4;
   const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
     }
   }
   __syncthreads();
-  double4v accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+  AccT accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
   double dsum = 0.0;
   const bool live = (ti * 32 < n) && (tj * 32 < n);
   const bool diag_tile = (ti == tj);
@@ -184,8 +190,8 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
       __syncthreads();
       for (int m = 0; m < 4; m++) {
         const int ww = pww + 8 * m;
-        Ua[prr][ww] = make_double2(pa[m].x, pa[m].y);  // U(g_r - w)
-        Ub[ww][prr] = make_double2(pb[m].x, pb[m].y);  // U(w - h_c)
+        Ua[prr][ww] = CT{pa[m].x, pa[m].y};  // U(g_r - w)
+        Ub[ww][prr] = CT{pb[m].x, pb[m].y};  // U(w - h_c)
         if (prr == 0) {
           wsgd[ww] = (double)psg[m];
           winv[ww] = 1.0 / (double)psg[m];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
@@ -196,18 +202,27 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
       const int wn = min(32, nw - w0);
       // entries past nw are zero, so the product may run over whole steps of 4
       for (int k4 = 0; k4 < wn; k4 += 4) {
-        const double2 a = Ua[wr + l16][k4 + l4];
-        const double2 u = Ub[k4 + l4][wc + l16];
-        const double iv = winv[k4 + l4];
-        const double2 bq = make_double2(u.x * iv, u.y * iv);  // U(w - h) / s_w
-        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.x, accr, 0, 0, 0);
-        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bq.y, accr, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.y, acci, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bq.x, acci, 0, 0, 0);
+        const CT a = Ua[wr + l16][k4 + l4];
+        const CT u = Ub[k4 + l4][wc + l16];
+        if constexpr (F32) {
+          const float iv = (float)winv[k4 + l4];
+          const float bx = u.x * iv, by = u.y * iv;  // U(w - h) / s_w
+          accr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, bx, accr, 0, 0, 0);
+          accr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, by, accr, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, by, acci, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, bx, acci, 0, 0, 0);
+        } else {
+          const double iv = winv[k4 + l4];
+          const double2 bq = make_double2(u.x * iv, u.y * iv);  // U(w - h) / s_w
+          accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.x, accr, 0, 0, 0);
+          accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bq.y, accr, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bq.y, acci, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bq.x, acci, 0, 0, 0);
+        }
       }
       if (diag_tile && tid < 32) {
         for (int w = 0; w < wn; w++) {
-          const double m = hypot(Ua[tid][w].x, Ua[tid][w].y);
+          const double m = hypot((double)Ua[tid][w].x, (double)Ua[tid][w].y);
           dsum += m * m / wsgd[w];
         }
       }
@@ -215,18 +230,19 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   }
   if (diag_tile && tid < 32) dsum_s[tid] = dsum;
   __syncthreads();
-  // write the tile: lane l holds rows wr + l/16 + 4e, column wc + l%16
+  // write the tile: lane l holds rows wr + l/16 + 4e (f64 MFMA) or wr + 4(l/16) + e (f32),
+  // column wc + l%16
   const double half = 0.5 * mlambda;
   const double pre = scaled ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]) : 0.0;
   for (int e = 0; e < 4; e++) {
-    const int r = wr + l4 + 4 * e, c = wc + l16;
+    const int r = F32 ? wr + 4 * l4 + e : wr + l4 + 4 * e, c = wc + l16;
     const int gi = ti * 32 + r, gj = tj * 32 + c;
     double vr = 0.0, vi = 0.0;
     if (gi < n && gj < n) {
       if (gi != gj) {
         const float2 u = lut_get(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]);
-        vr = (double)u.x - half * accr[e];
-        vi = (double)u.y - half * acci[e];
+        vr = (double)u.x - half * (double)accr[e];
+        vi = (double)u.y - half * (double)acci[e];
       } else {
         const double sd = dsum_s[r] * mlambda / 2.0;
         vr = 2.0 * (double)ssg[gi] / mlambda - sd;
@@ -248,7 +264,7 @@ void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                      double2* A) {
   const int tiles = B.npad / 32;
-  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
+  hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
                      mlambda, dynupz, dz, A, (float2*)nullptr, 1);
 }
 
@@ -256,14 +272,14 @@ void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const 
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                        float2* A32) {
   const int tiles = B.npad / 32;
-  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
+  hipLaunchKernelGGL(k_assemble<true>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
                      mlambda, dynupz, dz, (double2*)nullptr, A32, 1);
 }
 
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D) {
   const int tiles = B.npad / 32;
-  hipLaunchKernelGGL(k_assemble, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
+  hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
                      (const double*)nullptr, mlambda, dynupz, 0.0, D, (float2*)nullptr, 0);
 }
 
