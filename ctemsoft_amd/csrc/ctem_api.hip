@@ -684,12 +684,19 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 
 // Mixed precision: the same scaling-and-squaring with the exponential kept as E = I + F, so
 // that every product acts on the small part F and its rounding error scales with |F|^2:
-//   F_0 = B + B^2/2 + B^3 (I/6 + B/24 + B^2/120 + B^3/720),  B = A / 2^s, ||B||_1 <= 1/4
+//   F_0 = the degree-8 Taylor polynomial of exp(B) - I,  B = A / 2^s, ||B||_1 <= 1/2
 //   F_{i+1} = F_i^2 + 2 F_i            (E_{i+1} = E_i^2)
 // The depth powers use the same form, psi_{m+j} = psi_j + F_m psi_j with F_m = E^m - I.
-// Products run on split-BF16 MFMA with FP32 accumulation (ctem_mixed.hip); the norm and the
-// squaring counts come from the FP64 matrix, the final Lgh/Sgh contraction runs in FP64.
-constexpr double kThetaMixed = 0.25;
+// Products run on split-FP16 (or BF16) MFMA with FP32 accumulation (ctem_mixed.hip); the
+// depth recursion accumulates in FP64 and the final Lgh/Sgh contraction runs in FP64.
+constexpr double kThetaMixed = 0.5;
+// x1..x7, y2 of the three-product degree-8 Taylor evaluation (the structure of Bader, Blanes and
+// Casas, "Computing the matrix exponential with an optimized Taylor polynomial approximation",
+// Mathematics 7 (2019) 1174), solved here for the coefficients of B^2..B^8 of exp(B):
+// with x1 = 4 x2, B8 matches B^3/6 + ... + B^8/8! (coefficient error < 3e-17)
+constexpr float kBbc8[8] = {0.098979145598294002f, 0.024744786399573501f, 0.60892636975088432f,
+                            0.59860708413115948f,  0.17640400262879624f,  0.015427061856584176f,
+                            0.040505319112355415f, 0.13549236135285087f};
 
 int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32,
                          float* wmax, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
@@ -720,19 +727,7 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
-  auto cg = [&](float2* C, float* cmax, const float2* X, const float* xmax, const float2* Y,
-                const float* ymax, float betaI, std::initializer_list<std::pair<const float2*, float>> ms,
-                const int32_t* active, int step, double flops, int terms, int ncols, int64_t sy,
-                int64_t sc, int64_t sm) -> int32_t {
-    CGemm g{};
-    g.m = npad; g.n = ncols; g.k = npad;
-    g.ldx = g.ldy = g.ldc = g.ldm = npad;
-    g.X = X; g.sx = per; g.Y = Y; g.sy = sy; g.C = C; g.sc = sc; g.sm = sm;
-    g.alpha = 1.0f; g.betaI = betaI;
-    int q = 0;
-    for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
-    g.active = active; g.step = step; g.nb = nb; g.terms = terms; g.err = d_err;
-    g.xmax = xmax; g.ymax = ymax; g.cmax = cmax;
+  auto launch = [&](const CGemm& g, double flops) -> int32_t {
     GemmTimer t{tp, s};
     t.begin();
     launch_cgemm(s, g);
@@ -741,18 +736,39 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     if (tp) { tp->f_gemm += flops; tp->n_gemm++; }
     return CTEM_OK;
   };
+  // C = X*Y + betaI*I + sum beta_j M_j over nb square matrices
+  auto gemm = [&](float2* C, float* cmax, const float2* X, const float* xmax, const float2* Y, const float* ymax,
+                  float betaI, std::initializer_list<std::pair<const float2*, float>> ms, int terms) {
+    CGemm g{};
+    g.m = g.n = g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = X; g.sx = per; g.Y = Y; g.sy = per; g.C = C; g.sc = per; g.sm = per;
+    g.alpha = 1.0f; g.betaI = betaI;
+    int q = 0;
+    for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
+    g.nb = nb; g.terms = terms; g.err = d_err;
+    g.xmax = xmax; g.ymax = ymax; g.cmax = cmax;
+    return g;
+  };
   int32_t rc;
   const double f3 = 8.0 * n3;
-  if ((rc = cg(G2, mG2, B, mB, B, mB, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
-  if ((rc = cg(G3, mG3, B, mB, G2, mG2, 0.f, {}, nullptr, 0, f3, terms_taylor, npad, per, per, per))) return rc;
+  // degree-8 Taylor polynomial of exp(B) - I from three products
+  //   B2 = B*B,  B4 = B2*(x1 B + x2 B2),
+  //   B8 = (x3 B2 + B4)*(x4 I + x5 B + x6 B2 + x7 B4),
+  //   F0 = B + y2 B2 + B8
+  // each product's epilogue also writes the next operand (second output of the GEMM)
   {
-    const float2* M[4] = {B, G2, G3, nullptr};
-    const float beta[4] = {(float)inv_fact(4), (float)inv_fact(5), (float)inv_fact(6), 0.f};
-    launch_ccombo(s, npad, nb, P, (float)inv_fact(3), M, beta, mP);
-    HIPCHK(hipGetLastError());
+    CGemm g = gemm(G2, mG2, B, mB, B, mB, 0.f, {{B, 0.f}}, terms_taylor);
+    g.C2 = P; g.cmax2 = mP; g.alpha2 = kBbc8[1]; g.betaI2 = 0.f; g.beta2[0] = kBbc8[0];
+    if ((rc = launch(g, f3))) return rc;
   }
-  if ((rc = cg(F, mF, G3, mG3, P, mP, 0.f, {{B, 1.f}, {G2, 0.5f}}, nullptr, 0, f3, terms_taylor, npad, per, per, per)))
-    return rc;
+  {
+    CGemm g = gemm(G3, mG3, G2, mG2, P, mP, 0.f, {{G2, kBbc8[2]}, {B, 0.f}}, terms_taylor);
+    g.C2 = H; g.cmax2 = mH; g.alpha2 = kBbc8[6]; g.betaI2 = kBbc8[3];
+    g.beta2[0] = kBbc8[5]; g.beta2[1] = kBbc8[4];
+    if ((rc = launch(g, f3))) return rc;
+  }
+  if ((rc = launch(gemm(F, mF, G3, mG3, H, mH, 0.f, {{B, 1.f}, {G2, kBbc8[7]}}, terms_taylor), f3))) return rc;
   float2* buf[2] = {F, H};
   float* bmax[2] = {mF, mH};
   for (int i = 0; i < smax; i++) {
@@ -762,9 +778,9 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     float* out_max = bmax[(i + 1) & 1];
     // the output buffer's maxima were those of two steps back
     HIPCHK(hipMemsetAsync(out_max, 0, sizeof(float) * (size_t)nb, s));
-    if ((rc = cg(buf[(i + 1) & 1], out_max, X, bmax[i & 1], X, bmax[i & 1], 0.f, {{X, 2.f}}, d_sq, i, fl,
-                 terms_square, npad, per, per, per)))
-      return rc;
+    CGemm g = gemm(buf[(i + 1) & 1], out_max, X, bmax[i & 1], X, bmax[i & 1], 0.f, {{X, 2.f}}, terms_square);
+    g.active = d_sq; g.step = i;
+    if ((rc = launch(g, fl))) return rc;
   }
   launch_cparity_copy(s, npad, nb, d_sq, H, F);
   HIPCHK(hipGetLastError());

@@ -146,15 +146,17 @@ struct CGemm {
   const float* xmax;     // [nb] max |entry| of X and Y per matrix (split-FP16 scaling)
   const float* ymax;
   float* cmax;           // [nb] receives max |entry| of C per matrix (zeroed by the caller), or null
+  // optional second output from the same product:
+  // C2 = alpha2*X*Y + betaI2*I + sum_j beta2_j*M_j (same M, strides and leading dimension as C)
+  float2* C2;
+  float alpha2, betaI2, beta2[4];
+  float* cmax2;
 };
 void launch_cgemm(hipStream_t s, const CGemm& g);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,
                          int32_t smax, int32_t* sq, int32_t* err, float* amax);
-// C = betaI*I + sum_j beta_j*M_j per matrix, with the max |entry| into cmax[b]
-void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
-                   const float2* const M[4], const float beta[4], float* cmax);
 void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                          const float2* src, float2* dst);
 

@@ -22,6 +22,7 @@
 // (lane l): A[l%16][8*(l/16)+t], B[8*(l/16)+t][l%16], t = 0..7; result element e of lane l is
 // C[4*(l/16)+e][l%16].
 #include <hip/hip_bf16.h>
+#include <algorithm>
 #include <type_traits>
 #include <hip/hip_runtime.h>
 
@@ -210,28 +211,38 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
     kloop(std::false_type{});
   if (!wave_live) return;
   // undo the operand scaling (a power of two)
-  const float alpha = ldexpf(g.alpha, -(ex + ey));
-  float vmax = 0.0f;
+  const float unscale = ldexpf(1.0f, -(ex + ey));
+  const float alpha = g.alpha * unscale, alpha2 = g.alpha2 * unscale;
+  float2* C2 = g.C2 ? g.C2 + (int64_t)b * g.sc : nullptr;
+  float vmax = 0.0f, vmax2 = 0.0f;
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
       for (int e = 0; e < 4; e++) {
         const int row = bm + wm + 16 * i + 4 * l4 + e;
         const int col = bn + wn + 16 * j + l16;
         if (row < g.m && col < g.n) {
-          float vr = alpha * (crp[i][j][e] - crn[i][j][e]);
-          float vi = alpha * ci[i][j][e];
-          if (row == col) vr += g.betaI;
+          const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
+          float vr = alpha * pr, vi = alpha * pi;
+          float wr = alpha2 * pr, wi = alpha2 * pi;
+          if (row == col) { vr += g.betaI; wr += g.betaI2; }
           for (int q = 0; q < 4; q++)
             if (g.M[q]) {
               const float2 mv = g.M[q][(int64_t)b * g.sm + (int64_t)col * g.ldm + row];
               vr += g.beta[q] * mv.x;
               vi += g.beta[q] * mv.y;
+              wr += g.beta2[q] * mv.x;
+              wi += g.beta2[q] * mv.y;
             }
           C[(int64_t)col * g.ldc + row] = make_float2(vr, vi);
           vmax = nmax(vmax, nmax(fabsf(vr), fabsf(vi)));
+          if (C2) {
+            C2[(int64_t)col * g.ldc + row] = make_float2(wr, wi);
+            vmax2 = nmax(vmax2, nmax(fabsf(wr), fabsf(wi)));
+          }
         }
       }
   if (g.cmax) wave_max_to(vmax, g.cmax + b);
+  if (C2 && g.cmax2) wave_max_to(vmax2, g.cmax2 + b);
 }
 
 void launch_cgemm(hipStream_t s, const CGemm& g) {
@@ -245,8 +256,6 @@ void launch_cgemm(hipStream_t s, const CGemm& g) {
 }
 
 // ------------------------------------------------------------------------------------------
-static int slot_grid(int64_t per) { return (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64); }
-
 // 1-norm (max column sum of |a|, accumulated in FP64), squaring count s with
 // ||A||_1 / 2^s <= theta, in-place scaling by 2^-s and the max |entry| afterwards; one
 // workgroup per matrix
@@ -300,29 +309,6 @@ void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, dou
   hipLaunchKernelGGL(k_norm_scale32, dim3(nb), dim3(256), 0, s, npad, A, theta, smax, sq, err, amax);
 }
 
-struct CComboArgs { const float2* M[4]; float beta[4]; };
-__global__ void k_ccombo(int npad, float2* __restrict__ C, float betaI, CComboArgs a, float* __restrict__ cmax) {
-  const int64_t per = (int64_t)npad * npad;
-  const int64_t base = (int64_t)blockIdx.y * per;
-  float vmax = 0.0f;
-  for (int64_t loc = (int64_t)blockIdx.x * 256 + threadIdx.x; loc < per; loc += (int64_t)gridDim.x * 256) {
-    const int row = (int)(loc % npad), col = (int)(loc / npad);
-    float vr = (row == col) ? betaI : 0.f, vi = 0.f;
-    for (int q = 0; q < 4; q++)
-      if (a.M[q]) { const float2 v = a.M[q][base + loc]; vr += a.beta[q] * v.x; vi += a.beta[q] * v.y; }
-    C[base + loc] = make_float2(vr, vi);
-    vmax = nmax(vmax, nmax(fabsf(vr), fabsf(vi)));
-  }
-  wave_max_to(vmax, cmax + blockIdx.y);
-}
-void launch_ccombo(hipStream_t s, int32_t npad, int32_t nb, float2* C, float betaI,
-                   const float2* const M[4], const float beta[4], float* cmax) {
-  CComboArgs a;
-  for (int q = 0; q < 4; q++) { a.M[q] = M[q]; a.beta[q] = beta[q]; }
-  const int64_t per = (int64_t)npad * npad;
-  hipLaunchKernelGGL(k_ccombo, dim3(slot_grid(per), nb), dim3(256), 0, s, npad, C, betaI, a, cmax);
-}
-
 __global__ void k_cparity_copy(int npad, const int* __restrict__ sq, const float2* __restrict__ src,
                                float2* __restrict__ dst) {
   const int b = blockIdx.y;
@@ -334,7 +320,7 @@ __global__ void k_cparity_copy(int npad, const int* __restrict__ sq, const float
 void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                          const float2* src, float2* dst) {
   const int64_t per = (int64_t)npad * npad;
-  const int gx = (int)((per + 255) / 256 < 64 ? (per + 255) / 256 : 64);
+  const int gx = (int)std::min<int64_t>((per + 255) / 256, 64);
   hipLaunchKernelGGL(k_cparity_copy, dim3(gx, nb), dim3(256), 0, s, npad, sq, src, dst);
 }
 
