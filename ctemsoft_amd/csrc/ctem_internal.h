@@ -151,6 +151,7 @@ struct CGemm {
   float2* C2;
   float alpha2, betaI2, beta2[4];
   float* cmax2;
+  int32_t vec;           // set by launch_cgemm: 16-byte vector access to Y, C, C2 and M is aligned
 };
 void launch_cgemm(hipStream_t s, const CGemm& g);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by

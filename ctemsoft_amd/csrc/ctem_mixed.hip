@@ -47,7 +47,17 @@ template <> struct Split<true> {
   __device__ static f4v mma(V a, V b, f4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
 
-#define CG_KS 40  // LDS row stride in bf16 elements (32 used, padded against bank conflicts)
+// LDS tiles hold 32 k values per row (or column) as four 16-byte chunks of 8, with the chunk
+// index swizzled by the row: the fragment reads (8 lanes x 16 B of 8 different rows per LDS
+// cycle) and the 4-byte staging writes of the Y tile then fall on distinct banks without
+// padding
+__device__ __forceinline__ int cg_swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
+
+// two 16-bit values as one 32-bit word, the first in the low half
+template <typename E>
+__device__ __forceinline__ uint32_t pack2(E lo, E hi) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, lo) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
+}
 
 template <typename E>
 __device__ __forceinline__ void split3(float x, E& a, E& b, E& c) {
@@ -78,7 +88,7 @@ __device__ __forceinline__ int f16_scale_exp(float m) {
 }
 
 template <int TERMS, bool F16>
-__global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TERMS == 6 ? 2 : 3))) void k_cgemm(CGemm g) {
   typedef typename Split<F16>::E E;
   typedef typename Split<F16>::V V;
   constexpr int NP = (TERMS == 6) ? 3 : 2;  // 16-bit planes per real component
@@ -97,9 +107,9 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
     ey = f16_scale_exp(g.ymax[b]);
   }
 
-  // planes: [component re/im][term][row or col][k]
-  __shared__ __attribute__((aligned(16))) E As[2][NP][64][CG_KS];
-  __shared__ __attribute__((aligned(16))) E Bs[2][NP][64][CG_KS];
+  // planes: [component re/im][term][row or col][k, swizzled chunks]
+  __shared__ __attribute__((aligned(16))) E As[2][NP][64][32];
+  __shared__ __attribute__((aligned(16))) E Bs[2][NP][64][32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int l16 = lane & 15, l4 = lane >> 4;
@@ -116,56 +126,72 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   // register-prefetch pipeline; the global loads of the next chunk are in flight during the
   // MFMAs of the current one. X (column-major, rows contiguous): thread t loads row t & 63 at
   // k = 8*(t >> 6) .. +7, each load coalesced over the 64 rows of a wave. Y (column-major, k
-  // contiguous): thread t loads column t >> 2 at k = 8*(t & 3) .. +7 as four 16-byte loads,
-  // four lanes covering one column's 256 contiguous bytes.
-  const int arow = tid & 63, akq = (tid >> 6) * 8;
-  const int bcol = tid >> 2, bkq = (tid & 3) * 8;
+  // contiguous): in round j thread t loads column (t >> 4) + 16 j at k = 2*(t & 15), +1 as one
+  // 16-byte load, 16 lanes covering the column's 256 contiguous bytes.
+  const int arow = tid & 63, akq = tid >> 6;
+  const int bkp = tid & 15, bcol = tid >> 4;
   // loads are branch-free: indices are clamped into the matrix and out-of-range values zeroed
-  const bool arow_ok = bm + arow < g.m, bcol_ok = bn + bcol < g.n;
+  const bool arow_ok = bm + arow < g.m;
   const float2* xr0 = X + min(bm + arow, g.m - 1);
-  const float2* yc0 = Y + (int64_t)min(bn + bcol, g.n - 1) * g.ldy;
-  const bool yvec = (g.k % 8 == 0) && (g.ldy % 2 == 0);  // 16-byte loads of Y columns
-  float2 ra[8], rb[8];
+  float2 ra[8];
+  float4 rb[4];
   auto gload = [&](int k0) {
     for (int j = 0; j < 8; j++) {
-      const int kk = k0 + akq + j;
+      const int kk = k0 + 8 * akq + j;
       const float2 v = xr0[(int64_t)min(kk, g.k - 1) * g.ldx];
       ra[j] = (arow_ok && kk < g.k) ? v : make_float2(0.f, 0.f);
     }
-    if (yvec) {
-      const bool ok = bcol_ok && (k0 + bkq < g.k);
-      const float4* yc = (const float4*)(yc0 + min(k0 + bkq, g.k - 8));
-      for (int j = 0; j < 4; j++) {
-        const float4 v = yc[j];
-        rb[2 * j] = ok ? make_float2(v.x, v.y) : make_float2(0.f, 0.f);
-        rb[2 * j + 1] = ok ? make_float2(v.z, v.w) : make_float2(0.f, 0.f);
-      }
-    } else {
-      for (int j = 0; j < 8; j++) {
-        const int kk = k0 + bkq + j;
-        const float2 v = yc0[min(kk, g.k - 1)];
-        rb[j] = (bcol_ok && kk < g.k) ? v : make_float2(0.f, 0.f);
+    for (int j = 0; j < 4; j++) {
+      const int col = bn + bcol + 16 * j;
+      const int kk = k0 + 2 * bkp;
+      const float2* yc = Y + (int64_t)min(col, g.n - 1) * g.ldy;
+      if (g.vec) {
+        // k is even and ldy is even: the pair is 16-byte aligned; with k a multiple of 8 the
+        // pair lies wholly inside or wholly outside the matrix
+        const float4 v = *(const float4*)(yc + min(kk, g.k - 2));
+        rb[j] = (col < g.n && kk < g.k) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+      } else {
+        const float2 v0 = yc[min(kk, g.k - 1)], v1 = yc[min(kk + 1, g.k - 1)];
+        const bool ok = col < g.n;
+        rb[j] = make_float4((ok && kk < g.k) ? v0.x : 0.f, (ok && kk < g.k) ? v0.y : 0.f,
+                            (ok && kk + 1 < g.k) ? v1.x : 0.f, (ok && kk + 1 < g.k) ? v1.y : 0.f);
       }
     }
   };
   const float sx = ldexpf(1.0f, ex), sy = ldexpf(1.0f, ey);  // exact powers of two
-  auto stage = [&](const float2* v, E (*P)[NP][64][CG_KS], float sc, int row, int kq) {
+  auto split_scaled = [&](float x, float sc, E (&o)[NP]) {
+    if constexpr (F16) x *= sc;
+    E a, bb, c;
+    split3(x, a, bb, c);
+    o[0] = a; o[1] = bb;
+    if constexpr (NP == 3) o[2] = c;
+  };
+  auto stage_x = [&]() {
     __attribute__((aligned(16))) E pr[NP][8], pi[NP][8];
     for (int j = 0; j < 8; j++) {
-      E a, bb, c;
-      float xr = v[j].x, xi = v[j].y;
-      if constexpr (F16) {
-        xr *= sc;
-        xi *= sc;
-      }
-      split3(xr, a, bb, c);
-      pr[0][j] = a; pr[1][j] = bb; if constexpr (NP == 3) pr[NP - 1][j] = c;
-      split3(xi, a, bb, c);
-      pi[0][j] = a; pi[1][j] = bb; if constexpr (NP == 3) pi[NP - 1][j] = c;
+      E o[NP];
+      split_scaled(ra[j].x, sx, o);
+      for (int p = 0; p < NP; p++) pr[p][j] = o[p];
+      split_scaled(ra[j].y, sx, o);
+      for (int p = 0; p < NP; p++) pi[p][j] = o[p];
     }
+    const int ch = 8 * cg_swz(arow, akq);
     for (int p = 0; p < NP; p++) {
-      *(V*)&P[0][p][row][kq] = *(V*)pr[p];
-      *(V*)&P[1][p][row][kq] = *(V*)pi[p];
+      *(V*)&As[0][p][arow][ch] = *(V*)pr[p];
+      *(V*)&As[1][p][arow][ch] = *(V*)pi[p];
+    }
+  };
+  auto stage_y = [&]() {
+    for (int j = 0; j < 4; j++) {
+      const int col = bcol + 16 * j;
+      const int off = 8 * cg_swz(col, bkp >> 2) + 2 * (bkp & 3);
+      E o0[NP], o1[NP];
+      split_scaled(rb[j].x, sy, o0);
+      split_scaled(rb[j].z, sy, o1);
+      for (int p = 0; p < NP; p++) *(uint32_t*)&Bs[0][p][col][off] = pack2(o0[p], o1[p]);
+      split_scaled(rb[j].y, sy, o0);
+      split_scaled(rb[j].w, sy, o1);
+      for (int p = 0; p < NP; p++) *(uint32_t*)&Bs[1][p][col][off] = pack2(o0[p], o1[p]);
     }
   };
   gload(0);
@@ -175,18 +201,20 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   auto kloop = [&](auto live) {
   for (int k0 = 0; k0 < g.k; k0 += 32) {
     __syncthreads();
-    stage(ra, As, sx, arow, akq);
-    stage(rb, Bs, sy, bcol, bkq);
+    stage_x();
+    stage_y();
     __syncthreads();
     if (k0 + 32 < g.k) gload(k0 + 32);
     if constexpr (!decltype(live)::value) continue;
     V a[2][2][NP], bq[2][2][NP];  // [block][re/im][term]
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
+      const int ra_ = wm + 16 * i + l16, rb_ = wn + 16 * i + l16;
       for (int c = 0; c < 2; c++)
         for (int p = 0; p < NP; p++) {
-          a[i][c][p] = *(const V*)&As[c][p][wm + 16 * i + l16][8 * l4];
-          bq[i][c][p] = *(const V*)&Bs[c][p][wn + 16 * i + l16][8 * l4];
+          a[i][c][p] = *(const V*)&As[c][p][ra_][8 * cg_swz(ra_, l4)];
+          bq[i][c][p] = *(const V*)&Bs[c][p][rb_][8 * cg_swz(rb_, l4)];
         }
+    }
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++) {
 #define MM(acc, ca, pa, cb, pb) acc = Split<F16>::mma(a[i][ca][pa], bq[j][cb][pb], acc)
@@ -215,37 +243,73 @@ __global__ __launch_bounds__(256) void k_cgemm(CGemm g) {
   const float alpha = g.alpha * unscale, alpha2 = g.alpha2 * unscale;
   float2* C2 = g.C2 ? g.C2 + (int64_t)b * g.sc : nullptr;
   float vmax = 0.0f, vmax2 = 0.0f;
+  // each lane holds rows row0 .. row0+3 of one column: contiguous in the column-major outputs
+#pragma unroll
   for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int row0 = bm + wm + 16 * i + 4 * l4;
+      const int col = bn + wn + 16 * j + l16;
+      if (col >= g.n || row0 >= g.m) continue;
+      float vr[4], vi[4], wr[4], wi[4];
+#pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int row = bm + wm + 16 * i + 4 * l4 + e;
-        const int col = bn + wn + 16 * j + l16;
-        if (row < g.m && col < g.n) {
-          const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
-          float vr = alpha * pr, vi = alpha * pi;
-          float wr = alpha2 * pr, wi = alpha2 * pi;
-          if (row == col) { vr += g.betaI; wr += g.betaI2; }
+        const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
+        vr[e] = alpha * pr; vi[e] = alpha * pi;
+        wr[e] = alpha2 * pr; wi[e] = alpha2 * pi;
+        if (row0 + e == col) { vr[e] += g.betaI; wr[e] += g.betaI2; }
+      }
+      const int64_t mo = (int64_t)b * g.sm + (int64_t)col * g.ldm + row0;
+      const int64_t co = (int64_t)col * g.ldc + row0;
+      if (g.vec) {
+        // m is a multiple of 4: the four rows are all inside
+        for (int q = 0; q < 4; q++)
+          if (g.M[q]) {
+            const float4 m0 = *(const float4*)(g.M[q] + mo), m1 = *(const float4*)(g.M[q] + mo + 2);
+            const float mr[4] = {m0.x, m0.z, m1.x, m1.z}, mi[4] = {m0.y, m0.w, m1.y, m1.w};
+            for (int e = 0; e < 4; e++) {
+              vr[e] += g.beta[q] * mr[e]; vi[e] += g.beta[q] * mi[e];
+              wr[e] += g.beta2[q] * mr[e]; wi[e] += g.beta2[q] * mi[e];
+            }
+          }
+        *(float4*)(C + co) = make_float4(vr[0], vi[0], vr[1], vi[1]);
+        *(float4*)(C + co + 2) = make_float4(vr[2], vi[2], vr[3], vi[3]);
+        if (C2) {
+          *(float4*)(C2 + co) = make_float4(wr[0], wi[0], wr[1], wi[1]);
+          *(float4*)(C2 + co + 2) = make_float4(wr[2], wi[2], wr[3], wi[3]);
+        }
+        for (int e = 0; e < 4; e++) {
+          vmax = nmax(vmax, nmax(fabsf(vr[e]), fabsf(vi[e])));
+          vmax2 = nmax(vmax2, nmax(fabsf(wr[e]), fabsf(wi[e])));
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          if (row0 + e >= g.m) continue;
           for (int q = 0; q < 4; q++)
             if (g.M[q]) {
-              const float2 mv = g.M[q][(int64_t)b * g.sm + (int64_t)col * g.ldm + row];
-              vr += g.beta[q] * mv.x;
-              vi += g.beta[q] * mv.y;
-              wr += g.beta2[q] * mv.x;
-              wi += g.beta2[q] * mv.y;
+              const float2 mv = g.M[q][mo + e];
+              vr[e] += g.beta[q] * mv.x; vi[e] += g.beta[q] * mv.y;
+              wr[e] += g.beta2[q] * mv.x; wi[e] += g.beta2[q] * mv.y;
             }
-          C[(int64_t)col * g.ldc + row] = make_float2(vr, vi);
-          vmax = nmax(vmax, nmax(fabsf(vr), fabsf(vi)));
-          if (C2) {
-            C2[(int64_t)col * g.ldc + row] = make_float2(wr, wi);
-            vmax2 = nmax(vmax2, nmax(fabsf(wr), fabsf(wi)));
-          }
+          C[co + e] = make_float2(vr[e], vi[e]);
+          if (C2) C2[co + e] = make_float2(wr[e], wi[e]);
+          vmax = nmax(vmax, nmax(fabsf(vr[e]), fabsf(vi[e])));
+          vmax2 = nmax(vmax2, nmax(fabsf(wr[e]), fabsf(wi[e])));
         }
       }
+    }
   if (g.cmax) wave_max_to(vmax, g.cmax + b);
   if (C2 && g.cmax2) wave_max_to(vmax2, g.cmax2 + b);
 }
 
-void launch_cgemm(hipStream_t s, const CGemm& g) {
+void launch_cgemm(hipStream_t s, const CGemm& gin) {
+  CGemm g = gin;
+  auto al16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+  bool vec = g.m % 4 == 0 && g.k % 8 == 0 && g.ldy % 2 == 0 && g.ldc % 2 == 0 && g.ldm % 2 == 0 &&
+             g.sy % 2 == 0 && g.sc % 2 == 0 && g.sm % 2 == 0 && al16(g.Y) && al16(g.C) && al16(g.C2);
+  for (int q = 0; q < 4; q++) vec = vec && al16(g.M[q]);
+  g.vec = vec ? 1 : 0;
   const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
   if (g.terms == CTEM_SPLIT_FP16X3)
     hipLaunchKernelGGL((k_cgemm<3, true>), dim3(tm * tn * g.nb), dim3(256), 0, s, g);
