@@ -790,6 +790,106 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   return CTEM_OK;
 }
 
+// The mixed form with both stages on CTEM_SPLIT_FP16X3: the same Taylor polynomial and
+// squarings as expm_depth_mixed, with every matrix after the assembly kept in split form
+// (ctem_split.hip), so that the products read their operands without converting them.
+// Buffers (nb matrices each, 8 bytes per entry): 0 A as assembled (FP32), later L; 1 B;
+// 2 B2; 3 P; 4 R, then the squarings alternate between 5 (F0) and 4. wmax holds six FP32
+// maxima and six exponents per matrix.
+int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, float* wmax,
+                         double2* Psi64, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
+                         const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
+  const int64_t per = (int64_t)npad * npad;
+  uint16_t* S[6];
+  float* mx[6];
+  int32_t* ex[6];
+  for (int i = 0; i < 6; i++) {
+    S[i] = (uint16_t*)(W32 + i * per * nb);
+    mx[i] = wmax + i * nb;
+    ex[i] = (int32_t*)(wmax + (6 + i) * nb);
+  }
+  HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
+  launch_norm_split(s, npad, nb, W32, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
+  HIPCHK(hipGetLastError());
+  h_sq.resize(nb);
+  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int smax = 0;
+  for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  double n3 = 0.0;
+  for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
+  auto launch = [&](const SGemm& g, double flops) -> int32_t {
+    GemmTimer t{tp, s};
+    t.begin();
+    launch_sgemm(s, g);
+    t.end();
+    HIPCHK(hipGetLastError());
+    if (tp) { tp->f_gemm += flops; tp->n_gemm++; }
+    return CTEM_OK;
+  };
+  // C = X Y (+ terms added below)
+  auto mk = [&](int c, int x, int y) {
+    SGemm g{};
+    g.n = npad; g.nb = nb;
+    g.X = S[x]; g.xe = ex[x]; g.xmax = mx[x];
+    g.Y = S[y]; g.ye = ex[y]; g.ymax = mx[y];
+    g.C = S[c]; g.ce = ex[c]; g.cmax = mx[c];
+    g.alpha = 1.0f;
+    return g;
+  };
+  auto add_m = [&](SGemm& g, int q, int m, float beta, float beta2) {
+    g.M[q] = S[m]; g.me[q] = ex[m]; g.mmax[q] = mx[m];
+    g.beta[q] = beta; g.beta2[q] = beta2;
+  };
+  auto add_c2 = [&](SGemm& g, int c2, float alpha2, float betaI2) {
+    g.C2 = S[c2]; g.ce2 = ex[c2]; g.cmax2 = mx[c2];
+    g.alpha2 = alpha2; g.betaI2 = betaI2;
+  };
+  int32_t rc;
+  const double f3 = 8.0 * n3;
+  // B2 = B B, P = x2 B2 + x1 B
+  {
+    SGemm g = mk(2, 1, 1);
+    add_c2(g, 3, kBbc8[1], 0.f);
+    add_m(g, 0, 1, 0.f, kBbc8[0]);
+    if ((rc = launch(g, f3))) return rc;
+  }
+  // B4 = B2 P;  L = B4 + x3 B2,  R = x7 B4 + x4 I + x6 B2 + x5 B
+  {
+    SGemm g = mk(0, 2, 3);
+    add_c2(g, 4, kBbc8[6], kBbc8[3]);
+    add_m(g, 0, 2, kBbc8[2], kBbc8[5]);
+    add_m(g, 1, 1, 0.f, kBbc8[4]);
+    if ((rc = launch(g, f3))) return rc;
+  }
+  // F0 = L R + B + y2 B2
+  {
+    SGemm g = mk(5, 0, 4);
+    add_m(g, 0, 1, 1.f, 0.f);
+    add_m(g, 1, 2, kBbc8[7], 0.f);
+    if ((rc = launch(g, f3))) return rc;
+  }
+  const int buf[2] = {5, 4};
+  for (int i = 0; i < smax; i++) {
+    double fl = 0.0;
+    for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
+    const int x = buf[i & 1], out = buf[(i + 1) & 1];
+    // the output buffer's maxima were those of two steps back
+    HIPCHK(hipMemsetAsync(mx[out], 0, sizeof(float) * (size_t)nb, s));
+    SGemm g = mk(out, x, x);
+    add_m(g, 0, x, 2.f, 0.f);
+    g.active = d_sq; g.step = i;
+    if ((rc = launch(g, fl))) return rc;
+  }
+  // depth on E = I + F with FP64 accumulation; F of matrix b sits in buffer buf[sq[b] & 1],
+  // decoded to FP32 into buffer 0 (the depth kernel keeps the FP32 matrix on chip)
+  launch_split_decode(s, npad, nb, d_sq, S[5], ex[5], S[4], ex[4], W32);
+  HIPCHK(hipGetLastError());
+  launch_depth(s, npad, nb, nz, W32, 1, 1, Psi64, spsi, depth_grid);
+  HIPCHK(hipGetLastError());
+  return CTEM_OK;
+}
+
 }  // namespace
 
 // bucket sizes: multiples of 32
@@ -858,10 +958,14 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         rc = expm_depth_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
                              p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
-        if ((rc = c->w.wmax.ensure(6 * sizeof(float) * (size_t)nb))) return rc;
-        rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
-                              c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
-                              p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
+        if ((rc = c->w.wmax.ensure(12 * sizeof(float) * (size_t)nb))) return rc;
+        if (p->terms_taylor == CTEM_SPLIT_FP16X3 && p->terms_square == CTEM_SPLIT_FP16X3)
+          rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(), c->w.Psi.as<double2>(),
+                                spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+        else
+          rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
+                                c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
+                                p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
       }
       if (rc) return rc;
       launch_lgh_dots(s, B, V, R, L, c->w.Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,

@@ -16,6 +16,27 @@ __device__ __forceinline__ int xcd_logical_block(int bid, int nblk) {
   return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
 }
 
+// NaN-propagating max of non-negative values
+__device__ __forceinline__ float nmax(float a, float b) { return (a != a) ? a : ((b != b) ? b : fmaxf(a, b)); }
+
+// per-matrix max |entry|: wave reduction, one atomic per wave (non-negative floats order as
+// their bit patterns, a NaN above every finite value)
+__device__ __forceinline__ void wave_max_to(float v, float* slot) {
+  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax((unsigned int*)slot, __float_as_uint(v));
+  if ((threadIdx.x & 63) == 0 && v != v) atomicMax((unsigned int*)slot, 0x7fc00000u);
+}
+
+// scaling exponent of the split-FP16 operands: max |x| 2^e lies in [2^13, 2^14); e is held to
+// [-60, 60] so that products of two scale factors stay normal FP32 numbers (a matrix with
+// max |x| < 2^-46 is then scaled below 2^14, still far above the FP16 subnormal range)
+__device__ __forceinline__ int f16_scale_exp(float m) {
+  if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0;
+  int q;
+  frexpf(m, &q);
+  return min(max(14 - q, -60), 60);
+}
+
 // Reflection table on the device, structure of arrays in master-list order.
 struct DevRefl {
   int32_t nref;
@@ -154,6 +175,41 @@ struct CGemm {
   int32_t vec;           // set by launch_cgemm: 16-byte vector access to Y, C, C2 and M is aligned
 };
 void launch_cgemm(hipStream_t s, const CGemm& g);
+
+// Split-FP16 complex matrices (mixed form with CTEM_SPLIT_FP16X3): four FP16 values per entry
+// (real high, real low, imaginary high, imaginary low) and one exponent e per matrix: entry =
+// (high + low) * 2^-e. Storage is column-major in groups of 8 rows: column c of matrix b
+// starts at b * 4 n^2 + c * 4 n halfs, and holds for each group of rows 8g .. 8g+7 the four
+// planes one after the other (64 bytes), so that 8 consecutive rows of one plane are 16
+// contiguous bytes and all planes of an entry lie in one 64-byte block. A matrix takes 8 bytes
+// per entry, as float2 does. Alongside each buffer the true max |re|, |im| over the matrix is
+// kept in FP32 (bounds for the next exponents).
+__host__ __device__ __forceinline__ int64_t split_index(int n, int row, int col, int plane) {
+  return (int64_t)col * 4 * n + (row >> 3) * 32 + plane * 8 + (row & 7);
+}
+// C = alpha X Y + betaI I + sum_j beta_j M_j (and optionally C2 with alpha2, betaI2, beta2
+// from the same product), n a multiple of 32. The output exponents come from the bound
+// |alpha| n max|X| max|Y| + |betaI| + sum |beta_j| max|M_j|.
+struct SGemm {
+  int32_t n;
+  const uint16_t* X; const int32_t* xe; const float* xmax;
+  const uint16_t* Y; const int32_t* ye; const float* ymax;
+  uint16_t* C; int32_t* ce; float* cmax;
+  float alpha, betaI;
+  const uint16_t* M[2]; const int32_t* me[2]; const float* mmax[2]; float beta[2];
+  uint16_t* C2; int32_t* ce2; float* cmax2;
+  float alpha2, betaI2, beta2[2];
+  const int32_t* active; int32_t step;  // matrices with step >= active[b] are skipped
+  int32_t nb;
+};
+void launch_sgemm(hipStream_t s, const SGemm& g);
+// FP32 matrices A (as assembled) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in
+// split form into Bs with its exponent and max |entry|
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, uint16_t* Bs, double theta,
+                       int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax);
+// matrix b of the split pair (S[sq[b] & 1]) decoded to FP32 into out
+void launch_split_decode(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq, const uint16_t* S0,
+                         const int32_t* e0, const uint16_t* S1, const int32_t* e1, float2* out);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

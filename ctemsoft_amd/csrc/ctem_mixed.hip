@@ -68,25 +68,6 @@ __device__ __forceinline__ void split3(float x, E& a, E& b, E& c) {
   c = (E)r2;
 }
 
-// NaN-propagating max of non-negative values
-__device__ __forceinline__ float nmax(float a, float b) { return (a != a) ? a : ((b != b) ? b : fmaxf(a, b)); }
-
-// per-matrix max |entry|: wave reduction, one atomic per wave (non-negative floats order as
-// their bit patterns, a NaN above every finite value)
-__device__ __forceinline__ void wave_max_to(float v, float* slot) {
-  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
-  if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax((unsigned int*)slot, __float_as_uint(v));
-  if ((threadIdx.x & 63) == 0 && v != v) atomicMax((unsigned int*)slot, 0x7fc00000u);
-}
-
-// scaling exponent of the split-FP16 operands: max |x| 2^e lies in [2^13, 2^14)
-__device__ __forceinline__ int f16_scale_exp(float m) {
-  if (!(m > 0.0f) || !(m <= 3.4e38f)) return 0;
-  int q;
-  frexpf(m, &q);
-  return 14 - q;
-}
-
 template <int TERMS, bool F16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TERMS == 6 ? 2 : 3))) void k_cgemm(CGemm g) {
   typedef typename Split<F16>::E E;

@@ -1,0 +1,355 @@
+// Mixed form on split-FP16 matrices (the default CTEM_SPLIT_FP16X3 scheme): every matrix of the
+// Taylor stage and of the squarings is stored once, by the kernel that produces it, as four
+// FP16 planes (real high/low, imaginary high/low) with one exponent per matrix (layout:
+// split_index in ctem_internal.h). The products
+// then copy their tiles into LDS unchanged, and the three FP16 MFMA products per real product
+// (high*high + high*low + low*high, FP32 accumulation) see the same operands as the FP32 form
+// split on the fly in ctem_mixed.hip.
+//
+// Tile layout of k_sgemm (64 x 64 output tile per 256-thread workgroup, 32 x 32 per wave):
+// the X tile is staged as it lies in memory, k-major with the 64 rows contiguous, and the MFMA
+// A operand (8 consecutive k of one row per lane) is read with the LDS transpose read
+// ds_read_b64_tr_b16. Within a group of 16 lanes that read 4 halfs each, lane i receives
+// element i % 4 of lane 4e + i / 4 as its element e; lane 4e + j reads rows r0 + 4j .. +3 at
+// column k0 + e, so that lane i ends up with rows r0 + i at columns k0 .. k0 + 3.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "ctem_ebsd.h"
+#include "ctem_internal.h"
+
+namespace ctem {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+#define LDS_PTR(T, p) ((__attribute__((address_space(3))) T*)(p))
+
+// s -> high + low, both FP16 (round to nearest), |s| < 2^14
+__device__ __forceinline__ void split2(float s, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)s;
+  lo = (_Float16)(s - (float)hi);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sgemm(SGemm g) {
+  const int n = g.n;
+  const int tiles = (n + 63) >> 6;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
+  const int bm = (t % tiles) * 64, bn = (t / tiles) * 64;
+  if (g.active && g.step >= g.active[b]) return;
+  const int64_t per = (int64_t)n * n;
+  const int64_t mb = (int64_t)b * 4 * per;
+  const _Float16* X = (const _Float16*)g.X + mb;
+  const _Float16* Y = (const _Float16*)g.Y + mb;
+
+  // X tile: [plane][k 32][row 64], 32-byte row segments swizzled by k (segment ^ (k & 3));
+  // Y tile: [plane][col 64][k 32], 16-byte k chunks swizzled by the column (chunk ^ ((col >> 1) & 3))
+  __shared__ __attribute__((aligned(16))) _Float16 As[4 * 32 * 64];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  const bool wave_live = (bm + wm < n) && (bn + wn < n);
+
+  // staging, the 64-byte group block (8 rows, 4 planes) per thread: X rows bm + 8 (t & 7) .. +7
+  // of column k0 + (t >> 3), Y column bn + (t >> 2) at rows k0 + 8 (t & 3) .. +7. Rows and
+  // columns past the matrix are clamped onto its last ones: they feed only output rows and
+  // columns that are not written.
+  const int xk = tid >> 3, xr = 8 * (tid & 7);
+  const _Float16* xp = X + split_index(n, min(bm + xr, n - 8), xk, 0);
+  const int xlds = xk * 64 + 16 * ((xr >> 4) ^ (xk & 3)) + (xr & 8);
+  const int ycol = tid >> 2, yq = tid & 3;
+  const _Float16* yp = Y + split_index(n, 8 * yq, min(bn + ycol, n - 1), 0);
+  const int ylds = ycol * 32 + 8 * (yq ^ ((ycol >> 1) & 3));
+  u4v rx[4], ry[4];
+  auto gload = [&](int k0) {
+    #pragma unroll
+    for (int p = 0; p < 4; p++) {
+      rx[p] = *(const u4v*)(xp + (int64_t)k0 * 4 * n + 8 * p);
+      ry[p] = *(const u4v*)(yp + 4 * k0 + 8 * p);
+    }
+  };
+
+  f4v crp[2][2], crn[2][2], ci[2][2];
+  #pragma unroll
+  for (int i = 0; i < 2; i++)
+    #pragma unroll
+    for (int j = 0; j < 2; j++) {
+      crp[i][j] = (f4v){0, 0, 0, 0};
+      crn[i][j] = (f4v){0, 0, 0, 0};
+      ci[i][j] = (f4v){0, 0, 0, 0};
+    }
+  // fragment offsets: A (transpose read, lane reads rows r0 + 4 (l16 & 3) .. +3 at column
+  // 8 l4 + (l16 >> 2), and 4 columns further for the upper half), B (16 bytes at chunk l4)
+  const int kl = 8 * l4 + (l16 >> 2);
+  int aoff[2], boff[2];
+  #pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const int r0 = wm + 16 * i;
+    aoff[i] = kl * 64 + 16 * ((r0 >> 4) ^ (kl & 3)) + 4 * (l16 & 3);
+    const int col = wn + 16 * i + l16;
+    boff[i] = col * 32 + 8 * (l4 ^ ((col >> 1) & 3));
+  }
+
+  // exponents and maxima of the operands (read before the loop: its barriers would hold
+  // these loads back to the epilogue)
+  const float xmx = g.xmax[b], ymx = g.ymax[b];
+  const int exy = g.xe[b] + g.ye[b];
+  float mmx[2] = {0.f, 0.f};
+  int mex[2] = {0, 0};
+  #pragma unroll
+  for (int q = 0; q < 2; q++)
+    if (g.M[q]) { mmx[q] = g.mmax[q][b]; mex[q] = g.me[q][b]; }
+
+  gload(0);
+  // A wave whose 32x32 quadrant lies past the matrix edge only stages its share of the
+  // tiles; the choice is made once, outside the loop, so that the live copy of the loop keeps
+  // its accumulators in place across iterations.
+  auto kloop = [&](auto live) {
+    for (int k0 = 0; k0 < n; k0 += 32) {
+      __syncthreads();
+      #pragma unroll
+      for (int p = 0; p < 4; p++) {
+        *(u4v*)&As[p * 2048 + xlds] = rx[p];
+        *(u4v*)&Bs[p * 2048 + ylds] = ry[p];
+      }
+      __syncthreads();
+      if (k0 + 32 < n) gload(k0 + 32);
+      if constexpr (!decltype(live)::value) continue;
+      h8v a[2][2][2], bq[2][2][2];  // [block][re/im][high/low]
+      #pragma unroll
+      for (int i = 0; i < 2; i++)
+        #pragma unroll
+        for (int c = 0; c < 2; c++)
+          #pragma unroll
+          for (int h = 0; h < 2; h++) {
+            const int P = 2 * c + h;
+            const s4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i]]));
+            const s4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i] + 256]));
+            a[i][c][h] = __builtin_bit_cast(h8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            bq[i][c][h] = *(const h8v*)&Bs[P * 2048 + boff[i]];
+          }
+      #pragma unroll
+      for (int i = 0; i < 2; i++)
+        #pragma unroll
+        for (int j = 0; j < 2; j++) {
+#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[j][cb][hb], acc, 0, 0, 0)
+          // smallest terms first
+          MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
+          MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
+          MM(ci[i][j], 0, 1, 1, 0); MM(ci[i][j], 0, 0, 1, 1); MM(ci[i][j], 0, 0, 1, 0);
+          MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
+#undef MM
+        }
+    }
+  };
+  if (wave_live)
+    kloop(std::true_type{});
+  else
+    kloop(std::false_type{});
+
+  // output exponents from bounds on the entries (the same in every workgroup of the matrix)
+  const float xy = (float)n * xmx * ymx;
+  float bnd = fabsf(g.alpha) * xy + fabsf(g.betaI), bnd2 = fabsf(g.alpha2) * xy + fabsf(g.betaI2);
+  #pragma unroll
+  for (int q = 0; q < 2; q++)
+    if (g.M[q]) {
+      bnd += fabsf(g.beta[q]) * mmx[q];
+      bnd2 += fabsf(g.beta2[q]) * mmx[q];
+    }
+  const int ec = f16_scale_exp(bnd), ec2 = f16_scale_exp(bnd2);
+  if (t == 0 && tid == 0) {
+    g.ce[b] = ec;
+    if (g.C2) g.ce2[b] = ec2;
+  }
+  if (!wave_live) return;
+  // the accumulators carry 2^(ex + ey); the outputs are stored times 2^ec
+  const float fa = g.alpha * ldexpf(1.0f, ec - exy), fa2 = g.alpha2 * ldexpf(1.0f, ec2 - exy);
+  const float fi = g.betaI * ldexpf(1.0f, ec), fi2 = g.betaI2 * ldexpf(1.0f, ec2);
+  float fm[2] = {0.f, 0.f}, fm2[2] = {0.f, 0.f};
+  #pragma unroll
+  for (int q = 0; q < 2; q++)
+    if (g.M[q]) {
+      fm[q] = g.beta[q] * ldexpf(1.0f, ec - mex[q]);
+      fm2[q] = g.beta2[q] * ldexpf(1.0f, ec2 - mex[q]);
+    }
+  _Float16* C = (_Float16*)g.C + mb;
+  _Float16* C2 = g.C2 ? (_Float16*)g.C2 + mb : nullptr;
+  float vmax = 0.0f, vmax2 = 0.0f;
+  // each lane holds rows row0 .. row0+3 of one column: 8 contiguous bytes per plane, the four
+  // planes 16 bytes apart in one 64-byte group block
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int row0 = bm + wm + 16 * i + 4 * l4;
+      const int col = bn + wn + 16 * j + l16;
+      if (col >= n || row0 >= n) continue;
+      const int64_t o = split_index(n, row0, col, 0);
+      float vr[4], vi[4], wr[4], wi[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
+        vr[e] = fa * pr; vi[e] = fa * pi;
+        wr[e] = fa2 * pr; wi[e] = fa2 * pi;
+        if (row0 + e == col) { vr[e] += fi; wr[e] += fi2; }
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+        if (g.M[q]) {
+          const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
+          const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
+          const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const float mr = (float)m0[e] + (float)m1[e], mi = (float)m2[e] + (float)m3[e];
+            vr[e] += fm[q] * mr; vi[e] += fm[q] * mi;
+            wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
+          }
+        }
+      h4v o0, o1, o2, o3;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        _Float16 h, l;
+        split2(vr[e], h, l); o0[e] = h; o1[e] = l;
+        split2(vi[e], h, l); o2[e] = h; o3[e] = l;
+        vmax = nmax(vmax, nmax(fabsf(vr[e]), fabsf(vi[e])));
+      }
+      *(h4v*)(C + o) = o0; *(h4v*)(C + o + 8) = o1;
+      *(h4v*)(C + o + 16) = o2; *(h4v*)(C + o + 24) = o3;
+      if (C2) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          _Float16 h, l;
+          split2(wr[e], h, l); o0[e] = h; o1[e] = l;
+          split2(wi[e], h, l); o2[e] = h; o3[e] = l;
+          vmax2 = nmax(vmax2, nmax(fabsf(wr[e]), fabsf(wi[e])));
+        }
+        *(h4v*)(C2 + o) = o0; *(h4v*)(C2 + o + 8) = o1;
+        *(h4v*)(C2 + o + 16) = o2; *(h4v*)(C2 + o + 24) = o3;
+      }
+    }
+  // true maxima (the scale factors are powers of two)
+  wave_max_to(vmax * ldexpf(1.0f, -ec), g.cmax + b);
+  if (C2) wave_max_to(vmax2 * ldexpf(1.0f, -ec2), g.cmax2 + b);
+}
+
+void launch_sgemm(hipStream_t s, const SGemm& g) {
+  const int tiles = (g.n + 63) / 64;
+  hipLaunchKernelGGL(k_sgemm, dim3(tiles * tiles * g.nb), dim3(256), 0, s, g);
+}
+
+// 1-norm (max column sum of |a|, accumulated in FP64), squaring count s with
+// ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its exponent and max
+// |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
+__global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __restrict__ A,
+                                                    _Float16* __restrict__ Bs, double theta, int smax,
+                                                    int* __restrict__ sq, int* __restrict__ err,
+                                                    int* __restrict__ bexp, float* __restrict__ bmax) {
+  const int b = blockIdx.x;
+  const int64_t per = (int64_t)npad * npad;
+  const float2* Ab = A + (int64_t)b * per;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double wnorm[4];
+  __shared__ float wmx[4];
+  __shared__ int s_sh, e_sh;
+  double cnorm = 0.0;
+  float amax = 0.0f;
+  for (int j = wave; j < npad; j += 4) {
+    double sum = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const float2 v = Ab[(int64_t)j * npad + i];
+      sum += (double)hypotf(v.x, v.y);
+      amax = nmax(amax, nmax(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    cnorm = fmax(cnorm, sum);
+    if (!(sum == sum)) cnorm = sum;  // keep a NaN
+  }
+  for (int o = 32; o > 0; o >>= 1) amax = nmax(amax, __shfl_xor(amax, o));
+  if (lane == 0) { wnorm[wave] = cnorm; wmx[wave] = amax; }
+  __syncthreads();
+  if (tid == 0) {
+    double nrm = fmax(fmax(wnorm[0], wnorm[1]), fmax(wnorm[2], wnorm[3]));
+    float m = 0.0f;
+    for (int w = 0; w < 4; w++) {
+      if (!(wnorm[w] == wnorm[w])) nrm = wnorm[w];
+      m = nmax(m, wmx[w]);
+    }
+    int sc = 0;
+    if (!isfinite(nrm)) {
+      atomicOr(err, 2);
+    } else {
+      while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
+      if (nrm > theta) atomicOr(err, 4);
+    }
+    const float mb = ldexpf(m, -sc);
+    const int e = f16_scale_exp(mb);
+    s_sh = sc;
+    e_sh = e;
+    sq[b] = sc;
+    bexp[b] = e;
+    bmax[b] = mb;
+  }
+  __syncthreads();
+  const float f = ldexpf(1.0f, e_sh - s_sh);
+  // one thread per 8-row group of a column: 64 contiguous bytes out
+  _Float16* B = Bs + (int64_t)b * 4 * per;
+  const int ng = npad >> 3;
+  for (int w = tid; w < npad * ng; w += 256) {
+    const int col = w / ng, r0 = 8 * (w % ng);
+    h8v o[4];
+    #pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float2 v = Ab[(int64_t)col * npad + r0 + e];
+      _Float16 h, l;
+      split2(v.x * f, h, l);
+      o[0][e] = h; o[1][e] = l;
+      split2(v.y * f, h, l);
+      o[2][e] = h; o[3][e] = l;
+    }
+    h8v* dst = (h8v*)(B + split_index(npad, r0, col, 0));
+    #pragma unroll
+    for (int p = 0; p < 4; p++) dst[p] = o[p];
+  }
+}
+
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, uint16_t* Bs, double theta,
+                       int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax) {
+  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, (_Float16*)Bs, theta, smax, sq, err,
+                     bexp, bmax);
+}
+
+// one thread per 8-row group of a column: 64 bytes in, 64 bytes out
+__global__ void k_split_decode(int npad, const int* __restrict__ sq, const _Float16* __restrict__ S0,
+                               const int* __restrict__ e0, const _Float16* __restrict__ S1,
+                               const int* __restrict__ e1, float2* __restrict__ out) {
+  const int b = blockIdx.y;
+  const int par = sq[b] & 1;
+  const int64_t per = (int64_t)npad * npad;
+  const _Float16* S = (par ? S1 : S0) + (int64_t)b * 4 * per;
+  const float sc = ldexpf(1.0f, -(par ? e1 : e0)[b]);
+  const int ng = npad >> 3;
+  for (int w = blockIdx.x * 256 + threadIdx.x; w < npad * ng; w += gridDim.x * 256) {
+    const int col = w / ng, r0 = 8 * (w % ng);
+    const h8v* q = (const h8v*)(S + split_index(npad, r0, col, 0));
+    const h8v rh = q[0], rl = q[1], ih = q[2], il = q[3];
+    float4* o = (float4*)(out + b * per + (int64_t)col * npad + r0);
+    #pragma unroll
+    for (int e = 0; e < 4; e++)
+      o[e] = make_float4(((float)rh[2 * e] + (float)rl[2 * e]) * sc, ((float)ih[2 * e] + (float)il[2 * e]) * sc,
+                         ((float)rh[2 * e + 1] + (float)rl[2 * e + 1]) * sc, ((float)ih[2 * e + 1] + (float)il[2 * e + 1]) * sc);
+  }
+}
+
+void launch_split_decode(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq, const uint16_t* S0,
+                         const int32_t* e0, const uint16_t* S1, const int32_t* e1, float2* out) {
+  const int items = npad * (npad / 8);
+  const int gx = std::min((items + 255) / 256, 16);
+  hipLaunchKernelGGL(k_split_decode, dim3(gx, nb), dim3(256), 0, s, npad, sq, (const _Float16*)S0, e0,
+                     (const _Float16*)S1, e1, out);
+}
+
+}  // namespace ctem
