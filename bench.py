@@ -217,7 +217,7 @@ def main():
                      "peak": PEAK[head] / 1e12, "unit": "TFLOP/s",
                      "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),
                      "traffic": measured_traffic(wl.name, head),
-                     "kernel": "k_zgemm" if head == "fp64" else "k_cgemm",
+                     "kernel": "k_zgemm" if head == "fp64" else ("k_sgemm" if a.terms == "16,16" else "k_cgemm"),
                      "flops_per_step": gflops},
         "cpu_baseline": cpu,
     }
