@@ -201,6 +201,7 @@ struct SGemm {
   float alpha2, betaI2, beta2[2];
   const int32_t* active; int32_t step;  // matrices with step >= active[b] are skipped
   int32_t nb;
+  int32_t m0_is_x;       // set by launch_sgemm: M[0] is X (its term is taken from the staged X tiles)
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
 // FP32 matrices A (as assembled) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in

@@ -96,7 +96,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // exponents and maxima of the operands (read before the loop: its barriers would hold
   // these loads back to the epilogue)
   const float xmx = g.xmax[b], ymx = g.ymax[b];
-  const int exy = g.xe[b] + g.ye[b];
+  const int ey = g.ye[b];
+  const int exy = g.xe[b] + ey;
+  // with M[0] = X, beta_0 X enters the accumulators (units 2^-(ex+ey)) from the X tile staged
+  // at k0 = the output column block: X[rows][cols] = As[k = col - k0][rows]
+  const bool mx0 = g.m0_is_x != 0;
+  const float dm0 = mx0 ? (g.beta[0] / g.alpha) * ldexpf(1.0f, ey) : 0.0f;
   float mmx[2] = {0.f, 0.f};
   int mex[2] = {0, 0};
   #pragma unroll
@@ -143,6 +148,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
 #undef MM
         }
+      if (mx0 && k0 == bn + wn) {
+        #pragma unroll
+        for (int i = 0; i < 2; i++)
+          #pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const int kk = 16 * j + l16, r = wm + 16 * i + 4 * l4;
+            const int o = kk * 64 + 16 * ((r >> 4) ^ (kk & 3)) + (r & 15);
+            const h4v p0 = *(const h4v*)&As[o], p1 = *(const h4v*)&As[2048 + o];
+            const h4v p2 = *(const h4v*)&As[4096 + o], p3 = *(const h4v*)&As[6144 + o];
+            #pragma unroll
+            for (int e = 0; e < 4; e++) {
+              crp[i][j][e] += dm0 * ((float)p0[e] + (float)p1[e]);
+              ci[i][j][e] += dm0 * ((float)p2[e] + (float)p3[e]);
+            }
+          }
+      }
     }
   };
   if (wave_live)
@@ -198,7 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
 #pragma unroll
       for (int q = 0; q < 2; q++)
-        if (g.M[q]) {
+        if (g.M[q] && !(q == 0 && mx0)) {
           const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
           const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
           const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
@@ -236,7 +257,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   if (C2) wave_max_to(vmax2 * ldexpf(1.0f, -ec2), g.cmax2 + b);
 }
 
-void launch_sgemm(hipStream_t s, const SGemm& g) {
+void launch_sgemm(hipStream_t s, const SGemm& gin) {
+  SGemm g = gin;
+  // the staged X tiles give beta_0 M_0 only where M_0 is X and no second output needs it
+  g.m0_is_x = (g.M[0] == g.X && g.me[0] == g.xe && g.C2 == nullptr && g.alpha != 0.0f) ? 1 : 0;
   const int tiles = (g.n + 63) / 64;
   hipLaunchKernelGGL(k_sgemm, dim3(tiles * tiles * g.nb), dim3(256), 0, s, g);
 }
