@@ -24,6 +24,7 @@ typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
 #define LDS_PTR(T, p) ((__attribute__((address_space(3))) T*)(p))
 
 // s -> high + low, both FP16 (round to nearest), |s| < 2^14
@@ -200,14 +201,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   _Float16* C2 = g.C2 ? (_Float16*)g.C2 + mb : nullptr;
   float vmax = 0.0f, vmax2 = 0.0f;
   // each lane holds rows row0 .. row0+3 of one column: 8 contiguous bytes per plane, the four
-  // planes 16 bytes apart in one 64-byte group block
+  // planes 16 bytes apart in one 64-byte group block. The lane pair l4 = 2m, 2m+1 holds the
+  // whole group (rows 0-3 and 4-7) and trades halves, so that the even lane stores the real
+  // planes and the odd lane the imaginary planes as 16-byte pieces.
+  const bool odd = (l4 & 1) != 0;
+  auto store_split = [&](_Float16* dst, const float (&re)[4], const float (&im)[4], float& mx) {
+    h4v o0, o1, o2, o3;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      _Float16 h, l;
+      split2(re[e], h, l); o0[e] = h; o1[e] = l;
+      split2(im[e], h, l); o2[e] = h; o3[e] = l;
+      mx = nmax(mx, nmax(fabsf(re[e]), fabsf(im[e])));
+    }
+    // the even lane sends its imaginary rows 0-3, the odd lane its real rows 4-7
+    const u2v s0 = __builtin_bit_cast(u2v, odd ? o0 : o2), s1 = __builtin_bit_cast(u2v, odd ? o1 : o3);
+    u2v r0, r1;
+    r0.x = __shfl_xor(s0.x, 16); r0.y = __shfl_xor(s0.y, 16);
+    r1.x = __shfl_xor(s1.x, 16); r1.y = __shfl_xor(s1.y, 16);
+    const h4v q0 = __builtin_bit_cast(h4v, r0), q1 = __builtin_bit_cast(h4v, r1);
+    const h8v w0 = odd ? __builtin_shufflevector(q0, o2, 0, 1, 2, 3, 4, 5, 6, 7)
+                       : __builtin_shufflevector(o0, q0, 0, 1, 2, 3, 4, 5, 6, 7);
+    const h8v w1 = odd ? __builtin_shufflevector(q1, o3, 0, 1, 2, 3, 4, 5, 6, 7)
+                       : __builtin_shufflevector(o1, q1, 0, 1, 2, 3, 4, 5, 6, 7);
+    // dst points at this lane's rows in plane 0; the group starts 4 rows earlier on odd lanes
+    _Float16* gb = dst - (odd ? 4 : 0) + (odd ? 16 : 0);
+    *(h8v*)gb = w0;
+    *(h8v*)(gb + 8) = w1;
+  };
 #pragma unroll
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) {
+      // n is a multiple of 32: every lane of a live wave holds valid rows and columns
       const int row0 = bm + wm + 16 * i + 4 * l4;
       const int col = bn + wn + 16 * j + l16;
-      if (col >= n || row0 >= n) continue;
       const int64_t o = split_index(n, row0, col, 0);
       float vr[4], vi[4], wr[4], wi[4];
 #pragma unroll
@@ -230,27 +258,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
           }
         }
-      h4v o0, o1, o2, o3;
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        _Float16 h, l;
-        split2(vr[e], h, l); o0[e] = h; o1[e] = l;
-        split2(vi[e], h, l); o2[e] = h; o3[e] = l;
-        vmax = nmax(vmax, nmax(fabsf(vr[e]), fabsf(vi[e])));
-      }
-      *(h4v*)(C + o) = o0; *(h4v*)(C + o + 8) = o1;
-      *(h4v*)(C + o + 16) = o2; *(h4v*)(C + o + 24) = o3;
-      if (C2) {
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          _Float16 h, l;
-          split2(wr[e], h, l); o0[e] = h; o1[e] = l;
-          split2(wi[e], h, l); o2[e] = h; o3[e] = l;
-          vmax2 = nmax(vmax2, nmax(fabsf(wr[e]), fabsf(wi[e])));
-        }
-        *(h4v*)(C2 + o) = o0; *(h4v*)(C2 + o + 8) = o1;
-        *(h4v*)(C2 + o + 16) = o2; *(h4v*)(C2 + o + 24) = o3;
-      }
+      store_split(C + o, vr, vi, vmax);
+      if (C2) store_split(C2 + o, wr, wi, vmax2);
     }
   // true maxima (the scale factors are powers of two)
   wave_max_to(vmax * ldexpf(1.0f, -ec), g.cmax + b);
