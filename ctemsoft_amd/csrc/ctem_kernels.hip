@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   double2* Ab = A + (int64_t)b * npad * npad;
 
   __shared__ int rh[32][3], ch[32][3];
-  __shared__ double wsgd[32], winv[32], dsum_s[32];
+  __shared__ double winv[32], dsum_s[32], dpart_s[8][32];
   typedef typename std::conditional<F32, float2, double2>::type CT;  // staged LUT values
   typedef typename std::conditional<F32, f32x4, double4v>::type AccT;
   __shared__ CT Ua[32][33], Ub[32][33];
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   }
   __syncthreads();
   AccT accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
-  double dsum = 0.0;
+  double dpart = 0.0;  // diagonal tiles: this thread's share of sum_w |U(g_r - w)|^2 / s_w
   const bool live = (ti * 32 < n) && (tj * 32 < n);
   const bool diag_tile = (ti == tj);
   // register prefetch of the next chunk: thread t gathers rows/columns t & 31 for the weak
@@ -192,9 +192,11 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
         const int ww = pww + 8 * m;
         Ua[prr][ww] = CT{pa[m].x, pa[m].y};  // U(g_r - w)
         Ub[ww][prr] = CT{pb[m].x, pb[m].y};  // U(w - h_c)
-        if (prr == 0) {
-          wsgd[ww] = (double)psg[m];
-          winv[ww] = 1.0 / (double)psg[m];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
+        if (prr == 0) winv[ww] = 1.0 / (double)psg[m];  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
+        if (diag_tile) {
+          // |U(g_r - w)|^2 / s_w (gvectors.f90:1165); entries past nw are 0 with s = 1
+          const double m2 = hypot((double)pa[m].x, (double)pa[m].y);
+          dpart += m2 * m2 / (double)psg[m];
         }
       }
       __syncthreads();
@@ -220,15 +222,18 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
           acci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bq.x, acci, 0, 0, 0);
         }
       }
-      if (diag_tile && tid < 32) {
-        for (int w = 0; w < wn; w++) {
-          const double m = hypot((double)Ua[tid][w].x, (double)Ua[tid][w].y);
-          dsum += m * m / wsgd[w];
-        }
-      }
     }
   }
-  if (diag_tile && tid < 32) dsum_s[tid] = dsum;
+  // diagonal sums: the eight partial sums of each row, added in a fixed order
+  if (diag_tile) {
+    dpart_s[pww][prr] = dpart;
+    __syncthreads();
+    if (tid < 32) {
+      double d = 0.0;
+      for (int q = 0; q < 8; q++) d += dpart_s[q][tid];
+      dsum_s[tid] = d;
+    }
+  }
   __syncthreads();
   // write the tile: lane l holds rows wr + l/16 + 4e (f64 MFMA) or wr + 4(l/16) + e (f32),
   // column wc + l%16
