@@ -112,10 +112,10 @@ struct Crystal {
 // Work space of ctem_plan_run and ctem_plan_adjoint, shared by the plans of a context (calls
 // on one context are sequential); it grows to the largest chunk run and is kept.
 struct Work {
-  DBuf A, W, Psi, sq, dirs, partial, wmax;
+  DBuf A, W, Psi, sq, dirs, partial, wmax, colsum;
   DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
   void release() {
-    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
+    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
                     &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig})
       b->release();
   }
@@ -794,11 +794,12 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // squarings as expm_depth_mixed, with every matrix after the assembly kept in split form
 // (ctem_split.hip), so that the products read their operands without converting them.
 // Buffers (nb matrices each, 8 bytes per entry): 0 A as assembled (FP32), later L; 1 B;
-// 2 B2; 3 P; 4 R, then the squarings alternate between 5 (F0) and 4. wmax holds six FP32
-// maxima and six exponents per matrix.
-int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, float* wmax,
-                         double2* Psi64, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
-                         const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
+// 2 B2; 3 P, later the FP32 copy of the final F of every matrix (written by its last
+// product) for the depth kernel; 4 R, then the squarings alternate between 5 (F0) and 4.
+// wmax holds six FP32 maxima and six exponents per matrix.
+int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
+                         const float* amax, float* wmax, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
+                         int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
   uint16_t* S[6];
   float* mx[6];
@@ -809,7 +810,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
-  launch_norm_split(s, npad, nb, W32, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
+  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
@@ -862,11 +863,13 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     add_m(g, 1, 1, 0.f, kBbc8[4]);
     if ((rc = launch(g, f3))) return rc;
   }
-  // F0 = L R + B + y2 B2
+  // F0 = L R + B + y2 B2 (the final F where no squaring follows)
+  float2* Ff = W32 + 3 * per * nb;
   {
     SGemm g = mk(5, 0, 4);
     add_m(g, 0, 1, 1.f, 0.f);
     add_m(g, 1, 2, kBbc8[7], 0.f);
+    g.Cf = Ff; g.f_sq = d_sq; g.f_at = 0;
     if ((rc = launch(g, f3))) return rc;
   }
   const int buf[2] = {5, 4};
@@ -879,13 +882,11 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     SGemm g = mk(out, x, x);
     add_m(g, 0, x, 2.f, 0.f);
     g.active = d_sq; g.step = i;
+    g.Cf = Ff; g.f_sq = d_sq; g.f_at = i + 1;
     if ((rc = launch(g, fl))) return rc;
   }
-  // depth on E = I + F with FP64 accumulation; F of matrix b sits in buffer buf[sq[b] & 1],
-  // decoded to FP32 into buffer 0 (the depth kernel keeps the FP32 matrix on chip)
-  launch_split_decode(s, npad, nb, d_sq, S[5], ex[5], S[4], ex[4], W32);
-  HIPCHK(hipGetLastError());
-  launch_depth(s, npad, nb, nz, W32, 1, 1, Psi64, spsi, depth_grid);
+  // depth on E = I + F with FP64 accumulation (the depth kernel keeps the FP32 matrix on chip)
+  launch_depth(s, npad, nb, nz, Ff, 1, 1, Psi64, spsi, depth_grid);
   HIPCHK(hipGetLastError());
   return CTEM_OK;
 }
@@ -949,8 +950,21 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb);
       for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
+      const bool split = precision == CTEM_PREC_MIXED && p->terms_taylor == CTEM_SPLIT_FP16X3 &&
+                         p->terms_square == CTEM_SPLIT_FP16X3;
+      float* amax = nullptr;
+      if (precision == CTEM_PREC_MIXED) {
+        // six maxima and six exponents per matrix for the products, then the max of A
+        if ((rc = c->w.wmax.ensure(13 * sizeof(float) * (size_t)nb))) return rc;
+        amax = c->w.wmax.as<float>() + 12 * (size_t)nb;
+        HIPCHK(hipMemsetAsync(amax, 0, sizeof(float) * (size_t)nb, s));
+      }
+      if (split && (rc = c->w.colsum.ensure(sizeof(double) * (size_t)nb * npad * (npad / 32)))) return rc;
       if (precision == CTEM_PREC_FP64)
         launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.A.as<double2>());
+      else if (split)
+        launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>(),
+                          c->w.colsum.as<double>(), amax);
       else
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
       HIPCHK(hipGetLastError());
@@ -958,10 +972,10 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         rc = expm_depth_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
                              p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
       } else {
-        if ((rc = c->w.wmax.ensure(12 * sizeof(float) * (size_t)nb))) return rc;
-        if (p->terms_taylor == CTEM_SPLIT_FP16X3 && p->terms_square == CTEM_SPLIT_FP16X3)
-          rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(), c->w.Psi.as<double2>(),
-                                spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+        if (split)
+          rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
+                                c->w.wmax.as<float>(), c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(),
+                                p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
         else
           rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
                                 c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,

@@ -92,10 +92,12 @@ struct BeamView {        // read-only view of the selection lists
 void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                      double2* A);
-// the same scaled matrix rounded to FP32 (mixed form)
+// the same scaled matrix rounded to FP32 (mixed form); with colpart, also the per-tile
+// column sums of |a| (colpart[b][tile row][column], FP64) and the max |re|, |im| per matrix
+// into amax[b] (zeroed by the caller)
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                       float2* A32);
+                       float2* A32, double* colpart = nullptr, float* amax = nullptr);
 // same matrix, unscaled (DynMat itself), for parity checks
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D);
@@ -202,15 +204,17 @@ struct SGemm {
   const int32_t* active; int32_t step;  // matrices with step >= active[b] are skipped
   int32_t nb;
   int32_t m0_is_x;       // set by launch_sgemm: M[0] is X (its term is taken from the staged X tiles)
+  // optional FP32 copy of C (column-major float2, n x n per matrix) for the matrices with
+  // f_sq[b] == f_at: the last product of each matrix hands its result to the depth kernel
+  float2* Cf; const int32_t* f_sq; int32_t f_at;
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
-// FP32 matrices A (as assembled) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in
-// split form into Bs with its exponent and max |entry|
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, uint16_t* Bs, double theta,
-                       int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax);
-// matrix b of the split pair (S[sq[b] & 1]) decoded to FP32 into out
-void launch_split_decode(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq, const uint16_t* S0,
-                         const int32_t* e0, const uint16_t* S1, const int32_t* e1, float2* out);
+// FP32 matrices A (as assembled, with the column-sum partials and max |entry| of
+// launch_assemble32) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in split form
+// into Bs with its exponent and max |entry|
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
+                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
+                       int32_t* bexp, float* bmax);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

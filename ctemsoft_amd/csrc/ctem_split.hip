@@ -199,6 +199,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   _Float16* C = (_Float16*)g.C + mb;
   _Float16* C2 = g.C2 ? (_Float16*)g.C2 + mb : nullptr;
+  float2* Cf = (g.Cf && g.f_sq[b] == g.f_at) ? g.Cf + (int64_t)b * per : nullptr;
+  const float cunscale = ldexpf(1.0f, -ec);
   float vmax = 0.0f, vmax2 = 0.0f;
   // each lane holds rows row0 .. row0+3 of one column: 8 contiguous bytes per plane, the four
   // planes 16 bytes apart in one 64-byte group block. The lane pair l4 = 2m, 2m+1 holds the
@@ -260,6 +262,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
       store_split(C + o, vr, vi, vmax);
       if (C2) store_split(C2 + o, wr, wi, vmax2);
+      if (Cf) {
+        float4* q = (float4*)(Cf + (int64_t)col * n + row0);
+        q[0] = make_float4(vr[0] * cunscale, vi[0] * cunscale, vr[1] * cunscale, vi[1] * cunscale);
+        q[1] = make_float4(vr[2] * cunscale, vi[2] * cunscale, vr[3] * cunscale, vi[3] * cunscale);
+      }
     }
   // true maxima (the scale factors are powers of two)
   wave_max_to(vmax * ldexpf(1.0f, -ec), g.cmax + b);
@@ -274,45 +281,42 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
   hipLaunchKernelGGL(k_sgemm, dim3(tiles * tiles * g.nb), dim3(256), 0, s, g);
 }
 
-// 1-norm (max column sum of |a|, accumulated in FP64), squaring count s with
-// ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its exponent and max
-// |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
+// 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
+// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its
+// exponent and max |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
 __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __restrict__ A,
-                                                    _Float16* __restrict__ Bs, double theta, int smax,
-                                                    int* __restrict__ sq, int* __restrict__ err,
-                                                    int* __restrict__ bexp, float* __restrict__ bmax) {
+                                                    const double* __restrict__ colpart,
+                                                    const float* __restrict__ amax, _Float16* __restrict__ Bs,
+                                                    double theta, int smax, int* __restrict__ sq,
+                                                    int* __restrict__ err, int* __restrict__ bexp,
+                                                    float* __restrict__ bmax) {
   const int b = blockIdx.x;
   const int64_t per = (int64_t)npad * npad;
   const float2* Ab = A + (int64_t)b * per;
+  const int tiles = npad / 32;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ double wnorm[4];
-  __shared__ float wmx[4];
   __shared__ int s_sh, e_sh;
   double cnorm = 0.0;
-  float amax = 0.0f;
-  for (int j = wave; j < npad; j += 4) {
+  for (int j = tid; j < npad; j += 256) {
     double sum = 0.0;
-    for (int i = lane; i < npad; i += 64) {
-      const float2 v = Ab[(int64_t)j * npad + i];
-      sum += (double)hypotf(v.x, v.y);
-      amax = nmax(amax, nmax(fabsf(v.x), fabsf(v.y)));
-    }
-    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    for (int t = 0; t < tiles; t++) sum += colpart[((int64_t)b * tiles + t) * npad + j];
     cnorm = fmax(cnorm, sum);
     if (!(sum == sum)) cnorm = sum;  // keep a NaN
   }
-  for (int o = 32; o > 0; o >>= 1) amax = nmax(amax, __shfl_xor(amax, o));
-  if (lane == 0) { wnorm[wave] = cnorm; wmx[wave] = amax; }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(cnorm, o);
+    cnorm = (cnorm != cnorm) ? cnorm : ((v != v) ? v : fmax(cnorm, v));  // a NaN wins
+  }
+  if (lane == 0) wnorm[wave] = cnorm;
   __syncthreads();
   if (tid == 0) {
     double nrm = fmax(fmax(wnorm[0], wnorm[1]), fmax(wnorm[2], wnorm[3]));
-    float m = 0.0f;
-    for (int w = 0; w < 4; w++) {
+    for (int w = 0; w < 4; w++)
       if (!(wnorm[w] == wnorm[w])) nrm = wnorm[w];
-      m = nmax(m, wmx[w]);
-    }
+    const float m = amax[b];
     int sc = 0;
-    if (!isfinite(nrm)) {
+    if (!isfinite(nrm) || !(m == m)) {
       atomicOr(err, 2);
     } else {
       while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
@@ -349,40 +353,11 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
   }
 }
 
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, uint16_t* Bs, double theta,
-                       int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax) {
-  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, (_Float16*)Bs, theta, smax, sq, err,
-                     bexp, bmax);
-}
-
-// one thread per 8-row group of a column: 64 bytes in, 64 bytes out
-__global__ void k_split_decode(int npad, const int* __restrict__ sq, const _Float16* __restrict__ S0,
-                               const int* __restrict__ e0, const _Float16* __restrict__ S1,
-                               const int* __restrict__ e1, float2* __restrict__ out) {
-  const int b = blockIdx.y;
-  const int par = sq[b] & 1;
-  const int64_t per = (int64_t)npad * npad;
-  const _Float16* S = (par ? S1 : S0) + (int64_t)b * 4 * per;
-  const float sc = ldexpf(1.0f, -(par ? e1 : e0)[b]);
-  const int ng = npad >> 3;
-  for (int w = blockIdx.x * 256 + threadIdx.x; w < npad * ng; w += gridDim.x * 256) {
-    const int col = w / ng, r0 = 8 * (w % ng);
-    const h8v* q = (const h8v*)(S + split_index(npad, r0, col, 0));
-    const h8v rh = q[0], rl = q[1], ih = q[2], il = q[3];
-    float4* o = (float4*)(out + b * per + (int64_t)col * npad + r0);
-    #pragma unroll
-    for (int e = 0; e < 4; e++)
-      o[e] = make_float4(((float)rh[2 * e] + (float)rl[2 * e]) * sc, ((float)ih[2 * e] + (float)il[2 * e]) * sc,
-                         ((float)rh[2 * e + 1] + (float)rl[2 * e + 1]) * sc, ((float)ih[2 * e + 1] + (float)il[2 * e + 1]) * sc);
-  }
-}
-
-void launch_split_decode(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq, const uint16_t* S0,
-                         const int32_t* e0, const uint16_t* S1, const int32_t* e1, float2* out) {
-  const int items = npad * (npad / 8);
-  const int gx = std::min((items + 255) / 256, 16);
-  hipLaunchKernelGGL(k_split_decode, dim3(gx, nb), dim3(256), 0, s, npad, sq, (const _Float16*)S0, e0,
-                     (const _Float16*)S1, e1, out);
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
+                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
+                       int32_t* bexp, float* bmax) {
+  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, colpart, amax, (_Float16*)Bs, theta, smax,
+                     sq, err, bexp, bmax);
 }
 
 }  // namespace ctem
