@@ -85,6 +85,7 @@ struct Crystal {
   double mlambda = 0.0;
   float dynupz = 0.0f, sgcutoff = 0.0f;
   float nat = 0.0f;  // total orbit size, float(sum(nat))
+  float lutmax = 0.0f;  // max |re|, |im| over the float32 LUT
   std::vector<int32_t> h_hkl;
 
   ~Crystal() {
@@ -302,7 +303,11 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
     dbd[i] = x->dbdiff[i] ? 1 : 0;
   }
   std::vector<float> lutf(2 * (size_t)ncell);
-  for (size_t i = 0; i < 2 * (size_t)ncell; i++) lutf[i] = (float)x->lut[i];
+  float lutmax = 0.0f;
+  for (size_t i = 0; i < 2 * (size_t)ncell; i++) {
+    lutf[i] = (float)x->lut[i];
+    lutmax = std::max(lutmax, std::fabs(lutf[i]));
+  }
   std::vector<double> slut, dslut;
   build_slut(x, slo, sdim, slut, dslut);
   int nat = 0;
@@ -328,6 +333,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
     cr->slut_lo[i] = slo[i]; cr->slut_dim[i] = sdim[i];
   }
   cr->sncell = sncell;
+  cr->lutmax = lutmax;
   cr->mlambda = x->mlambda;
   cr->dynupz = x->dynupz;
   cr->sgcutoff = x->sgcutoff;
@@ -959,12 +965,15 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         amax = c->w.wmax.as<float>() + 12 * (size_t)nb;
         HIPCHK(hipMemsetAsync(amax, 0, sizeof(float) * (size_t)nb, s));
       }
-      if (split && (rc = c->w.colsum.ensure(sizeof(double) * (size_t)nb * npad * (npad / 32)))) return rc;
+      if (split && (rc = c->w.colsum.ensure(sizeof(double) * (size_t)nb * npad * ((npad + 63) / 64)))) return rc;
       if (precision == CTEM_PREC_FP64)
         launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.A.as<double2>());
-      else if (split)
-        launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>(),
-                          c->w.colsum.as<double>(), amax);
+      else if (split) {
+        int nwmax = 0;
+        for (int bb = 0; bb < nb; bb++) nwmax = std::max(nwmax, p->h_nnw[kv.second[c0 + bb]]);
+        launch_assemble_split(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz,
+                              p->cr->lutmax, nwmax, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax);
+      }
       else
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
       HIPCHK(hipGetLastError());

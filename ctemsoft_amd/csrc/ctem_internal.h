@@ -59,6 +59,21 @@ struct DevLut {
   int32_t nsite;
 };
 
+// U(g-h) as Compute_DynMat reads it: LUT(h,k,l) at the column-major offset of the Fortran
+// bounds, without a per-axis bounds check (gvectors.f90:1133-1160). When a beam set spans
+// more than the LUT box along an axis (hexagonal axes, where list entries reach |h+k| up to
+// twice the box), an offset that still lies inside the array reads the aliased entry, as the
+// reference does; an offset outside the array (which the reference reads from neighbouring
+// heap memory) returns 0 and raises error bit 8.
+__device__ __forceinline__ float2 lut_get(const DevLut& L, int h, int k, int l) {
+  const int idx = (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
+  if ((unsigned)idx >= (unsigned)L.ncell) {
+    atomicOr(L.err, 8);
+    return make_float2(0.0f, 0.0f);
+  }
+  return L.lut[idx];
+}
+
 struct SelectOut {       // per-direction beam sets (CSR over directions)
   int32_t* nns;          // [numk]
   int32_t* nnw;          // [numk]
@@ -92,12 +107,17 @@ struct BeamView {        // read-only view of the selection lists
 void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                      double2* A);
-// the same scaled matrix rounded to FP32 (mixed form); with colpart, also the per-tile
-// column sums of |a| (colpart[b][tile row][column], FP64) and the max |re|, |im| per matrix
-// into amax[b] (zeroed by the caller)
+// the same scaled matrix rounded to FP32 (mixed form)
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                       float2* A32, double* colpart = nullptr, float* amax = nullptr);
+                       float2* A32);
+// the same for the split path, with the weak-beam sum on split-FP16 MFMA (64 x 64 tiles);
+// also the column sums of |a| per 64-row tile (colpart[b][tile row][column], FP64) and the
+// max |re|, |im| per matrix into amax[b] (zeroed by the caller). lutmax = max |re|, |im| over
+// the LUT; nwmax = the largest weak-beam count of the bucket.
+void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                           const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax);
 // same matrix, unscaled (DynMat itself), for parity checks
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D);
@@ -210,7 +230,7 @@ struct SGemm {
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
 // FP32 matrices A (as assembled, with the column-sum partials and max |entry| of
-// launch_assemble32) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in split form
+// launch_assemble_split) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in split form
 // into Bs with its exponent and max |entry|
 void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
                        const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,

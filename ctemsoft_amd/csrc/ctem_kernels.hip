@@ -98,21 +98,6 @@ void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& 
 
 // ------------------------------------------------------------------------------------------
 // DynMat assembly
-// U(g-h) as Compute_DynMat reads it: LUT(h,k,l) at the column-major offset of the Fortran
-// bounds, without a per-axis bounds check (gvectors.f90:1133-1160). When a beam set spans
-// more than the LUT box along an axis (hexagonal axes, where list entries reach |h+k| up to
-// twice the box), an offset that still lies inside the array reads the aliased entry, as the
-// reference does; an offset outside the array (which the reference reads from neighbouring
-// heap memory) returns 0 and raises error bit 8.
-__device__ __forceinline__ float2 lut_get(const DevLut& L, int h, int k, int l) {
-  const int idx = (h - L.lo[0]) + L.dim[0] * ((k - L.lo[1]) + L.dim[1] * (l - L.lo[2]));
-  if ((unsigned)idx >= (unsigned)L.ncell) {
-    atomicOr(L.err, 8);
-    return make_float2(0.0f, 0.0f);
-  }
-  return L.lut[idx];
-}
-
 // S_p(g-h) table index; the table box holds every difference of two list entries
 __device__ __forceinline__ int64_t slut_index(const DevLut& L, int h, int k, int l) {
   return (int64_t)(h - L.slo[0]) + (int64_t)L.sdim[0] * ((k - L.slo[1]) + (int64_t)L.sdim[1] * (l - L.slo[2]));
@@ -129,8 +114,7 @@ template <bool F32>
 __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double* __restrict__ kn, double mlambda,
                                                   float dynupz, double dz, double2* __restrict__ A,
-                                                  float2* __restrict__ A32, int scaled,
-                                                  double* __restrict__ colpart, float* __restrict__ amax) {
+                                                  float2* __restrict__ A32, int scaled) {
 #pragma clang fp contract(off)
   const int npad = B.npad, tiles = npad / 32;
   const int b = blockIdx.x / (tiles * tiles);
@@ -240,8 +224,6 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   // column wc + l%16
   const double half = 0.5 * mlambda;
   const double pre = scaled ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]) : 0.0;
-  double csum = 0.0;  // this lane's four rows of its column, sum of |a| (FP32 form)
-  float fmx = 0.0f;
   for (int e = 0; e < 4; e++) {
     const int r = F32 ? wr + 4 * l4 + e : wr + l4 + 4 * e, c = wc + l16;
     const int gi = ti * 32 + r, gj = tj * 32 + c;
@@ -260,29 +242,10 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
     if (gi < npad && gj < npad) {
       // scaled: A = i * pre * DynMat, in FP64 or (A32) rounded to FP32
       const double2 v = scaled ? make_double2(-pre * vi, pre * vr) : make_double2(vr, vi);
-      if (A32) {
-        const float2 f = make_float2((float)v.x, (float)v.y);
-        A32[(int64_t)b * npad * npad + (int64_t)gj * npad + gi] = f;
-        csum += (double)hypotf(f.x, f.y);
-        fmx = nmax(fmx, nmax(fabsf(f.x), fabsf(f.y)));
-      } else {
+      if (A32)
+        A32[(int64_t)b * npad * npad + (int64_t)gj * npad + gi] = make_float2((float)v.x, (float)v.y);
+      else
         Ab[(int64_t)gj * npad + gi] = v;
-      }
-    }
-  }
-  if constexpr (F32) {
-    // the tile's share of the column 1-norms (eight partial sums per column, added in a fixed
-    // order) and of the max |entry|, for the norm and the split-form exponent of the matrix
-    if (colpart) {
-      __shared__ double cs[8][32];
-      cs[(wr >> 4) * 4 + l4][wc + l16] = csum;
-      __syncthreads();
-      if (tid < 32) {
-        double d = 0.0;
-        for (int q = 0; q < 8; q++) d += cs[q][tid];
-        colpart[((int64_t)b * tiles + ti) * npad + tj * 32 + tid] = d;
-      }
-      wave_max_to(fmx, amax + b);
     }
   }
 }
@@ -292,23 +255,22 @@ void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      double2* A) {
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
-                     mlambda, dynupz, dz, A, (float2*)nullptr, 1, (double*)nullptr, (float*)nullptr);
+                     mlambda, dynupz, dz, A, (float2*)nullptr, 1);
 }
 
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                       float2* A32, double* colpart, float* amax) {
+                       float2* A32) {
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble<true>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
-                     mlambda, dynupz, dz, (double2*)nullptr, A32, 1, colpart, amax);
+                     mlambda, dynupz, dz, (double2*)nullptr, A32, 1);
 }
 
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D) {
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
-                     (const double*)nullptr, mlambda, dynupz, 0.0, D, (float2*)nullptr, 0, (double*)nullptr,
-                     (float*)nullptr);
+                     (const double*)nullptr, mlambda, dynupz, 0.0, D, (float2*)nullptr, 0);
 }
 
 // ------------------------------------------------------------------------------------------

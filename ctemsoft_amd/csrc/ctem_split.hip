@@ -281,6 +281,244 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
   hipLaunchKernelGGL(k_sgemm, dim3(tiles * tiles * g.nb), dim3(256), 0, s, g);
 }
 
+// Mixed-form assembly for the split path: A = i pre DynMat (gvectors.f90:1123-1173) in FP32,
+// with the weak-beam sum W = sum_w U(g - w) U(w - h) / s_w as a split-FP16 MFMA product. One
+// 64 x 64 tile per 256-thread workgroup, 32 x 32 per wave. The direction's weak beams (hkl and
+// 1/s_w) are staged in LDS once; per chunk of 32 weak beams thread t gathers, one chunk ahead
+// in registers, U(g_r - w) for its tile row r = t & 63 and U(w - h_c) for its tile column
+// c = t & 63 at the weak beams 8 (t >> 6) .. +7 of the chunk, and writes them split into FP16
+// planes [plane][row or column][32 weak beams] (16-byte chunks swizzled as in k_sgemm). Scales:
+// U by 2^ea from the LUT's max entry, U / s_w by 2^eb from that max over the direction's
+// smallest |s_w|. The diagonal sums sum_w |U(g_r - w)|^2 / s_w (gvectors.f90:1165) are taken
+// in FP64 during the gathers and added in a fixed order.
+__device__ __forceinline__ int as_swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* __restrict__ kn,
+                      double mlambda, float dynupz, double dz, float lutmax, float2* __restrict__ A32,
+                      double* __restrict__ colpart, float* __restrict__ amax) {
+  const int npad = B.npad, tiles = (npad + 63) >> 6;
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
+  const int ti = t % tiles, tj = t / tiles;
+  const int bm = ti * 64, bn = tj * 64;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir], nw = V.nnw[dir];
+  const int nwpad = (nw + 31) & ~31;
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const float* ssg = V.ssg + V.soff[dir];
+  const int32_t* widx = V.widx + V.woff[dir];
+  const float* wsg = V.wsg + V.woff[dir];
+
+  extern __shared__ int4 wk[];  // [nwpad] weak beams: h, k, l, bits of fl32(1/s_w)
+  __shared__ int rh[64][3], chh[64][3];
+  __shared__ __attribute__((aligned(16))) _Float16 As[4 * 64 * 32];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
+  __shared__ float red[4];
+  __shared__ double dps[4][64], cs[8][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  const bool wave_live = (bm + wm < n) && (bn + wn < n);
+  const bool diag_tile = (ti == tj);
+
+  if (tid < 64) {
+    const int a = bm + tid;
+    for (int d = 0; d < 3; d++) rh[tid][d] = (a < n) ? R.hkl[3 * sidx[a] + d] : 0;
+  } else if (tid < 128) {
+    const int c = bn + tid - 64;
+    for (int d = 0; d < 3; d++) chh[tid - 64][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
+  }
+  float mi = 0.0f;
+  for (int w = tid; w < nwpad; w += 256) {
+    if (w < nw) {
+      const int id = widx[w];
+      const float sw = wsg[w];
+      const float iv = (float)(1.0 / (double)sw);  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
+      wk[w] = make_int4(R.hkl[3 * id], R.hkl[3 * id + 1], R.hkl[3 * id + 2], __float_as_int(iv));
+      mi = nmax(mi, fabsf(iv));
+    } else {
+      wk[w] = make_int4(0, 0, 0, 0);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) mi = nmax(mi, __shfl_xor(mi, o));
+  if (lane == 0) red[wave] = mi;
+  __syncthreads();
+  mi = nmax(nmax(red[0], red[1]), nmax(red[2], red[3]));
+  const int ea = f16_scale_exp(lutmax), eb = f16_scale_exp(lutmax * mi);
+  const float sa = ldexpf(1.0f, ea), sb = ldexpf(1.0f, eb);
+  // lut_get with the table's fields held in registers
+  const float2* lut = L.lut;
+  const int lo0 = L.lo[0], lo1 = L.lo[1], lo2 = L.lo[2], d0 = L.dim[0], d1 = L.dim[1], ncell = L.ncell;
+  int32_t* lerr = L.err;
+  auto U = [&](int h, int k, int l) __attribute__((always_inline)) -> float2 {
+    const int idx = (h - lo0) + d0 * ((k - lo1) + d1 * (l - lo2));
+    if ((unsigned)idx >= (unsigned)ncell) {
+      atomicOr(lerr, 8);
+      return make_float2(0.0f, 0.0f);
+    }
+    return lut[idx];
+  };
+
+  // gathers: row (or column) p = t & 63, weak beams w0 + pk .. +7
+  const int pr = tid & 63, pk = 8 * (tid >> 6);
+  const bool prow_ok = bm + pr < n, pcol_ok = bn + pr < n;
+  const int r0 = rh[pr][0], r1 = rh[pr][1], r2 = rh[pr][2];
+  const int c0 = chh[pr][0], c1 = chh[pr][1], c2 = chh[pr][2];
+  float2 ga[8], gb[8];
+  double dpart = 0.0;
+  auto gather = [&](int w0) __attribute__((always_inline)) {
+    #pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int w = w0 + pk + j;
+      const int4 q = wk[w];
+      ga[j] = (prow_ok && w < nw) ? U(r0 - q.x, r1 - q.y, r2 - q.z) : make_float2(0.f, 0.f);
+      gb[j] = (pcol_ok && w < nw) ? U(q.x - c0, q.y - c1, q.z - c2) : make_float2(0.f, 0.f);
+    }
+  };
+  const int alds = pr * 32 + 8 * as_swz(pr, tid >> 6);
+  auto stage = [&](int w0) __attribute__((always_inline)) {
+    h8v oa[4], ob[4];
+    #pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const float iv = __int_as_float(wk[w0 + pk + j].w);
+      _Float16 h, l;
+      split2(ga[j].x * sa, h, l); oa[0][j] = h; oa[1][j] = l;
+      split2(ga[j].y * sa, h, l); oa[2][j] = h; oa[3][j] = l;
+      split2(gb[j].x * iv * sb, h, l); ob[0][j] = h; ob[1][j] = l;  // U(w - h) / s_w
+      split2(gb[j].y * iv * sb, h, l); ob[2][j] = h; ob[3][j] = l;
+      if (diag_tile) {
+        // |U(g_r - w)|^2 / s_w; entries past nw are 0
+        const int w = w0 + pk + j;
+        if (w < nw) {
+          const double m2 = hypot((double)ga[j].x, (double)ga[j].y);
+          dpart += m2 * m2 / (double)wsg[w];
+        }
+      }
+    }
+    #pragma unroll
+    for (int p = 0; p < 4; p++) {
+      *(h8v*)&As[p * 2048 + alds] = oa[p];
+      *(h8v*)&Bs[p * 2048 + alds] = ob[p];
+    }
+  };
+
+  f4v crp[2][2], crn[2][2], ci[2][2];
+  #pragma unroll
+  for (int i = 0; i < 2; i++)
+    #pragma unroll
+    for (int j = 0; j < 2; j++) {
+      crp[i][j] = (f4v){0, 0, 0, 0};
+      crn[i][j] = (f4v){0, 0, 0, 0};
+      ci[i][j] = (f4v){0, 0, 0, 0};
+    }
+  int aoff[2], boff[2];
+  #pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const int ra = wm + 16 * i + l16, cb = wn + 16 * i + l16;
+    aoff[i] = ra * 32 + 8 * as_swz(ra, l4);
+    boff[i] = cb * 32 + 8 * as_swz(cb, l4);
+  }
+  gather(0);
+  auto kloop = [&](auto live) __attribute__((always_inline)) {
+    for (int w0 = 0; w0 < nwpad; w0 += 32) {
+      __syncthreads();
+      stage(w0);
+      __syncthreads();
+      if (w0 + 32 < nwpad) gather(w0 + 32);
+      if constexpr (!decltype(live)::value) continue;
+      h8v a[2][2][2], bq[2][2][2];  // [block][re/im][high/low]
+      #pragma unroll
+      for (int i = 0; i < 2; i++)
+        #pragma unroll
+        for (int c = 0; c < 2; c++)
+          #pragma unroll
+          for (int h = 0; h < 2; h++) {
+            a[i][c][h] = *(const h8v*)&As[(2 * c + h) * 2048 + aoff[i]];
+            bq[i][c][h] = *(const h8v*)&Bs[(2 * c + h) * 2048 + boff[i]];
+          }
+      #pragma unroll
+      for (int i = 0; i < 2; i++)
+        #pragma unroll
+        for (int j = 0; j < 2; j++) {
+#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[j][cb][hb], acc, 0, 0, 0)
+          MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
+          MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
+          MM(ci[i][j], 0, 1, 1, 0); MM(ci[i][j], 0, 0, 1, 1); MM(ci[i][j], 0, 0, 1, 0);
+          MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
+#undef MM
+        }
+    }
+  };
+  if (wave_live)
+    kloop(std::true_type{});
+  else
+    kloop(std::false_type{});
+
+  // diagonal sums of the tile rows: four partial sums per row, added in a fixed order
+  if (diag_tile) {
+    dps[tid >> 6][pr] = dpart;
+    __syncthreads();
+    if (tid < 64) dps[0][tid] = ((dps[0][tid] + dps[1][tid]) + dps[2][tid]) + dps[3][tid];
+    __syncthreads();
+  }
+  // write the tile: lane holds rows bm + wm + 16 i + 4 l4 + e of column bn + wn + 16 j + l16
+  const double half = 0.5 * mlambda;
+  const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
+  const double wsc = ldexp(1.0, -(ea + eb));
+  float2* Ab = A32 + (int64_t)b * npad * npad;
+  float fmx = 0.0f;
+  double csum[2] = {0.0, 0.0};
+  #pragma unroll
+  for (int i = 0; i < 2; i++)
+    #pragma unroll
+    for (int j = 0; j < 2; j++)
+      #pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int rr = wm + 16 * i + 4 * l4 + e, cc = wn + 16 * j + l16;
+        const int gi = bm + rr, gj = bn + cc;
+        double vr = 0.0, vi = 0.0;
+        if (gi < n && gj < n) {
+          if (gi != gj) {
+            const float2 u = U(rh[rr][0] - chh[cc][0], rh[rr][1] - chh[cc][1], rh[rr][2] - chh[cc][2]);
+            const float wr = crp[i][j][e] - crn[i][j][e], wi = ci[i][j][e];
+            vr = (double)u.x - half * ((double)wr * wsc);
+            vi = (double)u.y - half * ((double)wi * wsc);
+          } else {
+            const double sd = dps[0][rr] * mlambda / 2.0;
+            vr = 2.0 * (double)ssg[gi] / mlambda - sd;
+            vi = (double)dynupz;
+          }
+        }
+        if (gi < npad && gj < npad) {
+          const float2 f = make_float2((float)(-pre * vi), (float)(pre * vr));
+          Ab[(int64_t)gj * npad + gi] = f;
+          csum[j] += (double)hypotf(f.x, f.y);
+          fmx = nmax(fmx, nmax(fabsf(f.x), fabsf(f.y)));
+        }
+      }
+  // column 1-norm partials of the tile: eight per column (two waves x four lane groups),
+  // added in a fixed order
+  #pragma unroll
+  for (int j = 0; j < 2; j++) cs[(wm >> 5) * 4 + l4][wn + 16 * j + l16] = csum[j];
+  __syncthreads();
+  if (tid < 64 && bn + tid < npad) {
+    double d = 0.0;
+    for (int q = 0; q < 8; q++) d += cs[q][tid];
+    colpart[((int64_t)b * tiles + ti) * npad + bn + tid] = d;
+  }
+  wave_max_to(fmx, amax + b);
+}
+
+void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                           const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax) {
+  const int tiles = (B.npad + 63) / 64;
+  const size_t lds = sizeof(int4) * (size_t)std::max(32, (nwmax + 31) & ~31);
+  hipLaunchKernelGGL(k_assemble_split, dim3(tiles * tiles * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
+                     dynupz, dz, lutmax, A32, colpart, amax);
+}
+
 // 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
 // squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its
 // exponent and max |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
@@ -293,7 +531,7 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
   const int b = blockIdx.x;
   const int64_t per = (int64_t)npad * npad;
   const float2* Ab = A + (int64_t)b * per;
-  const int tiles = npad / 32;
+  const int tiles = (npad + 63) / 64;  // row partials per column (64-row assembly tiles)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ double wnorm[4];
   __shared__ int s_sh, e_sh;
