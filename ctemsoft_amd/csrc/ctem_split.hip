@@ -33,51 +33,96 @@ __device__ __forceinline__ void split2(float s, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(s - (float)hi);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_sgemm(SGemm g) {
+// Stores four consecutive entries (rows r .. r+3 of one column, values already scaled by 2^e)
+// in split form. dst points at this lane's rows in plane 0 of their 8-row group. The lane pair
+// at lanes l and l ^ 16 holds the whole group (the even lane of the pair rows 0-3, the odd lane
+// rows 4-7) and trades halves, so that the even lane stores the two real planes and the odd
+// lane the two imaginary planes as 16-byte pieces. All 64 lanes must take part.
+__device__ __forceinline__ void store_split4(_Float16* dst, const float (&re)[4], const float (&im)[4], bool odd) {
+  h4v o0, o1, o2, o3;
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    _Float16 h, l;
+    split2(re[e], h, l); o0[e] = h; o1[e] = l;
+    split2(im[e], h, l); o2[e] = h; o3[e] = l;
+  }
+  // the even lane sends its imaginary rows 0-3, the odd lane its real rows 4-7
+  const u2v s0 = __builtin_bit_cast(u2v, odd ? o0 : o2), s1 = __builtin_bit_cast(u2v, odd ? o1 : o3);
+  u2v r0, r1;
+  r0.x = __shfl_xor(s0.x, 16); r0.y = __shfl_xor(s0.y, 16);
+  r1.x = __shfl_xor(s1.x, 16); r1.y = __shfl_xor(s1.y, 16);
+  const h4v q0 = __builtin_bit_cast(h4v, r0), q1 = __builtin_bit_cast(h4v, r1);
+  const h8v w0 = odd ? __builtin_shufflevector(q0, o2, 0, 1, 2, 3, 4, 5, 6, 7)
+                     : __builtin_shufflevector(o0, q0, 0, 1, 2, 3, 4, 5, 6, 7);
+  const h8v w1 = odd ? __builtin_shufflevector(q1, o3, 0, 1, 2, 3, 4, 5, 6, 7)
+                     : __builtin_shufflevector(o1, q1, 0, 1, 2, 3, 4, 5, 6, 7);
+  // the group starts 4 rows earlier on odd lanes; their planes are the imaginary ones
+  _Float16* gb = dst - (odd ? 4 : 0) + (odd ? 16 : 0);
+  *(h8v*)gb = w0;
+  *(h8v*)(gb + 8) = w1;
+}
+
+// One workgroup of 4 waves per 64 x (32 JN) tile of C; wave w computes rows 32 (w >> 1) .. +31
+// and columns 16 JN (w & 1) .. +16 JN - 1 as 2 x JN MFMA subtiles of 16 x 16.
+template <int JN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2 : 3))) void k_sgemm(SGemm g) {
+  constexpr int BN = 32 * JN;  // tile columns
+  constexpr int YG = JN / 2;   // Y group blocks staged per thread
   const int n = g.n;
-  const int tiles = (n + 63) >> 6;
+  const int tm = (n + 63) >> 6, tn = (n + BN - 1) / BN;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
-  const int bm = (t % tiles) * 64, bn = (t / tiles) * 64;
+  const int b = lb / (tm * tn), t = lb % (tm * tn);
+  const int bm = (t % tm) * 64, bn = (t / tm) * BN;
   if (g.active && g.step >= g.active[b]) return;
   const int64_t per = (int64_t)n * n;
   const int64_t mb = (int64_t)b * 4 * per;
   const _Float16* X = (const _Float16*)g.X + mb;
   const _Float16* Y = (const _Float16*)g.Y + mb;
 
-  // X tile: [plane][k 32][row 64], 32-byte row segments swizzled by k (segment ^ (k & 3));
-  // Y tile: [plane][col 64][k 32], 16-byte k chunks swizzled by the column (chunk ^ ((col >> 1) & 3))
+  // X tile: [plane][k 32][row 64], 32-byte row segments swizzled by k (segment ^ (k & 3) ^
+  // ((k >> 3) & 3); the k >> 3 term gives the transpose reads of neighbouring 16-lane groups,
+  // at k and k + 8, different segments, and halved the LDS bank-conflict cycles measured with
+  // k & 3 alone);
+  // Y tile: [plane][col BN][k 32], 16-byte k chunks swizzled by the column (chunk ^ ((col >> 1) & 3))
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * 32 * 64];
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * BN * 32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * (16 * JN);
   const int l16 = lane & 15, l4 = lane >> 4;
-  const bool wave_live = (bm + wm < n) && (bn + wn < n);
+  // column subtiles of this wave inside the matrix (n is a multiple of 32): 0, 2 or JN
+  const int jlive = (bm + wm < n) ? max(0, min(JN, (n - bn - wn) / 16)) : 0;
 
   // staging, the 64-byte group block (8 rows, 4 planes) per thread: X rows bm + 8 (t & 7) .. +7
-  // of column k0 + (t >> 3), Y column bn + (t >> 2) at rows k0 + 8 (t & 3) .. +7. Rows and
-  // columns past the matrix are clamped onto its last ones: they feed only output rows and
+  // of column k0 + (t >> 3); Y columns bn + (t >> 2) + 64 m at rows k0 + 8 (t & 3) .. +7. Rows
+  // and columns past the matrix are clamped onto its last ones: they feed only output rows and
   // columns that are not written.
   const int xk = tid >> 3, xr = 8 * (tid & 7);
   const _Float16* xp = X + split_index(n, min(bm + xr, n - 8), xk, 0);
-  const int xlds = xk * 64 + 16 * ((xr >> 4) ^ (xk & 3)) + (xr & 8);
-  const int ycol = tid >> 2, yq = tid & 3;
-  const _Float16* yp = Y + split_index(n, 8 * yq, min(bn + ycol, n - 1), 0);
-  const int ylds = ycol * 32 + 8 * (yq ^ ((ycol >> 1) & 3));
-  u4v rx[4], ry[4];
-  auto gload = [&](int k0) {
+  const int xlds = xk * 64 + 16 * ((xr >> 4) ^ (xk & 3) ^ ((xk >> 3) & 3)) + (xr & 8);
+  const int yq = tid & 3;
+  const _Float16* yp[YG];
+  int ylds[YG];
+  #pragma unroll
+  for (int m = 0; m < YG; m++) {
+    const int ycol = (tid >> 2) + 64 * m;
+    yp[m] = Y + split_index(n, 8 * yq, min(bn + ycol, n - 1), 0);
+    ylds[m] = ycol * 32 + 8 * (yq ^ ((ycol >> 1) & 3));
+  }
+  u4v rx[4], ry[YG][4];
+  auto gload = [&](int k0) __attribute__((always_inline)) {
     #pragma unroll
-    for (int p = 0; p < 4; p++) {
-      rx[p] = *(const u4v*)(xp + (int64_t)k0 * 4 * n + 8 * p);
-      ry[p] = *(const u4v*)(yp + 4 * k0 + 8 * p);
-    }
+    for (int p = 0; p < 4; p++) rx[p] = *(const u4v*)(xp + (int64_t)k0 * 4 * n + 8 * p);
+    #pragma unroll
+    for (int m = 0; m < YG; m++)
+      #pragma unroll
+      for (int p = 0; p < 4; p++) ry[m][p] = *(const u4v*)(yp[m] + 4 * k0 + 8 * p);
   };
 
-  f4v crp[2][2], crn[2][2], ci[2][2];
+  f4v crp[2][JN], crn[2][JN], ci[2][JN];
   #pragma unroll
   for (int i = 0; i < 2; i++)
     #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < JN; j++) {
       crp[i][j] = (f4v){0, 0, 0, 0};
       crn[i][j] = (f4v){0, 0, 0, 0};
       ci[i][j] = (f4v){0, 0, 0, 0};
@@ -85,13 +130,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // fragment offsets: A (transpose read, lane reads rows r0 + 4 (l16 & 3) .. +3 at column
   // 8 l4 + (l16 >> 2), and 4 columns further for the upper half), B (16 bytes at chunk l4)
   const int kl = 8 * l4 + (l16 >> 2);
-  int aoff[2], boff[2];
+  int aoff[2], boff[JN];
   #pragma unroll
   for (int i = 0; i < 2; i++) {
     const int r0 = wm + 16 * i;
-    aoff[i] = kl * 64 + 16 * ((r0 >> 4) ^ (kl & 3)) + 4 * (l16 & 3);
-    const int col = wn + 16 * i + l16;
-    boff[i] = col * 32 + 8 * (l4 ^ ((col >> 1) & 3));
+    aoff[i] = kl * 64 + 16 * ((r0 >> 4) ^ (kl & 3) ^ ((kl >> 3) & 3)) + 4 * (l16 & 3);
+  }
+  #pragma unroll
+  for (int j = 0; j < JN; j++) {
+    const int col = wn + 16 * j + l16;
+    boff[j] = col * 32 + 8 * (l4 ^ ((col >> 1) & 3));
   }
 
   // exponents and maxima of the operands (read before the loop: its barriers would hold
@@ -99,8 +147,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const float xmx = g.xmax[b], ymx = g.ymax[b];
   const int ey = g.ye[b];
   const int exy = g.xe[b] + ey;
-  // with M[0] = X, beta_0 X enters the accumulators (units 2^-(ex+ey)) from the X tile staged
-  // at k0 = the output column block: X[rows][cols] = As[k = col - k0][rows]
+  // with M[0] = X, beta_0 X enters the accumulators (units 2^-(ex+ey)) from the X tiles staged
+  // at k0 = each 32-column block of the output: X[rows][cols] = As[k = col - k0][rows]
   const bool mx0 = g.m0_is_x != 0;
   const float dm0 = mx0 ? (g.beta[0] / g.alpha) * ldexpf(1.0f, ey) : 0.0f;
   float mmx[2] = {0.f, 0.f};
@@ -110,21 +158,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (g.M[q]) { mmx[q] = g.mmax[q][b]; mex[q] = g.me[q][b]; }
 
   gload(0);
-  // A wave whose 32x32 quadrant lies past the matrix edge only stages its share of the
-  // tiles; the choice is made once, outside the loop, so that the live copy of the loop keeps
-  // its accumulators in place across iterations.
-  auto kloop = [&](auto live) {
+  // The number of live column subtiles of the wave (0, 2 or JN) is chosen once, outside the
+  // loop, so that each copy of the loop keeps its accumulators in place across iterations.
+  auto kloop = [&](auto live) __attribute__((always_inline)) {
+    constexpr int JL = decltype(live)::value;
     for (int k0 = 0; k0 < n; k0 += 32) {
       __syncthreads();
       #pragma unroll
-      for (int p = 0; p < 4; p++) {
-        *(u4v*)&As[p * 2048 + xlds] = rx[p];
-        *(u4v*)&Bs[p * 2048 + ylds] = ry[p];
-      }
+      for (int p = 0; p < 4; p++) *(u4v*)&As[p * 2048 + xlds] = rx[p];
+      #pragma unroll
+      for (int m = 0; m < YG; m++)
+        #pragma unroll
+        for (int p = 0; p < 4; p++) *(u4v*)&Bs[p * 32 * BN + ylds[m]] = ry[m][p];
       __syncthreads();
       if (k0 + 32 < n) gload(k0 + 32);
-      if constexpr (!decltype(live)::value) continue;
-      h8v a[2][2][2], bq[2][2][2];  // [block][re/im][high/low]
+      if constexpr (JL == 0) continue;
+      h8v a[2][2][2];  // [block][re/im][high/low]
       #pragma unroll
       for (int i = 0; i < 2; i++)
         #pragma unroll
@@ -135,13 +184,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const s4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i]]));
             const s4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i] + 256]));
             a[i][c][h] = __builtin_bit_cast(h8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
-            bq[i][c][h] = *(const h8v*)&Bs[P * 2048 + boff[i]];
           }
       #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int j = 0; j < JL; j++) {
+        h8v bq[2][2];
         #pragma unroll
-        for (int j = 0; j < 2; j++) {
-#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[j][cb][hb], acc, 0, 0, 0)
+        for (int c = 0; c < 2; c++)
+          #pragma unroll
+          for (int h = 0; h < 2; h++) bq[c][h] = *(const h8v*)&Bs[(2 * c + h) * 32 * BN + boff[j]];
+        #pragma unroll
+        for (int i = 0; i < 2; i++) {
+#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[cb][hb], acc, 0, 0, 0)
           // smallest terms first
           MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
           MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
@@ -149,28 +202,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
 #undef MM
         }
-      if (mx0 && k0 == bn + wn) {
+      }
+      if (mx0) {
         #pragma unroll
-        for (int i = 0; i < 2; i++)
-          #pragma unroll
-          for (int j = 0; j < 2; j++) {
-            const int kk = 16 * j + l16, r = wm + 16 * i + 4 * l4;
-            const int o = kk * 64 + 16 * ((r >> 4) ^ (kk & 3)) + (r & 15);
-            const h4v p0 = *(const h4v*)&As[o], p1 = *(const h4v*)&As[2048 + o];
-            const h4v p2 = *(const h4v*)&As[4096 + o], p3 = *(const h4v*)&As[6144 + o];
+        for (int hb = 0; hb < JL / 2; hb++)
+          if (k0 == bn + wn + 32 * hb) {
             #pragma unroll
-            for (int e = 0; e < 4; e++) {
-              crp[i][j][e] += dm0 * ((float)p0[e] + (float)p1[e]);
-              ci[i][j][e] += dm0 * ((float)p2[e] + (float)p3[e]);
-            }
+            for (int i = 0; i < 2; i++)
+              #pragma unroll
+              for (int jj = 0; jj < 2; jj++) {
+                const int j = 2 * hb + jj;
+                const int kk = 16 * jj + l16, r = wm + 16 * i + 4 * l4;
+                const int o = kk * 64 + 16 * ((r >> 4) ^ (kk & 3) ^ ((kk >> 3) & 3)) + (r & 15);
+                const h4v p0 = *(const h4v*)&As[o], p1 = *(const h4v*)&As[2048 + o];
+                const h4v p2 = *(const h4v*)&As[4096 + o], p3 = *(const h4v*)&As[6144 + o];
+                #pragma unroll
+                for (int e = 0; e < 4; e++) {
+                  crp[i][j][e] += dm0 * ((float)p0[e] + (float)p1[e]);
+                  ci[i][j][e] += dm0 * ((float)p2[e] + (float)p3[e]);
+                }
+              }
           }
       }
     }
   };
-  if (wave_live)
-    kloop(std::true_type{});
+  if (jlive == JN)
+    kloop(std::integral_constant<int, JN>{});
+  else if (jlive == 2 && JN > 2)
+    kloop(std::integral_constant<int, 2>{});
   else
-    kloop(std::false_type{});
+    kloop(std::integral_constant<int, 0>{});
 
   // output exponents from bounds on the entries (the same in every workgroup of the matrix)
   const float xy = (float)n * xmx * ymx;
@@ -186,7 +247,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     g.ce[b] = ec;
     if (g.C2) g.ce2[b] = ec2;
   }
-  if (!wave_live) return;
+  if (jlive == 0) return;
   // the accumulators carry 2^(ex + ey); the outputs are stored times 2^ec
   const float fa = g.alpha * ldexpf(1.0f, ec - exy), fa2 = g.alpha2 * ldexpf(1.0f, ec2 - exy);
   const float fi = g.betaI * ldexpf(1.0f, ec), fi2 = g.betaI2 * ldexpf(1.0f, ec2);
@@ -203,71 +264,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const float cunscale = ldexpf(1.0f, -ec);
   float vmax = 0.0f, vmax2 = 0.0f;
   // each lane holds rows row0 .. row0+3 of one column: 8 contiguous bytes per plane, the four
-  // planes 16 bytes apart in one 64-byte group block. The lane pair l4 = 2m, 2m+1 holds the
-  // whole group (rows 0-3 and 4-7) and trades halves, so that the even lane stores the real
-  // planes and the odd lane the imaginary planes as 16-byte pieces.
+  // planes 16 bytes apart in one 64-byte group block (lane pairs l, l ^ 16 share a group)
   const bool odd = (l4 & 1) != 0;
-  auto store_split = [&](_Float16* dst, const float (&re)[4], const float (&im)[4], float& mx) {
-    h4v o0, o1, o2, o3;
+  auto store_split = [&](_Float16* dst, const float (&re)[4], const float (&im)[4], float& mx) __attribute__((always_inline)) {
+    store_split4(dst, re, im, odd);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      _Float16 h, l;
-      split2(re[e], h, l); o0[e] = h; o1[e] = l;
-      split2(im[e], h, l); o2[e] = h; o3[e] = l;
-      mx = nmax(mx, nmax(fabsf(re[e]), fabsf(im[e])));
-    }
-    // the even lane sends its imaginary rows 0-3, the odd lane its real rows 4-7
-    const u2v s0 = __builtin_bit_cast(u2v, odd ? o0 : o2), s1 = __builtin_bit_cast(u2v, odd ? o1 : o3);
-    u2v r0, r1;
-    r0.x = __shfl_xor(s0.x, 16); r0.y = __shfl_xor(s0.y, 16);
-    r1.x = __shfl_xor(s1.x, 16); r1.y = __shfl_xor(s1.y, 16);
-    const h4v q0 = __builtin_bit_cast(h4v, r0), q1 = __builtin_bit_cast(h4v, r1);
-    const h8v w0 = odd ? __builtin_shufflevector(q0, o2, 0, 1, 2, 3, 4, 5, 6, 7)
-                       : __builtin_shufflevector(o0, q0, 0, 1, 2, 3, 4, 5, 6, 7);
-    const h8v w1 = odd ? __builtin_shufflevector(q1, o3, 0, 1, 2, 3, 4, 5, 6, 7)
-                       : __builtin_shufflevector(o1, q1, 0, 1, 2, 3, 4, 5, 6, 7);
-    // dst points at this lane's rows in plane 0; the group starts 4 rows earlier on odd lanes
-    _Float16* gb = dst - (odd ? 4 : 0) + (odd ? 16 : 0);
-    *(h8v*)gb = w0;
-    *(h8v*)(gb + 8) = w1;
+    for (int e = 0; e < 4; e++) mx = nmax(mx, nmax(fabsf(re[e]), fabsf(im[e])));
   };
+
+  // n is a multiple of 32: every lane of a live subtile holds valid rows and columns
+  auto epi = [&](auto live) __attribute__((always_inline)) {
+    constexpr int JL = decltype(live)::value;
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      // n is a multiple of 32: every lane of a live wave holds valid rows and columns
-      const int row0 = bm + wm + 16 * i + 4 * l4;
-      const int col = bn + wn + 16 * j + l16;
-      const int64_t o = split_index(n, row0, col, 0);
-      float vr[4], vi[4], wr[4], wi[4];
+      for (int j = 0; j < JL; j++) {
+        const int row0 = bm + wm + 16 * i + 4 * l4;
+        const int col = bn + wn + 16 * j + l16;
+        const int64_t o = split_index(n, row0, col, 0);
+        float vr[4], vi[4], wr[4], wi[4];
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
-        vr[e] = fa * pr; vi[e] = fa * pi;
-        wr[e] = fa2 * pr; wi[e] = fa2 * pi;
-        if (row0 + e == col) { vr[e] += fi; wr[e] += fi2; }
-      }
-#pragma unroll
-      for (int q = 0; q < 2; q++)
-        if (g.M[q] && !(q == 0 && mx0)) {
-          const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
-          const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
-          const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
-#pragma unroll
-          for (int e = 0; e < 4; e++) {
-            const float mr = (float)m0[e] + (float)m1[e], mi = (float)m2[e] + (float)m3[e];
-            vr[e] += fm[q] * mr; vi[e] += fm[q] * mi;
-            wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
-          }
+        for (int e = 0; e < 4; e++) {
+          const float pr = crp[i][j][e] - crn[i][j][e], pi = ci[i][j][e];
+          vr[e] = fa * pr; vi[e] = fa * pi;
+          wr[e] = fa2 * pr; wi[e] = fa2 * pi;
+          if (row0 + e == col) { vr[e] += fi; wr[e] += fi2; }
         }
-      store_split(C + o, vr, vi, vmax);
-      if (C2) store_split(C2 + o, wr, wi, vmax2);
-      if (Cf) {
-        float4* q = (float4*)(Cf + (int64_t)col * n + row0);
-        q[0] = make_float4(vr[0] * cunscale, vi[0] * cunscale, vr[1] * cunscale, vi[1] * cunscale);
-        q[1] = make_float4(vr[2] * cunscale, vi[2] * cunscale, vr[3] * cunscale, vi[3] * cunscale);
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+          if (g.M[q] && !(q == 0 && mx0)) {
+            const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
+            const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
+            const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              const float mr = (float)m0[e] + (float)m1[e], mi = (float)m2[e] + (float)m3[e];
+              vr[e] += fm[q] * mr; vi[e] += fm[q] * mi;
+              wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
+            }
+          }
+        store_split(C + o, vr, vi, vmax);
+        if (C2) store_split(C2 + o, wr, wi, vmax2);
+        if (Cf) {
+          float4* qf = (float4*)(Cf + (int64_t)col * n + row0);
+          qf[0] = make_float4(vr[0] * cunscale, vi[0] * cunscale, vr[1] * cunscale, vi[1] * cunscale);
+          qf[1] = make_float4(vr[2] * cunscale, vi[2] * cunscale, vr[3] * cunscale, vi[3] * cunscale);
+        }
       }
-    }
+  };
+  if (jlive == JN)
+    epi(std::integral_constant<int, JN>{});
+  else
+    epi(std::integral_constant<int, 2>{});
   // true maxima (the scale factors are powers of two)
   wave_max_to(vmax * ldexpf(1.0f, -ec), g.cmax + b);
   if (C2) wave_max_to(vmax2 * ldexpf(1.0f, -ec2), g.cmax2 + b);
@@ -277,8 +325,11 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
   SGemm g = gin;
   // the staged X tiles give beta_0 M_0 only where M_0 is X and no second output needs it
   g.m0_is_x = (g.M[0] == g.X && g.me[0] == g.xe && g.C2 == nullptr && g.alpha != 0.0f) ? 1 : 0;
-  const int tiles = (g.n + 63) / 64;
-  hipLaunchKernelGGL(k_sgemm, dim3(tiles * tiles * g.nb), dim3(256), 0, s, g);
+  // 32 x 32 per wave: the wider 32 x 64 wave tile (JN = 4) needs two fewer waves per SIMD, which
+  // costs more than its 25% saving in LDS reads per MFMA gains (measured at n = 192, 224, 256)
+  constexpr int JN = 2;
+  const int tm = (g.n + 63) / 64, tn = (g.n + 32 * JN - 1) / (32 * JN);
+  hipLaunchKernelGGL(k_sgemm<JN>, dim3(tm * tn * g.nb), dim3(256), 0, s, g);
 }
 
 // Mixed-form assembly for the split path: A = i pre DynMat (gvectors.f90:1123-1173) in FP32,
@@ -295,8 +346,8 @@ __device__ __forceinline__ int as_swz(int row, int chunk) { return chunk ^ ((row
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* __restrict__ kn,
-                      double mlambda, float dynupz, double dz, float lutmax, float2* __restrict__ A32,
-                      double* __restrict__ colpart, float* __restrict__ amax) {
+                      double mlambda, float dynupz, double dz, float lutmax, uint16_t* __restrict__ As_out,
+                      int32_t* __restrict__ aexp, double* __restrict__ colpart, float* __restrict__ amax) {
   const int npad = B.npad, tiles = (npad + 63) >> 6;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
@@ -466,17 +517,38 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   const double half = 0.5 * mlambda;
   const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
   const double wsc = ldexp(1.0, -(ea + eb));
-  float2* Ab = A32 + (int64_t)b * npad * npad;
+  // exponent of A in split form, from a bound on its entries (the same in every workgroup of
+  // the matrix): off the diagonal |U| + (lambda/2) |W|, on it 2 |s_g| / lambda + (lambda/2) sum_w
+  // |U|^2 / |s_w|, and |dynupz|; |W| and the weak sum are at most nw 2 m^2 max 1/|s_w| with m the
+  // largest |re|, |im| of the LUT
+  double sgm = 0.0;
+  for (int a = tid; a < n; a += 256) sgm = fmax(sgm, fabs((double)ssg[a]));
+  for (int o = 32; o > 0; o >>= 1) sgm = fmax(sgm, __shfl_xor(sgm, o));
+  __syncthreads();
+  if (lane == 0) dps[3][wave] = sgm;
+  __syncthreads();
+  sgm = fmax(fmax(dps[3][0], dps[3][1]), fmax(dps[3][2], dps[3][3]));
+  const double wbound = 2.0 * (double)nw * (double)lutmax * (double)lutmax * (double)mi;
+  const double abound = 1.01 * pre * fmax(fmax((double)lutmax + half * wbound, 2.0 * sgm / mlambda + half * wbound),
+                                          fabs((double)dynupz));
+  const int eA = f16_scale_exp((float)abound);
+  if (t == 0 && tid == 0) aexp[b] = eA;
+  const float sA = ldexpf(1.0f, eA);
+  _Float16* Ab = (_Float16*)As_out + (int64_t)b * 4 * npad * npad;
+  const bool odd = (l4 & 1) != 0;
   float fmx = 0.0f;
   double csum[2] = {0.0, 0.0};
   #pragma unroll
   for (int i = 0; i < 2; i++)
     #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < 2; j++) {
+      const int cc = wn + 16 * j + l16, gj = bn + cc;
+      const int row0 = bm + wm + 16 * i + 4 * l4;
+      float2 f[4];
       #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int rr = wm + 16 * i + 4 * l4 + e, cc = wn + 16 * j + l16;
-        const int gi = bm + rr, gj = bn + cc;
+        const int rr = wm + 16 * i + 4 * l4 + e;
+        const int gi = bm + rr;
         double vr = 0.0, vi = 0.0;
         if (gi < n && gj < n) {
           if (gi != gj) {
@@ -490,13 +562,22 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
             vi = (double)dynupz;
           }
         }
-        if (gi < npad && gj < npad) {
-          const float2 f = make_float2((float)(-pre * vi), (float)(pre * vr));
-          Ab[(int64_t)gj * npad + gi] = f;
-          csum[j] += (double)hypotf(f.x, f.y);
-          fmx = nmax(fmx, nmax(fabsf(f.x), fabsf(f.y)));
-        }
+        f[e] = make_float2((float)(-pre * vi), (float)(pre * vr));
       }
+      // npad is a multiple of 32: a wave's 32 x 32 quadrant is all inside the matrix or all
+      // outside (the lane pairs of store_split4 then act together)
+      if (bm + wm < npad && bn + wn < npad) {
+        float re[4], im[4];
+        #pragma unroll
+        for (int e = 0; e < 4; e++) {
+          re[e] = f[e].x * sA;
+          im[e] = f[e].y * sA;
+          csum[j] += (double)hypotf(f[e].x, f[e].y);
+          fmx = nmax(fmx, nmax(fabsf(f[e].x), fabsf(f[e].y)));
+        }
+        store_split4(Ab + split_index(npad, row0, gj, 0), re, im, odd);
+      }
+    }
   // column 1-norm partials of the tile: eight per column (two waves x four lane groups),
   // added in a fixed order
   #pragma unroll
@@ -512,29 +593,27 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
 
 void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax) {
+                           float lutmax, int32_t nwmax, uint16_t* As, int32_t* aexp, double* colpart,
+                           float* amax) {
   const int tiles = (B.npad + 63) / 64;
   const size_t lds = sizeof(int4) * (size_t)std::max(32, (nwmax + 31) & ~31);
   hipLaunchKernelGGL(k_assemble_split, dim3(tiles * tiles * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
-                     dynupz, dz, lutmax, A32, colpart, amax);
+                     dynupz, dz, lutmax, As, aexp, colpart, amax);
 }
 
 // 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
-// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its
-// exponent and max |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
-__global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __restrict__ A,
-                                                    const double* __restrict__ colpart,
-                                                    const float* __restrict__ amax, _Float16* __restrict__ Bs,
+// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s: in split form only the
+// exponent changes (exponent of A + s) and the max |entry| scales by 2^-s; one workgroup per
+// matrix
+__global__ __launch_bounds__(256) void k_norm_split(int npad, const double* __restrict__ colpart,
+                                                    const float* __restrict__ amax, const int* __restrict__ aexp,
                                                     double theta, int smax, int* __restrict__ sq,
                                                     int* __restrict__ err, int* __restrict__ bexp,
                                                     float* __restrict__ bmax) {
   const int b = blockIdx.x;
-  const int64_t per = (int64_t)npad * npad;
-  const float2* Ab = A + (int64_t)b * per;
   const int tiles = (npad + 63) / 64;  // row partials per column (64-row assembly tiles)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ double wnorm[4];
-  __shared__ int s_sh, e_sh;
   double cnorm = 0.0;
   for (int j = tid; j < npad; j += 256) {
     double sum = 0.0;
@@ -560,42 +639,17 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
       while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
       if (nrm > theta) atomicOr(err, 4);
     }
-    const float mb = ldexpf(m, -sc);
-    const int e = f16_scale_exp(mb);
-    s_sh = sc;
-    e_sh = e;
     sq[b] = sc;
-    bexp[b] = e;
-    bmax[b] = mb;
-  }
-  __syncthreads();
-  const float f = ldexpf(1.0f, e_sh - s_sh);
-  // one thread per 8-row group of a column: 64 contiguous bytes out
-  _Float16* B = Bs + (int64_t)b * 4 * per;
-  const int ng = npad >> 3;
-  for (int w = tid; w < npad * ng; w += 256) {
-    const int col = w / ng, r0 = 8 * (w % ng);
-    h8v o[4];
-    #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      const float2 v = Ab[(int64_t)col * npad + r0 + e];
-      _Float16 h, l;
-      split2(v.x * f, h, l);
-      o[0][e] = h; o[1][e] = l;
-      split2(v.y * f, h, l);
-      o[2][e] = h; o[3][e] = l;
-    }
-    h8v* dst = (h8v*)(B + split_index(npad, r0, col, 0));
-    #pragma unroll
-    for (int p = 0; p < 4; p++) dst[p] = o[p];
+    bexp[b] = aexp[b] + sc;
+    bmax[b] = ldexpf(m, -sc);
   }
 }
 
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
-                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax) {
-  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, colpart, amax, (_Float16*)Bs, theta, smax,
-                     sq, err, bexp, bmax);
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,
+                       const int32_t* aexp, double theta, int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp,
+                       float* bmax) {
+  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, colpart, amax, aexp, theta, smax, sq, err,
+                     bexp, bmax);
 }
 
 }  // namespace ctem
