@@ -113,10 +113,10 @@ struct Crystal {
 // Work space of ctem_plan_run and ctem_plan_adjoint, shared by the plans of a context (calls
 // on one context are sequential); it grows to the largest chunk run and is kept.
 struct Work {
-  DBuf A, W, Psi, sq, dirs, partial, wmax, colsum;
+  DBuf A, W, Psi, sq, dirs, partial, wmax, colsum, order;
   DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
   void release() {
-    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
+    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &order, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
                     &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig})
       b->release();
   }
@@ -805,9 +805,9 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // alternate between 5 (F0) and 4. wmax holds six FP32 maxima and six exponents per matrix,
 // then the max and the exponent of A.
 int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
-                         const float* amax, const int32_t* aexp, float* wmax, double2* Psi64, int64_t spsi, int nz,
-                         int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
-                         int depth_grid) {
+                         const float* amax, const int32_t* aexp, float* wmax, DBuf& order, double2* Psi64,
+                         int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n,
+                         std::vector<int32_t>& h_sq, int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
   uint16_t* S[6];
   float* mx[6];
@@ -827,6 +827,12 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
+  // matrices by squaring count, largest first: squaring i runs on the first nact[i] of them
+  std::vector<int32_t> ord(nb);
+  for (int b = 0; b < nb; b++) ord[b] = b;
+  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return h_sq[a] > h_sq[c]; });
+  int32_t rc0;
+  if ((rc0 = upload(order, ord.data(), ord.size(), s))) return rc0;
   auto launch = [&](const SGemm& g, double flops) -> int32_t {
     GemmTimer t{tp, s};
     t.begin();
@@ -890,6 +896,9 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     SGemm g = mk(out, x, x);
     add_m(g, 0, x, 2.f, 0.f);
     g.active = d_sq; g.step = i;
+    int nact = 0;
+    while (nact < nb && h_sq[ord[nact]] > i) nact++;
+    g.order = order.as<int32_t>(); g.nord = nact;
     g.Cf = Ff; g.f_sq = d_sq; g.f_at = i + 1;
     if ((rc = launch(g, fl))) return rc;
   }
@@ -988,8 +997,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       } else {
         if (split)
           rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
-                                (const int32_t*)(amax + nb), c->w.wmax.as<float>(), c->w.Psi.as<double2>(), spsi,
-                                p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+                                (const int32_t*)(amax + nb), c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(),
+                                spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
         else
           rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
                                 c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,

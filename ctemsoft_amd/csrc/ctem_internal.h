@@ -225,6 +225,8 @@ struct SGemm {
   float alpha2, betaI2, beta2[2];
   const int32_t* active; int32_t step;  // matrices with step >= active[b] are skipped
   int32_t nb;
+  // optional: process only the matrices order[0 .. nord-1] (then active is not consulted)
+  const int32_t* order; int32_t nord;
   int32_t m0_is_x;       // set by launch_sgemm: M[0] is X (its term is taken from the staged X tiles)
   // optional FP32 copy of C (column-major float2, n x n per matrix) for the matrices with
   // f_sq[b] == f_at: the last product of each matrix hands its result to the depth kernel

@@ -71,9 +71,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   const int n = g.n;
   const int tm = (n + 63) >> 6, tn = (n + BN - 1) / BN;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int b = lb / (tm * tn), t = lb % (tm * tn);
+  const int t = lb % (tm * tn);
+  const int b = g.order ? g.order[lb / (tm * tn)] : lb / (tm * tn);
   const int bm = (t % tm) * 64, bn = (t / tm) * BN;
-  if (g.active && g.step >= g.active[b]) return;
+  if (!g.order && g.active && g.step >= g.active[b]) return;
   const int64_t per = (int64_t)n * n;
   const int64_t mb = (int64_t)b * 4 * per;
   const _Float16* X = (const _Float16*)g.X + mb;
@@ -325,11 +326,13 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
   SGemm g = gin;
   // the staged X tiles give beta_0 M_0 only where M_0 is X and no second output needs it
   g.m0_is_x = (g.M[0] == g.X && g.me[0] == g.xe && g.C2 == nullptr && g.alpha != 0.0f) ? 1 : 0;
+  const int nmat = g.order ? g.nord : g.nb;
+  if (nmat <= 0) return;
   // 32 x 32 per wave: the wider 32 x 64 wave tile (JN = 4) needs two fewer waves per SIMD, which
   // costs more than its 25% saving in LDS reads per MFMA gains (measured at n = 192, 224, 256)
   constexpr int JN = 2;
   const int tm = (g.n + 63) / 64, tn = (g.n + 32 * JN - 1) / (32 * JN);
-  hipLaunchKernelGGL(k_sgemm<JN>, dim3(tm * tn * g.nb), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_sgemm<JN>, dim3(tm * tn * nmat), dim3(256), 0, s, g);
 }
 
 // Mixed-form assembly for the split path: A = i pre DynMat (gvectors.f90:1123-1173) in FP32,
