@@ -799,15 +799,14 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // The mixed form with both stages on CTEM_SPLIT_FP16X3: the same Taylor polynomial and
 // squarings as expm_depth_mixed, with every matrix after the assembly kept in split form
 // (ctem_split.hip), so that the products read their operands without converting them.
-// Buffers (nb matrices each, 8 bytes per entry): 0 L; 1 A as assembled in split form, which
-// is B = A 2^-s with its exponent raised by s; 2 B2; 3 P, later the FP32 copy of the final F
-// of every matrix (written by its last product) for the depth kernel; 4 R, then the squarings
-// alternate between 5 (F0) and 4. wmax holds six FP32 maxima and six exponents per matrix,
-// then the max and the exponent of A.
+// Buffers (nb matrices each, 8 bytes per entry): 0 A as assembled (FP32), later L; 1 B; 2 B2;
+// 3 P, later the FP32 copy of the final F of every matrix (written by its last product) for
+// the depth kernel; 4 R, then the squarings alternate between 5 (F0) and 4. wmax holds six
+// FP32 maxima and six exponents per matrix, then the max of A.
 int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
-                         const float* amax, const int32_t* aexp, float* wmax, DBuf& order, double2* Psi64,
-                         int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n,
-                         std::vector<int32_t>& h_sq, int depth_grid) {
+                         const float* amax, float* wmax, DBuf& order, double2* Psi64, int64_t spsi, int nz,
+                         int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
+                         int depth_grid) {
   const int64_t per = (int64_t)npad * npad;
   uint16_t* S[6];
   float* mx[6];
@@ -818,7 +817,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
-  launch_norm_split(s, npad, nb, colpart, amax, aexp, kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
+  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
@@ -971,9 +970,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
                          p->terms_square == CTEM_SPLIT_FP16X3;
       float* amax = nullptr;
       if (precision == CTEM_PREC_MIXED) {
-        // six maxima and six exponents per matrix for the products, then the max and the
-        // split-form exponent of A
-        if ((rc = c->w.wmax.ensure(14 * sizeof(float) * (size_t)nb))) return rc;
+        // six maxima and six exponents per matrix for the products, then the max of A
+        if ((rc = c->w.wmax.ensure(13 * sizeof(float) * (size_t)nb))) return rc;
         amax = c->w.wmax.as<float>() + 12 * (size_t)nb;
         HIPCHK(hipMemsetAsync(amax, 0, sizeof(float) * (size_t)nb, s));
       }
@@ -983,10 +981,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       else if (split) {
         int nwmax = 0;
         for (int bb = 0; bb < nb; bb++) nwmax = std::max(nwmax, p->h_nnw[kv.second[c0 + bb]]);
-        // A in split form into buffer 1 of expm_depth_split
         launch_assemble_split(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz,
-                              p->cr->lutmax, nwmax, (uint16_t*)(c->w.W.as<float2>() + per * nb),
-                              (int32_t*)(amax + nb), c->w.colsum.as<double>(), amax);
+                              p->cr->lutmax, nwmax, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax);
       }
       else
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
@@ -997,8 +993,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       } else {
         if (split)
           rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
-                                (const int32_t*)(amax + nb), c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(),
-                                spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+                                c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(), spsi, p->nz,
+                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
         else
           rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
                                 c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,

@@ -111,15 +111,15 @@ void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const De
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                        float2* A32);
-// the same for the split path, with the weak-beam sum on split-FP16 MFMA (64 x 64 tiles),
-// written in split form into As with the exponent aexp[b] taken from a bound on the entries;
-// also the column sums of |a| (of the FP32 values) per 64-row tile (colpart[b][tile row]
-// [column], FP64) and the max |re|, |im| per matrix into amax[b] (zeroed by the caller).
-// lutmax = max |re|, |im| over the LUT; nwmax = the largest weak-beam count of the bucket.
+// the same for the split path, with the weak-beam sum on split-FP16 MFMA (64 x 64 tiles);
+// also the column sums of |a| per 64-row tile (colpart[b][tile row][column], FP64) and the
+// max |re|, |im| per matrix into amax[b] (zeroed by the caller). lutmax = max |re|, |im| over
+// the LUT; nwmax = the largest weak-beam count of the bucket. (An exponent for writing A in
+// split form straight from here would have to come from a bound on its entries; on corundum
+// such bounds run many bits above the true maximum and cost the low planes their precision.)
 void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                           float lutmax, int32_t nwmax, uint16_t* As, int32_t* aexp, double* colpart,
-                           float* amax);
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax);
 // same matrix, unscaled (DynMat itself), for parity checks
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D);
@@ -233,12 +233,12 @@ struct SGemm {
   float2* Cf; const int32_t* f_sq; int32_t f_at;
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
-// from the column-sum partials, max |entry| and exponent of launch_assemble_split: ||A||_1 per
-// matrix, squaring count s, and B = A 2^-s as the same split data with exponent aexp + s and
-// max |entry| amax 2^-s
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,
-                       const int32_t* aexp, double theta, int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp,
-                       float* bmax);
+// FP32 matrices A (as assembled, with the column-sum partials and max |entry| of
+// launch_assemble_split) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in split form
+// into Bs with its exponent (from the exact max) and max |entry|
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
+                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
+                       int32_t* bexp, float* bmax);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

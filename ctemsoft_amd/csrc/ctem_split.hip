@@ -349,8 +349,8 @@ __device__ __forceinline__ int as_swz(int row, int chunk) { return chunk ^ ((row
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* __restrict__ kn,
-                      double mlambda, float dynupz, double dz, float lutmax, uint16_t* __restrict__ As_out,
-                      int32_t* __restrict__ aexp, double* __restrict__ colpart, float* __restrict__ amax) {
+                      double mlambda, float dynupz, double dz, float lutmax, float2* __restrict__ A32,
+                      double* __restrict__ colpart, float* __restrict__ amax) {
   const int npad = B.npad, tiles = (npad + 63) >> 6;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
@@ -520,25 +520,7 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   const double half = 0.5 * mlambda;
   const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
   const double wsc = ldexp(1.0, -(ea + eb));
-  // exponent of A in split form, from a bound on its entries (the same in every workgroup of
-  // the matrix): off the diagonal |U| + (lambda/2) |W|, on it 2 |s_g| / lambda + (lambda/2) sum_w
-  // |U|^2 / |s_w|, and |dynupz|; |W| and the weak sum are at most nw 2 m^2 max 1/|s_w| with m the
-  // largest |re|, |im| of the LUT
-  double sgm = 0.0;
-  for (int a = tid; a < n; a += 256) sgm = fmax(sgm, fabs((double)ssg[a]));
-  for (int o = 32; o > 0; o >>= 1) sgm = fmax(sgm, __shfl_xor(sgm, o));
-  __syncthreads();
-  if (lane == 0) dps[3][wave] = sgm;
-  __syncthreads();
-  sgm = fmax(fmax(dps[3][0], dps[3][1]), fmax(dps[3][2], dps[3][3]));
-  const double wbound = 2.0 * (double)nw * (double)lutmax * (double)lutmax * (double)mi;
-  const double abound = 1.01 * pre * fmax(fmax((double)lutmax + half * wbound, 2.0 * sgm / mlambda + half * wbound),
-                                          fabs((double)dynupz));
-  const int eA = f16_scale_exp((float)abound);
-  if (t == 0 && tid == 0) aexp[b] = eA;
-  const float sA = ldexpf(1.0f, eA);
-  _Float16* Ab = (_Float16*)As_out + (int64_t)b * 4 * npad * npad;
-  const bool odd = (l4 & 1) != 0;
+  float2* Ab = A32 + (int64_t)b * npad * npad;
   float fmx = 0.0f;
   double csum[2] = {0.0, 0.0};
   #pragma unroll
@@ -567,18 +549,16 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
         }
         f[e] = make_float2((float)(-pre * vi), (float)(pre * vr));
       }
-      // npad is a multiple of 32: a wave's 32 x 32 quadrant is all inside the matrix or all
-      // outside (the lane pairs of store_split4 then act together)
-      if (bm + wm < npad && bn + wn < npad) {
-        float re[4], im[4];
+      // npad is a multiple of 32: the four rows are all inside the matrix or all outside
+      if (row0 < npad && gj < npad) {
+        float4* q = (float4*)(Ab + (int64_t)gj * npad + row0);
+        q[0] = make_float4(f[0].x, f[0].y, f[1].x, f[1].y);
+        q[1] = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
         #pragma unroll
         for (int e = 0; e < 4; e++) {
-          re[e] = f[e].x * sA;
-          im[e] = f[e].y * sA;
           csum[j] += (double)hypotf(f[e].x, f[e].y);
           fmx = nmax(fmx, nmax(fabsf(f[e].x), fabsf(f[e].y)));
         }
-        store_split4(Ab + split_index(npad, row0, gj, 0), re, im, odd);
       }
     }
   // column 1-norm partials of the tile: eight per column (two waves x four lane groups),
@@ -596,27 +576,29 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
 
 void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                           float lutmax, int32_t nwmax, uint16_t* As, int32_t* aexp, double* colpart,
-                           float* amax) {
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax) {
   const int tiles = (B.npad + 63) / 64;
   const size_t lds = sizeof(int4) * (size_t)std::max(32, (nwmax + 31) & ~31);
   hipLaunchKernelGGL(k_assemble_split, dim3(tiles * tiles * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
-                     dynupz, dz, lutmax, As, aexp, colpart, amax);
+                     dynupz, dz, lutmax, A32, colpart, amax);
 }
 
 // 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
-// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s: in split form only the
-// exponent changes (exponent of A + s) and the max |entry| scales by 2^-s; one workgroup per
-// matrix
-__global__ __launch_bounds__(256) void k_norm_split(int npad, const double* __restrict__ colpart,
-                                                    const float* __restrict__ amax, const int* __restrict__ aexp,
+// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its
+// exponent and max |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
+__global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __restrict__ A,
+                                                    const double* __restrict__ colpart,
+                                                    const float* __restrict__ amax, _Float16* __restrict__ Bs,
                                                     double theta, int smax, int* __restrict__ sq,
                                                     int* __restrict__ err, int* __restrict__ bexp,
                                                     float* __restrict__ bmax) {
   const int b = blockIdx.x;
+  const int64_t per = (int64_t)npad * npad;
+  const float2* Ab = A + (int64_t)b * per;
   const int tiles = (npad + 63) / 64;  // row partials per column (64-row assembly tiles)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ double wnorm[4];
+  __shared__ int s_sh, e_sh;
   double cnorm = 0.0;
   for (int j = tid; j < npad; j += 256) {
     double sum = 0.0;
@@ -642,17 +624,42 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const double* __re
       while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
       if (nrm > theta) atomicOr(err, 4);
     }
+    const float mb = ldexpf(m, -sc);
+    const int e = f16_scale_exp(mb);
+    s_sh = sc;
+    e_sh = e;
     sq[b] = sc;
-    bexp[b] = aexp[b] + sc;
-    bmax[b] = ldexpf(m, -sc);
+    bexp[b] = e;
+    bmax[b] = mb;
+  }
+  __syncthreads();
+  const float f = ldexpf(1.0f, e_sh - s_sh);
+  // one thread per 8-row group of a column: 64 contiguous bytes out
+  _Float16* B = Bs + (int64_t)b * 4 * per;
+  const int ng = npad >> 3;
+  for (int w = tid; w < npad * ng; w += 256) {
+    const int col = w / ng, r0 = 8 * (w % ng);
+    h8v o[4];
+    #pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float2 v = Ab[(int64_t)col * npad + r0 + e];
+      _Float16 h, l;
+      split2(v.x * f, h, l);
+      o[0][e] = h; o[1][e] = l;
+      split2(v.y * f, h, l);
+      o[2][e] = h; o[3][e] = l;
+    }
+    h8v* dst = (h8v*)(B + split_index(npad, r0, col, 0));
+    #pragma unroll
+    for (int p = 0; p < 4; p++) dst[p] = o[p];
   }
 }
 
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,
-                       const int32_t* aexp, double theta, int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp,
-                       float* bmax) {
-  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, colpart, amax, aexp, theta, smax, sq, err,
-                     bexp, bmax);
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
+                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
+                       int32_t* bexp, float* bmax) {
+  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, colpart, amax, (_Float16*)Bs, theta, smax,
+                     sq, err, bexp, bmax);
 }
 
 }  // namespace ctem

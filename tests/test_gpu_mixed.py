@@ -20,10 +20,13 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("name,nsample", [("fe_c1w", 1024), ("ni_c0w", None), ("fe_c1", 2048), ("ni_c0", None),
-                                          ("al2o3_c2w", 512), ("al2o3_c2", 2048), ("fe_c3w_10.0kV", 512),
+@pytest.mark.parametrize("name,nsample", [("fe_c1w", None), ("ni_c0w", None), ("fe_c1", 2048), ("ni_c0", None),
+                                          ("al2o3_c2w", None), ("al2o3_c2", 2048), ("fe_c3w_10.0kV", 512),
                                           ("fe_c3w_20.0kV", 512)])
 def test_mixed_matches_fp64(ctx, name, nsample):
+    """All directions of the fe_c1w and al2o3_c2w fixtures (one hemisphere each), samples of the
+    rest. Directions whose FP64 result is not finite (the reference's Bethe blow-up, a handful
+    on corundum) must be the same in both forms and are left out of the error."""
     from ctemsoft_amd import ebsd
     fx = load_golden(name)
     # corundum: the full-box LUT (a reference-sized one does not hold every g-h of its list)
@@ -36,8 +39,12 @@ def test_mixed_matches_fp64(ctx, name, nsample):
     d64 = plan.dots().copy()
     plan.run("mixed")
     dmx = plan.dots().copy()
-    rel = np.abs(dmx - d64) / np.abs(d64)
-    print(f"{name}: {len(ks)} directions, mixed vs fp64 max rel {rel.max():.2e}, mean {rel.mean():.2e}", flush=True)
+    fin = np.isfinite(d64).all(axis=-1) if d64.ndim > 1 else np.isfinite(d64)
+    fin_m = np.isfinite(dmx).all(axis=-1) if dmx.ndim > 1 else np.isfinite(dmx)
+    assert np.array_equal(fin, fin_m)
+    rel = np.abs(dmx[fin] - d64[fin]) / np.abs(d64[fin])
+    print(f"{name}: {len(ks)} directions ({int((~fin).sum())} not finite), mixed vs fp64 max rel {rel.max():.2e}, "
+          f"mean {rel.mean():.2e}", flush=True)
     sys.stdout.flush()
     assert rel.max() <= TOL
     plan.close()
