@@ -364,8 +364,10 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   const int32_t* widx = V.widx + V.woff[dir];
   const float* wsg = V.wsg + V.woff[dir];
 
-  extern __shared__ int4 wk[];  // [nwpad] weak beams: h, k, l, bits of fl32(1/s_w)
-  __shared__ int rh[64][3], chh[64][3];
+  // LUT offsets are linear in (h, k, l): idx(g - h) = lin(g) - lin(h) - lin(lo), lin(h, k, l) =
+  // h + d0 (k + d1 l), the same integer as the column-major offset lut_get computes
+  extern __shared__ int2 wk[];  // [nwpad] weak beams: lin(w), bits of fl32(1/s_w)
+  __shared__ int rl[64], cl[64];  // lin of the tile's rows (minus lin(lo)) and columns
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * 64 * 32];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
   __shared__ float red[4];
@@ -376,12 +378,17 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   const bool wave_live = (bm + wm < n) && (bn + wn < n);
   const bool diag_tile = (ti == tj);
 
+  const float2* lut = L.lut;
+  const int d0 = L.dim[0], d1 = L.dim[1], ncell = L.ncell;
+  const int linlo = L.lo[0] + d0 * (L.lo[1] + d1 * L.lo[2]);
+  int32_t* lerr = L.err;
+  auto lin = [&](const int32_t* h) __attribute__((always_inline)) { return h[0] + d0 * (h[1] + d1 * h[2]); };
   if (tid < 64) {
     const int a = bm + tid;
-    for (int d = 0; d < 3; d++) rh[tid][d] = (a < n) ? R.hkl[3 * sidx[a] + d] : 0;
+    rl[tid] = ((a < n) ? lin(R.hkl + 3 * sidx[a]) : 0) - linlo;
   } else if (tid < 128) {
     const int c = bn + tid - 64;
-    for (int d = 0; d < 3; d++) chh[tid - 64][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
+    cl[tid - 64] = (c < n) ? lin(R.hkl + 3 * sidx[c]) : 0;
   }
   float mi = 0.0f;
   for (int w = tid; w < nwpad; w += 256) {
@@ -389,10 +396,10 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
       const int id = widx[w];
       const float sw = wsg[w];
       const float iv = (float)(1.0 / (double)sw);  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
-      wk[w] = make_int4(R.hkl[3 * id], R.hkl[3 * id + 1], R.hkl[3 * id + 2], __float_as_int(iv));
+      wk[w] = make_int2(lin(R.hkl + 3 * id), __float_as_int(iv));
       mi = nmax(mi, fabsf(iv));
     } else {
-      wk[w] = make_int4(0, 0, 0, 0);
+      wk[w] = make_int2(0, 0);
     }
   }
   for (int o = 32; o > 0; o >>= 1) mi = nmax(mi, __shfl_xor(mi, o));
@@ -401,12 +408,8 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   mi = nmax(nmax(red[0], red[1]), nmax(red[2], red[3]));
   const int ea = f16_scale_exp(lutmax), eb = f16_scale_exp(lutmax * mi);
   const float sa = ldexpf(1.0f, ea), sb = ldexpf(1.0f, eb);
-  // lut_get with the table's fields held in registers
-  const float2* lut = L.lut;
-  const int lo0 = L.lo[0], lo1 = L.lo[1], lo2 = L.lo[2], d0 = L.dim[0], d1 = L.dim[1], ncell = L.ncell;
-  int32_t* lerr = L.err;
-  auto U = [&](int h, int k, int l) __attribute__((always_inline)) -> float2 {
-    const int idx = (h - lo0) + d0 * ((k - lo1) + d1 * (l - lo2));
+  // lut_get on a linear offset: outside the allocation -> 0 and error bit 8
+  auto U = [&](int idx) __attribute__((always_inline)) -> float2 {
     if ((unsigned)idx >= (unsigned)ncell) {
       atomicOr(lerr, 8);
       return make_float2(0.0f, 0.0f);
@@ -417,17 +420,16 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   // gathers: row (or column) p = t & 63, weak beams w0 + pk .. +7
   const int pr = tid & 63, pk = 8 * (tid >> 6);
   const bool prow_ok = bm + pr < n, pcol_ok = bn + pr < n;
-  const int r0 = rh[pr][0], r1 = rh[pr][1], r2 = rh[pr][2];
-  const int c0 = chh[pr][0], c1 = chh[pr][1], c2 = chh[pr][2];
+  const int rlp = rl[pr], clp = cl[pr] + linlo;
   float2 ga[8], gb[8];
   double dpart = 0.0;
   auto gather = [&](int w0) __attribute__((always_inline)) {
     #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int w = w0 + pk + j;
-      const int4 q = wk[w];
-      ga[j] = (prow_ok && w < nw) ? U(r0 - q.x, r1 - q.y, r2 - q.z) : make_float2(0.f, 0.f);
-      gb[j] = (pcol_ok && w < nw) ? U(q.x - c0, q.y - c1, q.z - c2) : make_float2(0.f, 0.f);
+      const int q = wk[w].x;
+      ga[j] = (prow_ok && w < nw) ? U(rlp - q) : make_float2(0.f, 0.f);        // U(g_r - w)
+      gb[j] = (pcol_ok && w < nw) ? U(q - clp) : make_float2(0.f, 0.f);        // U(w - h_c)
     }
   };
   const int alds = pr * 32 + 8 * as_swz(pr, tid >> 6);
@@ -435,7 +437,7 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
     h8v oa[4], ob[4];
     #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const float iv = __int_as_float(wk[w0 + pk + j].w);
+      const float iv = __int_as_float(wk[w0 + pk + j].y);
       _Float16 h, l;
       split2(ga[j].x * sa, h, l); oa[0][j] = h; oa[1][j] = l;
       split2(ga[j].y * sa, h, l); oa[2][j] = h; oa[3][j] = l;
@@ -537,7 +539,7 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
         double vr = 0.0, vi = 0.0;
         if (gi < n && gj < n) {
           if (gi != gj) {
-            const float2 u = U(rh[rr][0] - chh[cc][0], rh[rr][1] - chh[cc][1], rh[rr][2] - chh[cc][2]);
+            const float2 u = U(rl[rr] - cl[cc]);
             const float wr = crp[i][j][e] - crn[i][j][e], wi = ci[i][j][e];
             vr = (double)u.x - half * ((double)wr * wsc);
             vi = (double)u.y - half * ((double)wi * wsc);
@@ -578,7 +580,7 @@ void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, co
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                            float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax) {
   const int tiles = (B.npad + 63) / 64;
-  const size_t lds = sizeof(int4) * (size_t)std::max(32, (nwmax + 31) & ~31);
+  const size_t lds = sizeof(int2) * (size_t)std::max(32, (nwmax + 31) & ~31);
   hipLaunchKernelGGL(k_assemble_split, dim3(tiles * tiles * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
                      dynupz, dz, lutmax, A32, colpart, amax);
 }
