@@ -7,7 +7,8 @@
 //   k_norm_scale   1-norm, squaring count and power-of-two scaling per matrix
 //   k_zgemm        batched complex GEMM on v_mfma_f64_16x16x4_f64 with a fused polynomial
 //                  epilogue (Taylor / Paterson-Stockmeyer steps)
-//   k_lgh_dots     Lgh = (dz/thick) Psi diag(lam) Psi^H and Re sum Lgh (.) S_p (CTEMEBSDmaster
+//   k_lgh_dots     Lgh = (dz/thick) Psi diag(lam) Psi^H (upper-triangle tiles, Lgh being
+//                  Hermitian) and Re sum Lgh (.) S_p (CTEMEBSDmaster
 //                  .f90:456, with Lgh as in :697-720) in fixed-order partial sums
 //   k_scatter      float32 svals to the Laue-equivalent Lambert pixels (:462-479)
 #include <hip/hip_runtime.h>
@@ -648,11 +649,15 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
                                                   int nz, const double* __restrict__ lam,
                                                   double dzt, double* __restrict__ partial,
                                                   int keep_dir, double2* __restrict__ keep_lgh) {
-  const int npad = B.npad, tiles = npad / 32;
+  const int npad = B.npad, tiles = npad / 32, ntri = tiles * (tiles + 1) / 2;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int b = lb / (tiles * tiles);
-  const int t = lb % (tiles * tiles);
-  const int ti = t % tiles, tj = t / tiles;
+  const int b = lb / ntri;
+  const int t = lb % ntri;
+  // Lgh is Hermitian: only tiles on or above the diagonal (ti <= tj), t = tj (tj + 1) / 2 + ti
+  int tj = 0;
+  while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+  const int ti = t - tj * (tj + 1) / 2;
+  const bool offd = (ti != tj);
   const int dir = B.dirs[b];
   const int n = V.nns[dir];
   const int32_t* sidx = V.sidx + V.soff[dir];
@@ -713,12 +718,24 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
     const int gi = ti * 32 + r, gj = tj * 32 + c;
     if (live && gi < n && gj < n) {
       const double lr = dzt * accr[q], li = dzt * acci[q];
-      const int64_t cell = nsite ? slut_index(L, rh[r][0] - ch[c][0], rh[r][1] - ch[c][1], rh[r][2] - ch[c][2]) : 0;
+      const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
+      const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
       for (int p = 0; p < nsite && p < 4; p++) {
         const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
         part[p] += lr * sv.x - li * sv.y;
       }
-      if (dir == keep_dir) keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);
+      if (offd) {
+        // the mirrored entry Lgh(h, g) = conj Lgh(g, h) meets S_p(h - g)
+        const int64_t cell2 = nsite ? slut_index(L, -dh, -dk, -dl) : 0;
+        for (int p = 0; p < nsite && p < 4; p++) {
+          const double2 sv = L.slut[(int64_t)p * L.sncell + cell2];
+          part[p] += lr * sv.x + li * sv.y;
+        }
+      }
+      if (dir == keep_dir) {
+        keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);
+        if (offd) keep_lgh[(int64_t)gj * n + gi] = make_double2(lr, -li);
+      }
     }
   }
   // block reduction in fixed order: wave shuffles, then waves in order
@@ -730,7 +747,7 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   __syncthreads();
   if (tid < nsite && tid < 4) {
     const double v = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
-    partial[((int64_t)b * tiles * tiles + t) * nsite + tid] = v;
+    partial[((int64_t)b * ntri + t) * nsite + tid] = v;
   }
 }
 
@@ -749,11 +766,11 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const DevLut& L, const double2* Psi, int64_t spsi, int32_t nz,
                      const double* lam, double dzt, double* partial, double* dots,
                      int32_t keep_dir, double2* keep_lgh) {
-  const int tiles = B.npad / 32;
-  hipLaunchKernelGGL(k_lgh_dots, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
+  const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
+  hipLaunchKernelGGL(k_lgh_dots, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
                      spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
   hipLaunchKernelGGL(k_reduce_dots, dim3((B.nb + 127) / 128), dim3(128), 0, s, B.nb,
-                     tiles * tiles, L.nsite, B.dirs, partial, dots);
+                     ntri, L.nsite, B.dirs, partial, dots);
 }
 
 // ------------------------------------------------------------------------------------------
