@@ -86,6 +86,10 @@ struct Crystal {
   float dynupz = 0.0f, sgcutoff = 0.0f;
   float nat = 0.0f;  // total orbit size, float(sum(nat))
   float lutmax = 0.0f;  // max |re|, |im| over the float32 LUT
+  // the LUT is inversion-symmetric (U(-g) = U(g), a centrosymmetric structure with the origin
+  // on an inversion centre) to within 2^-20 of its max: DynMat is then complex symmetric to
+  // far below the mixed form's storage resolution
+  bool centro = false;
   std::vector<int32_t> h_hkl;
 
   ~Crystal() {
@@ -155,6 +159,7 @@ struct ctem_plan {
   double t_total = 0, t_gemm = 0, f_gemm = 0;
   int32_t n_gemm = 0;
   int32_t terms_taylor = CTEM_SPLIT_FP16X3, terms_square = CTEM_SPLIT_FP16X3;  // mixed-form split schemes
+  int32_t symmetry = 1;  // split mixed form: symmetric products when the crystal's LUT allows
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
 };
@@ -308,6 +313,18 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
     lutf[i] = (float)x->lut[i];
     lutmax = std::max(lutmax, std::fabs(lutf[i]));
   }
+  // inversion symmetry of the LUT: a box symmetric about the origin and |U(-g) - U(g)| small
+  bool centro = true;
+  for (int d = 0; d < 3; d++) centro = centro && (x->lut_lo[d] == -(x->lut_lo[d] + x->lut_dim[d] - 1));
+  if (centro) {
+    double lmax = 0.0, dmax = 0.0;
+    for (int64_t i = 0; i < ncell; i++) {
+      const int64_t j = ncell - 1 - i;  // the cell of (-h, -k, -l) in a box symmetric about 0
+      lmax = std::max(lmax, std::max(std::fabs(x->lut[2 * i]), std::fabs(x->lut[2 * i + 1])));
+      dmax = std::max(dmax, std::max(std::fabs(x->lut[2 * i] - x->lut[2 * j]), std::fabs(x->lut[2 * i + 1] - x->lut[2 * j + 1])));
+    }
+    centro = !(dmax > std::ldexp(lmax, -20));  // NaN entries: not symmetric
+  }
   std::vector<double> slut, dslut;
   build_slut(x, slo, sdim, slut, dslut);
   int nat = 0;
@@ -334,6 +351,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   }
   cr->sncell = sncell;
   cr->lutmax = lutmax;
+  cr->centro = centro;
   cr->mlambda = x->mlambda;
   cr->dynupz = x->dynupz;
   cr->sgcutoff = x->sgcutoff;
@@ -806,7 +824,7 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
                          const float* amax, float* wmax, DBuf& order, double2* Psi64, int64_t spsi, int nz,
                          int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
-                         int depth_grid) {
+                         int depth_grid, bool sym) {
   const int64_t per = (int64_t)npad * npad;
   uint16_t* S[6];
   float* mx[6];
@@ -824,8 +842,11 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   HIPCHK(hipStreamSynchronize(s));
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  // flops of one complex product: 8 n^3, or 8 n n (n + 1) / 2 for the entries on and above the
+  // diagonal of a symmetric result
+  auto pflops = [&](int b) { return sym ? 4.0 * h_n[b] * h_n[b] * (h_n[b] + 1.0) : 8.0 * h_n[b] * h_n[b] * h_n[b]; };
   double n3 = 0.0;
-  for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
+  for (int b = 0; b < nb; b++) n3 += pflops(b);
   // matrices by squaring count, largest first: squaring i runs on the first nact[i] of them
   std::vector<int32_t> ord(nb);
   for (int b = 0; b < nb; b++) ord[b] = b;
@@ -849,6 +870,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     g.Y = S[y]; g.ye = ex[y]; g.ymax = mx[y];
     g.C = S[c]; g.ce = ex[c]; g.cmax = mx[c];
     g.alpha = 1.0f;
+    g.sym = sym ? 1 : 0;
     return g;
   };
   auto add_m = [&](SGemm& g, int q, int m, float beta, float beta2) {
@@ -860,7 +882,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     g.alpha2 = alpha2; g.betaI2 = betaI2;
   };
   int32_t rc;
-  const double f3 = 8.0 * n3;
+  const double f3 = n3;
   // B2 = B B, P = x2 B2 + x1 B
   {
     SGemm g = mk(2, 1, 1);
@@ -888,7 +910,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   const int buf[2] = {5, 4};
   for (int i = 0; i < smax; i++) {
     double fl = 0.0;
-    for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += 8.0 * h_n[b] * h_n[b] * h_n[b];
+    for (int b = 0; b < nb; b++) if (i < h_sq[b]) fl += pflops(b);
     const int x = buf[i & 1], out = buf[(i + 1) & 1];
     // the output buffer's maxima were those of two steps back
     HIPCHK(hipMemsetAsync(mx[out], 0, sizeof(float) * (size_t)nb, s));
@@ -994,7 +1016,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         if (split)
           rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
                                 c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(), spsi, p->nz,
-                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count,
+                                p->symmetry == 1 && p->cr->centro);
         else
           rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
                                 c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
@@ -1029,6 +1052,14 @@ extern "C" int32_t ctem_plan_set_mixed_terms(ctem_plan* p, int32_t taylor_terms,
     return fail(CTEM_E_ARG, "split scheme must be CTEM_SPLIT_BF16X3 (3), CTEM_SPLIT_BF16X6 (6) or CTEM_SPLIT_FP16X3 (16)");
   p->terms_taylor = taylor_terms;
   p->terms_square = square_terms;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_set_symmetry(ctem_plan* p, int32_t mode, int32_t* in_effect) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if (mode != 0 && mode != 1) return fail(CTEM_E_ARG, "symmetry mode must be 0 (off) or 1 (auto)");
+  p->symmetry = mode;
+  if (in_effect) *in_effect = (mode == 1 && p->cr->centro) ? 1 : 0;
   return CTEM_OK;
 }
 

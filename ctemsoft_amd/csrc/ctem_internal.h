@@ -231,6 +231,10 @@ struct SGemm {
   // optional FP32 copy of C (column-major float2, n x n per matrix) for the matrices with
   // f_sq[b] == f_at: the last product of each matrix hands its result to the depth kernel
   float2* Cf; const int32_t* f_sq; int32_t f_at;
+  // symmetric operands and result (X, Y, M and C polynomials in one complex-symmetric matrix):
+  // only the 64 x 64 tiles on or above the diagonal are computed, and each off-diagonal tile
+  // also stores its transpose
+  int32_t sym;
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
 // FP32 matrices A (as assembled, with the column-sum partials and max |entry| of

@@ -71,9 +71,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   const int n = g.n;
   const int tm = (n + 63) >> 6, tn = (n + BN - 1) / BN;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int t = lb % (tm * tn);
-  const int b = g.order ? g.order[lb / (tm * tn)] : lb / (tm * tn);
-  const int bm = (t % tm) * 64, bn = (t / tm) * BN;
+  // symmetric mode (BN = 64): tiles (ti, tj) with ti <= tj, t = tj (tj + 1) / 2 + ti
+  const int ntile = g.sym ? tm * (tm + 1) / 2 : tm * tn;
+  const int t = lb % ntile;
+  const int b = g.order ? g.order[lb / ntile] : lb / ntile;
+  int bm, bn;
+  if (g.sym) {
+    int tj = 0;
+    while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+    bm = (t - tj * (tj + 1) / 2) * 64;
+    bn = tj * 64;
+  } else {
+    bm = (t % tm) * 64;
+    bn = (t / tm) * BN;
+  }
+  const bool mirror = g.sym && bm != bn;
   if (!g.order && g.active && g.step >= g.active[b]) return;
   const int64_t per = (int64_t)n * n;
   const int64_t mb = (int64_t)b * 4 * per;
@@ -85,8 +97,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   // at k and k + 8, different segments, and halved the LDS bank-conflict cycles measured with
   // k & 3 alone);
   // Y tile: [plane][col BN][k 32], 16-byte k chunks swizzled by the column (chunk ^ ((col >> 1) & 3))
-  __shared__ __attribute__((aligned(16))) _Float16 As[4 * 32 * 64];
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * BN * 32];
+  // (one array: after the k loop it holds the transposed copy of an off-diagonal output tile)
+  __shared__ __attribute__((aligned(16))) _Float16 Ls[4 * 32 * 64 + 4 * BN * 32];
+  _Float16* const As = Ls;
+  _Float16* const Bs = Ls + 4 * 32 * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * (16 * JN);
   const int l16 = lane & 15, l4 = lane >> 4;
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     g.ce[b] = ec;
     if (g.C2) g.ce2[b] = ec2;
   }
-  if (jlive == 0) return;
+  if (jlive == 0 && !mirror) return;
   // the accumulators carry 2^(ex + ey); the outputs are stored times 2^ec
   const float fa = g.alpha * ldexpf(1.0f, ec - exy), fa2 = g.alpha2 * ldexpf(1.0f, ec2 - exy);
   const float fi = g.betaI * ldexpf(1.0f, ec), fi2 = g.betaI2 * ldexpf(1.0f, ec2);
@@ -315,11 +329,98 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   };
   if (jlive == JN)
     epi(std::integral_constant<int, JN>{});
-  else
+  else if (jlive == 2)
     epi(std::integral_constant<int, 2>{});
   // true maxima (the scale factors are powers of two)
   wave_max_to(vmax * ldexpf(1.0f, -ec), g.cmax + b);
   if (C2) wave_max_to(vmax2 * ldexpf(1.0f, -ec2), g.cmax2 + b);
+  if (!mirror) return;
+
+  // Transposed copy of the tile (rows bm.., columns bn..) into columns bm.., rows bn.., one
+  // output at a time through LDS. The values are formed again from the accumulators (and M).
+  // Split outputs: LDS column R (tile row) holds the 64 rows (tile columns) as in memory,
+  // 32 chunks of 16 bytes (8-row group, plane), chunk index XOR (R >> 2) & 3 within each group
+  // of four; FP32 output: LDS column R holds 64 float2, index XOR 16 ((R >> 2) & 3).
+  auto outvals = [&](int i, int j, int second, float (&re)[4], float (&im)[4]) __attribute__((always_inline)) {
+    const int row0 = bm + wm + 16 * i + 4 * l4;
+    const int col = bn + wn + 16 * j + l16;
+    const int64_t o = split_index(n, row0, col, 0);
+    const float a = second ? fa2 : fa;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      re[e] = a * (crp[i][j][e] - crn[i][j][e]);
+      im[e] = a * ci[i][j][e];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+      if (g.M[q] && !(q == 0 && mx0)) {
+        const float f = second ? fm2[q] : fm[q];
+        const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
+        const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
+        const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          re[e] += f * ((float)m0[e] + (float)m1[e]);
+          im[e] += f * ((float)m2[e] + (float)m3[e]);
+        }
+      }
+  };
+  auto mpass = [&](auto live, int kind) __attribute__((always_inline)) {  // 0: C, 1: C2, 2: Cf
+    constexpr int JL = decltype(live)::value;
+    __syncthreads();  // the k loop (or the previous pass) is done with LDS
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < JL; j++) {
+        float re[4], im[4];
+        outvals(i, j, kind == 1, re, im);
+        const int cc = wn + 16 * j + l16;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int R = wm + 16 * i + 4 * l4 + e, sw = (R >> 2) & 3;
+          if (kind == 2) {
+            float2* Lf = (float2*)Ls;
+            Lf[R * 64 + (cc ^ (16 * sw))] = make_float2(re[e] * cunscale, im[e] * cunscale);
+          } else {
+            _Float16 h[4];
+            split2(re[e], h[0], h[1]);
+            split2(im[e], h[2], h[3]);
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++) {
+              const int dw = ((((cc >> 3) * 4 + pl) * 4) + ((cc & 7) >> 1)) ^ (4 * sw);
+              Ls[R * 256 + 2 * dw + (cc & 1)] = h[pl];
+            }
+          }
+        }
+      }
+    __syncthreads();
+    const int cq = tid & 31;
+#pragma unroll
+    for (int ps = 0; ps < 8; ps++) {
+      const int R = 8 * ps + (tid >> 5), sw = (R >> 2) & 3;
+      if (bm + R >= n) continue;
+      if (kind == 2) {
+        if (bn + 2 * cq < n) {
+          const float2* Lf = (const float2*)Ls;
+          *(u4v*)(Cf + (int64_t)(bm + R) * n + bn + 2 * cq) = *(const u4v*)&Lf[R * 64 + ((2 * cq) ^ (16 * sw))];
+        }
+      } else if (bn + 8 * (cq >> 2) < n) {
+        _Float16* dst = (kind == 1 ? C2 : C) + split_index(n, bn + 8 * (cq >> 2), bm + R, 0) + 8 * (cq & 3);
+        *(u4v*)dst = *(const u4v*)&Ls[R * 256 + 2 * ((4 * cq) ^ (4 * sw))];
+      }
+    }
+  };
+  auto mirror_out = [&](int kind) __attribute__((always_inline)) {
+    if (jlive == JN)
+      mpass(std::integral_constant<int, JN>{}, kind);
+    else if (jlive == 2)
+      mpass(std::integral_constant<int, 2>{}, kind);
+    else
+      mpass(std::integral_constant<int, 0>{}, kind);
+  };
+  mirror_out(0);
+  if (C2) mirror_out(1);
+  if (Cf) mirror_out(2);
 }
 
 void launch_sgemm(hipStream_t s, const SGemm& gin) {
@@ -332,7 +433,8 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
   // costs more than its 25% saving in LDS reads per MFMA gains (measured at n = 192, 224, 256)
   constexpr int JN = 2;
   const int tm = (g.n + 63) / 64, tn = (g.n + 32 * JN - 1) / (32 * JN);
-  hipLaunchKernelGGL(k_sgemm<JN>, dim3(tm * tn * nmat), dim3(256), 0, s, g);
+  const int ntile = g.sym ? tm * (tm + 1) / 2 : tm * tn;  // symmetric mode needs square tiles (JN = 2)
+  hipLaunchKernelGGL(k_sgemm<JN>, dim3(ntile * nmat), dim3(256), 0, s, g);
 }
 
 // Mixed-form assembly for the split path: A = i pre DynMat (gvectors.f90:1123-1173) in FP32,

@@ -32,7 +32,7 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
            "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
-           "ctem_plan_adjoint"]
+           "ctem_plan_set_symmetry", "ctem_plan_adjoint"]
 
 
 class CtemError(RuntimeError):
@@ -73,6 +73,7 @@ def lib():
             "ctem_plan_get_svals": [P, P], "ctem_plan_get_lgh": [P, P, I32, P],
             "ctem_plan_dynmat": [P, I32, P, I32, P], "ctem_plan_scatter": [P, P, I32, I32, I32, P],
             "ctem_plan_timing": [P, P, P, P, P], "ctem_plan_set_mixed_terms": [P, I32, I32],
+            "ctem_plan_set_symmetry": [P, I32, P],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
             "ctem_plan_adjoint": [P, P, P, P],
         }
@@ -274,6 +275,13 @@ class Plan:
     def set_mixed_terms(self, taylor=16, square=16):
         """Split schemes of the mixed form: 3 = BF16x3, 6 = BF16x6, 16 = FP16x3 (default)."""
         _check(lib().ctem_plan_set_mixed_terms(self._h, int(taylor), int(square)))
+
+    def set_symmetry(self, mode=1):
+        """Symmetric products in the split mixed form: 1 = when the crystal's structure-factor
+        table is inversion-symmetric (default), 0 = never. Returns whether they are in effect."""
+        on = C.c_int32()
+        _check(lib().ctem_plan_set_symmetry(self._h, int(mode), C.byref(on)))
+        return bool(on.value)
 
     def run(self, precision="fp64", keep_lgh=-1):
         _check(lib().ctem_plan_run(self._h, PREC[precision], int(keep_lgh)))

@@ -134,6 +134,13 @@ CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_
 /* Split schemes (CTEM_SPLIT_*) of the mixed form for the Taylor products (on the scaled
  * matrix, |B| <= 1/4) and for the squarings; default CTEM_SPLIT_FP16X3 for both. */
 CTEM_API int32_t ctem_plan_set_mixed_terms(ctem_plan* plan, int32_t taylor_terms, int32_t square_terms);
+/* No counterpart in the reference (a switch of the mixed form). mode 1 (default): when the
+ * crystal's structure-factor LUT is inversion-symmetric, U(-g) = U(g) to within 2^-20 of its max
+ * (centrosymmetric structure, origin on an inversion centre), the split mixed form treats the
+ * structure matrix as complex symmetric and computes each product's tiles on and above the
+ * diagonal only; mode 0: full products. *in_effect (may be NULL): 1 if the symmetric products
+ * will be used. The FP64 form always uses full products. */
+CTEM_API int32_t ctem_plan_set_symmetry(ctem_plan* plan, int32_t mode, int32_t* in_effect);
 
 /* Re sum_gh Lgh(g,h) Sgh(g,h,p) per direction and site, in double, before the reference's
  * float32 store (CTEMEBSDmaster.f90:456): dots[ik*nsite + p]. */

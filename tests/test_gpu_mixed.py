@@ -77,3 +77,52 @@ def test_split_scheme_rejects_unknown(ctx):
     with pytest.raises(ebsd.CtemError):
         plan.set_mixed_terms(4, 6)
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["fe_c1w", "al2o3_c2w"])
+def test_full_products_match_fp64(ctx, name):
+    """The split mixed form with symmetric products switched off (full products, the path of
+    crystals without inversion symmetry) on every direction, and the symmetric products
+    against it."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden(name)
+    st = ebsd.Setup.from_fixture(fx, load_golden("al2o3_c2") if name.startswith("al2o3") else None)
+    ctx.upload_crystal(st)
+    plan = ctx.plan(fx["kvec"], fx["kn"], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.run("fp64")
+    d64 = plan.dots().copy()
+    assert plan.set_symmetry(1)
+    plan.run("mixed")
+    dsym = plan.dots().copy()
+    assert not plan.set_symmetry(0)
+    plan.run("mixed")
+    dfull = plan.dots().copy()
+    fin = np.isfinite(d64)
+    assert np.array_equal(fin, np.isfinite(dfull)) and np.array_equal(fin, np.isfinite(dsym))
+    rel = np.abs(dfull[fin] - d64[fin]) / np.abs(d64[fin])
+    rs = np.abs(dsym[fin] - dfull[fin]) / np.abs(d64[fin])
+    print(f"{name}: full products vs fp64 max rel {rel.max():.2e}; symmetric vs full {rs.max():.2e}", flush=True)
+    assert rel.max() <= TOL
+    assert rs.max() <= TOL
+    plan.close()
+
+
+def test_symmetry_detection(ctx):
+    """Symmetric products are taken only for an inversion-symmetric LUT: an Fe LUT with one
+    entry of a +-g pair changed by 1e-3 of the max turns them off."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden("fe_c1w")
+    st = ebsd.Setup.from_fixture(fx)
+    ctx.upload_crystal(st)
+    plan = ctx.plan(fx["kvec"][:4], fx["kn"][:4], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    assert plan.set_symmetry(1)
+    assert not plan.set_symmetry(0)
+    plan.close()
+    lut = st.lut.copy()
+    cell = 3 * st.lut_dim[0] + 5  # a cell off the centre of the box
+    st.lut[2 * cell + 1] += 1e-3 * np.abs(lut).max()
+    ctx.upload_crystal(st)
+    plan = ctx.plan(fx["kvec"][:4], fx["kn"][:4], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    assert not plan.set_symmetry(1)
+    plan.close()
+    st.lut = lut
