@@ -232,8 +232,8 @@ struct SGemm {
   // f_sq[b] == f_at: the last product of each matrix hands its result to the depth kernel
   float2* Cf; const int32_t* f_sq; int32_t f_at;
   // symmetric operands and result (X, Y, M and C polynomials in one complex-symmetric matrix):
-  // only the 64 x 64 tiles on or above the diagonal are computed, and each off-diagonal tile
-  // also stores its transpose
+  // only the 64 x 64 tiles on or above the diagonal of C are computed and stored (C2 alike),
+  // and only those of X and Y are read (the rest through the symmetry); Cf is written in full
   int32_t sym;
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);

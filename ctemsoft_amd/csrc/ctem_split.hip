@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     bm = (t % tm) * 64;
     bn = (t / tm) * BN;
   }
-  const bool mirror = g.sym && bm != bn;
+  const bool offd = g.sym && bm != bn;
   if (!g.order && g.active && g.step >= g.active[b]) return;
   const int64_t per = (int64_t)n * n;
   const int64_t mb = (int64_t)b * 4 * per;
@@ -123,14 +123,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     yp[m] = Y + split_index(n, 8 * yq, min(bn + ycol, n - 1), 0);
     ylds[m] = ycol * 32 + 8 * (yq ^ ((ycol >> 1) & 3));
   }
+  // Symmetric mode (JN = 2): only the tiles on and above the diagonal are stored, so the
+  // chunks of X left of the diagonal tile (k0 < bm) are staged from X^T (the Y-style columns
+  // bm + (t >> 2), rows k0 + 8 (t & 3)) into the Y-style layout, and the chunks of Y below it
+  // (k0 >= bn + 64) from Y^T (column k0 + (t >> 3), rows bn + 8 (t & 7)) into the X-style
+  // layout; the fragment reads of such a chunk swap between the transpose read and the
+  // 16-byte read.
+  auto xtrans = [&](int k0) __attribute__((always_inline)) { return g.sym && k0 < bm; };
+  auto ytrans = [&](int k0) __attribute__((always_inline)) { return g.sym && k0 >= bn + 64; };
+  const _Float16* xtp = X + split_index(n, 8 * yq, min(bm + (tid >> 2), n - 1), 0);
+  const _Float16* ytp = Y + split_index(n, min(bn + xr, n - 8), xk, 0);
   u4v rx[4], ry[YG][4];
   auto gload = [&](int k0) __attribute__((always_inline)) {
+    const _Float16* xs = xtrans(k0) ? xtp + 4 * k0 : xp + (int64_t)k0 * 4 * n;
     #pragma unroll
-    for (int p = 0; p < 4; p++) rx[p] = *(const u4v*)(xp + (int64_t)k0 * 4 * n + 8 * p);
-    #pragma unroll
-    for (int m = 0; m < YG; m++)
+    for (int p = 0; p < 4; p++) rx[p] = *(const u4v*)(xs + 8 * p);
+    if constexpr (YG == 1) {
+      const _Float16* ys = ytrans(k0) ? ytp + (int64_t)k0 * 4 * n : yp[0] + 4 * k0;
       #pragma unroll
-      for (int p = 0; p < 4; p++) ry[m][p] = *(const u4v*)(yp[m] + 4 * k0 + 8 * p);
+      for (int p = 0; p < 4; p++) ry[0][p] = *(const u4v*)(ys + 8 * p);
+    } else {
+      #pragma unroll
+      for (int m = 0; m < YG; m++)
+        #pragma unroll
+        for (int p = 0; p < 4; p++) ry[m][p] = *(const u4v*)(yp[m] + 4 * k0 + 8 * p);
+    }
   };
 
   f4v crp[2][JN], crn[2][JN], ci[2][JN];
@@ -145,11 +162,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   // fragment offsets: A (transpose read, lane reads rows r0 + 4 (l16 & 3) .. +3 at column
   // 8 l4 + (l16 >> 2), and 4 columns further for the upper half), B (16 bytes at chunk l4)
   const int kl = 8 * l4 + (l16 >> 2);
-  int aoff[2], boff[JN];
+  int aoff[2], boff[JN], aofft[2], bofft[JN];  // (and the same reads on the swapped layouts)
   #pragma unroll
   for (int i = 0; i < 2; i++) {
-    const int r0 = wm + 16 * i;
+    const int r0 = wm + 16 * i, row = r0 + l16;
     aoff[i] = kl * 64 + 16 * ((r0 >> 4) ^ (kl & 3) ^ ((kl >> 3) & 3)) + 4 * (l16 & 3);
+    aofft[i] = row * 32 + 8 * (l4 ^ ((row >> 1) & 3));
+  }
+  #pragma unroll
+  for (int j = 0; j < JN; j++) {
+    const int c0 = wn + 16 * j;
+    bofft[j] = kl * 64 + 16 * ((c0 >> 4) ^ (kl & 3) ^ ((kl >> 3) & 3)) + 4 * (l16 & 3);
   }
   #pragma unroll
   for (int j = 0; j < JN; j++) {
@@ -178,35 +201,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   auto kloop = [&](auto live) __attribute__((always_inline)) {
     constexpr int JL = decltype(live)::value;
     for (int k0 = 0; k0 < n; k0 += 32) {
+      const bool xt = xtrans(k0), yt = ytrans(k0);
       __syncthreads();
-      #pragma unroll
-      for (int p = 0; p < 4; p++) *(u4v*)&As[p * 2048 + xlds] = rx[p];
-      #pragma unroll
-      for (int m = 0; m < YG; m++)
+      if constexpr (YG == 1) {
+        const int xo = xt ? ylds[0] : xlds, yo = yt ? xlds : ylds[0];
         #pragma unroll
-        for (int p = 0; p < 4; p++) *(u4v*)&Bs[p * 32 * BN + ylds[m]] = ry[m][p];
+        for (int p = 0; p < 4; p++) *(u4v*)&As[p * 2048 + xo] = rx[p];
+        #pragma unroll
+        for (int p = 0; p < 4; p++) *(u4v*)&Bs[p * 2048 + yo] = ry[0][p];
+      } else {
+        #pragma unroll
+        for (int p = 0; p < 4; p++) *(u4v*)&As[p * 2048 + xlds] = rx[p];
+        #pragma unroll
+        for (int m = 0; m < YG; m++)
+          #pragma unroll
+          for (int p = 0; p < 4; p++) *(u4v*)&Bs[p * 32 * BN + ylds[m]] = ry[m][p];
+      }
       __syncthreads();
       if (k0 + 32 < n) gload(k0 + 32);
       if constexpr (JL == 0) continue;
+      // 8 consecutive k of one row (or column) per lane: transpose reads on the X-style layout,
+      // 16-byte reads on the Y-style one
+      auto trread = [&](const _Float16* base) __attribute__((always_inline)) {
+        const s4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, base));
+        const s4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, base + 256));
+        return __builtin_bit_cast(h8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+      };
       h8v a[2][2][2];  // [block][re/im][high/low]
-      #pragma unroll
-      for (int i = 0; i < 2; i++)
+      if (!xt) {
         #pragma unroll
-        for (int c = 0; c < 2; c++)
+        for (int i = 0; i < 2; i++)
           #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            const int P = 2 * c + h;
-            const s4v lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i]]));
-            const s4v hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16(LDS_PTR(s4v, &As[P * 2048 + aoff[i] + 256]));
-            a[i][c][h] = __builtin_bit_cast(h8v, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
-          }
+          for (int P = 0; P < 4; P++) a[i][P >> 1][P & 1] = trread(&As[P * 2048 + aoff[i]]);
+      } else {
+        #pragma unroll
+        for (int i = 0; i < 2; i++)
+          #pragma unroll
+          for (int P = 0; P < 4; P++) a[i][P >> 1][P & 1] = *(const h8v*)&As[P * 2048 + aofft[i]];
+      }
       #pragma unroll
       for (int j = 0; j < JL; j++) {
         h8v bq[2][2];
-        #pragma unroll
-        for (int c = 0; c < 2; c++)
+        if (!yt) {
           #pragma unroll
-          for (int h = 0; h < 2; h++) bq[c][h] = *(const h8v*)&Bs[(2 * c + h) * 32 * BN + boff[j]];
+          for (int P = 0; P < 4; P++) bq[P >> 1][P & 1] = *(const h8v*)&Bs[P * 32 * BN + boff[j]];
+        } else {
+          #pragma unroll
+          for (int P = 0; P < 4; P++) bq[P >> 1][P & 1] = trread(&Bs[P * 32 * BN + bofft[j]]);
+        }
         #pragma unroll
         for (int i = 0; i < 2; i++) {
 #define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[cb][hb], acc, 0, 0, 0)
@@ -262,6 +304,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     g.ce[b] = ec;
     if (g.C2) g.ce2[b] = ec2;
   }
+  float2* Cf = (g.Cf && g.f_sq[b] == g.f_at) ? g.Cf + (int64_t)b * per : nullptr;
+  // the FP32 copy for the depth kernel is read in full: off-diagonal tiles also store their
+  // transpose there
+  const bool mirror = offd && Cf != nullptr;
   if (jlive == 0 && !mirror) return;
   // the accumulators carry 2^(ex + ey); the outputs are stored times 2^ec
   const float fa = g.alpha * ldexpf(1.0f, ec - exy), fa2 = g.alpha2 * ldexpf(1.0f, ec2 - exy);
@@ -275,9 +321,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     }
   _Float16* C = (_Float16*)g.C + mb;
   _Float16* C2 = g.C2 ? (_Float16*)g.C2 + mb : nullptr;
-  float2* Cf = (g.Cf && g.f_sq[b] == g.f_at) ? g.Cf + (int64_t)b * per : nullptr;
   const float cunscale = ldexpf(1.0f, -ec);
   float vmax = 0.0f, vmax2 = 0.0f;
+  // Transposed FP32 copy of an off-diagonal tile (rows bm.., columns bn..) into columns bm..,
+  // rows bn..: the lanes put their values (scaled like C) into LDS as float2, LDS column R (tile row)
+  // holding the 64 tile columns in 32 chunks of 16 bytes, chunk ch at
+  // ch ^ 2 ((ch >> 4) & 1) ^ (R & 1) ^ 8 ((R >> 2) & 1) (no bank conflicts for the puts, 32
+  // lanes at a time, nor for the 16-byte reads of eight consecutive rows per thread).
+  auto mchunk = [](int R, int ch) __attribute__((always_inline)) {
+    return ch ^ (2 * ((ch >> 4) & 1)) ^ (R & 1) ^ (8 * ((R >> 2) & 1));
+  };
+  auto mirror_put = [&](int i, int j, const float (&re)[4], const float (&im)[4]) __attribute__((always_inline)) {
+    float2* Lf = (float2*)Ls;
+    const int cc = wn + 16 * j + l16;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int R = wm + 16 * i + 4 * l4 + e;
+      Lf[R * 64 + 2 * mchunk(R, cc >> 1) + (cc & 1)] = make_float2(re[e], im[e]);
+    }
+  };
   // each lane holds rows row0 .. row0+3 of one column: 8 contiguous bytes per plane, the four
   // planes 16 bytes apart in one 64-byte group block (lane pairs l, l ^ 16 share a group)
   const bool odd = (l4 & 1) != 0;
@@ -325,8 +387,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
           qf[0] = make_float4(vr[0] * cunscale, vi[0] * cunscale, vr[1] * cunscale, vi[1] * cunscale);
           qf[1] = make_float4(vr[2] * cunscale, vi[2] * cunscale, vr[3] * cunscale, vi[3] * cunscale);
         }
+        if (mirror) mirror_put(i, j, vr, vi);
       }
   };
+  if (mirror) __syncthreads();  // every wave is done with the k loop's tiles
   if (jlive == JN)
     epi(std::integral_constant<int, JN>{});
   else if (jlive == 2)
@@ -336,91 +400,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   if (C2) wave_max_to(vmax2 * ldexpf(1.0f, -ec2), g.cmax2 + b);
   if (!mirror) return;
 
-  // Transposed copy of the tile (rows bm.., columns bn..) into columns bm.., rows bn.., one
-  // output at a time through LDS. The values are formed again from the accumulators (and M).
-  // Split outputs: LDS column R (tile row) holds the 64 rows (tile columns) as in memory,
-  // 32 chunks of 16 bytes (8-row group, plane), chunk index XOR (R >> 2) & 3 within each group
-  // of four; FP32 output: LDS column R holds 64 float2, index XOR 16 ((R >> 2) & 3).
-  auto outvals = [&](int i, int j, int second, float (&re)[4], float (&im)[4]) __attribute__((always_inline)) {
-    const int row0 = bm + wm + 16 * i + 4 * l4;
-    const int col = bn + wn + 16 * j + l16;
-    const int64_t o = split_index(n, row0, col, 0);
-    const float a = second ? fa2 : fa;
+  // the transposed tile from LDS: thread item (column R, rows 8 q .. 8 q + 7) reads 64 bytes
+  __syncthreads();
+  const float2* Lf = (const float2*)Ls;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      re[e] = a * (crp[i][j][e] - crn[i][j][e]);
-      im[e] = a * ci[i][j][e];
-    }
+  for (int p = 0; p < 2; p++) {
+    const int it = tid + 256 * p, R = it >> 3, q = it & 7;
+    if (bm + R >= n || bn + 8 * q >= n) continue;
+    f4v* df = (f4v*)(Cf + (int64_t)(bm + R) * n + bn + 8 * q);
 #pragma unroll
-    for (int q = 0; q < 2; q++)
-      if (g.M[q] && !(q == 0 && mx0)) {
-        const float f = second ? fm2[q] : fm[q];
-        const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
-        const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
-        const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          re[e] += f * ((float)m0[e] + (float)m1[e]);
-          im[e] += f * ((float)m2[e] + (float)m3[e]);
-        }
-      }
-  };
-  auto mpass = [&](auto live, int kind) __attribute__((always_inline)) {  // 0: C, 1: C2, 2: Cf
-    constexpr int JL = decltype(live)::value;
-    __syncthreads();  // the k loop (or the previous pass) is done with LDS
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < JL; j++) {
-        float re[4], im[4];
-        outvals(i, j, kind == 1, re, im);
-        const int cc = wn + 16 * j + l16;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-          const int R = wm + 16 * i + 4 * l4 + e, sw = (R >> 2) & 3;
-          if (kind == 2) {
-            float2* Lf = (float2*)Ls;
-            Lf[R * 64 + (cc ^ (16 * sw))] = make_float2(re[e] * cunscale, im[e] * cunscale);
-          } else {
-            _Float16 h[4];
-            split2(re[e], h[0], h[1]);
-            split2(im[e], h[2], h[3]);
-#pragma unroll
-            for (int pl = 0; pl < 4; pl++) {
-              const int dw = ((((cc >> 3) * 4 + pl) * 4) + ((cc & 7) >> 1)) ^ (4 * sw);
-              Ls[R * 256 + 2 * dw + (cc & 1)] = h[pl];
-            }
-          }
-        }
-      }
-    __syncthreads();
-    const int cq = tid & 31;
-#pragma unroll
-    for (int ps = 0; ps < 8; ps++) {
-      const int R = 8 * ps + (tid >> 5), sw = (R >> 2) & 3;
-      if (bm + R >= n) continue;
-      if (kind == 2) {
-        if (bn + 2 * cq < n) {
-          const float2* Lf = (const float2*)Ls;
-          *(u4v*)(Cf + (int64_t)(bm + R) * n + bn + 2 * cq) = *(const u4v*)&Lf[R * 64 + ((2 * cq) ^ (16 * sw))];
-        }
-      } else if (bn + 8 * (cq >> 2) < n) {
-        _Float16* dst = (kind == 1 ? C2 : C) + split_index(n, bn + 8 * (cq >> 2), bm + R, 0) + 8 * (cq & 3);
-        *(u4v*)dst = *(const u4v*)&Ls[R * 256 + 2 * ((4 * cq) ^ (4 * sw))];
-      }
-    }
-  };
-  auto mirror_out = [&](int kind) __attribute__((always_inline)) {
-    if (jlive == JN)
-      mpass(std::integral_constant<int, JN>{}, kind);
-    else if (jlive == 2)
-      mpass(std::integral_constant<int, 2>{}, kind);
-    else
-      mpass(std::integral_constant<int, 0>{}, kind);
-  };
-  mirror_out(0);
-  if (C2) mirror_out(1);
-  if (Cf) mirror_out(2);
+    for (int k = 0; k < 4; k++) df[k] = *(const f4v*)&Lf[R * 64 + 2 * mchunk(R, 4 * q + k)] * cunscale;
+  }
 }
 
 void launch_sgemm(hipStream_t s, const SGemm& gin) {
