@@ -835,7 +835,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
-  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1]);
+  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
@@ -990,6 +990,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
       const bool split = precision == CTEM_PREC_MIXED && p->terms_taylor == CTEM_SPLIT_FP16X3 &&
                          p->terms_square == CTEM_SPLIT_FP16X3;
+      // complex-symmetric DynMat (inversion-symmetric LUT): upper tiles only, split form alone
+      const bool sym = split && p->symmetry == 1 && p->cr->centro;
       float* amax = nullptr;
       if (precision == CTEM_PREC_MIXED) {
         // six maxima and six exponents per matrix for the products, then the max of A
@@ -1004,7 +1006,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         int nwmax = 0;
         for (int bb = 0; bb < nb; bb++) nwmax = std::max(nwmax, p->h_nnw[kv.second[c0 + bb]]);
         launch_assemble_split(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz,
-                              p->cr->lutmax, nwmax, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax);
+                              p->cr->lutmax, nwmax, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax, sym);
       }
       else
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
@@ -1016,8 +1018,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         if (split)
           rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
                                 c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(), spsi, p->nz,
-                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count,
-                                p->symmetry == 1 && p->cr->centro);
+                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count, sym);
         else
           rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
                                 c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,

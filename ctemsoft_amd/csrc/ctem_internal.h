@@ -119,7 +119,8 @@ void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const 
 // such bounds run many bits above the true maximum and cost the low planes their precision.)
 void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax);
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax,
+                           int32_t sym);
 // same matrix, unscaled (DynMat itself), for parity checks
 void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                    const DevLut& L, double mlambda, float dynupz, double2* D);
@@ -242,7 +243,7 @@ void launch_sgemm(hipStream_t s, const SGemm& g);
 // into Bs with its exponent (from the exact max) and max |entry|
 void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
                        const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax);
+                       int32_t* bexp, float* bmax, int32_t sym);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

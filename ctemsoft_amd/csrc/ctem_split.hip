@@ -436,17 +436,28 @@ void launch_sgemm(hipStream_t s, const SGemm& gin) {
 // planes [plane][row or column][32 weak beams] (16-byte chunks swizzled as in k_sgemm). Scales:
 // U by 2^ea from the LUT's max entry, U / s_w by 2^eb from that max over the direction's
 // smallest |s_w|. The diagonal sums sum_w |U(g_r - w)|^2 / s_w (gvectors.f90:1165) are taken
-// in FP64 during the gathers and added in a fixed order.
+// in FP64 during the gathers and added in a fixed order. With sym set (a complex-symmetric
+// DynMat) only the tiles on and above the diagonal are assembled; an off-diagonal tile then
+// also gives the column partials of its transpose (its row sums).
 __device__ __forceinline__ int as_swz(int row, int chunk) { return chunk ^ ((row >> 1) & 3); }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* __restrict__ kn,
                       double mlambda, float dynupz, double dz, float lutmax, float2* __restrict__ A32,
-                      double* __restrict__ colpart, float* __restrict__ amax) {
+                      double* __restrict__ colpart, float* __restrict__ amax, int sym) {
   const int npad = B.npad, tiles = (npad + 63) >> 6;
+  const int ntile = sym ? tiles * (tiles + 1) / 2 : tiles * tiles;
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int b = lb / (tiles * tiles), t = lb % (tiles * tiles);
-  const int ti = t % tiles, tj = t / tiles;
+  const int b = lb / ntile, t = lb % ntile;
+  int ti, tj;
+  if (sym) {  // t = tj (tj + 1) / 2 + ti, ti <= tj
+    tj = 0;
+    while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+    ti = t - tj * (tj + 1) / 2;
+  } else {
+    ti = t % tiles;
+    tj = t / tiles;
+  }
   const int bm = ti * 64, bn = tj * 64;
   const int dir = B.dirs[b];
   const int n = V.nns[dir], nw = V.nnw[dir];
@@ -463,7 +474,7 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * 64 * 32];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
   __shared__ float red[4];
-  __shared__ double dps[4][64], cs[8][64];
+  __shared__ double dps[4][64], cs[8][64], rs[2][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int l16 = lane & 15, l4 = lane >> 4;
@@ -575,27 +586,29 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
       __syncthreads();
       if (w0 + 32 < nwpad) gather(w0 + 32);
       if constexpr (!decltype(live)::value) continue;
-      h8v a[2][2][2], bq[2][2][2];  // [block][re/im][high/low]
+      h8v a[2][2][2], bq[2][2];  // [block][re/im][high/low]; B one column block at a time
       #pragma unroll
       for (int i = 0; i < 2; i++)
         #pragma unroll
         for (int c = 0; c < 2; c++)
           #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            a[i][c][h] = *(const h8v*)&As[(2 * c + h) * 2048 + aoff[i]];
-            bq[i][c][h] = *(const h8v*)&Bs[(2 * c + h) * 2048 + boff[i]];
-          }
+          for (int h = 0; h < 2; h++) a[i][c][h] = *(const h8v*)&As[(2 * c + h) * 2048 + aoff[i]];
       #pragma unroll
-      for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
         #pragma unroll
-        for (int j = 0; j < 2; j++) {
-#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[j][cb][hb], acc, 0, 0, 0)
+        for (int c = 0; c < 2; c++)
+          #pragma unroll
+          for (int h = 0; h < 2; h++) bq[c][h] = *(const h8v*)&Bs[(2 * c + h) * 2048 + boff[j]];
+        #pragma unroll
+        for (int i = 0; i < 2; i++) {
+#define MM(acc, ca, ha, cb, hb) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i][ca][ha], bq[cb][hb], acc, 0, 0, 0)
           MM(crp[i][j], 0, 1, 0, 0); MM(crp[i][j], 0, 0, 0, 1); MM(crp[i][j], 0, 0, 0, 0);
           MM(crn[i][j], 1, 1, 1, 0); MM(crn[i][j], 1, 0, 1, 1); MM(crn[i][j], 1, 0, 1, 0);
           MM(ci[i][j], 0, 1, 1, 0); MM(ci[i][j], 0, 0, 1, 1); MM(ci[i][j], 0, 0, 1, 0);
           MM(ci[i][j], 1, 1, 0, 0); MM(ci[i][j], 1, 0, 0, 1); MM(ci[i][j], 1, 0, 0, 0);
 #undef MM
         }
+      }
     }
   };
   if (wave_live)
@@ -617,8 +630,10 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
   float2* Ab = A32 + (int64_t)b * npad * npad;
   float fmx = 0.0f;
   double csum[2] = {0.0, 0.0};
+  const bool rows = sym && ti != tj;  // row sums wanted: column partials of the transpose
   #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < 2; i++) {
+    double rsum[4] = {0.0, 0.0, 0.0, 0.0};
     #pragma unroll
     for (int j = 0; j < 2; j++) {
       const int cc = wn + 16 * j + l16, gj = bn + cc;
@@ -650,11 +665,24 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
         q[1] = make_float4(f[2].x, f[2].y, f[3].x, f[3].y);
         #pragma unroll
         for (int e = 0; e < 4; e++) {
-          csum[j] += (double)hypotf(f[e].x, f[e].y);
+          const double m = (double)hypotf(f[e].x, f[e].y);
+          csum[j] += m;
+          rsum[e] += m;
           fmx = nmax(fmx, nmax(fabsf(f[e].x), fabsf(f[e].y)));
         }
       }
     }
+    if (rows) {
+      // the row sums over the wave's 32 columns: the 16 lanes of a row, in a fixed order
+      #pragma unroll
+      for (int e = 0; e < 4; e++) {
+        double v = rsum[e];
+        #pragma unroll
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 16);
+        if (l16 == 0) rs[wn >> 5][wm + 16 * i + 4 * l4 + e] = v;
+      }
+    }
+  }
   // column 1-norm partials of the tile: eight per column (two waves x four lane groups),
   // added in a fixed order
   #pragma unroll
@@ -665,16 +693,19 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
     for (int q = 0; q < 8; q++) d += cs[q][tid];
     colpart[((int64_t)b * tiles + ti) * npad + bn + tid] = d;
   }
+  // the row sums of the two waves of a row block: column partials of tile-row tj at columns bm..
+  if (rows && tid < 64 && bm + tid < npad) colpart[((int64_t)b * tiles + tj) * npad + bm + tid] = rs[0][tid] + rs[1][tid];
   wave_max_to(fmx, amax + b);
 }
 
 void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                            const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
-                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax) {
+                           float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax, int32_t sym) {
   const int tiles = (B.npad + 63) / 64;
+  const int ntile = sym ? tiles * (tiles + 1) / 2 : tiles * tiles;
   const size_t lds = sizeof(int2) * (size_t)std::max(32, (nwmax + 31) & ~31);
-  hipLaunchKernelGGL(k_assemble_split, dim3(tiles * tiles * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
-                     dynupz, dz, lutmax, A32, colpart, amax);
+  hipLaunchKernelGGL(k_assemble_split, dim3(ntile * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
+                     dynupz, dz, lutmax, A32, colpart, amax, sym);
 }
 
 // 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
@@ -685,7 +716,7 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
                                                     const float* __restrict__ amax, _Float16* __restrict__ Bs,
                                                     double theta, int smax, int* __restrict__ sq,
                                                     int* __restrict__ err, int* __restrict__ bexp,
-                                                    float* __restrict__ bmax) {
+                                                    float* __restrict__ bmax, int sym) {
   const int b = blockIdx.x;
   const int64_t per = (int64_t)npad * npad;
   const float2* Ab = A + (int64_t)b * per;
@@ -728,11 +759,13 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
   }
   __syncthreads();
   const float f = ldexpf(1.0f, e_sh - s_sh);
-  // one thread per 8-row group of a column: 64 contiguous bytes out
+  // one thread per 8-row group of a column: 64 contiguous bytes out (sym: only the groups of
+  // the 64 x 64 tiles on and above the diagonal, the only ones the symmetric products read)
   _Float16* B = Bs + (int64_t)b * 4 * per;
   const int ng = npad >> 3;
   for (int w = tid; w < npad * ng; w += 256) {
     const int col = w / ng, r0 = 8 * (w % ng);
+    if (sym && (r0 >> 6) > (col >> 6)) continue;
     h8v o[4];
     #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -751,9 +784,9 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
 
 void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
                        const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax) {
+                       int32_t* bexp, float* bmax, int32_t sym) {
   hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, colpart, amax, (_Float16*)Bs, theta, smax,
-                     sq, err, bexp, bmax);
+                     sq, err, bexp, bmax, sym);
 }
 
 }  // namespace ctem
