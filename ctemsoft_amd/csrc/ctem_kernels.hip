@@ -548,8 +548,10 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
 // 32 x 32 grid; thread (tr, tc) keeps the RT x RT block of rows tr*RT.. and columns tc*RT..
 // (RT = npad/32) in registers, or for npad = 224 five of its seven rows in registers and two
 // in LDS. Per step each thread forms its RT partial row sums in FP32 (RT terms each), the 32
-// threads of a row group (half a wave) add them with DPP lane moves, and the new vector
-// psi_{k+1} = E psi_k is formed and kept in FP64 in LDS (double-buffered) and written out.
+// threads of a row group (half a wave) add them with DPP lane moves (all RT rows together, so
+// that the lane moves of different rows interleave), and lane tc < RT of the group forms entry
+// tr*RT + tc of the new vector psi_{k+1} = E psi_k, which is kept in FP64 in LDS
+// (double-buffered, with an FP32 copy for the next step's products) and written out.
 // sum over the 32 lanes of each half wave, result in every lane: quad_perm [1,0,3,2] and
 // [2,3,0,1], row_ror 4 and 8 (DPP within a row of 16), then one exchange across rows
 __device__ __forceinline__ float half_wave_sum(float v) {
@@ -569,6 +571,7 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
   constexpr int LR = (RT == 7) ? 2 : 0;  // rows of the block kept in LDS
   constexpr int RR = RT - LR;            // rows kept in registers
   __shared__ double2 xs[2][NPAD];
+  __shared__ float2 xf32[2][NPAD];
   __shared__ float2 Elds[LR > 0 ? LR : 1][RT][1024];
   const int tid = threadIdx.x, tr = tid >> 5, tc = tid & 31;
   const int64_t per = (int64_t)NPAD * NPAD;
@@ -587,35 +590,48 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
       double2 v = make_double2((double)Eb[i].x, (double)Eb[i].y);
       if (add_identity && i == 0) v.x += 1.0;
       xs[0][i] = v;
+      xf32[0][i] = make_float2((float)v.x, (float)v.y);
       P[i] = v;
     }
     __syncthreads();
     for (int k = 1; k < nz; k++) {
       const int cur = (k - 1) & 1;
       float2 xf[RT];
-      for (int j = 0; j < RT; j++) {
-        const double2 v = xs[cur][tc * RT + j];
-        xf[j] = make_float2((float)v.x, (float)v.y);
-      }
+#pragma unroll
+      for (int j = 0; j < RT; j++) xf[j] = xf32[cur][tc * RT + j];
+      float ar[RT], ai[RT];
+#pragma unroll
       for (int i = 0; i < RT; i++) {
-        float ar = 0.f, ai = 0.f;
+        ar[i] = 0.f;
+        ai[i] = 0.f;
+#pragma unroll
         for (int j = 0; j < RT; j++) {
           float2 e;
           if (i < RR) e = er[i < RR ? i : 0][j];
           else e = Elds[LR > 0 ? i - RR : 0][j][tid];
-          ar = fmaf(e.x, xf[j].x, ar); ar = fmaf(-e.y, xf[j].y, ar);
-          ai = fmaf(e.x, xf[j].y, ai); ai = fmaf(e.y, xf[j].x, ai);
+          ar[i] = fmaf(e.x, xf[j].x, ar[i]); ar[i] = fmaf(-e.y, xf[j].y, ar[i]);
+          ai[i] = fmaf(e.x, xf[j].y, ai[i]); ai[i] = fmaf(e.y, xf[j].x, ai[i]);
         }
-        // the 32 partial sums of row r meet in FP32 (each holds RT terms); the recursion
-        // itself is carried in FP64
-        double sr = half_wave_sum(ar), si = half_wave_sum(ai);
-        if (tc == i) {
-          const int r = tr * RT + i;
-          if (add_identity) { sr += xs[cur][r].x; si += xs[cur][r].y; }
-          const double2 y = make_double2(sr, si);
-          xs[cur ^ 1][r] = y;
-          P[(int64_t)k * NPAD + r] = y;
-        }
+      }
+      // the 32 partial sums of row r meet in FP32 (each holds RT terms); the recursion
+      // itself is carried in FP64
+#pragma unroll
+      for (int i = 0; i < RT; i++) {
+        ar[i] = half_wave_sum(ar[i]);
+        ai[i] = half_wave_sum(ai[i]);
+      }
+      float sr = ar[0], si = ai[0];
+#pragma unroll
+      for (int i = 1; i < RT; i++)
+        if (tc == i) { sr = ar[i]; si = ai[i]; }
+      if (tc < RT) {
+        const int r = tr * RT + tc;
+        double yr = sr, yi = si;
+        if (add_identity) { yr += xs[cur][r].x; yi += xs[cur][r].y; }
+        const double2 y = make_double2(yr, yi);
+        xs[cur ^ 1][r] = y;
+        xf32[cur ^ 1][r] = make_float2((float)yr, (float)yi);
+        P[(int64_t)k * NPAD + r] = y;
       }
       __syncthreads();
     }
