@@ -548,19 +548,29 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
 // 32 x 32 grid; thread (tr, tc) keeps the RT x RT block of rows tr*RT.. and columns tc*RT..
 // (RT = npad/32) in registers, or for npad = 224 five of its seven rows in registers and two
 // in LDS. Per step each thread forms its RT partial row sums in FP32 (RT terms each), the 32
-// threads of a row group (half a wave) add them with DPP lane moves (all RT rows together, so
-// that the lane moves of different rows interleave), and lane tc < RT of the group forms entry
-// tr*RT + tc of the new vector psi_{k+1} = E psi_k, which is kept in FP64 in LDS
-// (double-buffered, with an FP32 copy for the next step's products) and written out.
-// sum over the 32 lanes of each half wave, result in every lane: quad_perm [1,0,3,2] and
-// [2,3,0,1], row_ror 4 and 8 (DPP within a row of 16), then one exchange across rows
-__device__ __forceinline__ float half_wave_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
-  v += __shfl_xor(v, 16, 32);
-  return v;
+// threads of a row group (half a wave) add them (a reduce-scatter of the 2 RT real and
+// imaginary sums within each 16-lane DPP row, then one exchange between the two rows), and lane
+// tc < RT of the group forms entry tr*RT + tc of the new vector psi_{k+1} = E psi_k, which is
+// kept in FP64 in LDS (double-buffered, with an FP32 copy for the next step's products) and
+// written out.
+// Reduce-scatter of 16 values over the 16 lanes of a DPP row (lanes 16 m .. 16 m + 15): at each
+// of four levels a lane keeps half of its values and adds its partner's copies of them (partners
+// at lane distance 8 by row_ror 8, mirrored within 8 lanes by row_half_mirror, then quad_perm
+// [2,3,0,1] and [1,0,3,2]); afterwards lane l holds the row's sum of value l % 16.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float row_reduce_scatter16(const float (&v)[16], int lane) {
+  const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
+  float a[8], c[4], d[2];
+#pragma unroll
+  for (int m = 0; m < 8; m++) a[m] = (b3 ? v[m + 8] : v[m]) + dpp_f32<0x128>(b3 ? v[m] : v[m + 8]);
+#pragma unroll
+  for (int m = 0; m < 4; m++) c[m] = (b2 ? a[m + 4] : a[m]) + dpp_f32<0x141>(b2 ? a[m] : a[m + 4]);
+#pragma unroll
+  for (int m = 0; m < 2; m++) d[m] = (b1 ? c[m + 2] : c[m]) + dpp_f32<0x4E>(b1 ? c[m] : c[m + 2]);
+  return (b0 ? d[1] : d[0]) + dpp_f32<0xB1>(b0 ? d[0] : d[1]);
 }
 
 template <int NPAD>
@@ -614,16 +624,18 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
         }
       }
       // the 32 partial sums of row r meet in FP32 (each holds RT terms); the recursion
-      // itself is carried in FP64
+      // itself is carried in FP64. Value i of the reduce-scatter is the real sum of row i,
+      // value 8 + i its imaginary sum.
+      float v[16];
 #pragma unroll
-      for (int i = 0; i < RT; i++) {
-        ar[i] = half_wave_sum(ar[i]);
-        ai[i] = half_wave_sum(ai[i]);
+      for (int i = 0; i < 8; i++) {
+        v[i] = (i < RT) ? ar[i < RT ? i : 0] : 0.f;
+        v[8 + i] = (i < RT) ? ai[i < RT ? i : 0] : 0.f;
       }
-      float sr = ar[0], si = ai[0];
-#pragma unroll
-      for (int i = 1; i < RT; i++)
-        if (tc == i) { sr = ar[i]; si = ai[i]; }
+      float sr = row_reduce_scatter16(v, tc);
+      sr += __shfl_xor(sr, 16, 32);
+      // lane tc < 8 takes the imaginary sum of row tc from lane tc + 8
+      const float si = dpp_f32<0x128>(sr);
       if (tc < RT) {
         const int r = tr * RT + tc;
         double yr = sr, yi = si;
