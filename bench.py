@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--terms", default="16,16",
                     help="split schemes of the mixed form, taylor,square: 3 = BF16x3, 6 = BF16x6, 16 = FP16x3")
+    ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
+                    help="symmetric products of the split mixed form for an inversion-symmetric LUT")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); the directions of a "
@@ -142,6 +144,7 @@ def main():
         plan = ctx.plan(k, kn, lam, dz, thick)
         ns, nw = plan.select()
         plan.set_mixed_terms(tt, ts)
+        sym = plan.set_symmetry(1 if a.symmetry == "auto" else 0)
         ns_all.append(ns); nw_all.append(nw)
         plans.append(plan)
         ndir += len(k)
@@ -207,7 +210,7 @@ def main():
         "config": {"workload": f"{wl.name}: {wl.describe()}",
                    "directions": int(ndir), "strong_beams_mean": round(float(ns.mean()), 1),
                    "weak_beams_mean": round(float(nw.mean()), 1), "precision": head,
-                   "mixed_terms": a.terms},
+                   "mixed_terms": a.terms, "symmetric_products": bool(sym)},
         "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
         "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,

@@ -136,6 +136,17 @@ interface
     integer(c_int32_t)        :: rc
   end function
 
+  ! mode 1: symmetric products of the split mixed form for an inversion-symmetric LUT, 0: off;
+  ! in_effect (may be c_null_ptr) receives whether they are used
+  function ctem_plan_set_symmetry(plan, mode, in_effect) &
+           bind(C, name='ctem_plan_set_symmetry') result(rc)
+    import :: c_int32_t, c_ptr
+    type(c_ptr), value        :: plan
+    integer(c_int32_t), value :: mode
+    type(c_ptr), value        :: in_effect
+    integer(c_int32_t)        :: rc
+  end function
+
   ! reverse mode: gdots(p, ik) = df/ddots -> glut = df/dLUT (complex, LUT order), gb(p) = df/dB_p
   function ctem_plan_adjoint(plan, gdots, glut, gb) bind(C, name='ctem_plan_adjoint') result(rc)
     import :: c_int32_t, c_double, c_double_complex, c_ptr
