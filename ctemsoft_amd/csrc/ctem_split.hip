@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   // at k and k + 8, different segments, and halved the LDS bank-conflict cycles measured with
   // k & 3 alone);
   // Y tile: [plane][col BN][k 32], 16-byte k chunks swizzled by the column (chunk ^ ((col >> 1) & 3))
-  // (one array: after the k loop it holds the transposed copy of an off-diagonal output tile)
+  // (one array: after the k loop it holds the transposed FP32 copy of an off-diagonal tile)
   __shared__ __attribute__((aligned(16))) _Float16 Ls[4 * 32 * 64 + 4 * BN * 32];
   _Float16* const As = Ls;
   _Float16* const Bs = Ls + 4 * 32 * 64;
@@ -469,7 +469,7 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
 
   // LUT offsets are linear in (h, k, l): idx(g - h) = lin(g) - lin(h) - lin(lo), lin(h, k, l) =
   // h + d0 (k + d1 l), the same integer as the column-major offset lut_get computes
-  extern __shared__ int2 wk[];  // [nwpad] weak beams: lin(w), bits of fl32(1/s_w)
+  extern __shared__ int4 wk[];  // [nwpad] weak beams: lin(w), bits of fl32(1/s_w), of 1/s_w (FP64)
   __shared__ int rl[64], cl[64];  // lin of the tile's rows (minus lin(lo)) and columns
   __shared__ __attribute__((aligned(16))) _Float16 As[4 * 64 * 32];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[4 * 64 * 32];
@@ -498,11 +498,12 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
     if (w < nw) {
       const int id = widx[w];
       const float sw = wsg[w];
-      const float iv = (float)(1.0 / (double)sw);  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
-      wk[w] = make_int2(lin(R.hkl + 3 * id), __float_as_int(iv));
+      const double rv = 1.0 / (double)sw;
+      const float iv = (float)rv;  // cmplx(1.D0/weaksg, 0) of gvectors.f90:1156
+      wk[w] = make_int4(lin(R.hkl + 3 * id), __float_as_int(iv), __double2loint(rv), __double2hiint(rv));
       mi = nmax(mi, fabsf(iv));
     } else {
-      wk[w] = make_int2(0, 0);
+      wk[w] = make_int4(0, 0, 0, 0);
     }
   }
   for (int o = 32; o > 0; o >>= 1) mi = nmax(mi, __shfl_xor(mi, o));
@@ -547,12 +548,10 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
       split2(gb[j].x * iv * sb, h, l); ob[0][j] = h; ob[1][j] = l;  // U(w - h) / s_w
       split2(gb[j].y * iv * sb, h, l); ob[2][j] = h; ob[3][j] = l;
       if (diag_tile) {
-        // |U(g_r - w)|^2 / s_w; entries past nw are 0
+        // |U(g_r - w)|^2 / s_w in FP64 (entries past nw are 0)
         const int w = w0 + pk + j;
-        if (w < nw) {
-          const double m2 = hypot((double)ga[j].x, (double)ga[j].y);
-          dpart += m2 * m2 / (double)wsg[w];
-        }
+        const double x = ga[j].x, y = ga[j].y;
+        dpart += (x * x + y * y) * __hiloint2double(wk[w].w, wk[w].z);
       }
     }
     #pragma unroll
@@ -703,7 +702,7 @@ void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, co
                            float lutmax, int32_t nwmax, float2* A32, double* colpart, float* amax, int32_t sym) {
   const int tiles = (B.npad + 63) / 64;
   const int ntile = sym ? tiles * (tiles + 1) / 2 : tiles * tiles;
-  const size_t lds = sizeof(int2) * (size_t)std::max(32, (nwmax + 31) & ~31);
+  const size_t lds = sizeof(int4) * (size_t)std::max(32, (nwmax + 31) & ~31);
   hipLaunchKernelGGL(k_assemble_split, dim3(ntile * B.nb), dim3(256), lds, s, B, V, R, L, kn, mlambda,
                      dynupz, dz, lutmax, A32, colpart, amax, sym);
 }
