@@ -672,6 +672,9 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
 
 // ------------------------------------------------------------------------------------------
 // Lgh and the Sgh contraction
+// NSP = nsite (1 or 2): the S_p gathers of the epilogue are issued before the depth loop and
+// wait in registers (same arithmetic and order); NSP = 0: gathered in the epilogue (any nsite)
+template <int NSP>
 __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double2* __restrict__ Psi, int64_t spsi,
                                                   int nz, const double* __restrict__ lam,
@@ -690,7 +693,9 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   const int n = V.nns[dir];
   const int32_t* sidx = V.sidx + V.soff[dir];
   const double2* P = Psi + (int64_t)b * spsi;
-  __shared__ double2 Pg[32][33], Ph[32][33];
+  // Psi tiles depth-major (Pg[k][g]): the staging writes and the operand reads of 16 lanes
+  // are contiguous 256-byte rows
+  __shared__ double2 Pg[32][32], Ph[32][32];
   __shared__ double lw[32];
   __shared__ int rh[32][3], ch[32][3];
   __shared__ double red[8][4];
@@ -707,6 +712,32 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       ch[tid][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
     }
   }
+  constexpr int NS1 = NSP > 0 ? NSP : 1;
+  double2 sp[NS1][4], sm[NS1][4];  // S_p(g - h) and S_p(h - g) of the lane's entries
+#pragma unroll
+  for (int p = 0; p < NS1; p++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) sp[p][q] = sm[p][q] = make_double2(0.0, 0.0);
+  if constexpr (NSP > 0) {
+    __syncthreads();  // rh, ch
+    if (live) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = wr + l4 + 4 * q, c = wc + l16;
+        if (ti * 32 + r < n && tj * 32 + c < n) {
+          const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
+          const int64_t cell = slut_index(L, dh, dk, dl);
+#pragma unroll
+          for (int p = 0; p < NSP; p++) sp[p][q] = L.slut[(int64_t)p * L.sncell + cell];
+          if (offd) {
+            const int64_t cell2 = slut_index(L, -dh, -dk, -dl);
+#pragma unroll
+            for (int p = 0; p < NSP; p++) sm[p][q] = L.slut[(int64_t)p * L.sncell + cell2];
+          }
+        }
+      }
+    }
+  }
   if (live) {
     for (int k0 = 0; k0 < nz; k0 += 32) {
       __syncthreads();
@@ -718,8 +749,8 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
           vg = P[(int64_t)k * npad + ti * 32 + rr];
           vh = P[(int64_t)k * npad + tj * 32 + rr];
         }
-        Pg[rr][kk] = vg;
-        Ph[rr][kk] = vh;
+        Pg[kk][rr] = vg;
+        Ph[kk][rr] = vh;
       }
       if (tid < 32) lw[tid] = (k0 + tid < nz) ? lam[k0 + tid] : 0.0;
       __syncthreads();
@@ -727,10 +758,10 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       // quadrant; entries past nz are zero
       const int kn = min(32, nz - k0);
       for (int k4 = 0; k4 < kn; k4 += 4) {
-        const double2 a = Pg[wr + l16][k4 + l4];
+        const double2 a = Pg[k4 + l4][wr + l16];
         const double w = lw[k4 + l4];
         const double ar = w * a.x, ai = w * a.y;
-        const double2 h = Ph[wc + l16][k4 + l4];
+        const double2 h = Ph[k4 + l4][wc + l16];
         accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, h.x, accr, 0, 0, 0);
         accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.y, accr, 0, 0, 0);
         acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.x, acci, 0, 0, 0);
@@ -746,18 +777,27 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
     const int gi = ti * 32 + r, gj = tj * 32 + c;
     if (live && gi < n && gj < n) {
       const double lr = dzt * accr[q], li = dzt * acci[q];
-      const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
-      const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
-      for (int p = 0; p < nsite && p < 4; p++) {
-        const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
-        part[p] += lr * sv.x - li * sv.y;
-      }
-      if (offd) {
+      if constexpr (NSP > 0) {
+#pragma unroll
+        for (int p = 0; p < NSP; p++) part[p] += lr * sp[p][q].x - li * sp[p][q].y;
         // the mirrored entry Lgh(h, g) = conj Lgh(g, h) meets S_p(h - g)
-        const int64_t cell2 = nsite ? slut_index(L, -dh, -dk, -dl) : 0;
+        if (offd)
+#pragma unroll
+          for (int p = 0; p < NSP; p++) part[p] += lr * sm[p][q].x + li * sm[p][q].y;
+      } else {
+        const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
+        const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
         for (int p = 0; p < nsite && p < 4; p++) {
-          const double2 sv = L.slut[(int64_t)p * L.sncell + cell2];
-          part[p] += lr * sv.x + li * sv.y;
+          const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
+          part[p] += lr * sv.x - li * sv.y;
+        }
+        if (offd) {
+          // the mirrored entry Lgh(h, g) = conj Lgh(g, h) meets S_p(h - g)
+          const int64_t cell2 = nsite ? slut_index(L, -dh, -dk, -dl) : 0;
+          for (int p = 0; p < nsite && p < 4; p++) {
+            const double2 sv = L.slut[(int64_t)p * L.sncell + cell2];
+            part[p] += lr * sv.x + li * sv.y;
+          }
         }
       }
       if (dir == keep_dir) {
@@ -795,8 +835,15 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const double* lam, double dzt, double* partial, double* dots,
                      int32_t keep_dir, double2* keep_lgh) {
   const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
-  hipLaunchKernelGGL(k_lgh_dots, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
-                     spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
+  if (L.nsite == 1)
+    hipLaunchKernelGGL(k_lgh_dots<1>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
+                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
+  else if (L.nsite == 2)
+    hipLaunchKernelGGL(k_lgh_dots<2>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
+                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
+  else
+    hipLaunchKernelGGL(k_lgh_dots<0>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
+                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
   hipLaunchKernelGGL(k_reduce_dots, dim3((B.nb + 127) / 128), dim3(128), 0, s, B.nb,
                      ntri, L.nsite, B.dirs, partial, dots);
 }
