@@ -708,12 +708,16 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 
 // Mixed precision: the same scaling-and-squaring with the exponential kept as E = I + F, so
 // that every product acts on the small part F and its rounding error scales with |F|^2:
-//   F_0 = the degree-8 Taylor polynomial of exp(B) - I,  B = A / 2^s, ||B||_1 <= 1/2
+//   F_0 = the degree-8 Taylor polynomial of exp(B) - I,  B = A / 2^s, ||B||_1 <= theta
 //   F_{i+1} = F_i^2 + 2 F_i            (E_{i+1} = E_i^2)
 // The depth powers use the same form, psi_{m+j} = psi_j + F_m psi_j with F_m = E^m - I.
 // Products run on split-FP16 (or BF16) MFMA with FP32 accumulation (ctem_mixed.hip); the
 // depth recursion accumulates in FP64 and the final Lgh/Sgh contraction runs in FP64.
+// theta: 1/2 for the split-BF16 options (truncation theta^9/9! = 5.4e-9); 3/4 for split-FP16
+// storage (2.1e-7, below the 2^-21 ~ 4.8e-7 rounding of one split-FP16 product, both errors
+// doubling with each squaring), which saves one squaring for about half the matrices
 constexpr double kThetaMixed = 0.5;
+constexpr double kThetaSplit = 0.75;
 // x1..x7, y2 of the three-product degree-8 Taylor evaluation (the structure of Bader, Blanes and
 // Casas, "Computing the matrix exponential with an optimized Taylor polynomial approximation",
 // Mathematics 7 (2019) 1174), solved here for the coefficients of B^2..B^8 of exp(B):
@@ -835,7 +839,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
-  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaMixed, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
+  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));

@@ -132,7 +132,8 @@ CTEM_API int32_t ctem_plan_get_beams(ctem_plan* plan, int32_t ik, int32_t* stron
 CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_lgh_ik);
 
 /* Split schemes (CTEM_SPLIT_*) of the mixed form for the Taylor products (on the scaled
- * matrix, |B| <= 1/4) and for the squarings; default CTEM_SPLIT_FP16X3 for both. */
+ * matrix, ||B||_1 <= 3/4 with CTEM_SPLIT_FP16X3 for both, 1/2 otherwise) and for the squarings;
+ * default CTEM_SPLIT_FP16X3 for both. */
 CTEM_API int32_t ctem_plan_set_mixed_terms(ctem_plan* plan, int32_t taylor_terms, int32_t square_terms);
 /* No counterpart in the reference (a switch of the mixed form). mode 1 (default): when the
  * crystal's structure-factor LUT is inversion-symmetric, U(-g) = U(g) to within 2^-20 of its max
