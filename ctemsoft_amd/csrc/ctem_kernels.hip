@@ -633,7 +633,12 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
         v[8 + i] = (i < RT) ? ai[i < RT ? i : 0] : 0.f;
       }
       float sr = row_reduce_scatter16(v, tc);
-      sr += __shfl_xor(sr, 16, 32);
+      {
+        // the other DPP row of the half wave (lane tc ^ 16): v_permlane16_swap trades the odd
+        // 16-lane rows of its first operand with the even rows of its second
+        const auto t = __builtin_amdgcn_permlane16_swap(__float_as_uint(sr), __float_as_uint(sr), false, false);
+        sr += __uint_as_float((tc & 16) ? t[0] : t[1]);
+      }
       // lane tc < 8 takes the imaginary sum of row tc from lane tc + 8
       const float si = dpp_f32<0x128>(sr);
       if (tc < RT) {

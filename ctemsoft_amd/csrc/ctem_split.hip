@@ -46,11 +46,20 @@ __device__ __forceinline__ void store_split4(_Float16* dst, const float (&re)[4]
     split2(re[e], h, l); o0[e] = h; o1[e] = l;
     split2(im[e], h, l); o2[e] = h; o3[e] = l;
   }
-  // the even lane sends its imaginary rows 0-3, the odd lane its real rows 4-7
-  const u2v s0 = __builtin_bit_cast(u2v, odd ? o0 : o2), s1 = __builtin_bit_cast(u2v, odd ? o1 : o3);
+  // the even lane sends its imaginary rows 0-3, the odd lane its real rows 4-7:
+  // v_permlane16_swap(x, y) trades the odd 16-lane rows of x with the even rows of y (returned
+  // as the new x, the new y), so the odd lane finds the even lane's y in the new x and the even
+  // lane the odd lane's x in the new y
+  const u2v a0 = __builtin_bit_cast(u2v, o0), a1 = __builtin_bit_cast(u2v, o1);
+  const u2v a2 = __builtin_bit_cast(u2v, o2), a3 = __builtin_bit_cast(u2v, o3);
   u2v r0, r1;
-  r0.x = __shfl_xor(s0.x, 16); r0.y = __shfl_xor(s0.y, 16);
-  r1.x = __shfl_xor(s1.x, 16); r1.y = __shfl_xor(s1.y, 16);
+#pragma unroll
+  for (int w = 0; w < 2; w++) {
+    const auto t0 = __builtin_amdgcn_permlane16_swap(a0[w], a2[w], false, false);
+    const auto t1 = __builtin_amdgcn_permlane16_swap(a1[w], a3[w], false, false);
+    r0[w] = odd ? t0[0] : t0[1];
+    r1[w] = odd ? t1[0] : t1[1];
+  }
   const h4v q0 = __builtin_bit_cast(h4v, r0), q1 = __builtin_bit_cast(h4v, r1);
   const h8v w0 = odd ? __builtin_shufflevector(q0, o2, 0, 1, 2, 3, 4, 5, 6, 7)
                      : __builtin_shufflevector(o0, q0, 0, 1, 2, 3, 4, 5, 6, 7);
