@@ -19,10 +19,60 @@ __device__ __forceinline__ int xcd_logical_block(int bid, int nblk) {
 // NaN-propagating max of non-negative values
 __device__ __forceinline__ float nmax(float a, float b) { return (a != a) ? a : ((b != b) ? b : fmaxf(a, b)); }
 
+// Cross-lane moves without LDS (the whole wave active). DPP within 16-lane rows: CTRL 0xB1 =
+// quad_perm [1,0,3,2] (lane ^ 1), 0x4E = [2,3,0,1] (lane ^ 2), 0x124 / 0x128 = row_ror 4 / 8,
+// 0x141 = row_half_mirror.
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_u32(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x) { return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(x))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  return __hiloint2double((int)dpp_u32<CTRL>((unsigned)__double2hiint(x)), (int)dpp_u32<CTRL>((unsigned)__double2loint(x)));
+}
+// The value of lane l ^ 16 (l ^ 32): v_permlane16_swap (v_permlane32_swap) trades the odd 16-lane
+// (32-lane) rows of its first operand with the even rows of its second and returns both; with
+// the same value in both, the odd rows find their partner's value in the first result and the
+// even rows in the second.
+__device__ __forceinline__ unsigned xchg16(unsigned v) {
+  const auto t = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return (threadIdx.x & 16) ? t[0] : t[1];
+}
+__device__ __forceinline__ unsigned xchg32(unsigned v) {
+  const auto t = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (threadIdx.x & 32) ? t[0] : t[1];
+}
+__device__ __forceinline__ float xchg16(float v) { return __uint_as_float(xchg16(__float_as_uint(v))); }
+__device__ __forceinline__ float xchg32(float v) { return __uint_as_float(xchg32(__float_as_uint(v))); }
+__device__ __forceinline__ double xchg16(double v) {
+  return __hiloint2double((int)xchg16((unsigned)__double2hiint(v)), (int)xchg16((unsigned)__double2loint(v)));
+}
+__device__ __forceinline__ double xchg32(double v) {
+  return __hiloint2double((int)xchg32((unsigned)__double2hiint(v)), (int)xchg32((unsigned)__double2loint(v)));
+}
+// sum over the 64 lanes, the same in every lane and in a fixed order: the quads (lane ^ 1,
+// lane ^ 2), the four quads of a row (row_ror 4, 8), then the rows (lane ^ 16, lane ^ 32)
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x128>(v);
+  v += xchg16(v);
+  v += xchg32(v);
+  return v;
+}
+
 // per-matrix max |entry|: wave reduction, one atomic per wave (non-negative floats order as
-// their bit patterns, a NaN above every finite value)
+// their bit patterns, a NaN above every finite value); the whole wave must be active
 __device__ __forceinline__ void wave_max_to(float v, float* slot) {
-  for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o));
+  v = nmax(v, dpp_f32<0xB1>(v));
+  v = nmax(v, dpp_f32<0x4E>(v));
+  v = nmax(v, dpp_f32<0x124>(v));
+  v = nmax(v, dpp_f32<0x128>(v));
+  v = nmax(v, xchg16(v));
+  v = nmax(v, xchg32(v));
   if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax((unsigned int*)slot, __float_as_uint(v));
   if ((threadIdx.x & 63) == 0 && v != v) atomicMax((unsigned int*)slot, 0x7fc00000u);
 }

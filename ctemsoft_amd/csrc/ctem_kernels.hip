@@ -557,10 +557,6 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
 // of four levels a lane keeps half of its values and adds its partner's copies of them (partners
 // at lane distance 8 by row_ror 8, mirrored within 8 lanes by row_half_mirror, then quad_perm
 // [2,3,0,1] and [1,0,3,2]); afterwards lane l holds the row's sum of value l % 16.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f32(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
-}
 __device__ __forceinline__ float row_reduce_scatter16(const float (&v)[16], int lane) {
   const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
   float a[8], c[4], d[2];
@@ -633,12 +629,7 @@ __global__ __launch_bounds__(1024) void k_depth_onchip(int nb, int nz, const flo
         v[8 + i] = (i < RT) ? ai[i < RT ? i : 0] : 0.f;
       }
       float sr = row_reduce_scatter16(v, tc);
-      {
-        // the other DPP row of the half wave (lane tc ^ 16): v_permlane16_swap trades the odd
-        // 16-lane rows of its first operand with the even rows of its second
-        const auto t = __builtin_amdgcn_permlane16_swap(__float_as_uint(sr), __float_as_uint(sr), false, false);
-        sr += __uint_as_float((tc & 16) ? t[0] : t[1]);
-      }
+      sr += xchg16(sr);  // the other DPP row of the half wave
       // lane tc < 8 takes the imaginary sum of row tc from lane tc + 8
       const float si = dpp_f32<0x128>(sr);
       if (tc < RT) {
@@ -811,10 +802,9 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       }
     }
   }
-  // block reduction in fixed order: wave shuffles, then waves in order
+  // block reduction in fixed order: within each wave (lane moves), then waves in order
   for (int p = 0; p < 4; p++) {
-    double v = part[p];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    const double v = wave_sum(part[p]);
     if (lane == 0) red[wave][p] = v;
   }
   __syncthreads();

@@ -515,7 +515,12 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
       wk[w] = make_int4(0, 0, 0, 0);
     }
   }
-  for (int o = 32; o > 0; o >>= 1) mi = nmax(mi, __shfl_xor(mi, o));
+  mi = nmax(mi, dpp_f32<0xB1>(mi));
+  mi = nmax(mi, dpp_f32<0x4E>(mi));
+  mi = nmax(mi, dpp_f32<0x124>(mi));
+  mi = nmax(mi, dpp_f32<0x128>(mi));
+  mi = nmax(mi, xchg16(mi));
+  mi = nmax(mi, xchg32(mi));
   if (lane == 0) red[wave] = mi;
   __syncthreads();
   mi = nmax(nmax(red[0], red[1]), nmax(red[2], red[3]));
@@ -685,8 +690,10 @@ void k_assemble_split(Bucket B, BeamView V, DevRefl R, DevLut L, const double* _
       #pragma unroll
       for (int e = 0; e < 4; e++) {
         double v = rsum[e];
-        #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 16);
+        v += dpp_f64<0xB1>(v);
+        v += dpp_f64<0x4E>(v);
+        v += dpp_f64<0x124>(v);
+        v += dpp_f64<0x128>(v);
         if (l16 == 0) rs[wn >> 5][wm + 16 * i + 4 * l4 + e] = v;
       }
     }
