@@ -389,7 +389,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
               wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
             }
           }
-        store_split(C + o, vr, vi, vmax);
+        // a matrix's last product hands over its FP32 copy only: no later product reads C
+        if (!Cf) store_split(C + o, vr, vi, vmax);
         if (C2) store_split(C2 + o, wr, wi, vmax2);
         if (Cf) {
           float4* qf = (float4*)(Cf + (int64_t)col * n + row0);
