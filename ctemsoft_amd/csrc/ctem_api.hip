@@ -1030,7 +1030,8 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
       }
       if (rc) return rc;
       launch_lgh_dots(s, B, V, R, L, c->w.Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
-                      c->w.partial.as<double>(), p->dots.as<double>(), p->keep_dir, p->keep_lgh.as<double2>());
+                      c->w.partial.as<double>(), p->dots.as<double>(), p->keep_dir, p->keep_lgh.as<double2>(),
+                      precision == CTEM_PREC_MIXED);
       HIPCHK(hipGetLastError());
     }
     off += nbk;
@@ -1173,7 +1174,7 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
       HIPCHK(hipGetLastError());
       // B: d dots_p / d B_p = Re sum Lgh dS_p/dB_p
       launch_lgh_dots(s, B, V, R, LdB, c->w.Psi.as<double2>(), spsi, nz, p->lam.as<double>(), dzt,
-                      c->w.partial.as<double>(), c->w.g_dots2.as<double>(), -1, nullptr);
+                      c->w.partial.as<double>(), c->w.g_dots2.as<double>(), -1, nullptr, 0);
       HIPCHK(hipGetLastError());
       // M = c (Lbar + Lbar^H), Q = M Psi
       launch_adj_m(s, B, V, R, L, c->w.g_dots.as<double>(), dzt, c->w.aM.as<double2>());
@@ -1365,7 +1366,7 @@ extern "C" int32_t ctem_calclgh3(ctem_ctx* c, const double* dmat, double* lgh, d
   L.dim[0] = L.dim[1] = L.dim[2] = 1;
   Bucket B{npad, 1, c->s_dirs.as<int32_t>()};
   launch_lgh_dots(s, B, V, R, L, c->s_Psi.as<double2>(), spsi, izz, c->s_lam.as<double>(), depthstep / thick,
-                  c->s_part.as<double>(), c->s_dots.as<double>(), 0, c->s_lgh.as<double2>());
+                  c->s_part.as<double>(), c->s_dots.as<double>(), 0, c->s_lgh.as<double2>(), 0);
   HIPCHK(hipGetLastError());
   std::vector<double> h(2 * (size_t)nn * nn);
   HIPCHK(hipMemcpyAsync(h.data(), c->s_lgh.p, 16 * (size_t)nn * nn, hipMemcpyDeviceToHost, s));

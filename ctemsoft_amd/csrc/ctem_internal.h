@@ -212,11 +212,12 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
                   int add_identity, double2* Psi, int64_t spsi, int32_t grid);
 
 // Lgh = (dz/thick) Psi diag(lam) Psi^H and the per-site contraction with S_p(g-h):
-// partial sums per 32x32 tile, reduced in fixed order into dots[dir*nsite + p].
+// partial sums per 32x32 tile, reduced in fixed order into dots[dir*nsite + p]. f32: the Lgh
+// products in FP32 (mixed form)
 void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double2* Psi, int64_t spsi, int32_t nz,
                      const double* lam, double dzt, double* partial, double* dots,
-                     int32_t keep_dir, double2* keep_lgh);
+                     int32_t keep_dir, double2* keep_lgh, int32_t f32);
 
 // scatter fl32 svals to the Laue-equivalent pixels of sr[nsite][2npy+1][2npx+1]
 void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,

@@ -669,8 +669,12 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
 // ------------------------------------------------------------------------------------------
 // Lgh and the Sgh contraction
 // NSP = nsite (1 or 2): the S_p gathers of the epilogue are issued before the depth loop and
-// wait in registers (same arithmetic and order); NSP = 0: gathered in the epilogue (any nsite)
-template <int NSP>
+// wait in registers (same arithmetic and order); NSP = 0: gathered in the epilogue (any nsite).
+// F32 (mixed form): Psi staged in FP32 and the products on v_mfma_f32_16x16x4_f32 with FP32
+// accumulation over the depths (the contraction with S_p stays FP64); measured against the FP64
+// form it adds ~1e-6 relative to the intensities of the mixed form (largest error over the Fe,
+// Ni and corundum tests 7.8e-6, from 4.8e-6 with FP64 Lgh)
+template <int NSP, bool F32>
 __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                   const double2* __restrict__ Psi, int64_t spsi,
                                                   int nz, const double* __restrict__ lam,
@@ -690,16 +694,22 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   const int32_t* sidx = V.sidx + V.soff[dir];
   const double2* P = Psi + (int64_t)b * spsi;
   // Psi tiles depth-major (Pg[k][g]): the staging writes and the operand reads of 16 lanes
-  // are contiguous 256-byte rows
-  __shared__ double2 Pg[32][32], Ph[32][32];
-  __shared__ double lw[32];
+  // are contiguous rows
+  typedef typename std::conditional<F32, float2, double2>::type PT;
+  typedef typename std::conditional<F32, float, double>::type WT;
+  typedef typename std::conditional<F32, f32x4, double4v>::type AccT;
+  __shared__ PT Pg[32][32], Ph[32][32];
+  __shared__ WT lw[32];
   __shared__ int rh[32][3], ch[32][3];
   __shared__ double red[8][4];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
   const int wr = (wave >> 1) * 16, wc = (wave & 1) * 16;
   const int nsite = L.nsite;
-  double4v accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+  AccT accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+  // accumulator entry q of lane l: row 4 (l / 16) + q (FP32 MFMA) or l / 16 + 4 q (FP64 MFMA)
+  // of the wave's quadrant, column l % 16
+  auto qrow = [&](int q) __attribute__((always_inline)) { return F32 ? wr + 4 * l4 + q : wr + l4 + 4 * q; };
   const bool live = (ti * 32 < n) && (tj * 32 < n);
   if (tid < 32) {
     const int a = ti * 32 + tid, c = tj * 32 + tid;
@@ -719,7 +729,7 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
     if (live) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int r = wr + l4 + 4 * q, c = wc + l16;
+        const int r = qrow(q), c = wc + l16;
         if (ti * 32 + r < n && tj * 32 + c < n) {
           const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
           const int64_t cell = slut_index(L, dh, dk, dl);
@@ -745,34 +755,45 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
           vg = P[(int64_t)k * npad + ti * 32 + rr];
           vh = P[(int64_t)k * npad + tj * 32 + rr];
         }
-        Pg[kk][rr] = vg;
-        Ph[kk][rr] = vh;
+        if constexpr (F32) {
+          Pg[kk][rr] = make_float2((float)vg.x, (float)vg.y);
+          Ph[kk][rr] = make_float2((float)vh.x, (float)vh.y);
+        } else {
+          Pg[kk][rr] = vg;
+          Ph[kk][rr] = vh;
+        }
       }
-      if (tid < 32) lw[tid] = (k0 + tid < nz) ? lam[k0 + tid] : 0.0;
+      if (tid < 32) lw[tid] = (WT)((k0 + tid < nz) ? lam[k0 + tid] : 0.0);
       __syncthreads();
-      // sum_k (lam_k psi_k(g)) conj(psi_k(h)) on v_mfma_f64_16x16x4_f64, each wave one 16x16
-      // quadrant; entries past nz are zero
+      // sum_k (lam_k psi_k(g)) conj(psi_k(h)) on v_mfma_f64_16x16x4_f64 (F32:
+      // v_mfma_f32_16x16x4_f32), each wave one 16x16 quadrant; entries past nz are zero
       const int kn = min(32, nz - k0);
       for (int k4 = 0; k4 < kn; k4 += 4) {
-        const double2 a = Pg[k4 + l4][wr + l16];
-        const double w = lw[k4 + l4];
-        const double ar = w * a.x, ai = w * a.y;
-        const double2 h = Ph[k4 + l4][wc + l16];
-        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, h.x, accr, 0, 0, 0);
-        accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.y, accr, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.x, acci, 0, 0, 0);
-        acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, h.y, acci, 0, 0, 0);
+        const PT a = Pg[k4 + l4][wr + l16];
+        const WT w = lw[k4 + l4];
+        const WT ar = w * a.x, ai = w * a.y;
+        const PT h = Ph[k4 + l4][wc + l16];
+        if constexpr (F32) {
+          accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, h.x, accr, 0, 0, 0);
+          accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, h.y, accr, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, h.x, acci, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f32_16x16x4f32(-ar, h.y, acci, 0, 0, 0);
+        } else {
+          accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, h.x, accr, 0, 0, 0);
+          accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.y, accr, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, h.x, acci, 0, 0, 0);
+          acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, h.y, acci, 0, 0, 0);
+        }
       }
     }
   }
   __syncthreads();
   double part[4] = {0, 0, 0, 0};
-  // lane l holds rows wr + l/16 + 4e, column wc + l%16 of the tile
   for (int q = 0; q < 4; q++) {
-    const int r = wr + l4 + 4 * q, c = wc + l16;
+    const int r = qrow(q), c = wc + l16;
     const int gi = ti * 32 + r, gj = tj * 32 + c;
     if (live && gi < n && gj < n) {
-      const double lr = dzt * accr[q], li = dzt * acci[q];
+      const double lr = dzt * (double)accr[q], li = dzt * (double)acci[q];
       if constexpr (NSP > 0) {
 #pragma unroll
         for (int p = 0; p < NSP; p++) part[p] += lr * sp[p][q].x - li * sp[p][q].y;
@@ -828,17 +849,22 @@ __global__ void k_reduce_dots(int nb, int ntile, int nsite, const int* __restric
 void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double2* Psi, int64_t spsi, int32_t nz,
                      const double* lam, double dzt, double* partial, double* dots,
-                     int32_t keep_dir, double2* keep_lgh) {
+                     int32_t keep_dir, double2* keep_lgh, int32_t f32) {
   const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
-  if (L.nsite == 1)
-    hipLaunchKernelGGL(k_lgh_dots<1>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
-                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
-  else if (L.nsite == 2)
-    hipLaunchKernelGGL(k_lgh_dots<2>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
-                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
-  else
-    hipLaunchKernelGGL(k_lgh_dots<0>, dim3(ntri * B.nb), dim3(256), 0, s, B, V, R, L, Psi,
-                       spsi, nz, lam, dzt, partial, keep_dir, keep_lgh);
+  const dim3 grid(ntri * B.nb);
+#define LGH_LAUNCH(NSP, F)                                                                          \
+  hipLaunchKernelGGL((k_lgh_dots<NSP, F>), grid, dim3(256), 0, s, B, V, R, L, Psi, spsi, nz, lam, dzt, \
+                     partial, keep_dir, keep_lgh)
+  if (f32) {
+    if (L.nsite == 1) LGH_LAUNCH(1, true);
+    else if (L.nsite == 2) LGH_LAUNCH(2, true);
+    else LGH_LAUNCH(0, true);
+  } else {
+    if (L.nsite == 1) LGH_LAUNCH(1, false);
+    else if (L.nsite == 2) LGH_LAUNCH(2, false);
+    else LGH_LAUNCH(0, false);
+  }
+#undef LGH_LAUNCH
   hipLaunchKernelGGL(k_reduce_dots, dim3((B.nb + 127) / 128), dim3(128), 0, s, B.nb,
                      ntri, L.nsite, B.dirs, partial, dots);
 }
