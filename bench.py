@@ -204,7 +204,7 @@ def main():
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(r["ms"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": (("fp16" if a.terms == "16,16" else "bf16/fp16") + "-split MFMA, f32 accumulate, f64 depth+contraction")
+        "dtype": (("fp16" if a.terms == "16,16" else "bf16/fp16") + "-split MFMA, f32 accumulate; f32 Lgh MFMA; f64 depth and contraction")
                  if head == "mixed" else "f64",
         "data": "synthetic wave vectors and depth profile; reference per-energy setup",
         "config": {"workload": f"{wl.name}: {wl.describe()}",

@@ -821,10 +821,11 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // The mixed form with both stages on CTEM_SPLIT_FP16X3: the same Taylor polynomial and
 // squarings as expm_depth_mixed, with every matrix after the assembly kept in split form
 // (ctem_split.hip), so that the products read their operands without converting them.
-// Buffers (nb matrices each, 8 bytes per entry): 0 A as assembled (FP32), later L; 1 B; 2 B2;
-// 3 P, later the FP32 copy of the final F of every matrix (written by its last product) for
-// the depth kernel; 4 R, then the squarings alternate between 5 (F0) and 4. wmax holds six
-// FP32 maxima and six exponents per matrix, then the max of A.
+// Buffers (nb matrices each, 8 bytes per entry): 0 A as assembled (FP32), read raw by the three
+// Taylor products as B = A 2^-s (split on the fly when staged); 1 L; 2 B2; 3 P, later the FP32
+// copy of the final F of every matrix (written by its last product) for the depth kernel; 4 R,
+// then the squarings alternate between 5 (F0) and 4. wmax holds six FP32 maxima and six
+// exponents per matrix (slot 0: L, slot 1: B, slots 2-5 as the buffers), then the max of A.
 int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
                          const float* amax, float* wmax, DBuf& order, double2* Psi64, int64_t spsi, int nz,
                          int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
@@ -839,7 +840,8 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
-  launch_norm_split(s, npad, nb, W32, colpart, amax, S[1], kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
+  // the squaring count, exponent and max of B = A 2^-s; the first products split A themselves
+  launch_norm_split(s, npad, nb, W32, colpart, amax, nullptr, kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
@@ -885,28 +887,37 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     g.C2 = S[c2]; g.ce2 = ex[c2]; g.cmax2 = mx[c2];
     g.alpha2 = alpha2; g.betaI2 = betaI2;
   };
+  // B as a raw operand: the assembled A (buffer 0) with B = A 2^-s (exponent and max in slot 1)
+  auto raw_b = [&](SGemm& g, int q, float beta, float beta2) {
+    g.M[q] = S[0]; g.me[q] = d_sq; g.mmax[q] = mx[1]; g.mraw[q] = 1;
+    g.beta[q] = beta; g.beta2[q] = beta2;
+  };
   int32_t rc;
   const double f3 = n3;
   // B2 = B B, P = x2 B2 + x1 B
   {
     SGemm g = mk(2, 1, 1);
+    g.X = g.Y = S[0]; g.xraw = 1; g.rs = d_sq;
     add_c2(g, 3, kBbc8[1], 0.f);
-    add_m(g, 0, 1, 0.f, kBbc8[0]);
+    raw_b(g, 0, 0.f, kBbc8[0]);
     if ((rc = launch(g, f3))) return rc;
   }
-  // B4 = B2 P;  L = B4 + x3 B2,  R = x7 B4 + x4 I + x6 B2 + x5 B
+  // B4 = B2 P;  L = B4 + x3 B2,  R = x7 B4 + x4 I + x6 B2 + x5 B  (L into buffer 1, its exponent
+  // and max in slot 0: A stays in buffer 0 for the next product)
   {
     SGemm g = mk(0, 2, 3);
+    g.C = S[1];
     add_c2(g, 4, kBbc8[6], kBbc8[3]);
     add_m(g, 0, 2, kBbc8[2], kBbc8[5]);
-    add_m(g, 1, 1, 0.f, kBbc8[4]);
+    raw_b(g, 1, 0.f, kBbc8[4]);
     if ((rc = launch(g, f3))) return rc;
   }
   // F0 = L R + B + y2 B2 (the final F where no squaring follows)
   float2* Ff = W32 + 3 * per * nb;
   {
     SGemm g = mk(5, 0, 4);
-    add_m(g, 0, 1, 1.f, 0.f);
+    g.X = S[1];
+    raw_b(g, 0, 1.f, 0.f);
     add_m(g, 1, 2, kBbc8[7], 0.f);
     g.Cf = Ff; g.f_sq = d_sq; g.f_at = 0;
     if ((rc = launch(g, f3))) return rc;

@@ -287,6 +287,10 @@ struct SGemm {
   // only the 64 x 64 tiles on or above the diagonal of C are computed and stored (C2 alike),
   // and only those of X and Y are read (the rest through the symmetry); Cf is written in full
   int32_t sym;
+  // raw operands: X = Y (xraw) and M[q] (mraw[q]) point at the assembled FP32 matrix A (float2,
+  // column-major); the operand is B = A 2^-rs[b] (rs the squaring counts), X and Y split on the
+  // fly into the staged tiles (scaled by 2^xe as split operands are), M[q] with me[q] = rs
+  int32_t xraw; const int32_t* rs; int32_t mraw[2];
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
 // FP32 matrices A (as assembled, with the column-sum partials and max |entry| of
@@ -294,7 +298,7 @@ void launch_sgemm(hipStream_t s, const SGemm& g);
 // into Bs with its exponent (from the exact max) and max |entry|
 void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
                        const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax, int32_t sym);
+                       int32_t* bexp, float* bmax, int32_t sym);  // Bs null: s, exponent and max only
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

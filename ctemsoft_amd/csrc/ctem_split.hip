@@ -203,6 +203,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
   #pragma unroll
   for (int q = 0; q < 2; q++)
     if (g.M[q]) { mmx[q] = g.mmax[q][b]; mex[q] = g.me[q][b]; }
+  // raw X = Y: the 64-byte block of 8 FP32 complex rows sits where the split group block of the
+  // same rows would (both 8 bytes per entry), and is split here into the four plane rows
+  const float fraw = g.xraw ? ldexpf(1.0f, g.xe[b] - g.rs[b]) : 0.0f;
+  auto split_block = [&](u4v (&r)[4]) __attribute__((always_inline)) {
+    h8v p0, p1, p2, p3;
+    #pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const f4v v = __builtin_bit_cast(f4v, r[q]);  // rows 2q, 2q + 1: re, im, re, im
+      #pragma unroll
+      for (int t = 0; t < 2; t++) {
+        _Float16 h, l;
+        split2(v[2 * t] * fraw, h, l); p0[2 * q + t] = h; p1[2 * q + t] = l;
+        split2(v[2 * t + 1] * fraw, h, l); p2[2 * q + t] = h; p3[2 * q + t] = l;
+      }
+    }
+    r[0] = __builtin_bit_cast(u4v, p0); r[1] = __builtin_bit_cast(u4v, p1);
+    r[2] = __builtin_bit_cast(u4v, p2); r[3] = __builtin_bit_cast(u4v, p3);
+  };
 
   gload(0);
   // The number of live column subtiles of the wave (0, 2 or JN) is chosen once, outside the
@@ -214,6 +232,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
       __syncthreads();
       if constexpr (YG == 1) {
         const int xo = xt ? ylds[0] : xlds, yo = yt ? xlds : ylds[0];
+        if (g.xraw) {
+          split_block(rx);
+          split_block(ry[0]);
+        }
         #pragma unroll
         for (int p = 0; p < 4; p++) *(u4v*)&As[p * 2048 + xo] = rx[p];
         #pragma unroll
@@ -379,14 +401,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
 #pragma unroll
         for (int q = 0; q < 2; q++)
           if (g.M[q] && !(q == 0 && mx0)) {
-            const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
-            const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
-            const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
+            float mr[4], mi[4];
+            if (g.mraw[q]) {
+              const float4* Mf = (const float4*)((const float2*)g.M[q] + (int64_t)b * per + (int64_t)col * n + row0);
+              const float4 u0 = Mf[0], u1 = Mf[1];
+              mr[0] = u0.x; mi[0] = u0.y; mr[1] = u0.z; mi[1] = u0.w;
+              mr[2] = u1.x; mi[2] = u1.y; mr[3] = u1.z; mi[3] = u1.w;
+            } else {
+              const _Float16* Mq = (const _Float16*)g.M[q] + mb + o;
+              const h4v m0 = *(const h4v*)Mq, m1 = *(const h4v*)(Mq + 8);
+              const h4v m2 = *(const h4v*)(Mq + 16), m3 = *(const h4v*)(Mq + 24);
+#pragma unroll
+              for (int e = 0; e < 4; e++) {
+                mr[e] = (float)m0[e] + (float)m1[e];
+                mi[e] = (float)m2[e] + (float)m3[e];
+              }
+            }
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-              const float mr = (float)m0[e] + (float)m1[e], mi = (float)m2[e] + (float)m3[e];
-              vr[e] += fm[q] * mr; vi[e] += fm[q] * mi;
-              wr[e] += fm2[q] * mr; wi[e] += fm2[q] * mi;
+              vr[e] += fm[q] * mr[e]; vi[e] += fm[q] * mi[e];
+              wr[e] += fm2[q] * mr[e]; wi[e] += fm2[q] * mi[e];
             }
           }
         // a matrix's last product hands over its FP32 copy only: no later product reads C
@@ -773,6 +807,7 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
     bexp[b] = e;
     bmax[b] = mb;
   }
+  if (!Bs) return;  // the products split A themselves
   __syncthreads();
   const float f = ldexpf(1.0f, e_sh - s_sh);
   // one thread per 8-row group of a column: 64 contiguous bytes out (sym: only the groups of
