@@ -516,6 +516,18 @@ __global__ __launch_bounds__(1024) void k_depth(int npad, int nb, int nz, const 
       for (int r = r0; r < npad; r += 256) {
         double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
         int c = c0;
+        // eight loads in flight before their products; even offsets into a, odd ones into b
+        for (; c + 8 <= c1; c += 8) {
+          double2 e[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) e[j] = ld2(Eb + (int64_t)(c + j) * npad + r);
+#pragma unroll
+          for (int j = 0; j < 8; j += 2) {
+            const double2 x0 = x[c + j], x1 = x[c + j + 1];
+            ar += e[j].x * x0.x - e[j].y * x0.y; ai += e[j].x * x0.y + e[j].y * x0.x;
+            br += e[j + 1].x * x1.x - e[j + 1].y * x1.y; bi += e[j + 1].x * x1.y + e[j + 1].y * x1.x;
+          }
+        }
         for (; c + 1 < c1; c += 2) {
           const double2 e0 = ld2(Eb + (int64_t)c * npad + r), e1 = ld2(Eb + (int64_t)(c + 1) * npad + r);
           const double2 x0 = x[c], x1 = x[c + 1];
@@ -664,6 +676,98 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
   else
     hipLaunchKernelGGL(k_depth<double2>, dim3(g), dim3(1024), lds, s, npad, nb, nz, (const double2*)E,
                        add_identity, Psi, spsi);
+}
+
+// Depth recursion over pairs of depths (mixed form, matrices too large to hold on chip): with
+// F = E - I and F2 = E^2 - I in FP32, psi_1 = e_1 + F e_1 (the first column), psi_2 = psi_1 +
+// F psi_1, then [psi_{k+2}, psi_{k+3}] = [psi_k, psi_{k+1}] + F2 [psi_k, psi_{k+1}]: each pass
+// over a matrix streamed from the caches serves two vectors, half the passes of k_depth. Rows
+// over 256 threads, columns in four quarters: FP32 products and sums within a quarter (from an
+// FP32 copy of the vectors), the quarters' sums and the recursion in FP64.
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_depth_pair(int npad, int nb, int nz, const float2* __restrict__ F,
+                                                     const float2* __restrict__ F2, double2* __restrict__ Psi,
+                                                     int64_t spsi) {
+  extern __shared__ double2 sm[];
+  double2* x = sm;                 // [2][npad] the two current vectors
+  double2* part = sm + 2 * npad;   // [4 quarters][2][npad]
+  float2* xf = (float2*)(sm + 10 * npad);  // [2][npad] FP32 copy of x
+  const int tid = threadIdx.x, q = tid >> 8, r0 = tid & 255;
+  const int64_t per = (int64_t)npad * npad;
+  const int cq = (npad + 3) / 4;
+  const int c0 = q * cq, c1 = min(npad, c0 + cq);
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const float2* Fb = F + (int64_t)b * per;
+    const float2* F2b = F2 + (int64_t)b * per;
+    double2* P = Psi + (int64_t)b * spsi;
+    // out[v] = in[v] + M in[v] for the vectors in slots 0 .. nv-1, into slots o .. o+nv-1,
+    // stored as depths k .. k+nv-1 (those below nz)
+    auto pass = [&](const float2* M, int nv, int o, int k) __attribute__((always_inline)) {
+      for (int r = r0; r < npad; r += 256) {
+        float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f;
+        auto mac = [&](const float2 e, int c) __attribute__((always_inline)) {
+          const float2 u = xf[c];
+          a0r = fmaf(e.x, u.x, a0r); a0r = fmaf(-e.y, u.y, a0r);
+          a0i = fmaf(e.x, u.y, a0i); a0i = fmaf(e.y, u.x, a0i);
+          if (nv == 2) {
+            const float2 w = xf[npad + c];
+            a1r = fmaf(e.x, w.x, a1r); a1r = fmaf(-e.y, w.y, a1r);
+            a1i = fmaf(e.x, w.y, a1i); a1i = fmaf(e.y, w.x, a1i);
+          }
+        };
+        // four loads in flight before their products (two workgroups per CU: 64 registers)
+        int c = c0;
+        for (; c + 4 <= c1; c += 4) {
+          float2 e[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++) e[j] = M[(int64_t)(c + j) * npad + r];
+#pragma unroll
+          for (int j = 0; j < 4; j++) mac(e[j], c + j);
+        }
+        for (; c < c1; c++) mac(M[(int64_t)c * npad + r], c);
+        part[(2 * q) * npad + r] = make_double2(a0r, a0i);
+        if (nv == 2) part[(2 * q + 1) * npad + r] = make_double2(a1r, a1i);
+      }
+      __syncthreads();
+      for (int i = tid; i < nv * npad; i += 1024) {
+        const int v = i / npad, r = i - v * npad;
+        double2 y = part[v * npad + r];
+        for (int qq = 1; qq < 4; qq++) {
+          const double2 t = part[(2 * qq + v) * npad + r];
+          y.x += t.x;
+          y.y += t.y;
+        }
+        y.x += x[v * npad + r].x;
+        y.y += x[v * npad + r].y;
+        if (k + v < nz) P[(int64_t)(k + v) * npad + r] = y;
+        part[v * npad + r] = y;  // staged; copied into the output slots after the barrier
+      }
+      __syncthreads();
+      for (int i = tid; i < nv * npad; i += 1024) {
+        const double2 y = part[i];
+        x[o * npad + i] = y;
+        xf[o * npad + i] = make_float2((float)y.x, (float)y.y);
+      }
+      __syncthreads();
+    };
+    __syncthreads();  // the previous direction is done with x
+    for (int i = tid; i < npad; i += 1024) {
+      const float2 f = Fb[i];
+      const double2 v = make_double2((double)f.x + (i == 0 ? 1.0 : 0.0), (double)f.y);
+      x[i] = v;
+      xf[i] = make_float2((float)v.x, (float)v.y);
+      P[i] = v;
+    }
+    __syncthreads();
+    if (nz > 1) pass(Fb, 1, 1, 1);
+    for (int k = 2; k < nz; k += 2) pass(F2b, 2, 0, k);
+  }
+}
+
+void launch_depth_pair(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const float2* F, const float2* F2,
+                       double2* Psi, int64_t spsi, int32_t grid) {
+  const size_t lds = (size_t)10 * npad * sizeof(double2) + (size_t)2 * npad * sizeof(float2);
+  const int g = std::min(nb, 2 * grid);  // two workgroups per CU fit
+  hipLaunchKernelGGL(k_depth_pair, dim3(g), dim3(1024), lds, s, npad, nb, nz, F, F2, Psi, spsi);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -335,9 +335,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     g.ce[b] = ec;
     if (g.C2) g.ce2[b] = ec2;
   }
-  float2* Cf = (g.Cf && g.f_sq[b] == g.f_at) ? g.Cf + (int64_t)b * per : nullptr;
-  // the FP32 copy for the depth kernel is read in full: off-diagonal tiles also store their
-  // transpose there
+  // the FP32 copy for the depth kernel (F, or E^2 - I after it) is read in full: off-diagonal
+  // tiles also store their transpose there; after the matrix's last product no C is stored
+  float2* Cf = nullptr;
+  bool last = false;
+  if (g.Cf && g.f_sq[b] == g.f_at) {
+    Cf = g.Cf + (int64_t)b * per;
+    last = g.Cf2 == nullptr;
+  } else if (g.Cf2 && g.f_sq[b] + 1 == g.f_at) {
+    Cf = g.Cf2 + (int64_t)b * per;
+    last = true;
+  }
   const bool mirror = offd && Cf != nullptr;
   if (jlive == 0 && !mirror) return;
   // the accumulators carry 2^(ex + ey); the outputs are stored times 2^ec
@@ -424,7 +432,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
             }
           }
         // a matrix's last product hands over its FP32 copy only: no later product reads C
-        if (!Cf) store_split(C + o, vr, vi, vmax);
+        if (!last) store_split(C + o, vr, vi, vmax);
         if (C2) store_split(C2 + o, wr, wi, vmax2);
         if (Cf) {
           float4* qf = (float4*)(Cf + (int64_t)col * n + row0);
