@@ -841,7 +841,7 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   }
   HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
   // the squaring count, exponent and max of B = A 2^-s; the first products split A themselves
-  launch_norm_split(s, npad, nb, W32, colpart, amax, nullptr, kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1], sym ? 1 : 0);
+  launch_norm_split(s, npad, nb, colpart, amax, kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1]);
   HIPCHK(hipGetLastError());
   h_sq.resize(nb);
   HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));

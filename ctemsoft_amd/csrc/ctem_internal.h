@@ -300,12 +300,10 @@ struct SGemm {
   int32_t xraw; const int32_t* rs; int32_t mraw[2];
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
-// FP32 matrices A (as assembled, with the column-sum partials and max |entry| of
-// launch_assemble_split) -> ||A||_1 per matrix, squaring count s, and B = A 2^-s in split form
-// into Bs with its exponent (from the exact max) and max |entry|
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
-                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax, int32_t sym);  // Bs null: s, exponent and max only
+// The column-sum partials and max |entry| of launch_assemble_split -> ||A||_1 per matrix, the
+// squaring count s, and the split exponent (from the exact max) and max |entry| of B = A 2^-s
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,
+                       double theta, int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax);
 // FP32 form of launch_norm_scale: ||A||_1 per matrix, squaring count, in-place scaling by
 // 2^-s, and the max |entry| of the scaled matrix into amax[b] (zeroed by the caller)
 void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, double theta,

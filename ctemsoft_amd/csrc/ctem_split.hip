@@ -767,21 +767,17 @@ void launch_assemble_split(hipStream_t s, const Bucket& B, const BeamView& V, co
 }
 
 // 1-norm (max column sum of |a|: the column partials of the assembly added in a fixed order),
-// squaring count s with ||A||_1 / 2^s <= theta, and B = A 2^-s written in split form with its
-// exponent and max |entry| (the max of A times 2^-s, exactly); one workgroup per matrix
-__global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __restrict__ A,
-                                                    const double* __restrict__ colpart,
-                                                    const float* __restrict__ amax, _Float16* __restrict__ Bs,
-                                                    double theta, int smax, int* __restrict__ sq,
-                                                    int* __restrict__ err, int* __restrict__ bexp,
-                                                    float* __restrict__ bmax, int sym) {
+// squaring count s with ||A||_1 / 2^s <= theta, and the split exponent and max |entry| of
+// B = A 2^-s (the max of A times 2^-s, exactly); one workgroup per matrix. B itself is not
+// stored: the Taylor products split A while staging it.
+__global__ __launch_bounds__(256) void k_norm_split(int npad, const double* __restrict__ colpart,
+                                                    const float* __restrict__ amax, double theta, int smax,
+                                                    int* __restrict__ sq, int* __restrict__ err,
+                                                    int* __restrict__ bexp, float* __restrict__ bmax) {
   const int b = blockIdx.x;
-  const int64_t per = (int64_t)npad * npad;
-  const float2* Ab = A + (int64_t)b * per;
   const int tiles = (npad + 63) / 64;  // row partials per column (64-row assembly tiles)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ double wnorm[4];
-  __shared__ int s_sh, e_sh;
   double cnorm = 0.0;
   for (int j = tid; j < npad; j += 256) {
     double sum = 0.0;
@@ -808,44 +804,16 @@ __global__ __launch_bounds__(256) void k_norm_split(int npad, const float2* __re
       if (nrm > theta) atomicOr(err, 4);
     }
     const float mb = ldexpf(m, -sc);
-    const int e = f16_scale_exp(mb);
-    s_sh = sc;
-    e_sh = e;
     sq[b] = sc;
-    bexp[b] = e;
+    bexp[b] = f16_scale_exp(mb);
     bmax[b] = mb;
-  }
-  if (!Bs) return;  // the products split A themselves
-  __syncthreads();
-  const float f = ldexpf(1.0f, e_sh - s_sh);
-  // one thread per 8-row group of a column: 64 contiguous bytes out (sym: only the groups of
-  // the 64 x 64 tiles on and above the diagonal, the only ones the symmetric products read)
-  _Float16* B = Bs + (int64_t)b * 4 * per;
-  const int ng = npad >> 3;
-  for (int w = tid; w < npad * ng; w += 256) {
-    const int col = w / ng, r0 = 8 * (w % ng);
-    if (sym && (r0 >> 6) > (col >> 6)) continue;
-    h8v o[4];
-    #pragma unroll
-    for (int e = 0; e < 8; e++) {
-      const float2 v = Ab[(int64_t)col * npad + r0 + e];
-      _Float16 h, l;
-      split2(v.x * f, h, l);
-      o[0][e] = h; o[1][e] = l;
-      split2(v.y * f, h, l);
-      o[2][e] = h; o[3][e] = l;
-    }
-    h8v* dst = (h8v*)(B + split_index(npad, r0, col, 0));
-    #pragma unroll
-    for (int p = 0; p < 4; p++) dst[p] = o[p];
   }
 }
 
-void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const float2* A, const double* colpart,
-                       const float* amax, uint16_t* Bs, double theta, int32_t smax, int32_t* sq, int32_t* err,
-                       int32_t* bexp, float* bmax, int32_t sym) {
-  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, A, colpart, amax, (_Float16*)Bs, theta, smax,
-                     sq, err, bexp, bmax, sym);
+void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,
+                       double theta, int32_t smax, int32_t* sq, int32_t* err, int32_t* bexp, float* bmax) {
+  hipLaunchKernelGGL(k_norm_split, dim3(nb), dim3(256), 0, s, npad, colpart, amax, theta, smax, sq, err, bexp,
+                     bmax);
 }
 
 }  // namespace ctem
