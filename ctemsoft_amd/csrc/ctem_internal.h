@@ -201,9 +201,6 @@ void launch_combo(hipStream_t s, int32_t npad, int32_t nb, double2* C, double be
 // copy matrices whose squaring count is odd from src to dst (so the result sits in dst)
 void launch_parity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                         const double2* src, double2* dst);
-// Psi[:,0] = E[:,0] per matrix; Psi has leading dimension npad and nzc columns
-void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
-                      int64_t spsi);
 
 // depth recursion on the GPU's last-level caches: Psi[b][:,k] = E_b^(k+1) e_1, FP64
 // accumulation; E is double2 (fp32 = 0) or float2 (fp32 = 1) holding E, or E - I when

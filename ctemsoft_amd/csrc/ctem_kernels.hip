@@ -469,18 +469,6 @@ void launch_parity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* 
   hipLaunchKernelGGL(k_parity_copy, dim3(gx, nb), dim3(256), 0, s, npad, sq, src, dst);
 }
 
-__global__ void k_first_col(int npad, const double2* __restrict__ E, double2* __restrict__ Psi,
-                            int64_t spsi) {
-  const int b = blockIdx.x;
-  for (int i = threadIdx.x; i < npad; i += blockDim.x)
-    Psi[(int64_t)b * spsi + i] = E[(int64_t)b * npad * npad + i];
-}
-
-void launch_first_col(hipStream_t s, int32_t npad, int32_t nb, const double2* E, double2* Psi,
-                      int64_t spsi) {
-  hipLaunchKernelGGL(k_first_col, dim3(nb), dim3(256), 0, s, npad, E, Psi, spsi);
-}
-
 // ------------------------------------------------------------------------------------------
 // depth recursion psi_{k+1} = E psi_k, k = 1..nz-1, psi_1 = E e_1 (CTEMEBSDmaster.f90:697-720
 // restated as a propagation). Persistent: one 1024-thread workgroup per CU walks the
