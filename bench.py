@@ -198,7 +198,7 @@ def main():
                                               ekev=None if not wl.energies else float(jobs[-1][0].split()[0]))
     gflops = r["gemm"]["gemm_flops"]
     line = {
-        "metric": "master-pattern directions/s, 1 GPU: mixed vs FP64 at <=1e-4 rel err",
+        "metric": "master-pattern directions/s, 1 GPU: mixed vs FP64 at \u22641e-4 rel err",
         "value": round(r["dps"], 2),
         "unit": "directions/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
