@@ -46,7 +46,7 @@ extern "C" {
 #define CTEM_PREC_MIXED 1   /* split 16-bit MFMA products with FP32 accumulation; FP64 norm,
                                squaring-count choice and depth accumulation; Lgh products in
                                FP32 and the final contraction in FP64 (intensities within 1e-4,
-                               measured <= 1.1e-5, of CTEM_PREC_FP64) */
+                               measured <= 1.2e-5, of CTEM_PREC_FP64) */
 
 /* split schemes of the mixed form's complex products (ctem_plan_set_mixed_terms) */
 #define CTEM_SPLIT_BF16X3 3   /* two BF16 planes, 3 MFMA products: ~2^-16 relative */
