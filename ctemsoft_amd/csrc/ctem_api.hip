@@ -914,17 +914,18 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   }
   // F0 = L R + B + y2 B2 (the final F where no squaring follows)
   float2* Ff = W32 + 3 * per * nb;
-  // matrices too large for the on-chip depth kernel: one squaring more gives E^2 - I (FP32 copy
-  // in buffer 6) for the depth recursion over pairs of depths
-  const bool pairs = npad > 224;
-  float2* Ff2 = pairs ? W32 + 6 * per * nb : nullptr;
-  const int extra = pairs ? 1 : 0;
+  // matrices too large for the on-chip depth kernel: two squarings more give E^2 - I and
+  // E^4 - I (FP32 copies in buffers 6 and 7) for the depth recursion over groups of four depths
+  const bool quads = npad > 224;
+  float2* Ff2 = quads ? W32 + 6 * per * nb : nullptr;
+  float2* Ff4 = quads ? W32 + 7 * per * nb : nullptr;
+  const int extra = quads ? 2 : 0;
   {
     SGemm g = mk(5, 0, 4);
     g.X = S[1];
     raw_b(g, 0, 1.f, 0.f);
     add_m(g, 1, 2, kBbc8[7], 0.f);
-    g.Cf = Ff; g.f_sq = d_sq; g.f_at = 0; g.Cf2 = Ff2;
+    g.Cf = Ff; g.f_sq = d_sq; g.f_at = 0; g.Cf2 = Ff2; g.Cf4 = Ff4;
     if ((rc = launch(g, f3))) return rc;
   }
   const int buf[2] = {5, 4};
@@ -940,13 +941,13 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     int nact = 0;
     while (nact < nb && h_sq[ord[nact]] + extra > i) nact++;
     g.order = order.as<int32_t>(); g.nord = nact;
-    g.Cf = Ff; g.f_sq = d_sq; g.f_at = i + 1; g.Cf2 = Ff2;
+    g.Cf = Ff; g.f_sq = d_sq; g.f_at = i + 1; g.Cf2 = Ff2; g.Cf4 = Ff4;
     if ((rc = launch(g, fl))) return rc;
   }
   // depth on E = I + F with FP64 accumulation (the depth kernel keeps the FP32 matrix on chip,
-  // or for larger matrices steps two depths at a time with E^2)
-  if (pairs)
-    launch_depth_pair(s, npad, nb, nz, Ff, Ff2, Psi64, spsi, depth_grid);
+  // or for larger matrices steps four depths at a time with E^4)
+  if (quads)
+    launch_depth_quad(s, npad, nb, nz, Ff, Ff2, Ff4, Psi64, spsi, depth_grid);
   else
     launch_depth(s, npad, nb, nz, Ff, 1, 1, Psi64, spsi, depth_grid);
   HIPCHK(hipGetLastError());

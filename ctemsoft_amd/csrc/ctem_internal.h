@@ -209,8 +209,8 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
                   int add_identity, double2* Psi, int64_t spsi, int32_t grid);
 // the depth recursion from F = E - I and F2 = E^2 - I (FP32, column-major): psi_1 = E e_1,
 // psi_2 = E psi_1, then psi_{k+2} = E^2 psi_k for two depths per pass over F2
-void launch_depth_pair(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const float2* F, const float2* F2,
-                       double2* Psi, int64_t spsi, int32_t grid);
+void launch_depth_quad(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const float2* F, const float2* F2,
+                       const float2* F4, double2* Psi, int64_t spsi, int32_t grid);
 
 // Lgh = (dz/thick) Psi diag(lam) Psi^H and the per-site contraction with S_p(g-h):
 // partial sums per 32x32 tile, reduced in fixed order into dots[dir*nsite + p]. f32: the Lgh
@@ -285,8 +285,11 @@ struct SGemm {
   // f_sq[b] == f_at: the last product of each matrix hands its result to the depth kernel
   float2* Cf; const int32_t* f_sq; int32_t f_at;
   // optional FP32 copy of the product after it (f_sq[b] + 1 == f_at): E^2 - I for the depth
-  // recursion over pairs of depths; with Cf2 set, a matrix's product at f_sq still stores C
+  // recursion over groups of depths; with Cf2 set, a matrix's product at f_sq still stores C
   float2* Cf2;
+  // optional FP32 copy of the product two after it (f_sq[b] + 2 == f_at): E^4 - I (needs Cf2;
+  // with Cf4 set, the product at f_sq + 1 still stores C)
+  float2* Cf4;
   // symmetric operands and result (X, Y, M and C polynomials in one complex-symmetric matrix):
   // only the 64 x 64 tiles on or above the diagonal of C are computed and stored (C2 alike),
   // and only those of X and Y are read (the rest through the symmetry); Cf is written in full

@@ -666,43 +666,47 @@ void launch_depth(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const voi
                        add_identity, Psi, spsi);
 }
 
-// Depth recursion over pairs of depths (mixed form, matrices too large to hold on chip): with
-// F = E - I and F2 = E^2 - I in FP32, psi_1 = e_1 + F e_1 (the first column), psi_2 = psi_1 +
-// F psi_1, then [psi_{k+2}, psi_{k+3}] = [psi_k, psi_{k+1}] + F2 [psi_k, psi_{k+1}]: each pass
-// over a matrix streamed from the caches serves two vectors, half the passes of k_depth. Rows
-// over 256 threads, columns in four quarters: FP32 products and sums within a quarter (from an
-// FP32 copy of the vectors), the quarters' sums and the recursion in FP64.
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_depth_pair(int npad, int nb, int nz, const float2* __restrict__ F,
-                                                     const float2* __restrict__ F2, double2* __restrict__ Psi,
-                                                     int64_t spsi) {
+// Depth recursion over groups of four depths (mixed form, matrices too large to hold on chip):
+// with F = E - I, F2 = E^2 - I and F4 = E^4 - I in FP32, psi_0 = e_1 + F e_1 (the first column
+// of E), psi_1 = psi_0 + F psi_0, [psi_2, psi_3] = [psi_0, psi_1] + F2 [psi_0, psi_1], then
+// [psi_{k+4} .. psi_{k+7}] = [psi_k .. psi_{k+3}] + F4 [psi_k .. psi_{k+3}]: each pass over a
+// matrix streamed from memory serves four vectors, a quarter of the passes of k_depth. Rows over
+// 512 threads, columns in two halves: FP32 products and sums within a half (from an FP32 copy of
+// the vectors), the halves' sums and the recursion in FP64. LDS: 160 npad bytes; 86 registers,
+// one workgroup per CU (a 64-register build for two spills, and measured 7% slower on corundum).
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_depth_quad(
+    int npad, int nb, int nz, const float2* __restrict__ F, const float2* __restrict__ F2,
+    const float2* __restrict__ F4, double2* __restrict__ Psi, int64_t spsi) {
   extern __shared__ double2 sm[];
-  double2* x = sm;                 // [2][npad] the two current vectors
-  double2* part = sm + 2 * npad;   // [4 quarters][2][npad]
-  float2* xf = (float2*)(sm + 10 * npad);  // [2][npad] FP32 copy of x
-  const int tid = threadIdx.x, q = tid >> 8, r0 = tid & 255;
+  double2* x = sm;                          // [4][npad] the current vectors
+  float2* part = (float2*)(sm + 4 * npad);  // [2 halves][4][npad] FP32 partial sums
+  float2* xf = part + 8 * npad;             // [4][npad] FP32 copy of x
+  const int tid = threadIdx.x, h = tid >> 9, r0 = tid & 511;
   const int64_t per = (int64_t)npad * npad;
-  const int cq = (npad + 3) / 4;
-  const int c0 = q * cq, c1 = min(npad, c0 + cq);
+  const int ch = (npad + 1) / 2;
+  const int c0 = h * ch, c1 = min(npad, c0 + ch);
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const float2* Fb = F + (int64_t)b * per;
     const float2* F2b = F2 + (int64_t)b * per;
+    const float2* F4b = F4 + (int64_t)b * per;
     double2* P = Psi + (int64_t)b * spsi;
-    // out[v] = in[v] + M in[v] for the vectors in slots 0 .. nv-1, into slots o .. o+nv-1,
-    // stored as depths k .. k+nv-1 (those below nz)
-    auto pass = [&](const float2* M, int nv, int o, int k) __attribute__((always_inline)) {
-      for (int r = r0; r < npad; r += 256) {
-        float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f;
+    // out[v] = in[v] + M in[v] for the vectors in slots 0 .. NV-1, into slots o .. o+NV-1,
+    // stored as depths k .. k+NV-1 (those below nz)
+    auto pass = [&](const float2* M, auto nvc, int o, int k) __attribute__((always_inline)) {
+      constexpr int NV = decltype(nvc)::value;
+      for (int r = r0; r < npad; r += 512) {
+        float ar[NV], ai[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) { ar[v] = 0.f; ai[v] = 0.f; }
         auto mac = [&](const float2 e, int c) __attribute__((always_inline)) {
-          const float2 u = xf[c];
-          a0r = fmaf(e.x, u.x, a0r); a0r = fmaf(-e.y, u.y, a0r);
-          a0i = fmaf(e.x, u.y, a0i); a0i = fmaf(e.y, u.x, a0i);
-          if (nv == 2) {
-            const float2 w = xf[npad + c];
-            a1r = fmaf(e.x, w.x, a1r); a1r = fmaf(-e.y, w.y, a1r);
-            a1i = fmaf(e.x, w.y, a1i); a1i = fmaf(e.y, w.x, a1i);
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const float2 u = xf[v * npad + c];
+            ar[v] = fmaf(e.x, u.x, ar[v]); ar[v] = fmaf(-e.y, u.y, ar[v]);
+            ai[v] = fmaf(e.x, u.y, ai[v]); ai[v] = fmaf(e.y, u.x, ai[v]);
           }
         };
-        // four loads in flight before their products (two workgroups per CU: 64 registers)
+        // four loads in flight before their products
         int c = c0;
         for (; c + 4 <= c1; c += 4) {
           float2 e[4];
@@ -712,32 +716,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
           for (int j = 0; j < 4; j++) mac(e[j], c + j);
         }
         for (; c < c1; c++) mac(M[(int64_t)c * npad + r], c);
-        part[(2 * q) * npad + r] = make_double2(a0r, a0i);
-        if (nv == 2) part[(2 * q + 1) * npad + r] = make_double2(a1r, a1i);
+#pragma unroll
+        for (int v = 0; v < NV; v++) part[(4 * h + v) * npad + r] = make_float2(ar[v], ai[v]);
       }
       __syncthreads();
-      for (int i = tid; i < nv * npad; i += 1024) {
+      // one thread per (vector, row) reads x there and writes the output slot, so the output
+      // slots may be the input slots
+      for (int i = tid; i < NV * npad; i += 1024) {
         const int v = i / npad, r = i - v * npad;
-        double2 y = part[v * npad + r];
-        for (int qq = 1; qq < 4; qq++) {
-          const double2 t = part[(2 * qq + v) * npad + r];
-          y.x += t.x;
-          y.y += t.y;
-        }
-        y.x += x[v * npad + r].x;
-        y.y += x[v * npad + r].y;
+        const float2 p0 = part[v * npad + r], p1 = part[(4 + v) * npad + r];
+        const double2 xi = x[v * npad + r];
+        const double2 y = make_double2(xi.x + ((double)p0.x + (double)p1.x), xi.y + ((double)p0.y + (double)p1.y));
         if (k + v < nz) P[(int64_t)(k + v) * npad + r] = y;
-        part[v * npad + r] = y;  // staged; copied into the output slots after the barrier
-      }
-      __syncthreads();
-      for (int i = tid; i < nv * npad; i += 1024) {
-        const double2 y = part[i];
-        x[o * npad + i] = y;
-        xf[o * npad + i] = make_float2((float)y.x, (float)y.y);
+        x[(o + v) * npad + r] = y;
+        xf[(o + v) * npad + r] = make_float2((float)y.x, (float)y.y);
       }
       __syncthreads();
     };
-    __syncthreads();  // the previous direction is done with x
+    __syncthreads();  // the previous direction is done with x and xf
     for (int i = tid; i < npad; i += 1024) {
       const float2 f = Fb[i];
       const double2 v = make_double2((double)f.x + (i == 0 ? 1.0 : 0.0), (double)f.y);
@@ -746,16 +742,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
       P[i] = v;
     }
     __syncthreads();
-    if (nz > 1) pass(Fb, 1, 1, 1);
-    for (int k = 2; k < nz; k += 2) pass(F2b, 2, 0, k);
+    if (nz > 1) pass(Fb, std::integral_constant<int, 1>{}, 1, 1);
+    if (nz > 2) pass(F2b, std::integral_constant<int, 2>{}, 2, 2);
+    for (int k = 4; k < nz; k += 4) pass(F4b, std::integral_constant<int, 4>{}, 0, k);
   }
 }
 
-void launch_depth_pair(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const float2* F, const float2* F2,
-                       double2* Psi, int64_t spsi, int32_t grid) {
-  const size_t lds = (size_t)10 * npad * sizeof(double2) + (size_t)2 * npad * sizeof(float2);
-  const int g = std::min(nb, 2 * grid);  // two workgroups per CU fit
-  hipLaunchKernelGGL(k_depth_pair, dim3(g), dim3(1024), lds, s, npad, nb, nz, F, F2, Psi, spsi);
+void launch_depth_quad(hipStream_t s, int32_t npad, int32_t nb, int32_t nz, const float2* F, const float2* F2,
+                       const float2* F4, double2* Psi, int64_t spsi, int32_t grid) {
+  const size_t lds = (size_t)4 * npad * sizeof(double2) + (size_t)12 * npad * sizeof(float2);
+  const int g = std::min(nb, 2 * grid);  // two rounds of one workgroup per CU
+  hipLaunchKernelGGL(k_depth_quad, dim3(g), dim3(1024), lds, s, npad, nb, nz, F, F2, F4, Psi, spsi);
 }
 
 // ------------------------------------------------------------------------------------------

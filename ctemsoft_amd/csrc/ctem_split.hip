@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     g.ce[b] = ec;
     if (g.C2) g.ce2[b] = ec2;
   }
-  // the FP32 copy for the depth kernel (F, or E^2 - I after it) is read in full: off-diagonal
+  // the FP32 copy for the depth kernel (F, and E^2 - I, E^4 - I after it) is read in full: off-diagonal
   // tiles also store their transpose there; after the matrix's last product no C is stored
   float2* Cf = nullptr;
   bool last = false;
@@ -344,6 +344,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     last = g.Cf2 == nullptr;
   } else if (g.Cf2 && g.f_sq[b] + 1 == g.f_at) {
     Cf = g.Cf2 + (int64_t)b * per;
+    last = g.Cf4 == nullptr;
+  } else if (g.Cf4 && g.f_sq[b] + 2 == g.f_at) {
+    Cf = g.Cf4 + (int64_t)b * per;
     last = true;
   }
   const bool mirror = offd && Cf != nullptr;
