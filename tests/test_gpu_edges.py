@@ -89,10 +89,11 @@ def test_one_depth_step_against_oracle(ctx, oracle_mod):
 
 
 @pytest.mark.parametrize("name", ["fe_c1w", "al2o3_c2w"])
-@pytest.mark.parametrize("nz", [1, 2, 3])
+@pytest.mark.parametrize("nz", [1, 2, 3, 5, 6])
 def test_short_depth_profiles_mixed_vs_fp64(ctx, name, nz):
-    """One to three depth steps: the on-chip recursion (Fe) and the pair recursion (corundum,
-    whose first pass gives depth 1 and whose pair passes start at depth 2) against FP64."""
+    """One to six depth steps: the on-chip recursion (Fe) and the four-depth recursion
+    (corundum, whose first pass gives depth 1, its E^2 pass depths 2 and 3, and whose E^4 passes
+    start at depth 4) against FP64."""
     fx = _upload(ctx, name)
     ks = np.sort(np.random.default_rng(nz).choice(len(fx["kvec"]), 256, replace=False))
     plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"][:nz], float(fx["dz"]), float(fx["thick"]))
