@@ -2,11 +2,18 @@
 """Benchmark: EBSD master-pattern directions/s on one MI355X.
 
 One step = the whole master pattern of the workload: every incident direction of both
-Lambert hemispheres through beam selection, DynMat assembly, the depth-integrated Lgh and
-the Sgh contraction, with the inputs (per-energy setup and wave vectors) resident in HBM
-before the timed region. `value` is directions per second of the mixed-precision form; the
-FP64 form is timed in the same run and reported beside it with the ratio and the error of
-the mixed form against it.
+Lambert hemispheres through beam selection (ctem_plan_select, run again in every step),
+DynMat assembly, the depth-integrated Lgh and the Sgh contraction, with the inputs
+(per-energy setup and wave vectors) resident in HBM before the timed region. `value` is
+directions per second of the mixed-precision form; the FP64 form is timed in the same run
+and reported beside it with the ratio and the error of the mixed form against it.
+
+roofline: the dominant kernel is the matrix-exponential GEMM. `achieved` prices its work
+with the path's algorithmic count, 8 n^3 (q + j) per direction with the Pade degree q and
+squaring count j that the reference's MatrixExponential would pick for that direction's
+matrix (math.f90:208-219; ctem_plan_alg_flops), divided by the GEMM kernels' time measured
+with HIP events on the plan's stream; `kernel_flops_tflops` is the GEMM's own product count
+(8 n^3, or n^2 (n+1) 4 in the symmetric mode, per product launched) over the same time.
 
     python bench.py [--steps K] [--warmup W] [--workload fe_c1w] [--precision both|fp64|mixed]
 
@@ -90,12 +97,14 @@ def measured_traffic(workload, head):
     return None
 
 
-def cpu_baseline(wl, lam, k_count, seconds=20.0, ekev=None):
+def cpu_baseline(wl, lam, k_count, seconds=20.0, ekev=None, all_cores=True):
     """Reference beam loop (compiled from the reference sources, MKL sequential) on one
-    host core over a seeded sample of the workload's northern directions."""
+    host core over a seeded sample of the workload's northern directions; then the same loop
+    as one process per host core (the reference's loop is serial, so its all-cores form is
+    independent processes over disjoint directions), over a sample sized for ~`seconds`."""
     from oracle import refrun
     if not refrun.have_reference():
-        return None
+        return None, None
     rng = np.random.default_rng(wl.seed)
     with tempfile.TemporaryDirectory(prefix="ctem_cpu_") as work:
         # calibrate on 4 directions, then size the sample to ~`seconds`
@@ -104,10 +113,21 @@ def cpu_baseline(wl, lam, k_count, seconds=20.0, ekev=None):
         n = int(max(8, min(512, 4 * seconds / t4)))
         ks = np.sort(rng.choice(k_count, n, replace=False))
         t = refrun.time_reference(work, wl, lam, ks, ekev)
-    return dict(value=n / t, unit="directions/s", cores=1, kind="reference",
-                sample=f"{n} seeded directions of {wl.name}"
-                       f"{'' if ekev is None else f' at {ekev:g} kV'} (northern hemisphere), Compute_DynMat + "
-                       f"CalcLgh3 per direction, amdflang -O2 + MKL 2021.4 sequential, {t:.1f} s")
+        one = dict(value=n / t, unit="directions/s", cores=1, kind="reference",
+                   sample=f"{n} seeded directions of {wl.name}"
+                          f"{'' if ekev is None else f' at {ekev:g} kV'} (northern hemisphere), Compute_DynMat + "
+                          f"CalcLgh3 per direction, amdflang -O2 + MKL 2021.4 sequential, {t:.1f} s")
+        allc = None
+        if all_cores:
+            procs = max(1, min(16, len(os.sched_getaffinity(0))))
+            if procs > 1:
+                m = int(min(k_count, max(procs, procs * n)))
+                ks2 = np.sort(rng.choice(k_count, m, replace=False))
+                t2 = refrun.time_reference_parallel(work, wl, lam, ks2, procs, ekev)
+                allc = dict(value=m / t2, unit="directions/s", cores=procs, kind="reference",
+                            sample=f"{m} seeded directions in {procs} processes (one per core, disjoint "
+                                   f"directions), same build, {t2:.1f} s of beam loop")
+    return one, allc
 
 
 def main():
@@ -149,6 +169,12 @@ def main():
         plans.append(plan)
         ndir += len(k)
     ns, nw = np.concatenate(ns_all), np.concatenate(nw_all)
+
+    def step(prec):
+        # one pass of the path: beam selection and the whole master pattern of every bin
+        for plan in plans:
+            plan.select()
+            plan.run(prec)
     name, cus = ctx.device_info()
     log(f"device {name}, {cus} CUs; workload {wl.name}: {len(jobs)} energy bin(s), {ndir} directions, nz={wl.nz}")
     log(f"strong beams {ns.min()}..{ns.max()} (mean {ns.mean():.1f}); weak {nw.min()}..{nw.max()} (mean {nw.mean():.1f})")
@@ -157,8 +183,7 @@ def main():
     for prec in precs:
         try:
             for w in range(a.warmup):
-                for plan in plans:
-                    plan.run(prec)
+                step(prec)
                 log(f"{prec}: warmup {w + 1}/{a.warmup} done")
         except ebsd.CtemError as e:
             log(f"{prec}: not available ({e})")
@@ -166,14 +191,16 @@ def main():
         times = []
         for s_ in range(a.steps):
             t0 = time.perf_counter()
-            for plan in plans:
-                plan.run(prec)
+            step(prec)
             times.append(time.perf_counter() - t0)
             log(f"{prec}: step {s_ + 1}/{a.steps} {times[-1] * 1e3:.1f} ms")
         tims = [plan.timing() for plan in plans]
         g = dict(gemm_ms=sum(t["gemm_ms"] for t in tims), gemm_flops=sum(t["gemm_flops"] for t in tims),
                  gemm_launches=sum(t["gemm_launches"] for t in tims))
         dots = np.concatenate([plan.dots().copy() for plan in plans])
+        if prec == "fp64":
+            afl = [plan.alg_flops() for plan in plans]
+            alg = {k: sum(f[k] for f in afl) for k in ("total", "products", "solve")}
         ms = float(np.mean(times) * 1e3)
         res[prec] = dict(ms=ms, dps=ndir / (ms * 1e-3), gemm=g, dots=dots,
                          gemm_tflops=g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12 if g["gemm_ms"] > 0 else 0.0)
@@ -194,9 +221,13 @@ def main():
         err = float(np.max(np.abs(dmx[fin] - d64[fin]) / np.abs(d64[fin]))) if fin.any() else None
         if np.isfinite(dmx[~fin]).all(1).any():
             err = float("inf")
-    cpu = None if a.no_cpu else cpu_baseline(wl, jobs[-1][2], len(plans[-1].k) // wl.hemispheres,
-                                              ekev=None if not wl.energies else float(jobs[-1][0].split()[0]))
+    cpu, cpu_all = (None, None) if a.no_cpu else cpu_baseline(
+        wl, jobs[-1][2], len(plans[-1].k) // wl.hemispheres,
+        ekev=None if not wl.energies else float(jobs[-1][0].split()[0]))
     gflops = r["gemm"]["gemm_flops"]
+    gsec = r["gemm"]["gemm_ms"] * 1e-3
+    # the path's algorithmic product flops (reference Pade rule) over the GEMM kernels' time
+    ach = alg["products"] / gsec / 1e12 if ("fp64" in res and gsec > 0) else None
     line = {
         "metric": "master-pattern directions/s, 1 GPU: mixed vs FP64 at \u22641e-4 rel err",
         "value": round(r["dps"], 2),
@@ -216,13 +247,21 @@ def main():
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,
         "mixed_max_rel_err_vs_fp64": err,
         "nonfinite_directions": nonfinite if err is not None else None,
-        "roofline": {"bound": "mfma", "achieved": round(r["gemm_tflops"], 3),
+        "roofline": {"bound": "mfma", "achieved": None if ach is None else round(ach, 3),
                      "peak": PEAK[head] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),
+                     "frac": None if ach is None else round(ach * 1e12 / PEAK[head], 4),
                      "traffic": measured_traffic(wl.name, head),
                      "kernel": "k_zgemm" if head == "fp64" else ("k_sgemm" if a.terms == "16,16" else "k_cgemm"),
-                     "flops_per_step": gflops},
+                     "alg_flops_per_step": alg["products"] if "fp64" in res else None,
+                     "kernel_ms_per_step": round(r["gemm"]["gemm_ms"], 3),
+                     "kernel_launches_per_step": r["gemm"]["gemm_launches"],
+                     "kernel_flops_tflops": round(r["gemm_tflops"], 3),
+                     "kernel_flops_frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),
+                     "kernel_flops_per_step": gflops},
+        "path_alg_flops_per_step": alg["total"] if "fp64" in res else None,
+        "path_alg_tflops": round(alg["total"] / (r["ms"] * 1e-3) / 1e12, 3) if "fp64" in res else None,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(line), flush=True)
     for plan in plans:

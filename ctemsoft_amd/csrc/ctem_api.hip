@@ -117,10 +117,10 @@ struct Crystal {
 // Work space of ctem_plan_run and ctem_plan_adjoint, shared by the plans of a context (calls
 // on one context are sequential); it grows to the largest chunk run and is kept.
 struct Work {
-  DBuf A, W, Psi, sq, dirs, partial, wmax, colsum, order;
+  DBuf A, W, Psi, sq, dirs, partial, wmax, colsum, order, amod;
   DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
   void release() {
-    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &order, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
+    for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &order, &amod, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
                     &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig})
       b->release();
   }
@@ -158,6 +158,10 @@ struct ctem_plan {
   // timing of the last run
   double t_total = 0, t_gemm = 0, f_gemm = 0;
   int32_t n_gemm = 0;
+  // algorithmic flops of the last FP64 run by the reference's Pade rule (math.f90:208-219):
+  // total F = 8n^2w + 8n^3(q+j) + (32/3)n^3 + 16n^2Z + 4n^2P, its product part 8n^3(q+j) and
+  // its solve part (32/3)n^3, summed over the directions
+  double fa_total = -1, fa_prod = -1, fa_solve = -1;
   int32_t terms_taylor = CTEM_SPLIT_FP16X3, terms_square = CTEM_SPLIT_FP16X3;  // mixed-form split schemes
   int32_t symmetry = 1;  // split mixed form: symmetric products when the crystal's LUT allows
   std::vector<hipEvent_t> evpool;
@@ -629,7 +633,8 @@ constexpr int kSmax = 48;
 // receives the squaring counts.
 double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                    int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n,
-                   std::vector<int32_t>& h_sq, int32_t& rc_out) {
+                   std::vector<int32_t>& h_sq, int32_t& rc_out, double* d_amod = nullptr,
+                   std::vector<double>* h_amod = nullptr) {
   rc_out = CTEM_OK;
   const int64_t per = (int64_t)npad * npad;
   double2* A2 = W;
@@ -638,10 +643,14 @@ double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, d
   double2* H = W + 3 * per * nb;
   double2* T = W + 4 * per * nb;
 #define EXPM_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { rc_out = fail(CTEM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); return nullptr; } } while (0)
-  launch_norm_scale(s, npad, nb, A, kTheta64, kSmax, d_sq, d_err);
+  launch_norm_scale(s, npad, nb, A, kTheta64, kSmax, d_sq, d_err, d_amod);
   EXPM_CHK(hipGetLastError());
   h_sq.resize(nb);
   EXPM_CHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  if (d_amod && h_amod) {
+    h_amod->resize(nb);
+    EXPM_CHK(hipMemcpyAsync(h_amod->data(), d_amod, 8 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  }
   EXPM_CHK(hipStreamSynchronize(s));
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
@@ -697,9 +706,10 @@ double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, d
 // FP64 form: expm, then psi_k = E^k e_1 by the depth recursion
 int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                         double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
-                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid) {
+                        const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid,
+                        double* d_amod = nullptr, std::vector<double>* h_amod = nullptr) {
   int32_t rc;
-  double2* E = expm_fp64(tp, s, npad, nb, A, W, d_sq, d_err, h_n, h_sq, rc);
+  double2* E = expm_fp64(tp, s, npad, nb, A, W, d_sq, d_err, h_n, h_sq, rc, d_amod, h_amod);
   if (!E) return rc;
   launch_depth(s, npad, nb, nz, E, 0, 0, Psi, spsi, depth_grid);
   HIPCHK(hipGetLastError());
@@ -959,6 +969,25 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // bucket sizes: multiples of 32
 static int npad_of(int n) { return std::max(32, (n + 31) / 32 * 32); }
 
+// Algorithmic flops of one direction: F = 8n^2w + 8n^3(q+j) + (32/3)n^3 + 16n^2Z + 4n^2P (the
+// Bethe product, the Pade powers and squarings, the Pade solve, propagation with psi psi^H
+// accumulation, the S contraction), with the Pade degree q and squaring count j that the
+// reference's MatrixExponential picks for this matrix: modA = max |a_ij| of
+// A = 2 pi i dz DynMat / fl32(2kn), ilev = nint(log10(modA)) + 3 clamped to 1..6
+// (math.f90:208-219). amod is max |a_ij| of A.
+static void alg_flops(int n, int w, int nz, int nsite, double amod, double /*dz*/, double& tot,
+                      double& prod, double& solve) {
+  static const int qPade[6] = {2, 3, 4, 4, 4, 4}, jPade[6] = {0, 0, 1, 5, 8, 11};
+  int ilev = (amod > 0.0 && std::isfinite(amod)) ? (int)std::lround(std::log10(amod)) + 3 : 1;
+  ilev = std::min(6, std::max(1, ilev));
+  const double q = qPade[ilev - 1], j = jPade[ilev - 1];
+  const double n2 = (double)n * n, n3 = n2 * n;
+  const double fp = 8.0 * n3 * (q + j), fs = 32.0 / 3.0 * n3;
+  prod += fp;
+  solve += fs;
+  tot += 8.0 * n2 * w + fp + fs + 16.0 * n2 * nz + 4.0 * n2 * nsite;
+}
+
 extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
   if (!p) return fail(CTEM_E_ARG, "null plan");
   if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
@@ -975,6 +1004,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
   p->t_total = p->t_gemm = p->f_gemm = 0.0;
   p->n_gemm = 0;
   p->evused = 0;
+  double fa[3] = {0.0, 0.0, 0.0};
   if ((rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * std::max(1, p->cr->nsite)))) return rc;
   p->keep_dir = keep_lgh_ik;
   p->keep_n = (keep_lgh_ik >= 0) ? p->h_nns[keep_lgh_ik] : 0;
@@ -1037,8 +1067,17 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
       HIPCHK(hipGetLastError());
       if (precision == CTEM_PREC_FP64) {
+        if ((rc = c->w.amod.ensure(8 * (size_t)nb))) return rc;
+        std::vector<double> hmod;
         rc = expm_depth_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
-                             p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count);
+                             p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count,
+                             c->w.amod.as<double>(), &hmod);
+        if (!rc) {
+          for (int b = 0; b < nb; b++) {
+            const int d = kv.second[c0 + b];
+            alg_flops(hn[b], p->h_nnw[d], p->nz, p->cr->nsite, hmod[b], p->dz, fa[0], fa[1], fa[2]);
+          }
+        }
       } else {
         if (split)
           rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
@@ -1068,6 +1107,7 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
     p->t_gemm += g;
   }
   if ((rc = check_err_flag(p->err, s, "ctem_plan_run"))) return rc;
+  if (precision == CTEM_PREC_FP64) { p->fa_total = fa[0]; p->fa_prod = fa[1]; p->fa_solve = fa[2]; }
   p->ran = true;
   return CTEM_OK;
 }
@@ -1085,6 +1125,8 @@ extern "C" int32_t ctem_plan_set_mixed_terms(ctem_plan* p, int32_t taylor_terms,
 extern "C" int32_t ctem_plan_set_symmetry(ctem_plan* p, int32_t mode, int32_t* in_effect) {
   if (!p) return fail(CTEM_E_ARG, "null plan");
   if (mode != 0 && mode != 1) return fail(CTEM_E_ARG, "symmetry mode must be 0 (off) or 1 (auto)");
+  if (!p->cr) p->cr = p->ctx->cr;
+  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_set_symmetry: no crystal uploaded");
   p->symmetry = mode;
   if (in_effect) *in_effect = (mode == 1 && p->cr->centro) ? 1 : 0;
   return CTEM_OK;
@@ -1294,18 +1336,23 @@ extern "C" int32_t ctem_plan_scatter(ctem_plan* p, const int32_t* ij, int32_t np
   if (!ij || !sr || npx < 0 || npy < 0) return fail(CTEM_E_ARG, "bad arguments");
   ctem_ctx* c = p->ctx;
   hipStream_t s = c->stream;
+  if (isym < 1 || isym > 13) return fail(CTEM_E_ARG, "ctem_plan_scatter: Laue group %d not in 1..13", isym);
   const size_t plane = (size_t)(2 * npx + 1) * (2 * npy + 1);
   const size_t tot = plane * std::max(1, p->cr->nsite);
-  DBuf d_sr;
+  DBuf d_sr, d_owner;
   int32_t rc;
-  if ((rc = upload(p->ij, ij, (size_t)2 * p->numk, s)) || (rc = d_sr.ensure(4 * tot))) return rc;
+  if ((rc = upload(p->ij, ij, (size_t)2 * p->numk, s)) || (rc = d_sr.ensure(4 * tot)) ||
+      (rc = d_owner.ensure(4 * plane)))
+    return rc;
   HIPCHK(hipMemcpyAsync(d_sr.p, sr, 4 * tot, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(d_owner.p, 0xff, 4 * plane, s));
   launch_scatter(s, p->numk, p->ij.as<int32_t>(), p->dots.as<double>(), p->cr->nsite, p->cr->nat, npx, npy,
-                 isym, d_sr.as<float>());
+                 isym, d_owner.as<int32_t>(), d_sr.as<float>());
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(sr, d_sr.p, 4 * tot, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   d_sr.release();
+  d_owner.release();
   if (e != hipSuccess) return fail(CTEM_E_HIP, "scatter: %s", hipGetErrorString(e));
   return CTEM_OK;
 }
@@ -1317,6 +1364,14 @@ extern "C" int32_t ctem_plan_timing(ctem_plan* p, double* total_ms, double* gemm
   if (gemm_ms) *gemm_ms = p->t_gemm;
   if (gemm_flops) *gemm_flops = p->f_gemm;
   if (gemm_launches) *gemm_launches = p->n_gemm;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_alg_flops(ctem_plan* p, double* total, double* products, double* solve) {
+  if (!p || p->fa_total < 0) return fail(CTEM_E_STATE, "ctem_plan_alg_flops: run the plan in FP64 first");
+  if (total) *total = p->fa_total;
+  if (products) *products = p->fa_prod;
+  if (solve) *solve = p->fa_solve;
   return CTEM_OK;
 }
 

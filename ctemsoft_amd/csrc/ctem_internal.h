@@ -176,9 +176,10 @@ void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevR
                    const DevLut& L, double mlambda, float dynupz, double2* D);
 
 // ||A||_1 per matrix, squaring count s with ||A||_1 / 2^s <= theta, in-place scaling by 2^-s.
-// sq[b] receives s; err is set on a non-finite norm or s > smax.
+// sq[b] receives s; err is set on a non-finite norm or s > smax. amod (may be null) receives
+// max |a_ij| of each unscaled matrix.
 void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, double theta,
-                       int32_t smax, int32_t* sq, int32_t* err);
+                       int32_t smax, int32_t* sq, int32_t* err, double* amod);
 
 // Batched complex GEMM C = alpha*X*Y + betaI*I + sum_j beta_j*M_j on FP64 MFMA.
 struct ZGemm {
@@ -220,9 +221,11 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const double* lam, double dzt, double* partial, double* dots,
                      int32_t keep_dir, double2* keep_lgh, int32_t f32);
 
-// scatter fl32 svals to the Laue-equivalent pixels of sr[nsite][2npy+1][2npx+1]
+// scatter fl32 svals to the Laue-equivalent pixels of sr[nsite][2npy+1][2npx+1]; owner
+// [(2npy+1)(2npx+1)] int32 must hold -1 on entry (the last direction in list order wins a pixel)
 void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,
-                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym, float* sr);
+                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym,
+                    int32_t* owner, float* sr);
 
 // ---- mixed precision (ctem_mixed.hip) ----
 // Complex GEMM on split 16-bit MFMA (BF16 or FP16 planes) with FP32 accumulation for FP32

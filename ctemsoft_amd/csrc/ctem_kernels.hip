@@ -10,7 +10,8 @@
 //   k_lgh_dots     Lgh = (dz/thick) Psi diag(lam) Psi^H (upper-triangle tiles, Lgh being
 //                  Hermitian) and Re sum Lgh (.) S_p (CTEMEBSDmaster
 //                  .f90:456, with Lgh as in :697-720) in fixed-order partial sums
-//   k_scatter      float32 svals to the Laue-equivalent Lambert pixels (:462-479)
+//   k_scatter      float32 svals to the Laue-equivalent Lambert pixels (:462-479), the last
+//                  direction in list order winning a pixel, as in the reference's serial loop
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -276,28 +277,34 @@ void launch_dynmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevR
 
 // ------------------------------------------------------------------------------------------
 // norm and scaling
+// amod (may be null) receives max |a_ij| of the unscaled matrix (the quantity the reference's
+// MatrixExponential picks its Pade degree and squaring count from, math.f90:217-219)
 __global__ __launch_bounds__(256) void k_norm_scale(int npad, double2* __restrict__ A,
                                                     double theta, int smax, int* __restrict__ sq,
-                                                    int* __restrict__ err) {
+                                                    int* __restrict__ err, double* __restrict__ amod) {
   const int b = blockIdx.x;
   double2* Ab = A + (int64_t)b * npad * npad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ double wmax[4];
+  __shared__ double wmax[4], wemax[4];
   __shared__ int s_sh;
-  double cmax = 0.0;
+  double cmax = 0.0, emax = 0.0;
   for (int j = wave; j < npad; j += 4) {
     double sum = 0.0;
     for (int i = lane; i < npad; i += 64) {
       const double2 v = Ab[(int64_t)j * npad + i];
-      sum += hypot(v.x, v.y);
+      const double h = hypot(v.x, v.y);
+      sum += h;
+      emax = fmax(emax, h);
     }
     for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
     cmax = fmax(cmax, sum);
     if (!(sum == sum)) cmax = sum;  // keep a NaN
   }
-  if (lane == 0) wmax[wave] = cmax;
+  for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o));
+  if (lane == 0) { wmax[wave] = cmax; wemax[wave] = emax; }
   __syncthreads();
   if (tid == 0) {
+    if (amod) amod[b] = fmax(fmax(wemax[0], wemax[1]), fmax(wemax[2], wemax[3]));
     double nrm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
     for (int w = 0; w < 4; w++) if (!(wmax[w] == wmax[w])) nrm = wmax[w];
     int s = 0;
@@ -321,8 +328,8 @@ __global__ __launch_bounds__(256) void k_norm_scale(int npad, double2* __restric
 }
 
 void launch_norm_scale(hipStream_t s, int32_t npad, int32_t nb, double2* A, double theta,
-                       int32_t smax, int32_t* sq, int32_t* err) {
-  hipLaunchKernelGGL(k_norm_scale, dim3(nb), dim3(256), 0, s, npad, A, theta, smax, sq, err);
+                       int32_t smax, int32_t* sq, int32_t* err, double* amod) {
+  hipLaunchKernelGGL(k_norm_scale, dim3(nb), dim3(256), 0, s, npad, A, theta, smax, sq, err, amod);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -960,27 +967,50 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
 
 // ------------------------------------------------------------------------------------------
 // Lambert scatter
+// Several directions can map to one pixel (hexagonal grids: the (i,0) row is listed twice and
+// neighbouring hexagonal cells share square-array pixels). The reference's serial loop leaves
+// the value of the last direction in list order (CTEMEBSDmaster.f90:410-479); the scatter keeps
+// that order in two passes: the largest direction index that reaches each pixel, then the
+// stores of that direction only.
+__global__ void k_scatter_owner(int numk, const int32_t* __restrict__ ij, int npx, int npy, int isym,
+                                int32_t* __restrict__ owner) {
+  const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ik >= numk) return;
+  int32_t q[24];
+  const int ne = laue_equiv(ij[2 * ik], ij[2 * ik + 1], isym, q);
+  for (int e = 0; e < ne; e++) {
+    const int i = q[2 * e], j = q[2 * e + 1];
+    if (i < -npx || i > npx || j < -npy || j > npy) continue;
+    atomicMax(&owner[(int64_t)(j + npy) * (2 * npx + 1) + (i + npx)], ik);
+  }
+}
+
 __global__ void k_scatter(int numk, const int32_t* __restrict__ ij, const double* __restrict__ dots,
-                          int nsite, float nat, int npx, int npy, int isym, float* __restrict__ sr) {
+                          int nsite, float nat, int npx, int npy, int isym,
+                          const int32_t* __restrict__ owner, float* __restrict__ sr) {
   const int ik = blockIdx.x * blockDim.x + threadIdx.x;
   if (ik >= numk) return;
   int32_t q[24];
   const int ne = laue_equiv(ij[2 * ik], ij[2 * ik + 1], isym, q);
   const int64_t plane = (int64_t)(2 * npx + 1) * (2 * npy + 1);
-  for (int p = 0; p < nsite; p++) {
-    const float v = __fdiv_rn((float)dots[(int64_t)ik * nsite + p], nat);
-    for (int e = 0; e < ne; e++) {
-      const int i = q[2 * e], j = q[2 * e + 1];
-      if (i < -npx || i > npx || j < -npy || j > npy) continue;
-      sr[p * plane + (int64_t)(j + npy) * (2 * npx + 1) + (i + npx)] = v;
-    }
+  for (int e = 0; e < ne; e++) {
+    const int i = q[2 * e], j = q[2 * e + 1];
+    if (i < -npx || i > npx || j < -npy || j > npy) continue;
+    const int64_t pix = (int64_t)(j + npy) * (2 * npx + 1) + (i + npx);
+    if (owner[pix] != ik) continue;
+    for (int p = 0; p < nsite; p++)
+      sr[p * plane + pix] = __fdiv_rn((float)dots[(int64_t)ik * nsite + p], nat);
   }
 }
 
 void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,
-                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym, float* sr) {
+                    int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym,
+                    int32_t* owner, float* sr) {
+  if (numk <= 0) return;
+  hipLaunchKernelGGL(k_scatter_owner, dim3((numk + 127) / 128), dim3(128), 0, s, numk, ij, npx, npy,
+                     isym, owner);
   hipLaunchKernelGGL(k_scatter, dim3((numk + 127) / 128), dim3(128), 0, s, numk, ij, dots, nsite,
-                     nat, npx, npy, isym, sr);
+                     nat, npx, npy, isym, (const int32_t*)owner, sr);
 }
 
 }  // namespace ctem

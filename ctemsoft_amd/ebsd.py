@@ -32,7 +32,7 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
            "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
-           "ctem_plan_set_symmetry", "ctem_plan_adjoint"]
+           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops"]
 
 
 class CtemError(RuntimeError):
@@ -75,7 +75,7 @@ def lib():
             "ctem_plan_timing": [P, P, P, P, P], "ctem_plan_set_mixed_terms": [P, I32, I32],
             "ctem_plan_set_symmetry": [P, I32, P],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
-            "ctem_plan_adjoint": [P, P, P, P],
+            "ctem_plan_adjoint": [P, P, P, P], "ctem_plan_alg_flops": [P, P, P, P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -333,3 +333,10 @@ class Plan:
         t, g, f, n = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
         _check(lib().ctem_plan_timing(self._h, C.byref(t), C.byref(g), C.byref(f), C.byref(n)))
         return dict(total_ms=t.value, gemm_ms=g.value, gemm_flops=f.value, gemm_launches=n.value)
+
+    def alg_flops(self):
+        """Algorithmic flops of the last FP64 run by the reference's Pade rule: total path,
+        matrix-exponential products 8n^3(q+j), Pade solve (32/3)n^3."""
+        t, q, v = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().ctem_plan_alg_flops(self._h, C.byref(t), C.byref(q), C.byref(v)))
+        return dict(total=t.value, products=q.value, solve=v.value)

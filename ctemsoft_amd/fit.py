@@ -25,6 +25,7 @@ def pixel_owner(ij, npx, npy, isym):
     owner = -np.ones((2 * npy + 1) * (2 * npx + 1), np.int64)
     for d, (i, j) in enumerate(np.asarray(ij)):
         q = ebsd.laue_equiv(int(i), int(j), int(isym))
+        q = q[(np.abs(q[:, 0]) <= npx) & (np.abs(q[:, 1]) <= npy)]
         owner[(q[:, 1] + npy) * (2 * npx + 1) + (q[:, 0] + npx)] = d
     return owner
 

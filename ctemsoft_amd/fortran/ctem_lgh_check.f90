@@ -1,5 +1,5 @@
 ! Fortran host check of the ISO_C_BINDING boundary: reads one direction's DynMat and depth
-! profile from `lgh_in.bin` (stream file written by tests/test_gpu_fortran.py), calls
+! profile from `lgh_in.bin` (stream file written by tests/test_fortran_boundary.py), calls
 ! ctem_calclgh3 -- the GPU stand-in for CalcLgh3 (CTEMEBSDmaster.f90:610-722) with the same
 ! arguments -- and writes Lgh to `lgh_out.bin`. Exit status 0 on success, 1 on a library error.
 program ctem_lgh_check

@@ -176,6 +176,15 @@ CTEM_API int32_t ctem_plan_scatter(ctem_plan* plan, const int32_t* ij, int32_t n
 CTEM_API int32_t ctem_plan_timing(ctem_plan* plan, double* total_ms, double* gemm_ms, double* gemm_flops,
                          int32_t* gemm_launches);
 
+/* No counterpart in the reference (measurement). The algorithmic flop count of the last
+ * FP64 run of the plan, summed over its directions: total = 8n^2w + 8n^3(q+j) + (32/3)n^3 +
+ * 16n^2Z + 4n^2P per direction (n strong and w weak beams, Z depths, P sites), products =
+ * 8n^3(q+j), solve = (32/3)n^3, where q and j are the Pade degree and squaring count that
+ * the reference's MatrixExponential (math.f90:208-219) picks from max |a_ij| of
+ * A = 2 pi i dz DynMat / fl32(2kn). The count does not depend on the algorithm the library
+ * runs; it prices the work of the path. */
+CTEM_API int32_t ctem_plan_alg_flops(ctem_plan* plan, double* total, double* products, double* solve);
+
 /* ----- single-direction drop-in for CalcLgh3 ------------------------------------------ */
 /* Same arguments and meaning as the reference's external subroutine CalcLgh3
  * (CTEMEBSDmaster.f90:610-630): DMat and Lgh are nn x nn complex, column-major (Fortran),

@@ -46,17 +46,25 @@ def _p(a):
 class Crystal:
     """Per-energy setup arrays, as dumped from the reference (or built by the host)."""
 
-    def __init__(self, fx):
+    def __init__(self, fx, lut_from=None):
+        """lut_from: a fixture of the same crystal and energy holding `lut_full`, the LUT over
+        the box of every difference of two list entries (hexagonal axes), used in place of the
+        reference-sized LUT."""
         self.hkl = np.ascontiguousarray(fx["hkl"], np.int32)
         ucg = np.asarray(fx["ucg"], np.complex128)
         self.ucg = np.ascontiguousarray(np.stack([ucg.real, ucg.imag], -1).ravel())
         lut = np.asarray(fx["lut"], np.complex128)  # [l][k][h] = Fortran LUT(h,k,l)
-        self.lut_dim = np.array(lut.shape[::-1], np.int32)
-        self.lut_lo = np.asarray(fx["lut_lo"], np.int32).copy()
-        self.lut = np.ascontiguousarray(np.stack([lut.real, lut.imag], -1).ravel())
+        lo = np.asarray(fx["lut_lo"], np.int32).copy()
         dbd = np.asarray(fx["dbdiff"])
-        h, k, l = (self.hkl - self.lut_lo).T
+        h, k, l = (self.hkl - lo).T
         self.dbdiff_ref = np.ascontiguousarray(dbd[l, k, h].astype(np.uint8))
+        if lut_from is not None:
+            assert np.array_equal(np.asarray(lut_from["hkl"]), np.asarray(fx["hkl"]))
+            lut = np.asarray(lut_from["lut_full"], np.complex128)
+            lo = np.asarray(lut_from["lut_full_lo"], np.int32).copy()
+        self.lut_dim = np.array(lut.shape[::-1], np.int32)
+        self.lut_lo = lo
+        self.lut = np.ascontiguousarray(np.stack([lut.real, lut.imag], -1).ravel())
         self.mlambda = float(fx["mLambda"])
         self.dynupz = float(fx["DynUpz"])
         self.weakcutoff, self.cutoff, self.sgcutoff = (float(v) for v in fx["cutoffs"])

@@ -210,6 +210,59 @@ def run_program(work: str, wl: workloads.Workload, counts, tail) -> dict:
     return read_master(os.path.join(work, "master.data"))
 
 
+def run_sym(work: str, nbox: int, npx: int, dsm: np.ndarray, wavelength: float) -> dict:
+    """Laue-equivalence table over the box -nbox..nbox squared (isym 1..13) and the
+    RoscaLambert direction lists of every Laue group on both grids, from the reference's
+    Apply2DLaueSymmetry and Calckvectors (oracle/ref_sym.f90). dsm is the direct structure
+    matrix as a row-major 3x3 array."""
+    d = np.asarray(dsm, np.float64)
+    vals = ", ".join(repr(float(v)) for v in d.T.reshape(-1))  # Fortran column order
+    with open(os.path.join(work, "refsym.nml"), "w") as f:
+        f.write(f" &refsym\n nbox = {nbox},\n npx = {npx},\n dsm = {vals},\n"
+                f" wavelength = {float(wavelength)!r},\n outname = 'sym.bin',\n /\n")
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && {REF_BIN}/ctemref_sym > sym.log 2>&1"],
+                   cwd=work, check=True)
+    with open(os.path.join(work, "sym.bin"), "rb") as f:
+        s = Stream(f.read())
+    magic, nb, npx_ = s.get("<i4", 3)
+    assert magic == 20261 and nb == nbox and npx_ == npx
+    w = 2 * nbox + 1
+    raw = s.get("<i4", 13 * w * w * 25).reshape(13, w, w, 25)  # [isym-1][ipy][ipx]
+    out = dict(laue_n=raw[..., 0].astype(np.int8), laue_q=raw[..., 1:].astype(np.int8))
+    assert np.array_equal(out["laue_q"].astype(np.int32), raw[..., 1:])
+    for _ in range(24):
+        isym, ihex, numk = (int(v) for v in s.get("<i4", 3))
+        rec = np.frombuffer(s.b, np.dtype([("i", "<i4"), ("j", "<i4"), ("k", "<f8", 3), ("kn", "<f8")]),
+                            numk, s.o)
+        s.o += rec.itemsize * numk
+        out[f"kv{isym}_{ihex}_ij"] = np.stack([rec["i"], rec["j"]], 1).astype(np.int32)
+        out[f"kv{isym}_{ihex}_k"] = rec["k"].copy()
+        out[f"kv{isym}_{ihex}_kn"] = rec["kn"].copy()
+    assert s.o == len(s.b), "trailing bytes in symmetry dump"
+    return out
+
+
+def time_reference_parallel(work: str, wl, lam: np.ndarray, ksel, procs: int,
+                            ekev: float | None = None) -> float:
+    """Wall time of the reference beam loop over the directions ksel split into `procs`
+    disjoint parts, one single-threaded driver process per part running at once, minus the
+    time of one process on one direction (the per-energy setup every process repeats)."""
+    parts = [p for p in np.array_split(np.asarray(ksel), procs) if len(p)]
+    dirs = []
+    for i in range(len(parts)):
+        d = os.path.join(work, f"p{i}")
+        os.makedirs(d, exist_ok=True)
+        dirs.append(d)
+    t0 = time.perf_counter()
+    run_driver(dirs[0], wl, lam, [int(parts[0][0])], 2, ekev)
+    t_setup = time.perf_counter() - t0
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(len(parts)) as ex:
+        list(ex.map(lambda a: run_driver(a[0], wl, lam, list(a[1]), 2, ekev), zip(dirs, parts)))
+    return (time.perf_counter() - t0) - t_setup
+
+
 def have_reference() -> bool:
     return os.path.exists(os.path.join(REF_BIN, "ctemref_driver"))
 

@@ -32,7 +32,7 @@ REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
 sys.path.insert(0, REPO)
 
 from ctemsoft_amd import workloads  # noqa: E402
-from oracle.refrun import read_master, run_driver, run_program  # noqa: E402
+from oracle.refrun import read_master, run_driver, run_program, run_sym  # noqa: E402
 
 def _setup_arrays(d: dict) -> dict:
     out = dict(isym=d["isym"], usehex=d["usehex"], numk=d["numk"], sgnum=d["sgnum"],
@@ -141,6 +141,24 @@ def setup_series(name: str) -> str:
     return fn
 
 
+def golden_sym() -> str:
+    """Laue-equivalence table (isym 1..13, pixels -16..16 squared) and the RoscaLambert
+    direction lists of every Laue group on the square and hexagonal grids (npx 11), from the
+    reference's Apply2DLaueSymmetry and Calckvectors. The structure matrix is corundum's
+    (hexagonal axes) and the wavelength that of 15 kV; both enter only AddkVector's final
+    transformation."""
+    fx = np.load(os.path.join(HERE, "al2o3_c2.npz"))
+    with tempfile.TemporaryDirectory(prefix="ctemsym_") as work:
+        out = run_sym(work, 16, 11, fx["dsm"], float(fx["mLambda"]))
+    out["dsm"] = fx["dsm"]
+    out["mLambda"] = fx["mLambda"]
+    out["nbox"] = 16
+    out["npx"] = 11
+    fn = os.path.join(HERE, "sym_tables.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -161,6 +179,8 @@ def main():
                                       ekev=e, tag=f"{w}_{e:.1f}kV", corners=False))
             if w == "fe_c3w":
                 print(setup_series(w))
+        elif w == "sym":
+            print(golden_sym())
         else:
             raise SystemExit(f"unknown workload {w}")
 

@@ -193,3 +193,64 @@ def test_south_hemisphere_mirror(ctx):
         out.append(p.dots()[:, 0])
         p.close()
     assert np.allclose(out[0], out[1], rtol=2e-6, atol=0)
+
+
+@pytest.mark.parametrize("name,count", [("ni_c0", 256), ("fe_c1", 256), ("al2o3_c2", 256), ("ni_c0w", 256),
+                                        ("fe_c1w", 64), ("al2o3_c2w", 48)])
+def test_dots_many_directions_against_oracle(ctx, oracle_mod, name, count):
+    """FP64 dots of seeded directions of each workload against the oracle (extended-precision
+    Lgh), every site; the oracle runs on a thread pool (its C calls release the interpreter)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from ctemsoft_amd import ebsd
+    fx = load_golden(name)
+    # corundum: the LUT over the box of every g-h of its hexagonal list, in both
+    full = load_golden("al2o3_c2") if name.startswith("al2o3") else None
+    st = ebsd.Setup.from_fixture(fx, full)
+    ctx.upload_crystal(st)
+    rng = np.random.default_rng(1000 + count)
+    ks = np.sort(rng.choice(len(fx["kvec"]), min(count, len(fx["kvec"])), replace=False))
+    _say("dots vs oracle", name, len(ks), "directions")
+    plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.run("fp64")
+    d = plan.dots()
+    plan.close()
+    cr = oracle_mod.Crystal(fx, full)
+    def one(k):
+        return oracle_mod.master_direction(cr, fx["kvec"][k], fx["kn"][k], fx["lam"], float(fx["dz"]),
+                                           float(fx["thick"]))[0]
+    with ThreadPoolExecutor(16) as ex:
+        ref = np.array(list(ex.map(one, ks)))
+    fin = np.isfinite(ref).all(1)
+    rel = np.abs(d[fin] - ref[fin]) / np.abs(ref[fin])
+    _say(f"  max rel {rel.max():.2e}, non-finite in the oracle {int((~fin).sum())}")
+    assert rel.max() <= 1e-11
+    assert np.array_equal(np.isfinite(d), np.isfinite(ref))
+
+
+def test_hexagonal_scatter_keeps_last_direction(ctx):
+    """Corundum (Laue group 12 on the hexagonal grid, where directions share pixels): the GPU
+    scatter equals the reference's serial loop -- every direction in list order overwriting its
+    Laue-equivalent pixels (CTEMEBSDmaster.f90:462-479) -- bit for bit, on every pixel."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden("al2o3_c2")
+    nk = len(fx["kij"])
+    st = ebsd.Setup.from_fixture(fx, fx)
+    ctx.upload_crystal(st)
+    plan = ctx.plan(fx["kvec"], fx["kn"], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.run("fp64")
+    sv = plan.svals()
+    npx, npy = ebsd.lambert_dims(200, 1)
+    isym = int(fx["isym"])
+    assert isym == 12
+    sr = plan.scatter(fx["kij"], npx, npy, isym)
+    plan.close()
+    ref = np.zeros_like(sr)
+    shared = np.zeros(sr.shape[1:], np.int32)
+    for ik in range(nk):
+        q = ebsd.laue_equiv(int(fx["kij"][ik, 0]), int(fx["kij"][ik, 1]), isym)
+        q = q[(np.abs(q[:, 0]) <= npx) & (np.abs(q[:, 1]) <= npy)]
+        ref[:, q[:, 1] + npy, q[:, 0] + npx] = sv[ik][:, None]
+        shared[q[:, 1] + npy, q[:, 0] + npx] += 1
+    _say(f"  {nk} directions, pixels reached by more than one direction: {int((shared > 1).sum())}")
+    assert (shared > 1).sum() > 1000
+    assert np.array_equal(sr.view(np.int32), ref.view(np.int32))

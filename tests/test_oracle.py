@@ -216,3 +216,39 @@ def test_corundum_beam_sets_reach_beyond_lut_box(oracle_mod):
     d = g[:len(st), None, :] - g[None, :, :]
     outside = ((d < lo) | (d > hi)).any(-1)
     assert outside.sum() > 100
+
+
+def _sym_cases():
+    fx = load_golden("sym_tables")
+    nb, npx = int(fx["nbox"]), int(fx["npx"])
+    return fx, nb, npx, int(round(2 * npx / np.sqrt(3.0)))
+
+
+def test_laue_equivalents_match_reference_table(oracle_mod):
+    """Apply2DLaueSymmetry (Lambert.f90:1942-2046) for every Laue group 1..13 over the pixel
+    box -16..16 squared, against the reference's own table (tests/golden/sym_tables.npz)."""
+    fx, nb, _, _ = _sym_cases()
+    bad = []
+    for isym in range(1, 14):
+        for iy in range(-nb, nb + 1):
+            for ix in range(-nb, nb + 1):
+                n = int(fx["laue_n"][isym - 1, iy + nb, ix + nb])
+                ref = fx["laue_q"][isym - 1, iy + nb, ix + nb].reshape(12, 2)[:n].astype(np.int32)
+                got = np.asarray(oracle_mod.laue_equiv(ix, iy, isym), np.int32).reshape(-1, 2)
+                if got.shape != ref.shape or not np.array_equal(got, ref):
+                    bad.append((isym, ix, iy))
+    assert not bad, f"{len(bad)} pixels differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("hexgrid", [0, 1])
+@pytest.mark.parametrize("isym", list(range(1, 13)))
+def test_kvectors_every_laue_group_match_reference(isym, hexgrid, oracle_mod):
+    """Calckvectors/AddkVector (kvectors.f90:421-542,839-947) for every Laue group on the square
+    and hexagonal grids against the reference's lists (hexagonal grids take npy = npyhex, as
+    the program passes it; points outside the hexagon are NaN in both)."""
+    fx, _, npx, npyhex = _sym_cases()
+    npy = npyhex if hexgrid else npx
+    ij, k, kn = oracle_mod.kvectors(npx, npy, isym, hexgrid, fx["dsm"], float(fx["mLambda"]))
+    assert np.array_equal(ij, fx[f"kv{isym}_{hexgrid}_ij"])
+    assert np.array_equal(k, fx[f"kv{isym}_{hexgrid}_k"], equal_nan=True)
+    assert np.array_equal(kn, fx[f"kv{isym}_{hexgrid}_kn"])
