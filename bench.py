@@ -142,6 +142,10 @@ def main():
                     help="split schemes of the mixed form, taylor,square: 3 = BF16x3, 6 = BF16x6, 16 = FP16x3")
     ap.add_argument("--symmetry", default="auto", choices=["auto", "off"],
                     help="symmetric products of the split mixed form for an inversion-symmetric LUT")
+    ap.add_argument("--tune", default="on", choices=["on", "off", "fresh"],
+                    help="launch autotuner (bucket multiple, chunk size) before timing; fresh ignores its cache")
+    ap.add_argument("--graph", default="auto", choices=["auto", "on", "off"],
+                    help="timed steps as captured HIP graphs (auto: on for energy series, all bins in one graph)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {a.gpus} requested); the directions of a "
@@ -170,11 +174,23 @@ def main():
         ndir += len(k)
     ns, nw = np.concatenate(ns_all), np.concatenate(nw_all)
 
-    def step(prec):
-        # one pass of the path: beam selection and the whole master pattern of every bin
+    use_graph = a.graph == "on" or (a.graph == "auto" and len(plans) > 1)
+    tuned = None
+    if a.tune != "off":
+        from ctemsoft_amd import autotune
+        tprec = "mixed" if a.precision != "fp64" else "fp64"
+        tuned, _ = autotune.tune(ctx, plans, tprec, use_cache=(a.tune == "on"), log=log)
+
+    def step(prec, graph=use_graph):
+        # one pass of the path: beam selection and the whole master pattern of every bin (with
+        # graphs: every bin's assembly .. contraction replayed as one captured launch)
         for plan in plans:
             plan.select()
-            plan.run(prec)
+        if graph:
+            ebsd.run_plans(plans, prec)
+        else:
+            for plan in plans:
+                plan.run(prec)
     name, cus = ctx.device_info()
     log(f"device {name}, {cus} CUs; workload {wl.name}: {len(jobs)} energy bin(s), {ndir} directions, nz={wl.nz}")
     log(f"strong beams {ns.min()}..{ns.max()} (mean {ns.mean():.1f}); weak {nw.min()}..{nw.max()} (mean {nw.mean():.1f})")
@@ -194,6 +210,11 @@ def main():
             step(prec)
             times.append(time.perf_counter() - t0)
             log(f"{prec}: step {s_ + 1}/{a.steps} {times[-1] * 1e3:.1f} ms")
+        tot_gpu_ms = sum(plan.timing()["total_ms"] for plan in plans)
+        if use_graph:
+            # a graph replay records no per-kernel events: one more (untimed) eager pass gives the
+            # GEMM kernels' event times for the roofline
+            step(prec, graph=False)
         tims = [plan.timing() for plan in plans]
         g = dict(gemm_ms=sum(t["gemm_ms"] for t in tims), gemm_flops=sum(t["gemm_flops"] for t in tims),
                  gemm_launches=sum(t["gemm_launches"] for t in tims))
@@ -202,7 +223,7 @@ def main():
             afl = [plan.alg_flops() for plan in plans]
             alg = {k: sum(f[k] for f in afl) for k in ("total", "products", "solve")}
         ms = float(np.mean(times) * 1e3)
-        res[prec] = dict(ms=ms, dps=ndir / (ms * 1e-3), gemm=g, dots=dots,
+        res[prec] = dict(ms=ms, dps=ndir / (ms * 1e-3), gemm=g, dots=dots, gpu_ms=tot_gpu_ms,
                          gemm_tflops=g["gemm_flops"] / (g["gemm_ms"] * 1e-3) / 1e12 if g["gemm_ms"] > 0 else 0.0)
         log(f"{prec}: {ms:.1f} ms/step, {res[prec]['dps']:.0f} directions/s, GEMM {g['gemm_ms']:.1f} ms "
             f"({res[prec]['gemm_tflops']:.2f} TFLOP/s over {g['gemm_launches']} launches)")
@@ -241,7 +262,11 @@ def main():
         "config": {"workload": f"{wl.name}: {wl.describe()}",
                    "directions": int(ndir), "strong_beams_mean": round(float(ns.mean()), 1),
                    "weak_beams_mean": round(float(nw.mean()), 1), "precision": head,
-                   "mixed_terms": a.terms, "symmetric_products": bool(sym)},
+                   "mixed_terms": a.terms, "symmetric_products": bool(sym),
+                   "launch": {"graph": use_graph, "bins_per_graph": len(plans) if use_graph else 0,
+                              "bucket_multiple": tuned[0] if tuned else 32,
+                              "chunk_gib": tuned[1] if tuned else 24.0,
+                              "autotuned": tuned is not None}},
         "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
         "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,
@@ -254,6 +279,8 @@ def main():
                      "kernel": "k_zgemm" if head == "fp64" else ("k_sgemm" if a.terms == "16,16" else "k_cgemm"),
                      "alg_flops_per_step": alg["products"] if "fp64" in res else None,
                      "kernel_ms_per_step": round(r["gemm"]["gemm_ms"], 3),
+                     "kernel_ms_source": ("HIP events of an eager pass after the timed graph steps"
+                                          if use_graph else "HIP events over the timed steps"),
                      "kernel_launches_per_step": r["gemm"]["gemm_launches"],
                      "kernel_flops_tflops": round(r["gemm_tflops"], 3),
                      "kernel_flops_frac": round(r["gemm_tflops"] * 1e12 / PEAK[head], 4),

@@ -71,6 +71,9 @@ int32_t upload(DBuf& d, const T* h, size_t n, hipStream_t s) {
 // Taylor coefficients 1/j!
 double inv_fact(int j) { double f = 1.0; for (int i = 2; i <= j; i++) f *= i; return 1.0 / f; }
 
+// bucket sizes: multiples of 32 (or of a coarser multiple of 32)
+int npad_of(int n, int multiple = 32) { return std::max(multiple, (n + multiple - 1) / multiple * multiple); }
+
 }  // namespace
 
 // One uploaded crystal (one energy bin). A plan keeps the crystal that was current when it
@@ -136,6 +139,20 @@ struct ctem_ctx {
   DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<ctem_plan*> plans;  // live plans, destroyed with the context
+  // bumped whenever a work-space buffer is reallocated: graphs captured before hold the old
+  // addresses and are captured again
+  uint64_t wgen = 0;
+  uint64_t plan_serial = 0;  // plans are numbered for graph keys
+  // one captured graph over a list of plans (ctem_plans_run)
+  hipGraphExec_t mgraph = nullptr;
+  std::vector<uint64_t> mgraph_key;
+};
+
+// squaring counts of one chunk of matrices and their maximum
+struct ChunkSched {
+  bool known = false;
+  std::vector<int32_t> sq;
+  int smax = 0;
 };
 
 struct ctem_plan {
@@ -166,6 +183,23 @@ struct ctem_plan {
   int32_t symmetry = 1;  // split mixed form: symmetric products when the crystal's LUT allows
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
+  // bucket layout from the last selection: (npad, directions) per bucket, and the bucket order
+  // of all directions on the device
+  std::vector<std::pair<int, std::vector<int32_t>>> buckets;
+  DBuf dirs;
+  // squaring counts of each chunk from an earlier run; the same directions and crystal give
+  // the same matrices, so later runs need no read-back. Key: (form, npad, chunk start).
+  std::map<std::tuple<int, int, int>, ChunkSched> sched;
+  uint64_t serial = 0;
+  // tuning: padded sizes are multiples of bucket_multiple (32 or 64); a chunk of one bucket
+  // holds as many directions as fit in chunk_bytes of work space
+  int32_t bucket_multiple = 32;
+  size_t chunk_bytes = (size_t)24 << 30;
+  uint64_t bufsig = 0;  // addresses of the beam-set buffers of the last selection
+  // graph mode: the whole run captured once and replayed (ctem_plan_set_graph)
+  int32_t graph_mode = 0;
+  hipGraphExec_t graph = nullptr;
+  std::vector<uint64_t> graph_key;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -200,6 +234,7 @@ extern "C" int32_t ctem_ctx_destroy(ctem_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   while (!c->plans.empty()) ctem_plan_destroy(c->plans.back());
+  if (c->mgraph) (void)hipGraphExecDestroy(c->mgraph);
   c->cr.reset();
   c->w.release();
   for (DBuf* b : {&c->s_A, &c->s_W, &c->s_Psi, &c->s_sq, &c->s_err, &c->s_lam, &c->s_lgh, &c->s_dirs,
@@ -499,6 +534,7 @@ extern "C" int32_t ctem_plan_create(ctem_ctx* c, int32_t numk, const double* k, 
   ctem_plan* p = new ctem_plan();
   p->ctx = c;
   p->cr = c->cr;
+  p->serial = ++c->plan_serial;
   p->numk = numk; p->nz = nz; p->dz = dz; p->thick = thick;
   std::vector<double> lamd(lam, lam + nz);
   int32_t rc;
@@ -517,8 +553,14 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
   if (!p) return CTEM_OK;
   (void)hipSetDevice(p->ctx->device);
   (void)hipStreamSynchronize(p->ctx->stream);
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  if (p->ctx->mgraph) {  // a graph over a plan list may hold this plan's buffers
+    (void)hipGraphExecDestroy(p->ctx->mgraph);
+    p->ctx->mgraph = nullptr;
+    p->ctx->mgraph_key.clear();
+  }
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
-                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh})
+                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->dirs})
     b->release();
   p->cr.reset();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
@@ -532,6 +574,7 @@ static int32_t check_err_flag(DBuf& err, hipStream_t s, const char* what) {
   int32_t h = 0;
   HIPCHK(hipMemcpyAsync(&h, err.p, sizeof h, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (h & 32) return fail(CTEM_E_STATE, "%s: a matrix needs more squarings than the cached run schedule", what);
   if (h & 1) return fail(CTEM_E_RANGE, "%s: an excitation-error angle left the reproduced cosf range (|g| too large for k)", what);
   if (h & 2) return fail(CTEM_E_NUMERIC, "%s: non-finite dynamical matrix", what);
   if (h & 4) return fail(CTEM_E_NUMERIC, "%s: matrix norm needs more squarings than allowed", what);
@@ -582,7 +625,22 @@ extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw
   o.widx = p->widx.as<int32_t>(); o.wsg = p->wsg.as<float>();
   launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), o, 1);
   HIPCHK(hipGetLastError());
+  // bucket layout: directions grouped by padded size, in list order within a bucket
+  {
+    std::map<int, std::vector<int32_t>> bk;
+    for (int i = 0; i < nk; i++) bk[npad_of(p->h_nns[i], p->bucket_multiple)].push_back(i);
+    p->buckets.assign(bk.begin(), bk.end());
+    std::vector<int32_t> order;
+    order.reserve(nk);
+    for (auto& kv : p->buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
+    if ((rc = upload(p->dirs, order.data(), order.size(), s))) return rc;
+  }
   HIPCHK(hipStreamSynchronize(s));
+  const void* bufs[] = {p->nns.p, p->nnw.p, p->soff.p, p->woff.p, p->sidx.p, p->ssg.p, p->widx.p, p->wsg.p,
+                        p->dirs.p, p->err.p};
+  uint64_t h = 1469598103934665603ull;
+  for (const void* q : bufs) h = (h ^ (uint64_t)(uintptr_t)q) * 1099511628211ull;
+  p->bufsig = h;
   p->selected = true;
   if (nns_out) memcpy(nns_out, p->h_nns.data(), 4 * (size_t)nk);
   if (nnw_out) memcpy(nnw_out, p->h_nnw.data(), 4 * (size_t)nk);
@@ -631,10 +689,12 @@ constexpr int kSmax = 48;
 // W holds 5*nb*npad^2 work space; on exit E_b = expm(A_b) is in the fifth slice of W (the
 // returned pointer). h_n[b] are the true sizes (for the algorithmic flop count), h_sq
 // receives the squaring counts.
+// cs (may be null): with cs->known the squaring counts of this chunk are taken from it and not
+// read back (no host synchronisation); otherwise they are read back and stored in it.
 double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                    int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n,
                    std::vector<int32_t>& h_sq, int32_t& rc_out, double* d_amod = nullptr,
-                   std::vector<double>* h_amod = nullptr) {
+                   std::vector<double>* h_amod = nullptr, ChunkSched* cs = nullptr) {
   rc_out = CTEM_OK;
   const int64_t per = (int64_t)npad * npad;
   double2* A2 = W;
@@ -645,15 +705,21 @@ double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, d
 #define EXPM_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { rc_out = fail(CTEM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); return nullptr; } } while (0)
   launch_norm_scale(s, npad, nb, A, kTheta64, kSmax, d_sq, d_err, d_amod);
   EXPM_CHK(hipGetLastError());
-  h_sq.resize(nb);
-  EXPM_CHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
-  if (d_amod && h_amod) {
-    h_amod->resize(nb);
-    EXPM_CHK(hipMemcpyAsync(h_amod->data(), d_amod, 8 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  if (cs && cs->known) {
+    h_sq = cs->sq;
+  } else {
+    h_sq.resize(nb);
+    EXPM_CHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+    if (d_amod && h_amod) {
+      h_amod->resize(nb);
+      EXPM_CHK(hipMemcpyAsync(h_amod->data(), d_amod, 8 * (size_t)nb, hipMemcpyDeviceToHost, s));
+    }
+    EXPM_CHK(hipStreamSynchronize(s));
   }
-  EXPM_CHK(hipStreamSynchronize(s));
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  if (cs && !cs->known) { cs->sq = h_sq; cs->smax = smax; cs->known = true; }
+  if (cs) launch_sq_check(s, nb, d_sq, smax, d_err);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
 
@@ -707,9 +773,10 @@ double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, d
 int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                         double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
                         const std::vector<int>& h_n, std::vector<int32_t>& h_sq, int depth_grid,
-                        double* d_amod = nullptr, std::vector<double>* h_amod = nullptr) {
+                        double* d_amod = nullptr, std::vector<double>* h_amod = nullptr,
+                        ChunkSched* cs = nullptr) {
   int32_t rc;
-  double2* E = expm_fp64(tp, s, npad, nb, A, W, d_sq, d_err, h_n, h_sq, rc, d_amod, h_amod);
+  double2* E = expm_fp64(tp, s, npad, nb, A, W, d_sq, d_err, h_n, h_sq, rc, d_amod, h_amod, cs);
   if (!E) return rc;
   launch_depth(s, npad, nb, nz, E, 0, 0, Psi, spsi, depth_grid);
   HIPCHK(hipGetLastError());
@@ -739,7 +806,7 @@ constexpr float kBbc8[8] = {0.098979145598294002f, 0.024744786399573501f, 0.6089
 int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32,
                          float* wmax, double2* Psi64, int64_t spsi, int nz, int32_t* d_sq,
                          int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
-                         int terms_taylor, int terms_square, int depth_grid) {
+                         int terms_taylor, int terms_square, int depth_grid, ChunkSched* cs) {
   const int64_t per = (int64_t)npad * npad;
   float2* B = W32;
   float2* G2 = W32 + per * nb;
@@ -754,15 +821,21 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   float* mP = wmax + 3 * nb;
   float* mF = wmax + 4 * nb;
   float* mH = wmax + 5 * nb;
-  HIPCHK(hipMemsetAsync(wmax, 0, 6 * sizeof(float) * (size_t)nb, s));
+  launch_fill_zero(s, wmax, 6 * sizeof(float) * (size_t)nb);
   // B (the first buffer) holds A in FP32 as assembled; scale it by 2^-s in place
   launch_norm_scale32(s, npad, nb, B, kThetaMixed, kSmax, d_sq, d_err, mB);
   HIPCHK(hipGetLastError());
-  h_sq.resize(nb);
-  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (cs->known) {
+    h_sq = cs->sq;
+  } else {
+    h_sq.resize(nb);
+    HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  if (!cs->known) { cs->sq = h_sq; cs->smax = smax; cs->known = true; }
+  launch_sq_check(s, nb, d_sq, smax, d_err);
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += (double)h_n[b] * h_n[b] * h_n[b];
   auto launch = [&](const CGemm& g, double flops) -> int32_t {
@@ -815,7 +888,7 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     float2* X = buf[i & 1];
     float* out_max = bmax[(i + 1) & 1];
     // the output buffer's maxima were those of two steps back
-    HIPCHK(hipMemsetAsync(out_max, 0, sizeof(float) * (size_t)nb, s));
+    launch_fill_zero(s, out_max, sizeof(float) * (size_t)nb);
     CGemm g = gemm(buf[(i + 1) & 1], out_max, X, bmax[i & 1], X, bmax[i & 1], 0.f, {{X, 2.f}}, terms_square);
     g.active = d_sq; g.step = i;
     if ((rc = launch(g, fl))) return rc;
@@ -837,9 +910,9 @@ int32_t expm_depth_mixed(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 // then the squarings alternate between 5 (F0) and 4. wmax holds six FP32 maxima and six
 // exponents per matrix (slot 0: L, slot 1: B, slots 2-5 as the buffers), then the max of A.
 int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2* W32, const double* colpart,
-                         const float* amax, float* wmax, DBuf& order, double2* Psi64, int64_t spsi, int nz,
+                         const float* amax, float* wmax, double2* Psi64, int64_t spsi, int nz,
                          int32_t* d_sq, int32_t* d_err, const std::vector<int>& h_n, std::vector<int32_t>& h_sq,
-                         int depth_grid, bool sym) {
+                         int depth_grid, bool sym, ChunkSched* cs) {
   const int64_t per = (int64_t)npad * npad;
   uint16_t* S[6];
   float* mx[6];
@@ -849,26 +922,26 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     mx[i] = wmax + i * nb;
     ex[i] = (int32_t*)(wmax + (6 + i) * nb);
   }
-  HIPCHK(hipMemsetAsync(wmax, 0, 12 * sizeof(float) * (size_t)nb, s));
+  launch_fill_zero(s, wmax, 12 * sizeof(float) * (size_t)nb);
   // the squaring count, exponent and max of B = A 2^-s; the first products split A themselves
   launch_norm_split(s, npad, nb, colpart, amax, kThetaSplit, kSmax, d_sq, d_err, ex[1], mx[1]);
   HIPCHK(hipGetLastError());
-  h_sq.resize(nb);
-  HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (cs->known) {
+    h_sq = cs->sq;
+  } else {
+    h_sq.resize(nb);
+    HIPCHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
   int smax = 0;
   for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  if (!cs->known) { cs->sq = h_sq; cs->smax = smax; cs->known = true; }
+  launch_sq_check(s, nb, d_sq, smax, d_err);
   // flops of one complex product: 8 n^3, or 8 n n (n + 1) / 2 for the entries on and above the
   // diagonal of a symmetric result
   auto pflops = [&](int b) { return sym ? 4.0 * h_n[b] * h_n[b] * (h_n[b] + 1.0) : 8.0 * h_n[b] * h_n[b] * h_n[b]; };
   double n3 = 0.0;
   for (int b = 0; b < nb; b++) n3 += pflops(b);
-  // matrices by squaring count, largest first: squaring i runs on the first nact[i] of them
-  std::vector<int32_t> ord(nb);
-  for (int b = 0; b < nb; b++) ord[b] = b;
-  std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t c) { return h_sq[a] > h_sq[c]; });
-  int32_t rc0;
-  if ((rc0 = upload(order, ord.data(), ord.size(), s))) return rc0;
   auto launch = [&](const SGemm& g, double flops) -> int32_t {
     GemmTimer t{tp, s};
     t.begin();
@@ -944,13 +1017,12 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
     for (int b = 0; b < nb; b++) if (i < h_sq[b] + extra) fl += pflops(b);
     const int x = buf[i & 1], out = buf[(i + 1) & 1];
     // the output buffer's maxima were those of two steps back
-    HIPCHK(hipMemsetAsync(mx[out], 0, sizeof(float) * (size_t)nb, s));
+    launch_fill_zero(s, mx[out], sizeof(float) * (size_t)nb);
     SGemm g = mk(out, x, x);
     add_m(g, 0, x, 2.f, 0.f);
-    g.active = d_sq; g.step = i;
-    int nact = 0;
-    while (nact < nb && h_sq[ord[nact]] + extra > i) nact++;
-    g.order = order.as<int32_t>(); g.nord = nact;
+    // every matrix of the chunk in the grid; those whose squarings (plus the extra two) are done
+    // leave at once (counts stay on the device)
+    g.active = d_sq; g.step = i; g.active_add = extra;
     g.Cf = Ff; g.f_sq = d_sq; g.f_at = i + 1; g.Cf2 = Ff2; g.Cf4 = Ff4;
     if ((rc = launch(g, fl))) return rc;
   }
@@ -966,8 +1038,6 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
 
 }  // namespace
 
-// bucket sizes: multiples of 32
-static int npad_of(int n) { return std::max(32, (n + 31) / 32 * 32); }
 
 // Algorithmic flops of one direction: F = 8n^2w + 8n^3(q+j) + (32/3)n^3 + 16n^2Z + 4n^2P (the
 // Bethe product, the Pade powers and squarings, the Pade solve, propagation with psi psi^H
@@ -988,73 +1058,95 @@ static void alg_flops(int n, int w, int nz, int nsite, double amod, double /*dz*
   tot += 8.0 * n2 * w + fp + fs + 16.0 * n2 * nz + 4.0 * n2 * nsite;
 }
 
-extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
-  if (!p) return fail(CTEM_E_ARG, "null plan");
-  if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
-    return fail(CTEM_E_ARG, "unknown precision %d", precision);
+namespace {
+
+// form of the matrix exponential a run uses (the squaring counts differ between forms)
+int run_form(const ctem_plan* p, int32_t precision) {
+  if (precision == CTEM_PREC_FP64) return 0;
+  return (p->terms_taylor == CTEM_SPLIT_FP16X3 && p->terms_square == CTEM_SPLIT_FP16X3) ? 1 : 2;
+}
+
+size_t chunk_budget(const ctem_plan* p) { return p->chunk_bytes; }
+
+// Device memory a run of the plan needs, chunk by chunk; grows the context's work space (and
+// the plan's buffers) to it before anything is enqueued, so that enqueueing allocates nothing.
+int32_t run_reserve(ctem_plan* p, int32_t precision) {
   ctem_ctx* c = p->ctx;
-  if (!p->cr) p->cr = c->cr;
-  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
-  if (keep_lgh_ik >= p->numk) return fail(CTEM_E_ARG, "keep_lgh_ik out of range");
+  const int ns = std::max(1, p->cr->nsite);
   int32_t rc;
-  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
-  HIPCHK(hipSetDevice(c->device));
-  hipStream_t s = c->stream;
-  const int nk = p->numk;
-  p->t_total = p->t_gemm = p->f_gemm = 0.0;
-  p->n_gemm = 0;
-  p->evused = 0;
-  double fa[3] = {0.0, 0.0, 0.0};
-  if ((rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * std::max(1, p->cr->nsite)))) return rc;
-  p->keep_dir = keep_lgh_ik;
-  p->keep_n = (keep_lgh_ik >= 0) ? p->h_nns[keep_lgh_ik] : 0;
-  if (p->keep_n && (rc = p->keep_lgh.ensure(16 * (size_t)p->keep_n * p->keep_n))) return rc;
-  if ((rc = p->err.ensure(4))) return rc;
-  HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
-  // buckets by padded size
-  std::map<int, std::vector<int32_t>> buckets;
-  for (int i = 0; i < nk; i++) buckets[npad_of(p->h_nns[i])].push_back(i);
-  std::vector<int32_t> order;
-  for (auto& kv : buckets) order.insert(order.end(), kv.second.begin(), kv.second.end());
-  if ((rc = upload(c->w.dirs, order.data(), order.size(), s))) return rc;
-  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
-             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
-  DevRefl R = p->cr->refl();
-  DevLut L = p->cr->lutv(p->err.as<int32_t>());
-  const double dzt = p->dz / p->thick;
-  hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
-  HIPCHK(hipEventRecord(e0, s));
-  // work-memory budget per chunk
-  const size_t budget = (size_t)24 << 30;
-  size_t off = 0;
-  for (auto& kv : buckets) {
+  if ((rc = p->dots.ensure(8 * (size_t)std::max(1, p->numk) * ns)) || (rc = p->err.ensure(4))) return rc;
+  size_t a = 0, w = 0, psi = 0, sq = 0, part = 0, wmax = 0, col = 0, amod = 0;
+  for (auto& kv : p->buckets) {
     const int npad = kv.first;
     const int nbk = (int)kv.second.size();
     const int64_t per = (int64_t)npad * npad;
     const int64_t spsi = (int64_t)npad * p->nz;
     const size_t per_dir = (size_t)(6 * per + spsi) * sizeof(double2) + 64;
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, chunk_budget(p) / per_dir));
+    const size_t nb = (size_t)std::min(chunk, nbk);
+    a = std::max(a, nb * per * sizeof(double2));
+    w = std::max(w, 5 * nb * per * sizeof(double2));
+    psi = std::max(psi, nb * spsi * sizeof(double2));
+    sq = std::max(sq, 4 * nb);
+    part = std::max(part, 8 * nb * (npad / 32) * (npad / 32) * ns);
+    wmax = std::max(wmax, 13 * sizeof(float) * nb);
+    col = std::max(col, sizeof(double) * nb * npad * ((npad + 63) / 64));
+    amod = std::max(amod, 8 * nb);
+  }
+  const void* before[8] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p, c->w.partial.p, c->w.wmax.p,
+                           c->w.colsum.p, c->w.amod.p};
+  if ((rc = c->w.A.ensure(a)) || (rc = c->w.W.ensure(w)) || (rc = c->w.Psi.ensure(psi)) ||
+      (rc = c->w.sq.ensure(sq)) || (rc = c->w.partial.ensure(part)) ||
+      (precision == CTEM_PREC_MIXED && (rc = c->w.wmax.ensure(wmax))) ||
+      (precision == CTEM_PREC_MIXED && (rc = c->w.colsum.ensure(col))) ||
+      (precision == CTEM_PREC_FP64 && (rc = c->w.amod.ensure(amod))))
+    return rc;
+  const void* after[8] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p, c->w.partial.p, c->w.wmax.p,
+                          c->w.colsum.p, c->w.amod.p};
+  for (int i = 0; i < 8; i++)
+    if (before[i] != after[i]) c->wgen++;
+  return CTEM_OK;
+}
+
+// Every kernel of one run of the plan, in stream order. With `timed`, the GEMM launches are
+// bracketed by events; with `sync_ok`, chunks whose squaring counts are not yet known read them
+// back (otherwise such a chunk is an error: the caller warms the schedule first).
+int32_t run_enqueue(ctem_plan* p, int32_t precision, bool timed, bool sync_ok, double fa[3]) {
+  ctem_ctx* c = p->ctx;
+  hipStream_t s = c->stream;
+  const int form = run_form(p, precision);
+  launch_fill_zero(s, p->err.p, 4);
+  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
+             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
+  DevRefl R = p->cr->refl();
+  DevLut L = p->cr->lutv(p->err.as<int32_t>());
+  const double dzt = p->dz / p->thick;
+  ctem_plan* tp = timed ? p : nullptr;
+  size_t off = 0;
+  int32_t rc;
+  for (auto& kv : p->buckets) {
+    const int npad = kv.first;
+    const int nbk = (int)kv.second.size();
+    const int64_t per = (int64_t)npad * npad;
+    const int64_t spsi = (int64_t)npad * p->nz;
+    const size_t per_dir = (size_t)(6 * per + spsi) * sizeof(double2) + 64;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, chunk_budget(p) / per_dir));
     for (int c0 = 0; c0 < nbk; c0 += chunk) {
       const int nb = std::min(chunk, nbk - c0);
-      if ((rc = c->w.A.ensure((size_t)nb * per * sizeof(double2))) || (rc = c->w.W.ensure((size_t)5 * nb * per * sizeof(double2))) ||
-          (rc = c->w.Psi.ensure((size_t)nb * spsi * sizeof(double2))) || (rc = c->w.sq.ensure(4 * (size_t)nb)) ||
-          (rc = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * std::max(1, p->cr->nsite))))
-        return rc;
-      Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
+      ChunkSched& cs = p->sched[std::make_tuple(form, npad, c0)];
+      if (!cs.known && !sync_ok) return fail(CTEM_E_STATE, "run schedule not known for a chunk");
+      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb);
       for (int b = 0; b < nb; b++) hn[b] = p->h_nns[kv.second[c0 + b]];
-      const bool split = precision == CTEM_PREC_MIXED && p->terms_taylor == CTEM_SPLIT_FP16X3 &&
-                         p->terms_square == CTEM_SPLIT_FP16X3;
+      const bool split = form == 1;
       // complex-symmetric DynMat (inversion-symmetric LUT): upper tiles only, split form alone
       const bool sym = split && p->symmetry == 1 && p->cr->centro;
       float* amax = nullptr;
       if (precision == CTEM_PREC_MIXED) {
         // six maxima and six exponents per matrix for the products, then the max of A
-        if ((rc = c->w.wmax.ensure(13 * sizeof(float) * (size_t)nb))) return rc;
         amax = c->w.wmax.as<float>() + 12 * (size_t)nb;
-        HIPCHK(hipMemsetAsync(amax, 0, sizeof(float) * (size_t)nb, s));
+        launch_fill_zero(s, amax, sizeof(float) * (size_t)nb);
       }
-      if (split && (rc = c->w.colsum.ensure(sizeof(double) * (size_t)nb * npad * ((npad + 63) / 64)))) return rc;
       if (precision == CTEM_PREC_FP64)
         launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.A.as<double2>());
       else if (split) {
@@ -1062,31 +1154,29 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
         for (int bb = 0; bb < nb; bb++) nwmax = std::max(nwmax, p->h_nnw[kv.second[c0 + bb]]);
         launch_assemble_split(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz,
                               p->cr->lutmax, nwmax, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax, sym);
-      }
-      else
+      } else
         launch_assemble32(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, c->w.W.as<float2>());
       HIPCHK(hipGetLastError());
       if (precision == CTEM_PREC_FP64) {
-        if ((rc = c->w.amod.ensure(8 * (size_t)nb))) return rc;
+        const bool first = !cs.known;
         std::vector<double> hmod;
-        rc = expm_depth_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
+        rc = expm_depth_fp64(tp, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.Psi.as<double2>(), spsi,
                              p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count,
-                             c->w.amod.as<double>(), &hmod);
-        if (!rc) {
+                             c->w.amod.as<double>(), &hmod, &cs);
+        if (!rc && first && fa) {
           for (int b = 0; b < nb; b++) {
             const int d = kv.second[c0 + b];
             alg_flops(hn[b], p->h_nnw[d], p->nz, p->cr->nsite, hmod[b], p->dz, fa[0], fa[1], fa[2]);
           }
         }
+      } else if (split) {
+        rc = expm_depth_split(tp, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
+                              c->w.wmax.as<float>(), c->w.Psi.as<double2>(), spsi, p->nz,
+                              c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count, sym, &cs);
       } else {
-        if (split)
-          rc = expm_depth_split(p, s, npad, nb, c->w.W.as<float2>(), c->w.colsum.as<double>(), amax,
-                                c->w.wmax.as<float>(), c->w.order, c->w.Psi.as<double2>(), spsi, p->nz,
-                                c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, p->h_sq, c->cu_count, sym);
-        else
-          rc = expm_depth_mixed(p, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
-                                c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
-                                p->h_sq, p->terms_taylor, p->terms_square, c->cu_count);
+        rc = expm_depth_mixed(tp, s, npad, nb, c->w.W.as<float2>(), c->w.wmax.as<float>(),
+                              c->w.Psi.as<double2>(), spsi, p->nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
+                              p->h_sq, p->terms_taylor, p->terms_square, c->cu_count, &cs);
       }
       if (rc) return rc;
       launch_lgh_dots(s, B, V, R, L, c->w.Psi.as<double2>(), spsi, p->nz, p->lam.as<double>(), dzt,
@@ -1096,19 +1186,218 @@ extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_l
     }
     off += nbk;
   }
-  HIPCHK(hipEventRecord(e1, s));
+  return CTEM_OK;
+}
+
+bool sched_complete(const ctem_plan* p, int form) {
+  for (auto& kv : p->buckets) {
+    const int npad = kv.first;
+    const int nbk = (int)kv.second.size();
+    const int64_t per = (int64_t)npad * npad;
+    const int64_t spsi = (int64_t)npad * p->nz;
+    const size_t per_dir = (size_t)(6 * per + spsi) * sizeof(double2) + 64;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, chunk_budget(p) / per_dir));
+    for (int c0 = 0; c0 < nbk; c0 += chunk) {
+      auto it = p->sched.find(std::make_tuple(form, npad, c0));
+      if (it == p->sched.end() || !it->second.known) return false;
+    }
+  }
+  return true;
+}
+
+void graph_drop(hipGraphExec_t& g) {
+  if (g) (void)hipGraphExecDestroy(g);
+  g = nullptr;
+}
+
+// what a captured run depends on besides the device data: form, options and the addresses of
+// the work space (its generation)
+std::vector<uint64_t> graph_key_of(const ctem_plan* p, int32_t precision) {
+  return {p->serial, (uint64_t)precision, (uint64_t)p->terms_taylor, (uint64_t)p->terms_square,
+          (uint64_t)p->symmetry, p->ctx->wgen, p->bufsig, (uint64_t)(uintptr_t)p->dots.p,
+          (uint64_t)(uintptr_t)p->cr.get(), (uint64_t)p->bucket_multiple, (uint64_t)p->chunk_bytes};
+}
+
+}  // namespace
+
+// Everything a run needs that is not device work: the crystal, the beam sets and their bucket
+// layout, the work space. Returns CTEM_OK when the plan can be enqueued.
+static int32_t run_prepare(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
+  if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
+    return fail(CTEM_E_ARG, "unknown precision %d", precision);
+  ctem_ctx* c = p->ctx;
+  if (!p->cr) p->cr = c->cr;
+  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
+  if (keep_lgh_ik >= p->numk) return fail(CTEM_E_ARG, "keep_lgh_ik out of range");
+  int32_t rc;
+  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  p->keep_dir = keep_lgh_ik;
+  p->keep_n = (keep_lgh_ik >= 0) ? p->h_nns[keep_lgh_ik] : 0;
+  if (p->keep_n && (rc = p->keep_lgh.ensure(16 * (size_t)p->keep_n * p->keep_n))) return rc;
+  return run_reserve(p, precision);
+}
+
+static int32_t run_finish(ctem_plan* p, int32_t precision, const double fa[3], bool have_fa) {
+  ctem_ctx* c = p->ctx;
+  hipStream_t s = c->stream;
+  HIPCHK(hipEventRecord(c->ev[1], s));
   HIPCHK(hipStreamSynchronize(s));
   float ms = 0.0f;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   p->t_total = ms;
   for (size_t i = 0; i + 1 < p->evused; i += 2) {
     float g = 0.0f;
     HIPCHK(hipEventElapsedTime(&g, p->evpool[i], p->evpool[i + 1]));
     p->t_gemm += g;
   }
+  int32_t rc;
   if ((rc = check_err_flag(p->err, s, "ctem_plan_run"))) return rc;
-  if (precision == CTEM_PREC_FP64) { p->fa_total = fa[0]; p->fa_prod = fa[1]; p->fa_solve = fa[2]; }
+  if (precision == CTEM_PREC_FP64 && have_fa) { p->fa_total = fa[0]; p->fa_prod = fa[1]; p->fa_solve = fa[2]; }
   p->ran = true;
+  return CTEM_OK;
+}
+
+static void reset_timing(ctem_plan* p) {
+  p->t_total = p->t_gemm = p->f_gemm = 0.0;
+  p->n_gemm = 0;
+  p->evused = 0;
+}
+
+extern "C" int32_t ctem_plan_run(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  int32_t rc;
+  if ((rc = run_prepare(p, precision, keep_lgh_ik))) return rc;
+  ctem_ctx* c = p->ctx;
+  hipStream_t s = c->stream;
+  const int form = run_form(p, precision);
+  reset_timing(p);
+  double fa[3] = {0.0, 0.0, 0.0};
+  const bool have_fa = precision == CTEM_PREC_FP64 && !sched_complete(p, form);
+  // graph mode, once every chunk's squaring counts are known and no Lgh is kept: the run is
+  // captured once and then replayed as one launch
+  if (p->graph_mode && keep_lgh_ik < 0 && sched_complete(p, form)) {
+    const std::vector<uint64_t> key = graph_key_of(p, precision);
+    if (!p->graph || p->graph_key != key) {
+      graph_drop(p->graph);
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      rc = run_enqueue(p, precision, false, false, nullptr);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (e != hipSuccess) return fail(CTEM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { p->graph = nullptr; return fail(CTEM_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+      p->graph_key = key;
+    }
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    HIPCHK(hipGraphLaunch(p->graph, s));
+    rc = run_finish(p, precision, fa, false);
+  } else {
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if ((rc = run_enqueue(p, precision, true, true, fa))) return rc;
+    rc = run_finish(p, precision, fa, have_fa);
+  }
+  if (rc == CTEM_E_STATE && strstr(g_err.c_str(), "schedule")) {
+    // a matrix needed more squarings than an earlier run of the plan: run again with the counts
+    // read back
+    for (auto it = p->sched.begin(); it != p->sched.end();)
+      it = (std::get<0>(it->first) == form) ? p->sched.erase(it) : std::next(it);
+    graph_drop(p->graph);
+    reset_timing(p);
+    fa[0] = fa[1] = fa[2] = 0.0;
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    if ((rc = run_enqueue(p, precision, true, true, fa))) return rc;
+    rc = run_finish(p, precision, fa, precision == CTEM_PREC_FP64);
+  }
+  return rc;
+}
+
+// No counterpart in the reference: tuning of the batched launch (see the header).
+extern "C" int32_t ctem_plan_set_tuning(ctem_plan* p, int32_t bucket_multiple, double chunk_gib) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if (bucket_multiple != 32 && bucket_multiple != 64)
+    return fail(CTEM_E_ARG, "bucket multiple must be 32 or 64 (got %d)", bucket_multiple);
+  if (!(chunk_gib >= 0.25 && chunk_gib <= 256.0)) return fail(CTEM_E_ARG, "chunk size must be 0.25..256 GiB");
+  const size_t bytes = (size_t)(chunk_gib * (double)(1ull << 30));
+  if (bucket_multiple != p->bucket_multiple || bytes != p->chunk_bytes) {
+    p->bucket_multiple = bucket_multiple;
+    p->chunk_bytes = bytes;
+    p->sched.clear();
+    graph_drop(p->graph);
+    p->selected = false;  // the bucket layout follows at the next run
+  }
+  return CTEM_OK;
+}
+
+// No counterpart in the reference: graph mode of a plan (see the header).
+extern "C" int32_t ctem_plan_set_graph(ctem_plan* p, int32_t on) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if (on != 0 && on != 1) return fail(CTEM_E_ARG, "graph mode must be 0 or 1");
+  p->graph_mode = on;
+  if (!on) graph_drop(p->graph);
+  return CTEM_OK;
+}
+
+// No counterpart in the reference: the plans of an energy series as one captured graph.
+extern "C" int32_t ctem_plans_run(ctem_plan* const* plans, int32_t nplans, int32_t precision) {
+  if (!plans || nplans < 1) return fail(CTEM_E_ARG, "ctem_plans_run: empty plan list");
+  ctem_ctx* c = plans[0] ? plans[0]->ctx : nullptr;
+  if (!c) return fail(CTEM_E_ARG, "ctem_plans_run: null plan");
+  for (int i = 0; i < nplans; i++)
+    if (!plans[i] || plans[i]->ctx != c) return fail(CTEM_E_ARG, "ctem_plans_run: plans of one context only");
+  int32_t rc;
+  // warm: every plan's bucket layout, work space and squaring counts (eager runs where needed)
+  for (int i = 0; i < nplans; i++) {
+    ctem_plan* p = plans[i];
+    if ((rc = run_prepare(p, precision, -1))) return rc;
+    if (!sched_complete(p, run_form(p, precision)))
+      if ((rc = ctem_plan_run(p, precision, -1))) return rc;
+  }
+  for (int i = 0; i < nplans; i++)
+    if ((rc = run_reserve(plans[i], precision))) return rc;
+  std::vector<uint64_t> key{(uint64_t)nplans};
+  for (int i = 0; i < nplans; i++) {
+    auto k = graph_key_of(plans[i], precision);
+    key.insert(key.end(), k.begin(), k.end());
+  }
+  hipStream_t s = c->stream;
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->mgraph || c->mgraph_key != key) {
+    graph_drop(c->mgraph);
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = CTEM_OK;
+    for (int i = 0; i < nplans && !rc; i++) {
+      plans[i]->keep_dir = -1;
+      rc = run_enqueue(plans[i], precision, false, false, nullptr);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(CTEM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->mgraph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { c->mgraph = nullptr; return fail(CTEM_E_HIP, "graph instantiate: %s", hipGetErrorString(e)); }
+    c->mgraph_key = key;
+  }
+  for (int i = 0; i < nplans; i++) reset_timing(plans[i]);
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  HIPCHK(hipGraphLaunch(c->mgraph, s));
+  HIPCHK(hipEventRecord(c->ev[1], s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  for (int i = 0; i < nplans; i++) {
+    ctem_plan* p = plans[i];
+    p->t_total = (i == 0) ? ms : 0.0;
+    if ((rc = check_err_flag(p->err, s, "ctem_plans_run"))) {
+      graph_drop(c->mgraph);
+      return rc;
+    }
+    p->keep_n = 0;
+    p->ran = true;
+  }
   return CTEM_OK;
 }
 

@@ -279,7 +279,8 @@ struct SGemm {
   const uint16_t* M[2]; const int32_t* me[2]; const float* mmax[2]; float beta[2];
   uint16_t* C2; int32_t* ce2; float* cmax2;
   float alpha2, betaI2, beta2[2];
-  const int32_t* active; int32_t step;  // matrices with step >= active[b] are skipped
+  const int32_t* active; int32_t step;  // matrices with step >= active[b] + active_add are skipped
+  int32_t active_add;
   int32_t nb;
   // optional: process only the matrices order[0 .. nord-1] (then active is not consulted)
   const int32_t* order; int32_t nord;
@@ -303,6 +304,11 @@ struct SGemm {
   int32_t xraw; const int32_t* rs; int32_t mraw[2];
 };
 void launch_sgemm(hipStream_t s, const SGemm& g);
+// zero `bytes` bytes (a multiple of 4) of device memory with a kernel (no memset node, so that
+// the sequence can be captured into a graph)
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes);
+// sets bit 32 of *err when some sq[b] > smax (a run launched with a stale squaring schedule)
+void launch_sq_check(hipStream_t s, int32_t nb, const int32_t* sq, int32_t smax, int32_t* err);
 // The column-sum partials and max |entry| of launch_assemble_split -> ||A||_1 per matrix, the
 // squaring count s, and the split exponent (from the exact max) and max |entry| of B = A 2^-s
 void launch_norm_split(hipStream_t s, int32_t npad, int32_t nb, const double* colpart, const float* amax,

@@ -1003,6 +1003,29 @@ __global__ void k_scatter(int numk, const int32_t* __restrict__ ij, const double
   }
 }
 
+__global__ void k_fill_zero32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0u;
+}
+
+// flag 32 when a matrix needs more squarings than the schedule a run was launched with
+__global__ void k_sq_check(int nb, const int32_t* __restrict__ sq, int smax, int32_t* __restrict__ err) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb && sq[b] > smax) atomicOr(err, 32);
+}
+
+void launch_sq_check(hipStream_t s, int32_t nb, const int32_t* sq, int32_t smax, int32_t* err) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_sq_check, dim3((nb + 255) / 256), dim3(256), 0, s, nb, sq, smax, err);
+}
+
+void launch_fill_zero(hipStream_t s, void* p, size_t bytes) {
+  const int64_t n = (int64_t)(bytes / 4);
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_zero32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)p, n);
+}
+
 void launch_scatter(hipStream_t s, int32_t numk, const int32_t* ij, const double* dots,
                     int32_t nsite, float nat, int32_t npx, int32_t npy, int32_t isym,
                     int32_t* owner, float* sr) {

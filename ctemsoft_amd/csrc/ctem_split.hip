@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(JN == 4 ? 2
     bn = (t / tm) * BN;
   }
   const bool offd = g.sym && bm != bn;
-  if (!g.order && g.active && g.step >= g.active[b]) return;
+  if (!g.order && g.active && g.step >= g.active[b] + g.active_add) return;
   const int64_t per = (int64_t)n * n;
   const int64_t mb = (int64_t)b * 4 * per;
   const _Float16* X = (const _Float16*)g.X + mb;

@@ -32,7 +32,8 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
            "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
-           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops"]
+           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops",
+           "ctem_plan_set_graph", "ctem_plans_run", "ctem_plan_set_tuning"]
 
 
 class CtemError(RuntimeError):
@@ -76,6 +77,8 @@ def lib():
             "ctem_plan_set_symmetry": [P, I32, P],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
             "ctem_plan_adjoint": [P, P, P, P], "ctem_plan_alg_flops": [P, P, P, P],
+            "ctem_plan_set_graph": [P, I32], "ctem_plans_run": [P, I32, I32],
+            "ctem_plan_set_tuning": [P, I32, D],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -102,6 +105,39 @@ def lambert_dims(npx, usehex):
         v = np.float32(2.0) * np.float32(npx) / np.sqrt(np.float32(3.0))
         return int(npx), int(np.floor(float(v) + 0.5))
     return int(npx), int(npx)
+
+
+# First space-group number of each of the 32 crystallographic point groups in International
+# Tables order, and the Laue class (1..11: -1, 2/m, mmm, 4/m, 4/mmm, -3, -3m, 6/m, 6/mmm, m-3,
+# m-3m) of each point group (symmetryvars.f90:242-244,286-287).
+_SG_FIRST_OF_PG = (1, 2, 3, 6, 10, 16, 25, 47, 75, 81, 83, 89, 99, 111, 123, 143, 147, 149, 156, 162,
+                   168, 174, 175, 177, 183, 187, 191, 195, 200, 207, 215, 221)
+_LAUE_OF_PG = (1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 6, 6, 7, 7, 7, 8, 8, 8, 9, 9, 9, 9, 10, 10,
+               11, 11, 11)
+# trigonal -3m space groups whose Lambert sampling takes the second setting of the Laue group
+# (isym 12; CTEMEBSDmaster.f90:254-282)
+_LAUE_SECOND_SETTING = (149, 151, 153, 156, 158, 160, 161, 164, 165, 166, 167)
+
+
+def run_plans(plans, precision="fp64"):
+    """All plans (of one context) as one captured HIP graph (ctem_plans_run)."""
+    arr = (C.c_void_p * len(plans))(*[p._h.value for p in plans])
+    _check(lib().ctem_plans_run(C.cast(arr, C.c_void_p), len(plans), PREC[precision]))
+
+
+def laue_group(sgnum):
+    """(isym, usehex) that the master-pattern program derives from the space-group number
+    (CTEMEBSDmaster.f90:249-282): the Laue class of the space group's point group (the last point
+    group whose first space group is <= sgnum), 12 in place of 7 for the listed trigonal groups,
+    and the hexagonal Lambert grid for Laue classes 6..9 and 12."""
+    sgnum = int(sgnum)
+    if not 1 <= sgnum <= 230:
+        raise ValueError(f"space group {sgnum} not in 1..230")
+    pg = max(i for i, first in enumerate(_SG_FIRST_OF_PG) if first <= sgnum)
+    isym = _LAUE_OF_PG[pg]
+    if isym == 7 and sgnum in _LAUE_SECOND_SETTING:
+        isym = 12
+    return isym, int(6 <= isym <= 9 or isym == 12)
 
 
 def kvectors(npx, npy, isym, usehex, dsm, mlambda, hemisphere=1):
@@ -285,6 +321,15 @@ class Plan:
 
     def run(self, precision="fp64", keep_lgh=-1):
         _check(lib().ctem_plan_run(self._h, PREC[precision], int(keep_lgh)))
+
+    def set_tuning(self, bucket_multiple=32, chunk_gib=24.0):
+        """Bucket granularity of the padded sizes and work space per chunk (ctem_plan_set_tuning)."""
+        _check(lib().ctem_plan_set_tuning(self._h, int(bucket_multiple), float(chunk_gib)))
+
+    def set_graph(self, on=True):
+        """Capture the plan's run into a HIP graph once its squaring counts are known, and
+        replay it (ctem_plan_set_graph)."""
+        _check(lib().ctem_plan_set_graph(self._h, 1 if on else 0))
 
     def dots(self):
         d = np.zeros(self.numk * self.nsite)

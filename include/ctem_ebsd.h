@@ -133,6 +133,29 @@ CTEM_API int32_t ctem_plan_get_beams(ctem_plan* plan, int32_t ik, int32_t* stron
  * ctem_plan_get_lgh. */
 CTEM_API int32_t ctem_plan_run(ctem_plan* plan, int32_t precision, int32_t keep_lgh_ik);
 
+/* No counterpart in the reference (the launch pattern of the batched loop). on = 1: once a run
+ * of the plan has read back the squaring counts of its matrices (the first run in each
+ * precision; the same directions and crystal give the same counts on every run), later runs
+ * need no host synchronisation inside, and the plan captures its whole run -- assembly,
+ * matrix exponentials, depth recursion and contraction of every bucket -- into a HIP graph
+ * once and replays it as one launch. The graph is captured again when the options, the
+ * selection or the context's work space change. A run that needs more squarings than
+ * recorded (not possible for unchanged inputs) is detected on the device and run again
+ * without the graph. Per-kernel timing (ctem_plan_timing) is not recorded in a graph replay;
+ * the total is. Runs that keep an Lgh (keep_lgh_ik >= 0) do not use the graph. Default 0. */
+CTEM_API int32_t ctem_plan_set_graph(ctem_plan* plan, int32_t on);
+/* No counterpart in the reference (launch tuning; results do not depend on it). Directions are
+ * grouped into buckets of one padded matrix size, a multiple of bucket_multiple (32, default,
+ * or 64), and each bucket runs in chunks of as many directions as fit in chunk_gib GiB of work
+ * space (0.25..256, default 24). ctemsoft_amd/autotune.py picks both per GPU and workload. */
+CTEM_API int32_t ctem_plan_set_tuning(ctem_plan* plan, int32_t bucket_multiple, double chunk_gib);
+/* No counterpart in the reference. Run several plans of one context (the energy bins of a
+ * series) as one captured HIP graph: every plan's bucket layout and squaring counts are
+ * established first (eager runs where missing), then all their runs are captured into one
+ * graph, kept on the context and replayed while the same plans, options and work space are
+ * used. Results are those of ctem_plan_run on each plan, bit for bit. */
+CTEM_API int32_t ctem_plans_run(ctem_plan* const* plans, int32_t nplans, int32_t precision);
+
 /* Split schemes (CTEM_SPLIT_*) of the mixed form for the Taylor products (on the scaled
  * matrix, ||B||_1 <= 3/4 with CTEM_SPLIT_FP16X3 for both, 1/2 otherwise) and for the squarings;
  * default CTEM_SPLIT_FP16X3 for both. */

@@ -7,7 +7,9 @@
 !   - Apply2DLaueSymmetry (Lambert.f90:1942-2046) for isym = 1..13 over every pixel (ipx, ipy)
 !     of the box -nbox..nbox squared: nequiv and iequiv(2,12);
 !   - Calckvectors 'RoscaLambert' (kvectors.f90:421-542, AddkVector :839-947) for isym = 1..12
-!     on the square and the hexagonal grid: numk, then (i, j, k(3), kn) per direction.
+!     on the square and the hexagonal grid: numk, then (i, j, k(3), kn) per direction;
+!   - the Laue group and grid of every space group 1..230 by the program's rule
+!     (CTEMEBSDmaster.f90:249-282) on the reference's SGPG/PGLaueinv tables.
 ! The structure matrix and wavelength come from the namelist (they only enter through the
 ! final transformation of AddkVector). The output is one little-endian stream file parsed by
 ! tests/golden/make_golden.py.
@@ -16,6 +18,7 @@ program ctemref_sym
 use local
 use constants
 use crystalvars
+use symmetryvars
 use diffraction
 use kvectors
 use Lambert
@@ -23,6 +26,8 @@ use Lambert
 IMPLICIT NONE
 
 integer(kind=irg) :: nbox, npx, npy, npyhex, isym, ipx, ipy, iequiv(2,12), nequiv, ihex, numk, ijmax, ik
+integer(kind=irg) :: isg, i, j
+integer(kind=irg),parameter :: LaueTest(11) = (/ 149, 151, 153, 156, 158, 160, 161, 164, 165, 166, 167 /)
 real(kind=dbl)    :: dsm(3,3), wavelength
 character(fnlen)  :: outname
 logical           :: usehex
@@ -75,6 +80,21 @@ do ihex = 0, 1
       kp => kp%next
     end do
   end do
+end do
+! the Laue group and grid the master-pattern program picks for every space group
+! (CTEMEBSDmaster.f90:249-282, with the reference's SGPG and PGLaueinv tables)
+do isg = 1, 230
+  j = 0
+  do i = 1, 32
+    if (SGPG(i).le.isg) j = i
+  end do
+  isym = PGLaueinv(j)
+  if (isym.eq.7) then
+    do i = 1, 11
+      if (isg.eq.LaueTest(i)) isym = 12
+    end do
+  end if
+  write(ou) isg, isym, merge(1, 0, ((isym.ge.6).and.(isym.le.9)).or.(isym.eq.12))
 end do
 close(ou)
 

@@ -238,6 +238,10 @@ def run_sym(work: str, nbox: int, npx: int, dsm: np.ndarray, wavelength: float) 
         out[f"kv{isym}_{ihex}_ij"] = np.stack([rec["i"], rec["j"]], 1).astype(np.int32)
         out[f"kv{isym}_{ihex}_k"] = rec["k"].copy()
         out[f"kv{isym}_{ihex}_kn"] = rec["kn"].copy()
+    lg = s.get("<i4", 3 * 230).reshape(230, 3)
+    assert np.array_equal(lg[:, 0], np.arange(1, 231))
+    out["laue_isym"] = lg[:, 1].astype(np.int8)
+    out["laue_usehex"] = lg[:, 2].astype(np.int8)
     assert s.o == len(s.b), "trailing bytes in symmetry dump"
     return out
 

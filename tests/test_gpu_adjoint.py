@@ -142,3 +142,42 @@ def test_directional_derivative_whole_lut(ctx):
     pred = float(np.sum(glut.real * (dp + dm).real + glut.imag * (dp + dm).imag))
     _say(f"directional: fd {fp - fm:.6e} adjoint {pred:.6e}")
     assert abs((fp - fm) - pred) <= 1e-4 * abs(pred)
+
+
+def test_gradient_matches_complex_step(ctx, oracle_mod):
+    """The reverse mode against the complex-step derivative (oracle/complex_step.py: the same
+    forward carried in complex arithmetic on the real parameters, exact to rounding) of
+    sum_j w_j dots_j over a few fcc-Ni directions: the real and imaginary parts of the LUT
+    entries with the largest gradients, and the site's Debye-Waller factor B. The beam sets
+    are held fixed, as the reverse mode holds them."""
+    from oracle import complex_step as cs
+    fx = load_golden("ni_c0w")
+    st = cs.setup_from_fixture(fx)
+    cr = oracle_mod.Crystal(fx)
+    rng = np.random.default_rng(11)
+    ks = np.sort(rng.choice(len(fx["kvec"]), 3, replace=False))
+    ctx.upload_crystal(_setup(fx))
+    plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.run("fp64")
+    w = rng.standard_normal(plan.dots().shape)
+    glut, gb = plan.adjoint(w)
+    plan.close()
+    cases = []
+    for j, k in enumerate(ks):
+        beams = oracle_mod.select_beams(cr, fx["kvec"][k])
+        cases.append((beams, float(fx["kn"][k]), w[j, 0]))
+    args = lambda c: (st, c[0], c[1], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    flat = np.argsort(-np.abs(glut).ravel())[:3]
+    gmax = np.abs(glut).max()
+    for cell in flat:
+        idx = np.unravel_index(cell, glut.shape)
+        for part in ("re", "im"):
+            ref = sum(c[2] * cs.derivative_lut(*args(c), st["lut"], st["site_b"], int(cell), part)[0] for c in cases)
+            ad = glut[idx].real if part == "re" else glut[idx].imag
+            _say(f"cell {idx} {part}: adjoint {ad:.12e} complex step {ref:.12e}")
+            # the device contracts with the reference's float32-rounded S terms, the complex step
+            # with the smooth S they round: ~1e-7 relative between the two
+            assert abs(ad - ref) <= 1e-7 * gmax + 1e-6 * abs(ref), (idx, part, ad, ref)
+    refb = sum(c[2] * cs.derivative_b(*args(c), st["lut"], st["site_b"], 0)[0] for c in cases)
+    _say(f"B: adjoint {gb[0]:.12e} complex step {refb:.12e}")
+    assert abs(gb[0] - refb) <= 1e-8 * abs(refb)

@@ -1,6 +1,6 @@
 """Master-pattern residual and gradient (ctemsoft_amd.fit) for the adjoint workload's model:
-fcc Ni, target = the same model with B x 1.05 and LUT x (1 + 0.01 N(0,1)); a reduced Lambert
-grid keeps the finite differences quick."""
+fcc Ni, target = the same model with B x 1.05 and LUT x (1 + 0.01 N(0,1)), on the workload's own
+Lambert grid (npx 100, 5,151 directions) and on a reduced one (npx 20) for the descent step."""
 import sys
 
 import numpy as np
@@ -25,9 +25,10 @@ def _problem(npx=20):
     return fx, st, wl, ij, k, kn, fit.perturbed_target_setup(st, wl.seed)
 
 
-def test_master_gradient_directional_and_descent():
+@pytest.mark.parametrize("npx", [20, 100])
+def test_master_gradient_directional_and_descent(npx):
     from ctemsoft_amd import ebsd, fit
-    fx, st, wl, ij, k, kn, st_t = _problem()
+    fx, st, wl, ij, k, kn, st_t = _problem(npx)
     npx = int(ij[:, 0].max())
     isym = int(st.isym)
     ctx = ebsd.Context(0)
@@ -41,9 +42,18 @@ def test_master_gradient_directional_and_descent():
     ctx.upload_crystal(st)
     plan = ctx.plan(k, kn, lam, wl.dz, thick)
     owner = fit.pixel_owner(ij, npx, npx, isym)
+    import time
+    t0 = time.perf_counter()
     loss, glut, gb = fit.master_loss_grad(plan, ij, npx, npx, isym, target, owner)
-    _say(f"{len(k)} directions, loss {loss:.6e}, |grad LUT| {np.abs(glut).max():.3e}, grad B {gb}")
+    t1 = time.perf_counter()
+    _say(f"{len(k)} directions, loss {loss:.6e}, |grad LUT| {np.abs(glut).max():.3e}, grad B {gb}, "
+         f"forward + reverse {t1 - t0:.2f} s")
     assert loss > 0 and np.isfinite(glut).all() and np.isfinite(gb).all()
+    # the loss from the mixed forward (the inner loop of a fit) against the FP64 one: the
+    # residuals are ~1e-2 of the intensities, the mixed form's error ~1e-5 of them
+    loss_m, _ = fit.master_loss(plan, ij, npx, npx, isym, target, precision="mixed")
+    _say(f"loss fp64 {loss:.8e} mixed {loss_m:.8e}")
+    assert abs(loss_m - loss) <= 1e-2 * loss
     plan.close()
 
     d = st.lut_dim
@@ -72,6 +82,9 @@ def test_master_gradient_directional_and_descent():
     _say(f"directional: fd {fd:.6e} adjoint {pred:.6e}")
     assert abs(fd - pred) <= 2e-3 * abs(pred)
 
+    if npx != 20:
+        ctx.close()
+        return
     # a step against the gradient lowers the residual
     g = glut.astype(np.complex128)
     step = 0.2 * loss / float(np.sum(np.abs(g) ** 2))

@@ -121,3 +121,21 @@ def test_target_setup_shares_factors_between_g_and_minus_g():
     assert np.allclose(f[m], f[::-1, ::-1, ::-1][m], rtol=1e-6)
     assert 0.005 < np.std(np.abs(f[m])) < 0.02
     assert np.allclose(tt.site_b, st.site_b * np.float32(1.05))
+
+
+def test_laue_group_of_every_space_group():
+    """The host's Laue-group pick (isym, usehex) for space groups 1..230 against the reference's
+    rule on its own tables (CTEMEBSDmaster.f90:249-282) and against the Laue classes of
+    International Tables (space-group ranges of the 11 classes), with the fixtures' crystals."""
+    from ctemsoft_amd import ebsd
+    fx = load_golden("sym_tables")
+    ranges = [(1, 2, 1), (3, 15, 2), (16, 74, 3), (75, 88, 4), (89, 142, 5), (143, 148, 6), (149, 167, 7),
+              (168, 176, 8), (177, 194, 9), (195, 206, 10), (207, 230, 11)]
+    for g in range(1, 231):
+        isym, hexg = ebsd.laue_group(g)
+        assert (isym, hexg) == (int(fx["laue_isym"][g - 1]), int(fx["laue_usehex"][g - 1])), g
+        cls = next(c for a, b, c in ranges if a <= g <= b)
+        assert isym == cls or (cls == 7 and isym == 12), g
+    for name in ("ni_c0", "fe_c1", "al2o3_c2"):
+        f = load_golden(name)
+        assert ebsd.laue_group(int(f["sgnum"])) == (int(f["isym"]), int(f["usehex"]))
