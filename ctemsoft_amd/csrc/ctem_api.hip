@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -139,6 +140,7 @@ struct ctem_ctx {
   DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
   hipEvent_t ev[2] = {nullptr, nullptr};
   std::vector<ctem_plan*> plans;  // live plans, destroyed with the context
+  unsigned char* scratch = nullptr;  // this device's static work space (ctem_calclgh3_static)
   // bumped whenever a work-space buffer is reallocated: graphs captured before hold the old
   // addresses and are captured again
   uint64_t wgen = 0;
@@ -203,6 +205,37 @@ struct ctem_plan {
 };
 
 // ------------------------------------------------------------------------------------------
+// ------------------------------------------------------------------------------------------
+// Work space of the allocation-free CalcLgh3 entry (ctem_calclgh3_static): one array in device
+// memory per GPU, for matrices of up to kStaticMaxN strong beams and kStaticMaxZ depth steps.
+// Each GPU the library runs on holds it until the process ends.
+constexpr int kStaticMaxN = 256;
+constexpr int kStaticMaxZ = 256;
+constexpr size_t kStaticMat = (size_t)kStaticMaxN * kStaticMaxN * sizeof(double2);  // 1 MiB
+constexpr size_t kStaticBytes = 8 * kStaticMat + 64 * 1024;                         // 8 MiB + 64 KiB
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+// device address of `scratch` on each device, looked up once per device with that device
+// current (the host-side name is only a stand-in for the symbol, not a device address)
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;
+
+int32_t scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CTEM_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return CTEM_OK;
+}
+}  // namespace
+
 extern "C" const char* ctem_last_error(void) { return g_err.c_str(); }
 
 extern "C" int32_t ctem_ctx_create(int32_t device, ctem_ctx** out) {
@@ -225,6 +258,10 @@ extern "C" int32_t ctem_ctx_create(int32_t device, ctem_ctx** out) {
   if (e != hipSuccess) { delete c; return fail(CTEM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e)); }
   (void)hipEventCreate(&c->ev[0]);
   (void)hipEventCreate(&c->ev[1]);
+  void* sp = nullptr;
+  int32_t rc = scratch_address(device, &sp);
+  if (rc) { ctem_ctx_destroy(c); return rc; }
+  c->scratch = (unsigned char*)sp;
   *out = c;
   return CTEM_OK;
 }
@@ -1680,6 +1717,96 @@ __global__ void k_scale_dmat(int nn, int npad, const double2* __restrict__ D, do
   A[e] = v;
 }
 }  // namespace
+
+namespace {
+__global__ void k_fill_zero(uint32_t* __restrict__ p, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = 0u;
+}
+__global__ void k_widen_lam(int n, const float* __restrict__ f, double* __restrict__ d) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) d[e] = (double)f[e];
+}
+// row-major nn x nn -> column-major
+__global__ void k_transpose_c(int nn, const double2* __restrict__ R, double2* __restrict__ Cm) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nn * nn) return;
+  const int i = (int)(e % nn), j = (int)(e / nn);  // Cm(i, j) at j*nn + i
+  Cm[e] = R[(int64_t)i * nn + j];
+}
+}  // namespace
+
+extern "C" int32_t ctem_calclgh3_static(ctem_ctx* c, const double* dmat, double* lgh, double thick, double kn,
+                                        int32_t nn, int32_t gzero, double depthstep, const float* lambdae,
+                                        int32_t izz) {
+  if (!c || !dmat || !lgh || !lambdae) return fail(CTEM_E_ARG, "ctem_calclgh3_static: null argument");
+  if (nn < 1 || izz < 1 || !(thick > 0) || !(depthstep > 0) || !(kn > 0))
+    return fail(CTEM_E_ARG, "ctem_calclgh3_static: bad sizes");
+  if (gzero != 1) return fail(CTEM_E_RANGE, "ctem_calclgh3_static: only gzero = 1 (the transmitted beam first) is supported");
+  if (nn > kStaticMaxN || izz > kStaticMaxZ)
+    return fail(CTEM_E_RANGE, "ctem_calclgh3_static: at most %d strong beams and %d depth steps (got %d, %d); "
+                "use ctem_calclgh3", kStaticMaxN, kStaticMaxZ, nn, izz);
+  if (!c->scratch) return fail(CTEM_E_STATE, "ctem_calclgh3_static: no work space on this context");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int npad = npad_of(nn);
+  const int64_t per = (int64_t)npad * npad;
+  const int64_t spsi = (int64_t)npad * izz;
+  // carve the work space (all offsets multiples of 256 bytes)
+  unsigned char* base = c->scratch;
+  double2* A = (double2*)base;                          // 1 matrix
+  double2* W = (double2*)(base + kStaticMat);           // 5 matrices
+  double2* Psi = (double2*)(base + 6 * kStaticMat);     // npad x izz <= 1 matrix
+  double2* Lr = (double2*)(base + 7 * kStaticMat);      // DynMat in, then Lgh (row-major)
+  unsigned char* small = base + 8 * kStaticMat;         // 64 KiB of small buffers
+  int32_t* d_sq = (int32_t*)(small);
+  int32_t* d_err = (int32_t*)(small + 256);
+  int32_t* d_dirs = (int32_t*)(small + 512);
+  int32_t* d_nns = (int32_t*)(small + 768);
+  int64_t* d_off = (int64_t*)(small + 1024);
+  double* d_dots = (double*)(small + 1280);
+  int32_t* d_zero = (int32_t*)(small + 2048);           // 4 npad + 16 bytes <= 1040
+  double* d_part = (double*)(small + 4096);             // 8 (npad/32)^2 <= 512 bytes
+  float* d_lamf = (float*)(small + 8192);               // 4 izz <= 1 KiB
+  double* d_lam = (double*)(small + 12288);             // 8 izz <= 2 KiB
+  // the zeroed small buffers: counts, flags, the zero index list, partial sums
+  hipLaunchKernelGGL(k_fill_zero, dim3(16), dim3(256), 0, s, (uint32_t*)small, (int64_t)(8192 / 4));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(d_lamf, lambdae, 4 * (size_t)izz, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_widen_lam, dim3((izz + 255) / 256), dim3(256), 0, s, izz, (const float*)d_lamf, d_lam);
+  HIPCHK(hipMemcpyAsync(Lr, dmat, 16 * (size_t)nn * nn, hipMemcpyHostToDevice, s));
+  const double pre = 2.0 * kPi * depthstep / (double)(float)(2.0 * kn);
+  hipLaunchKernelGGL(k_scale_dmat, dim3((unsigned)((per + 255) / 256)), dim3(256), 0, s, nn, npad,
+                     (const double2*)Lr, pre, A);
+  HIPCHK(hipGetLastError());
+  std::vector<int> hn{nn};
+  std::vector<int32_t> hsq;
+  int32_t rc = expm_depth_fp64(nullptr, s, npad, 1, A, W, Psi, spsi, izz, d_sq, d_err, hn, hsq, c->cu_count);
+  if (rc) return rc;
+  const int32_t nnv[2] = {nn, 0};
+  HIPCHK(hipMemcpyAsync(d_nns, nnv, 8, hipMemcpyHostToDevice, s));
+  BeamView V{d_nns, d_nns + 1, d_off, d_off + 1, d_zero, nullptr, nullptr, nullptr};
+  DevRefl R{};
+  R.hkl = d_zero;
+  DevLut L{};
+  L.nsite = 0;
+  L.dim[0] = L.dim[1] = L.dim[2] = 1;
+  Bucket B{npad, 1, d_dirs};
+  launch_lgh_dots(s, B, V, R, L, Psi, spsi, izz, d_lam, depthstep / thick, d_part, d_dots, 0, Lr, 0);
+  HIPCHK(hipGetLastError());
+  // column-major for the Fortran caller, into the first work matrix (free after the depth stage)
+  hipLaunchKernelGGL(k_transpose_c, dim3((unsigned)(((int64_t)nn * nn + 255) / 256)), dim3(256), 0, s, nn,
+                     (const double2*)Lr, W);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(lgh, W, 16 * (size_t)nn * nn, hipMemcpyDeviceToHost, s));
+  int32_t h = 0;
+  HIPCHK(hipMemcpyAsync(&h, d_err, sizeof h, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h & 2) return fail(CTEM_E_NUMERIC, "ctem_calclgh3_static: non-finite dynamical matrix");
+  if (h & 4) return fail(CTEM_E_NUMERIC, "ctem_calclgh3_static: matrix norm needs more squarings than allowed");
+  if (h) return fail(CTEM_E_NUMERIC, "ctem_calclgh3_static: error flag %d", h);
+  return CTEM_OK;
+}
 
 extern "C" int32_t ctem_calclgh3(ctem_ctx* c, const double* dmat, double* lgh, double thick, double kn,
                                  int32_t nn, int32_t gzero, double depthstep, const float* lambdae,

@@ -32,7 +32,7 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
            "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
-           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops",
+           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops", "ctem_calclgh3_static",
            "ctem_plan_set_graph", "ctem_plans_run", "ctem_plan_set_tuning"]
 
 
@@ -76,6 +76,7 @@ def lib():
             "ctem_plan_timing": [P, P, P, P, P], "ctem_plan_set_mixed_terms": [P, I32, I32],
             "ctem_plan_set_symmetry": [P, I32, P],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
+            "ctem_calclgh3_static": [P, P, P, D, D, I32, I32, D, P, I32],
             "ctem_plan_adjoint": [P, P, P, P], "ctem_plan_alg_flops": [P, P, P, P],
             "ctem_plan_set_graph": [P, I32], "ctem_plans_run": [P, I32, I32],
             "ctem_plan_set_tuning": [P, I32, D],

@@ -171,6 +171,20 @@ interface
     integer(c_int32_t)        :: rc
   end function
 
+  ! ctem_calclgh3 without device allocations: work space in a fixed per-GPU array (nn <= 256,
+  ! izz <= 256); concurrent calls on one GPU race (no counterpart in the reference)
+  function ctem_calclgh3_static(ctx, dmat, lgh, thick, kn, nn, gzero, depthstep, lambdae, izz) &
+           bind(C, name='ctem_calclgh3_static') result(rc)
+    import :: c_int32_t, c_double, c_double_complex, c_float, c_ptr
+    type(c_ptr), value        :: ctx
+    complex(c_double_complex), intent(in)  :: dmat(*)
+    complex(c_double_complex), intent(out) :: lgh(*)
+    real(c_double), value     :: thick, kn, depthstep
+    integer(c_int32_t), value :: nn, gzero, izz
+    real(c_float), intent(in) :: lambdae(*)
+    integer(c_int32_t)        :: rc
+  end function
+
 end interface
 
 contains

@@ -217,6 +217,18 @@ CTEM_API int32_t ctem_calclgh3(ctem_ctx* ctx, const double* dmat, double* lgh, d
                       int32_t nn, int32_t gzero, double depthstep, const float* lambdae,
                       int32_t izz);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): ctem_calclgh3 with
+ * the same arguments and results, whose work space is a fixed array in device memory that the
+ * library holds per GPU (8 MiB + 64 KiB, kept until the process ends) instead of buffers
+ * allocated on the context. It calls no hipMalloc or hipFree. Limits: nn <= 256 strong beams
+ * and izz <= 256 depth steps (CTEM_E_RANGE beyond). Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several host threads, or contexts on the same device
+ * used concurrently) race and must be serialised by the caller. */
+CTEM_API int32_t ctem_calclgh3_static(ctem_ctx* ctx, const double* dmat, double* lgh, double thick, double kn,
+                             int32_t nn, int32_t gzero, double depthstep, const float* lambdae,
+                             int32_t izz);
+
 #ifdef __cplusplus
 }
 #endif
