@@ -12,17 +12,22 @@ import torch
 from conftest import REPO, lambert_dims, load_golden
 
 
-def _declared():
-    txt = open(os.path.join(REPO, "include", "ctem_ebsd.h")).read()
+def _declared(headers=("ctem_ebsd.h",)):
+    txt = "".join(open(os.path.join(REPO, "include", h)).read() for h in headers)
     return sorted(set(re.findall(r"CTEM_API\s+[\w\s\*]+?\b(ctem_\w+)\s*\(", txt)))
 
 
+def _all_headers():
+    return tuple(sorted(f for f in os.listdir(os.path.join(REPO, "include")) if f.endswith(".h")))
+
+
 def test_library_exports_every_declared_symbol():
-    from ctemsoft_amd import ebsd
+    from ctemsoft_amd import ebsd, xtal
     L = ebsd.lib()
     decl = _declared()
     assert decl == sorted(ebsd.EXPORTS)
-    for name in decl:
+    assert _declared(("ctem_setup.h",)) == sorted(xtal.EXPORTS)
+    for name in _declared(_all_headers()):
         assert hasattr(L, name), name
 
 
@@ -31,7 +36,7 @@ def test_exports_match_nm():
     from ctemsoft_amd import ebsd
     out = subprocess.run(["nm", "-D", "--defined-only", ebsd.LIBPATH], capture_output=True, text=True).stdout
     exported = sorted(l.split()[-1] for l in out.splitlines() if " T " in l and l.split()[-1].startswith("ctem_"))
-    assert exported == _declared()
+    assert exported == _declared(_all_headers())
 
 
 @pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
