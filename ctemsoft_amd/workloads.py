@@ -25,6 +25,7 @@ cutoffs so that the strong-beam counts land in the quoted ranges:
   ni_c0w / ni_c4w   weakcutoff 240, cutoff 480    (quoted 50-150 / 100-250 strong beams)
   fe_c1w / fe_c3w   weakcutoff 320, cutoff 640    (quoted 80-300 / 60-350)
   al2o3_c2w         weakcutoff 320, cutoff 640    (quoted 200-800)
+  al2o3_c2x         weakcutoff 640, cutoff 1280   (the upper half of 200-800: 509-616 strong beams)
 
 Debye-Waller B is in nm^2, as in the reference's .xtal files (crystal.f90:1163).
 """
@@ -151,7 +152,16 @@ _WIDE_CUTOFFS = {"ni_c0": (240.0, 480.0), "ni_c4": (240.0, 480.0), "fe_c1": (320
                  "fe_c3": (320.0, 640.0), "al2o3_c2": (320.0, 640.0)}
 
 
+# the upper half of corundum's quoted 200-800 strong beams (509-616 measured: matrices past 512)
+_WIDER_CUTOFFS = {"al2o3_c2": (640.0, 1280.0)}
+
+
 def get(name: str) -> Workload:
+    if name.endswith("x") and name[:-1] in _WIDER_CUTOFFS:
+        wl = _WORKLOADS[name[:-1]]()
+        wl.name = name
+        wl.weakcutoff, wl.cutoff = _WIDER_CUTOFFS[name[:-1]]
+        return wl
     base = name[:-1] if name.endswith("w") and name[:-1] in _WIDE_CUTOFFS else name
     try:
         wl = _WORKLOADS[base]()
@@ -164,4 +174,4 @@ def get(name: str) -> Workload:
 
 
 def names():
-    return sorted(list(_WORKLOADS) + [k + "w" for k in _WIDE_CUTOFFS])
+    return sorted(list(_WORKLOADS) + [k + "w" for k in _WIDE_CUTOFFS] + [k + "x" for k in _WIDER_CUTOFFS])

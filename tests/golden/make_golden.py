@@ -179,6 +179,23 @@ def main():
                                       ekev=e, tag=f"{w}_{e:.1f}kV", corners=False))
             if w == "fe_c3w":
                 print(setup_series(w))
+        elif w == "al2o3_c2x":
+            # matrices past 512 beams: beam sets on 32 directions, the reference's Lgh (zgeev) on
+            # the two largest of 3 seeded directions, kept as upper triangles (Lgh is Hermitian);
+            # the LUT and DynMat are al2o3_c2's and the oracle's (both pinned to the reference)
+            fn = golden_workload(w, nsel_dyn=2, nsel_beams=32, full_program=False, corners=False)
+            d = dict(np.load(fn))
+            sizes = [d[f"dyn{j}_lgh"].shape[0] for j in range(len(d["dyn_ik"]))]
+            keep = [j for j in np.argsort(sizes)[::-1][:2]]
+            out = {k: v for k, v in d.items() if not k.startswith("dyn") and k not in ("lut", "dbdiff")}
+            out["dyn_ik"] = d["dyn_ik"][keep]
+            for jj, j in enumerate(keep):
+                L = d[f"dyn{j}_lgh"]
+                iu = np.triu_indices(L.shape[0])
+                out[f"dyn{jj}_lgh_n"] = np.int32(L.shape[0])
+                out[f"dyn{jj}_lgh_upper"] = L[iu]
+            np.savez_compressed(fn, **out)
+            print(fn, [d[f"dyn{j}_lgh"].shape[0] for j in keep])
         elif w == "sym":
             print(golden_sym())
         else:
