@@ -501,3 +501,372 @@ int orc_laue_equiv(int ipx, int ipy, int isym, int32_t* q) {
 #undef FL
   return ne;
 }
+
+/* ---- EBSD detector patterns (CTEMEBSD.f90) ---------------------------------------------- */
+
+/* InitLambertParameters (Lambert.f90:231-262): sqrt(pi/2) and sqrt(pi)/2 with pi = 4 atan(1) */
+static double lps_spio2(void) { return sqrt(4.0 * atan(1.0) * 0.5); }
+static double lps_spi2(void) { return sqrt(4.0 * atan(1.0)) * 0.5; }
+
+/* Lambert2DSquareInverseSingle (Lambert.f90:405-433) in float32, times scl (the caller's
+ * ixy = scl * LambertSphereToSquare(dc)) */
+int orc_lambert_inverse_single(const float xyz[3], float scl, float xy[2]) {
+  const float spi2 = (float)lps_spi2();
+  float r0, r1;
+  int ierr = 0;
+  const float ss = xyz[0] * xyz[0] + xyz[1] * xyz[1] + xyz[2] * xyz[2];
+  if (fabsf(1.0f - ss) > 1.0e-6f) {
+    r0 = r1 = 0.0f;
+    ierr = 1;
+  } else if (fabsf(xyz[2]) == 1.0f) {
+    r0 = r1 = 0.0f;
+  } else if (fabsf(xyz[1]) <= fabsf(xyz[0])) {
+    const float q = fabsf(xyz[0]) / xyz[0] * sqrtf(2.0f * (1.0f - xyz[2]));
+    r0 = q * spi2;
+    r1 = q * atanf(xyz[1] / xyz[0]) / spi2;
+  } else {
+    const float q = fabsf(xyz[1]) / xyz[1] * sqrtf(2.0f * (1.0f - xyz[2]));
+    r0 = q * atanf(xyz[0] / xyz[1]) / spi2;
+    r1 = q * spi2;
+  }
+  xy[0] = scl * r0;
+  xy[1] = scl * r1;
+  return ierr;
+}
+
+/* element (k, ix, iy) of a Fortran array a(nk, -nx:nx, -ny:ny) at its column-major offset
+ * without a bounds check, as the compiled program reads it (an index one past a row reads the
+ * next row); offsets outside the array give 0 */
+static float at3_i(const int32_t* a, int nk, int nx, int ny, int k, int ix, int iy) {
+  const int64_t o = (int64_t)k + (int64_t)nk * ((ix + nx) + (int64_t)(2 * nx + 1) * (iy + ny));
+  const int64_t tot = (int64_t)nk * (2 * nx + 1) * (2 * ny + 1);
+  return (o >= 0 && o < tot) ? (float)a[o] : 0.0f;
+}
+static float at3_f(const float* a, int nx, int ny, int nk, int ix, int iy, int k) {
+  const int64_t o = (int64_t)(ix + nx) + (int64_t)(2 * nx + 1) * ((iy + ny) + (int64_t)(2 * ny + 1) * k);
+  const int64_t tot = (int64_t)nk * (2 * nx + 1) * (2 * ny + 1);
+  return (o >= 0 && o < tot) ? a[o] : 0.0f;
+}
+
+int orc_ebsd_detector(float L, float thetac, float delta, int numsx, int numsy, float xpc, float ypc,
+                      float sig, int numEbins, int nsx, int nsy, const int32_t* accum_e, int Emin,
+                      int Emax, float* rg, float* wdet, int64_t* num_el) {
+  const float dtor = 0.0174533f; /* CTEMEBSD.f90:148 */
+  const int64_t np = (int64_t)numsx * numsy;
+  float* sx_ = (float*)malloc(sizeof(float) * numsx);
+  float* sy_ = (float*)malloc(sizeof(float) * numsy);
+  if (!sx_ || !sy_) return -1;
+  /* :414-415 */
+  for (int i = 0; i < numsx; i++) sx_[i] = -((xpc - (1.0f - (float)numsx) * 0.5f) - (float)i) * delta;
+  for (int j = 0; j < numsy; j++) sy_[j] = ((ypc - (1.0f - (float)numsy) * 0.5f) - (float)j) * delta;
+  /* :421-423 */
+  const float alp = (float)(0.5 * CPI - (double)((sig - thetac) * dtor));
+  const float ca = cosf(alp), sa = sinf(alp);
+  /* :431-442 */
+  const float L2 = L * L;
+  float* rgx = rg;
+  float* rgy = rg + np;
+  float* rgz = rg + 2 * np;
+  for (int j = 0; j < numsy; j++) {
+    const float sx = L2 + sy_[j] * sy_[j];
+    const float Ls = ca * sy_[j] + L * sa;
+    const float Lc = -sa * sy_[j] + L * ca;
+    for (int i = 0; i < numsx; i++) {
+      const float rhos = 1.0f / sqrtf(sx + sx_[i] * sx_[i]);
+      const int64_t p = (int64_t)j * numsx + i;
+      rgx[p] = Ls * rhos;
+      rgy[p] = sx_[i] * rhos;
+      rgz[p] = Lc * rhos;
+    }
+  }
+  /* :445-450 */
+  for (int64_t p = 0; p < np; p++) {
+    const float z = sqrtf(rgx[p] * rgx[p] + rgy[p] * rgy[p] + rgz[p] * rgz[p]);
+    rgx[p] = rgx[p] / z;
+    rgy[p] = rgy[p] / z;
+    rgz[p] = rgz[p] / z;
+  }
+  /* :469-502 */
+  const float scl = (float)((double)(float)nsx / lps_spio2());
+  memset(wdet, 0, sizeof(float) * (size_t)np * numEbins);
+  for (int j = 0; j < numsy; j++)
+    for (int i = 0; i < numsx; i++) {
+      const int64_t p = (int64_t)j * numsx + i;
+      float dc[3] = {rgx[p], rgy[p], rgz[p]};
+      if (dc[2] < 0.0f) { dc[0] = -dc[0]; dc[1] = -dc[1]; dc[2] = -dc[2]; }
+      float ixy[2];
+      orc_lambert_inverse_single(dc, scl, ixy);
+      const float x = ixy[0];
+      ixy[0] = ixy[1];
+      ixy[1] = -x;
+      const int nix = (int)((float)nsx + ixy[0]) - nsx;
+      const int niy = (int)((float)nsy + ixy[1]) - nsy;
+      const float dx = ixy[0] - (float)nix, dy = ixy[1] - (float)niy;
+      const float dxm = 1.0f - dx, dym = 1.0f - dy;
+      for (int k = Emin - 1; k < Emax; k++)
+        wdet[p * numEbins + k] = at3_i(accum_e, numEbins, nsx, nsy, k, nix, niy) * dxm * dym +
+                                 at3_i(accum_e, numEbins, nsx, nsy, k, nix + 1, niy) * dx * dym +
+                                 at3_i(accum_e, numEbins, nsx, nsy, k, nix, niy + 1) * dxm * dy +
+                                 at3_i(accum_e, numEbins, nsx, nsy, k, nix + 1, niy + 1) * dx * dy;
+    }
+  /* :503, :508: num_el = nint(sum(accum_e_detector)) over the whole array (every bin), as the
+   * Fortran runtime's SUM of a REAL(4) array evaluates it: compensated (Kahan) summation in single
+   * precision, in array element order; nint rounds half away from zero */
+  for (int64_t e = 0; e < np * numEbins; e++) wdet[e] = wdet[e] * 0.25f;
+  float tot = 0.0f, comp = 0.0f;
+  for (int64_t e = 0; e < np * numEbins; e++) {
+    const float y = wdet[e] - comp;
+    const float t = tot + y;
+    comp = (t - tot) - y;
+    tot = t;
+  }
+  *num_el = (int64_t)lroundf(tot);
+  free(sx_);
+  free(sy_);
+  return 0;
+}
+
+/* SUM of a REAL(4) array as the compiled program evaluates it (sum(dc**2), CTEMEBSD.f90:540):
+ * the Fortran runtime's compensated (Kahan) summation in single precision, in element order */
+static float sum3_kahan(float a, float b, float c) {
+  const float v[3] = {a, b, c};
+  float s = 0.0f, comp = 0.0f;
+  for (int i = 0; i < 3; i++) {
+    const float y = v[i] - comp;
+    const float t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  }
+  return s;
+}
+
+int orc_ebsd_pattern(int numsx, int numsy, const float* rg, const float* wdet, int numEbins, int Emin,
+                     int Emax, int npx, int npy, const float* sr, const float q[4], double prefactor,
+                     float* out) {
+  const int64_t np = (int64_t)numsx * numsy;
+  const float scl = (float)((double)(float)npx / lps_spio2());
+  const float q1[4] = {q[0], -q[1], -q[2], -q[3]}; /* conjg (quaternions.f90:306-313) */
+  for (int j = 0; j < numsy; j++)
+    for (int i = 0; i < numsx; i++) {
+      const int64_t p = (int64_t)j * numsx + i;
+      const float qq[4] = {0.0f, rg[p], rg[np + p], rg[2 * np + p]};
+      float t[4], r[4];
+      /* quat_mult (quaternions.f90:255-265): t = qq * conjg(q), r = q * t */
+      t[0] = qq[0] * q1[0] - qq[1] * q1[1] - qq[2] * q1[2] - qq[3] * q1[3];
+      t[1] = qq[0] * q1[1] + qq[1] * q1[0] + qq[2] * q1[3] - qq[3] * q1[2];
+      t[2] = qq[0] * q1[2] - qq[1] * q1[3] + qq[2] * q1[0] + qq[3] * q1[1];
+      t[3] = qq[0] * q1[3] + qq[1] * q1[2] - qq[2] * q1[1] + qq[3] * q1[0];
+      r[0] = q[0] * t[0] - q[1] * t[1] - q[2] * t[2] - q[3] * t[3];
+      r[1] = q[0] * t[1] + q[1] * t[0] + q[2] * t[3] - q[3] * t[2];
+      r[2] = q[0] * t[2] - q[1] * t[3] + q[2] * t[0] + q[3] * t[1];
+      r[3] = q[0] * t[3] + q[1] * t[2] - q[2] * t[1] + q[3] * t[0];
+      (void)r[0];
+      float dc[3] = {r[1], r[2], r[3]};
+      const float nrm = sqrtf(sum3_kahan(dc[0] * dc[0], dc[1] * dc[1], dc[2] * dc[2]));
+      dc[0] = dc[0] / nrm;
+      dc[1] = dc[1] / nrm;
+      dc[2] = dc[2] / nrm;
+      if (dc[2] < 0.0f) { dc[0] = -dc[0]; dc[1] = -dc[1]; dc[2] = -dc[2]; }
+      float ixy[2];
+      orc_lambert_inverse_single(dc, scl, ixy);
+      const int nix = (int)((float)npx + ixy[0]) - npx;
+      const int niy = (int)((float)npy + ixy[1]) - npy;
+      const float dx = ixy[0] - (float)nix, dy = ixy[1] - (float)niy;
+      const float dxm = 1.0f - dx, dym = 1.0f - dy;
+      float acc = 0.0f;
+      for (int k = Emin - 1; k < Emax; k++)
+        acc = acc + wdet[p * numEbins + k] * (at3_f(sr, npx, npy, numEbins, nix, niy, k) * dxm * dym +
+                                              at3_f(sr, npx, npy, numEbins, nix + 1, niy, k) * dx * dym +
+                                              at3_f(sr, npx, npy, numEbins, nix, niy + 1, k) * dxm * dy +
+                                              at3_f(sr, npx, npy, numEbins, nix + 1, niy + 1, k) * dx * dy);
+      out[p] = (float)(prefactor * (double)acc);
+    }
+  return 0;
+}
+
+/* noise.f90: ran (:136-165) and POIDEV (:167-228) with their saved variables */
+struct orc_noise {
+  int32_t idum, ix, iy, started;
+  float am, oldm, g, sq, alxm;
+};
+static float orc_ran(struct orc_noise* s) {
+  const int32_t IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836;
+  if (!s->started) { s->ix = -1; s->iy = -1; s->started = 1; }
+  if (s->idum <= 0 || s->iy < 0) {
+    s->am = nextafterf(1.0f, 0.0f) / (float)IM;
+    const int32_t a = s->idum < 0 ? -s->idum : s->idum;
+    s->iy = (888889999 ^ a) | 1;
+    s->ix = 777755555 ^ a;
+    s->idum = a + 1;
+  }
+  uint32_t x = (uint32_t)s->ix;
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  s->ix = (int32_t)x;
+  const int32_t k = s->iy / IQ;
+  s->iy = IA * (s->iy - k * IQ) - IR * k;
+  if (s->iy < 0) s->iy += IM;
+  return s->am * (float)((IM & (s->ix ^ s->iy)) | 1);
+}
+/* alogam (noise.f90:47-133; Pike and Hill, CACM algorithm 291) */
+static double orc_alogam(double x) {
+  if (x <= 0.0) return 0.0;
+  double y = x, f, z;
+  if (x < 7.0) {
+    f = 1.0;
+    z = y;
+    while (z < 7.0) {
+      f = f * z;
+      z = z + 1.0;
+    }
+    y = z;
+    f = -log(f);
+  } else {
+    f = 0.0;
+  }
+  z = 1.0 / y / y;
+  return f + (y - 0.5) * log(y) - y + 0.918938533204673 +
+         (((-0.000595238095238 * z + 0.000793650793651) * z - 0.002777777777778) * z + 0.083333333333333) / y;
+}
+static float orc_poidev(float xm, struct orc_noise* s) {
+  const float PI = 3.141592654f;
+  float em, t, y;
+  if (xm < 20.0f) {
+    if (xm != s->oldm) {
+      s->oldm = xm;
+      s->g = expf(-xm);
+    }
+    em = -1.0f;
+    t = 1.0f;
+    for (;;) {
+      em = em + 1.0f;
+      t = t * orc_ran(s);
+      if (t <= s->g) break;
+    }
+  } else {
+    if (xm != s->oldm) {
+      s->oldm = xm;
+      s->sq = sqrtf(2.0f * xm);
+      s->alxm = logf(xm);
+      s->g = (float)((double)(xm * s->alxm) - orc_alogam((double)(xm + 1.0f)));
+    }
+    for (;;) {
+      for (;;) {
+        y = tanf(PI * orc_ran(s));
+        em = s->sq * y + xm;
+        if (em >= 0.0f) break;
+      }
+      em = (float)(int32_t)em;
+      t = (float)((double)(0.9f * (1.0f + y * y)) *
+                  exp((double)(em * s->alxm) - orc_alogam((double)(em + 1.0f)) - (double)s->g));
+      if (orc_ran(s) <= t) break;
+    }
+  }
+  return em;
+}
+void orc_poisson_noise(float* pattern, int numsx, int numsy, int64_t* st) {
+  struct orc_noise s;
+  if (st[1] == 0) { /* first use: the saved variables' initial values */
+    memset(&s, 0, sizeof(s));
+    s.idum = (int32_t)st[0];
+    s.oldm = -1.0f;
+  } else {
+    memcpy(&s, &st[2], sizeof(s));
+    s.idum = (int32_t)st[0];
+  }
+  for (int i = 0; i < numsx; i++)
+    for (int j = 0; j < numsy; j++) {
+      float* v = &pattern[(int64_t)j * numsx + i];
+      *v = orc_poidev(*v, &s);
+    }
+  st[0] = s.idum;
+  st[1] = 1;
+  memcpy(&st[2], &s, sizeof(s));
+}
+
+/* eu2qu (rotations.f90:1322-1351), float32; e in radians */
+void orc_eu2qu(const float e[3], float q[4]) {
+  const float ee[3] = {0.5f * e[0], 0.5f * e[1], 0.5f * e[2]};
+  const float cPhi = cosf(ee[1]), sPhi = sinf(ee[1]);
+  const float cm = cosf(ee[0] - ee[2]), sm = sinf(ee[0] - ee[2]);
+  const float cp = cosf(ee[0] + ee[2]), sp = sinf(ee[0] + ee[2]);
+  const float qq = cPhi * cp;
+  if (qq < 0.0f) {
+    q[0] = -qq; q[1] = sPhi * cm; q[2] = sPhi * sm; q[3] = cPhi * sp;
+  } else {
+    q[0] = qq; q[1] = -sPhi * cm; q[2] = -sPhi * sm; q[3] = -cPhi * sp;
+  }
+}
+
+/* axisangle rotation of the program (CTEMEBSD.f90:264-270): qax = ax2qu(a) with a(4) in degrees
+ * times dtor (ax2om, om2qu: rotations.f90; quat_norm's sum in element order, quaternions.f90),
+ * then q = quat_mult(qax, q) for each of the n quaternions */
+void orc_apply_axisangle(int n, const float ax[4], float* quats) {
+  if (ax[3] == 0.0f) return;
+  const float a3 = ax[3] * 0.0174533f;
+  const float c = cosf(a3), s = sinf(a3), omc = 1.0f - c;
+  float m[3][3];
+  for (int i = 0; i < 3; i++) m[i][i] = ax[i] * ax[i] * omc + c;
+  float q = omc * ax[0] * ax[1];
+  m[0][1] = q + s * ax[2];
+  m[1][0] = q - s * ax[2];
+  q = omc * ax[1] * ax[2];
+  m[1][2] = q + s * ax[0];
+  m[2][1] = q - s * ax[0];
+  q = omc * ax[2] * ax[0];
+  m[2][0] = q + s * ax[1];
+  m[0][2] = q - s * ax[1];
+  float r[4];
+  r[0] = sqrtf(m[0][0] + m[1][1] + m[2][2] + 1.0f) / 2.0f;
+  r[1] = sqrtf(fabsf(m[0][0] - m[1][1] - m[2][2] + 1.0f)) / 2.0f;
+  r[2] = sqrtf(fabsf(-m[0][0] + m[1][1] - m[2][2] + 1.0f)) / 2.0f;
+  r[3] = sqrtf(fabsf(-m[0][0] - m[1][1] + m[2][2] + 1.0f)) / 2.0f;
+  if (m[2][1] - m[1][2] < 0.0f) r[1] = -r[1];
+  if (m[0][2] - m[2][0] < 0.0f) r[2] = -r[2];
+  if (m[1][0] - m[0][1] < 0.0f) r[3] = -r[3];
+  const float sc = 1.0f / sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+  for (int i = 0; i < 4; i++) r[i] = sc * r[i];
+  for (int k = 0; k < n; k++) {
+    float* x = quats + 4 * k;
+    const float t0 = r[0] * x[0] - r[1] * x[1] - r[2] * x[2] - r[3] * x[3];
+    const float t1 = r[0] * x[1] + r[1] * x[0] + r[2] * x[3] - r[3] * x[2];
+    const float t2 = r[0] * x[2] - r[1] * x[3] + r[2] * x[0] + r[3] * x[1];
+    const float t3 = r[0] * x[3] + r[1] * x[2] - r[2] * x[1] + r[3] * x[0];
+    x[0] = t0; x[1] = t1; x[2] = t2; x[3] = t3;
+  }
+}
+
+/* Kahan sum in element order (the Fortran runtime's SUM of REAL(4) values) */
+static float kahan_add(float* s, float* c, float x) {
+  const float y = x - *c;
+  const float t = *s + y;
+  *c = (t - *s) - y;
+  *s = t;
+  return t;
+}
+
+/* sr = sum(srtmp, 4) (CTEMEBSD.f90:404): per element over the nset sites */
+void orc_site_sum(int64_t per, int nset, const float* srtmp, float* sr) {
+  for (int64_t e = 0; e < per; e++) {
+    float s = 0.0f, c = 0.0f;
+    for (int q = 0; q < nset; q++) kahan_add(&s, &c, srtmp[e + per * q]);
+    sr[e] = s;
+  }
+}
+
+/* binning and gamma (CTEMEBSD.f90:575-590); binning divides numsx and numsy */
+void orc_bin_pattern(int numsx, int numsy, int binning, int gamma, float gammavalue, const float* pat,
+                     float* binned) {
+  const int bx = numsx / binning, by = numsy / binning;
+  const float bindx = 1.0f / ((float)binning * (float)binning);
+  for (int jb = 0; jb < by; jb++)
+    for (int ib = 0; ib < bx; ib++) {
+      float s = 0.0f, c = 0.0f;
+      for (int j = jb * binning; j < (jb + 1) * binning; j++)
+        for (int i = ib * binning; i < (ib + 1) * binning; i++) kahan_add(&s, &c, pat[(int64_t)j * numsx + i]);
+      binned[(int64_t)jb * bx + ib] = binning == 1 ? pat[(int64_t)jb * numsx + ib] : s * bindx;
+    }
+  if (gamma)
+    for (int64_t e = 0; e < (int64_t)bx * by; e++) binned[e] = powf(binned[e], gammavalue);
+}

@@ -58,6 +58,50 @@ int orc_kvectors(int npx, int npy, int isym, int usehex, const double dsm[9], do
 /* Laue-equivalent Lambert pixels: Apply2DLaueSymmetry, Lambert.f90:1942-2046 */
 int orc_laue_equiv(int ipx, int ipy, int isym, int32_t* iequiv /* 2*12 */);
 
+/* ---- EBSD detector patterns (CTEMEBSD.f90) ---------------------------------------------- */
+
+/* Float32 Lambert square inverse, Lambert2DSquareInverseSingle (Lambert.f90:405-433), times
+ * the grid scale scl: xy[0], xy[1]; returns ierr (1 when |1 - |xyz|^2| > 1e-6: xy = 0) */
+int orc_lambert_inverse_single(const float xyz[3], float scl, float xy[2]);
+
+/* Detector set-up, CTEMEBSD.f90:414-518: pixel direction cosines rg[3][numsy][numsx]
+ * (x fastest; the reference's rgx/rgy/rgz(numsx,numsy)), the MC energy histograms
+ * accum_e(numEbins, -nsx:nsx, -nsy:nsy) (int32, Fortran order) interpolated to the pixels and
+ * scaled by 0.25 for bins Emin..Emax (1-based, inclusive; the other bins 0), as
+ * wdet[numsy][numsx][numEbins]; num_el = nint(sum(wdet)). sig is the MC file's sample tilt
+ * (float32 as the program keeps it), thetac the detector tilt [deg], L and delta [micron],
+ * xpc/ypc the pattern centre [pixels]. */
+int orc_ebsd_detector(float L, float thetac, float delta, int numsx, int numsy, float xpc, float ypc,
+                      float sig, int numEbins, int nsx, int nsy, const int32_t* accum_e, int Emin,
+                      int Emax, float* rg, float* wdet, int64_t* num_el);
+
+/* One noise-free pattern, CTEMEBSD.f90:525-556: the rotated pixel directions (quaternion q,
+ * float32 quat_mult, quaternions.f90:255-265), the Lambert gather of the site-summed master
+ * sr(-npx:npx, -npy:npy, nE) over bins Emin..Emax weighted by wdet, times prefactor (double,
+ * stored float32). out[numsy][numsx] (x fastest). */
+int orc_ebsd_pattern(int numsx, int numsy, const float* rg, const float* wdet, int numEbins, int Emin,
+                     int Emax, int npx, int npy, const float* sr, const float q[4], double prefactor,
+                     float* out);
+
+/* Poisson deviates, POIDEV with the generator ran (noise.f90:136-228, Numerical Recipes), with
+ * the routines' saved variables in st[] (zero-initialise, then st[0] = idum, e.g. -1). Applied
+ * in the program's order: i (x) outer, j (y) inner. */
+void orc_poisson_noise(float* pattern, int numsx, int numsy, int64_t* st);
+
+/* the program's extra rotation: q = quat_mult(ax2qu(axisangle), q), axisangle[3] in degrees
+ * (CTEMEBSD.f90:264-270; rotations.f90 ax2om/om2qu) */
+void orc_apply_axisangle(int n, const float axisangle[4], float* quats);
+
+/* sr = sum(srtmp, 4): per element over nset sites (CTEMEBSD.f90:404), the runtime's SUM */
+void orc_site_sum(int64_t per, int nset, const float* srtmp, float* sr);
+
+/* binning (binning divides numsx, numsy) and gamma, CTEMEBSD.f90:575-590 */
+void orc_bin_pattern(int numsx, int numsy, int binning, int gamma, float gammavalue, const float* pat,
+                     float* binned);
+
+/* Euler angles (radians, Bunge) to a quaternion, eu2qu (rotations.f90:1322-1351), float32 */
+void orc_eu2qu(const float e[3], float q[4]);
+
 #ifdef __cplusplus
 }
 #endif

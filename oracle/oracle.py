@@ -35,6 +35,14 @@ def lib():
         L.orc_kvectors.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_double, C.c_int,
                                    P, P, P]
         L.orc_laue_equiv.argtypes = [C.c_int, C.c_int, C.c_int, P]
+        F, I = C.c_float, C.c_int
+        L.orc_ebsd_detector.argtypes = [F, F, F, I, I, F, F, F, I, I, I, P, I, I, P, P, P]
+        L.orc_ebsd_pattern.argtypes = [I, I, P, P, I, I, I, I, I, P, P, C.c_double, P]
+        L.orc_poisson_noise.argtypes = [P, I, I, P]
+        L.orc_eu2qu.argtypes = [P, P]
+        L.orc_apply_axisangle.argtypes = [I, P, P]
+        L.orc_site_sum.argtypes = [C.c_int64, I, P, P]
+        L.orc_bin_pattern.argtypes = [I, I, I, I, F, P, P]
         _LIB = L
     return _LIB
 
@@ -159,3 +167,80 @@ def master_direction(cr: Crystal, kvec, kn, lam, dz, thick):
     L = lgh(dm, kn, dz, thick, lam)
     dots = np.array([lgh_dot_sgh(L, sgh(cr, st, p)) for p in range(cr.nsite)])
     return dots, st, wk
+
+
+# ---- EBSD detector patterns (CTEMEBSD.f90) ----------------------------------------------------
+
+def ebsd_detector(geom: dict, accum_e, ehistmin: float, ebinsize: float, mcsig: float):
+    """The program's detector set-up: (rg [3][ny][nx], wdet [ny][nx][nE], Emin, Emax, num_el,
+    prefactor). geom: &EBSDdata entries; accum_e [2nsy+1][2nsx+1][nE] int32."""
+    acc = np.ascontiguousarray(accum_e, np.int32)
+    ny2, nx2, ne = acc.shape
+    f32 = np.float32
+    emin = int(np.floor(abs((float(f32(geom["energymin"])) - ehistmin) / ebinsize) + 0.5)
+               * np.sign((float(f32(geom["energymin"])) - ehistmin) / ebinsize)) + 1
+    emax = int(np.floor(abs((float(f32(geom["energymax"])) - ehistmin) / ebinsize) + 0.5)
+               * np.sign((float(f32(geom["energymax"])) - ehistmin) / ebinsize)) + 1
+    emin, emax = min(max(emin, 1), ne), min(max(emax, 1), ne)
+    nx, ny = int(geom["numsx"]), int(geom["numsy"])
+    rg = np.zeros(3 * nx * ny, np.float32)
+    wdet = np.zeros(nx * ny * ne, np.float32)
+    nel = C.c_int64()
+    lib().orc_ebsd_detector(geom["L"], geom["thetac"], geom["delta"], nx, ny, geom["xpc"], geom["ypc"],
+                            float(f32(mcsig)), ne, (nx2 - 1) // 2, (ny2 - 1) // 2, _p(acc), emin, emax, _p(rg),
+                            _p(wdet), C.byref(nel))
+    pref = 0.25 * 6.241e18 * float(geom["beamcurrent"]) * float(geom["dwelltime"]) * 1.0e-15 / float(nel.value)
+    return rg, wdet, emin, emax, int(nel.value), pref
+
+
+def ebsd_pattern(geom: dict, det, sr, quat) -> np.ndarray:
+    """One noise-free pattern [ny][nx] from the site-summed master sr [nE][2npy+1][2npx+1]."""
+    rg, wdet, emin, emax, _, pref = det
+    s = np.ascontiguousarray(sr, np.float32)
+    ne, ny2, nx2 = s.shape
+    nx, ny = int(geom["numsx"]), int(geom["numsy"])
+    q = np.ascontiguousarray(quat, np.float32)
+    out = np.zeros(nx * ny, np.float32)
+    lib().orc_ebsd_pattern(nx, ny, _p(rg), _p(wdet), ne, emin, emax, (nx2 - 1) // 2, (ny2 - 1) // 2, _p(s), _p(q),
+                           pref, _p(out))
+    return out.reshape(ny, nx)
+
+
+def poisson_noise(pattern, state) -> np.ndarray:
+    p = np.array(pattern, np.float32)
+    ny, nx = p.shape
+    lib().orc_poisson_noise(_p(p), nx, ny, _p(state))
+    return p
+
+
+def eu2qu_deg(eul) -> np.ndarray:
+    e = (np.asarray(eul, np.float32).reshape(-1, 3) * np.float32(0.0174533)).astype(np.float32)
+    q = np.zeros((len(e), 4), np.float32)
+    for i in range(len(e)):
+        ei = np.ascontiguousarray(e[i])
+        qi = np.zeros(4, np.float32)
+        lib().orc_eu2qu(_p(ei), _p(qi))
+        q[i] = qi
+    return q
+
+
+def apply_axisangle(quats, ax) -> np.ndarray:
+    q = np.array(quats, np.float32).reshape(-1, 4)
+    a = np.ascontiguousarray(ax, np.float32)
+    lib().orc_apply_axisangle(len(q), _p(a), _p(q))
+    return q
+
+
+def site_sum(srtmp) -> np.ndarray:
+    s = np.ascontiguousarray(srtmp, np.float32)
+    out = np.zeros(s.shape[1:], np.float32)
+    lib().orc_site_sum(out.size, s.shape[0], _p(s), _p(out))
+    return out
+
+
+def bin_pattern(pattern, binning: int, gamma: bool, gammavalue: float) -> np.ndarray:
+    p = np.ascontiguousarray(pattern, np.float32)
+    ny, nx = p.shape
+    out = np.zeros((ny // binning, nx // binning), np.float32)
+    lib().orc_bin_pattern(nx, ny, binning, int(gamma), float(gammavalue), _p(p), _p(out))
+    return out

@@ -281,3 +281,54 @@ def time_reference(work: str, wl, lam: np.ndarray, ksel, ekev: float | None = No
     run_driver(work, wl, lam, [int(ksel[0])], 2, ekev)
     t2 = time.perf_counter()
     return (t1 - t0) - (t2 - t1)
+
+
+# ----------------------------------------------------------------------------------------
+# The detector-pattern program (CTEMEBSD.f90), built as oracle/_ref/CTEMEBSD and, with the
+# noise module's POIDEV replaced by the identity, oracle/_ref/CTEMEBSD_clean.
+
+def write_mc_energy_file(path: str, accum_e, ehistmin: float, ebinsize: float, mcsig: float,
+                         ekev: float = 20.0, xtalname: str = "crystal.xtal") -> None:
+    """MC file (CTEMMC.f90:264-276) with energy histograms accum_e [2nsy+1][2nsx+1][numEbins]
+    (C order == Fortran accum_e(numEbins, -nsx:nsx, -nsy:nsy)) and an empty depth array."""
+    acc = np.ascontiguousarray(accum_e, np.int32)
+    ny2, nx2, ne = acc.shape
+    nzb = 2
+    recs = [
+        _rec(_chars("CTEMMC.f90", 132)),
+        _rec(_chars("2.0/2013", 8)),
+        _rec(_chars(xtalname, 132)),
+        _rec(struct.pack("<4iqi", ne, nzb, nx2, ny2, int(acc.astype(np.int64).sum()), 1)),
+        _rec(struct.pack("<5d", ekev, ehistmin, ebinsize, 100.0, 1.0)),
+        _rec(struct.pack("<2d", mcsig, 0.0)),
+        _rec(_chars("full", 4)),
+        _rec(acc.tobytes()),
+        _rec(np.zeros((ne, nzb, nx2 // 10 + 1, ny2 // 10 + 1), np.int32).tobytes()),
+    ]
+    with open(path, "wb") as f:
+        f.write(b"".join(recs))
+
+
+def run_ebsd_program(work: str, nml: dict, clean: bool, timeout: int = 300) -> np.ndarray:
+    """Run CTEMEBSD (or CTEMEBSD_clean) in `work` on &EBSDdata `nml` (the angle, MC and master
+    files already there); returns the output file's patterns float32 [n][ny][nx]."""
+    def fmt(v):
+        if isinstance(v, str):
+            return f"'{v}'"
+        if isinstance(v, (list, tuple)):
+            return ", ".join(repr(float(x)) for x in v)
+        return repr(v)
+    with open(os.path.join(work, "CTEMEBSD.nml"), "w") as f:
+        f.write(" &EBSDdata\n" + "".join(f" {k} = {fmt(v)},\n" for k, v in nml.items()) + " /\n")
+    exe = os.path.join(REF_BIN, "CTEMEBSD_clean" if clean else "CTEMEBSD")
+    subprocess.run(["timeout", "-k", "5", str(timeout), exe, "CTEMEBSD.nml"], cwd=work, check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    with open(os.path.join(work, nml.get("datafile", "EBSDout.data")), "rb") as f:
+        data = f.read()
+    recs, off = [], 0
+    while off < len(data):
+        (n,) = struct.unpack_from("<i", data, off)
+        recs.append(data[off + 4: off + 4 + n])
+        off += 8 + n
+    nx, ny, num = struct.unpack("<3i", recs[0])
+    return np.stack([np.frombuffer(r, np.float32).copy() for r in recs[1:1 + num]]), (nx, ny, num)

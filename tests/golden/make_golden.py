@@ -159,6 +159,76 @@ def golden_sym() -> str:
     return fn
 
 
+def golden_ebsd() -> str:
+    """Detector patterns from the reference's CTEMEBSD program (its output file) and from the same
+    program with POIDEV replaced by the identity (the noise-free patterns), for two cases on a
+    64 x 48 detector: (a) Euler angles, raw output; (b) quaternions with the optional axis-angle
+    rotation, 2 x 2 binning and gamma, a three-site master (the program's site sum). The master is
+    the reference program's fcc-Ni master (ni_c0) extended to three energy bins with seeded
+    perturbations; the MC histograms are a seeded smooth field. The energy window covers every
+    bin: with a narrower window the program sums unassigned memory into num_el
+    (CTEMEBSD.f90:470-508) and its output is undefined."""
+    from oracle.refrun import run_ebsd_program, write_mc_energy_file
+    from ctemsoft_amd.masterfile import write_master
+    ni = np.load(os.path.join(HERE, "ni_c0.npz"))
+    sr0 = np.asarray(ni["master_sr"], np.float32)  # [1][1][2npy+1][2npx+1]
+    rng = np.random.default_rng(20260312)
+    nE, nsx = 3, 20
+    out = {}
+    for case in ("a", "b"):
+        nset = 1 if case == "a" else 3
+        sr = np.concatenate([sr0 * np.float32(0.8 + 0.2 * e) for e in range(nE)], axis=1)
+        sr = np.concatenate([sr * np.float32(1.0 + 0.37 * q) for q in range(nset)], axis=0)
+        sr = (sr + rng.random(sr.shape, dtype=np.float32) * np.float32(0.01)).astype(np.float32)
+        xs = np.arange(-nsx, nsx + 1)
+        X, Y = np.meshgrid(xs, xs, indexing="xy")  # [y][x]
+        base = np.exp(-((X - 3) ** 2 + (Y + 2) ** 2) / 300.0)
+        acc = np.stack([np.round(base * (4000 * (e + 1)) + rng.integers(0, 50, base.shape)) for e in range(nE)],
+                       axis=-1).astype(np.int32)  # [y][x][E]
+        if case == "a":
+            ang = np.array([[10.0, 20.0, 30.0], [120.5, 45.25, 300.0], [0.0, 0.0, 0.0], [250.0, 80.0, 33.3]], np.float32)
+            nml = dict(L=15000.0, thetac=10.0, delta=50.0, numsx=64, numsy=48, xpc=3.5, ypc=-2.25,
+                       anglefile="angles.txt", eulerconvention="tsl", FZfile="master.data", energyfile="mc.data",
+                       datafile="ebsdout.data", beamcurrent=14.513, dwelltime=100.0, energymin=17.6,
+                       energymax=20.4, binning=1, gammavalue=1.0, scalingmode="not", axisangle=[0.0, 0.0, 0.0, 0.0])
+            kind = "eu"
+        else:
+            q = rng.normal(size=(5, 4)).astype(np.float64)
+            q[:, 0] = np.abs(q[:, 0])
+            ang = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+            nml = dict(L=18000.0, thetac=6.5, delta=40.0, numsx=64, numsy=48, xpc=-1.25, ypc=2.5,
+                       anglefile="angles.txt", eulerconvention="tsl", FZfile="master.data", energyfile="mc.data",
+                       datafile="ebsdout.data", beamcurrent=20.0, dwelltime=50.0, energymin=18.0,
+                       energymax=20.0, binning=2, gammavalue=0.7, scalingmode="gam",
+                       axisangle=[0.0, 0.6, 0.8, 12.5])
+            kind = "qu"
+        res = {}
+        with tempfile.TemporaryDirectory(prefix="ctemdet_") as work:
+            write_mc_energy_file(os.path.join(work, "mc.data"), acc, 18.0, 1.0, 70.0)
+            write_master(os.path.join(work, "master.data"), sr, [18.0, 19.0, 20.0], [28] * nset)
+            with open(os.path.join(work, "angles.txt"), "w") as f:
+                f.write(f"{kind}\n{len(ang)}\n")
+                for r in ang:
+                    f.write(" ".join(repr(float(v)) for v in r) + "\n")
+            res["noisy"], hdr = run_ebsd_program(work, nml, clean=False)
+            res["clean"], _ = run_ebsd_program(work, nml, clean=True)
+            with open(os.path.join(work, "angles.txt")) as f:
+                angles_txt = f.read()
+        out[f"{case}_sr"] = sr
+        out[f"{case}_accum_e"] = acc
+        out[f"{case}_angles"] = ang
+        out[f"{case}_angles_txt"] = np.frombuffer(angles_txt.encode(), np.uint8)
+        out[f"{case}_kind"] = kind
+        out[f"{case}_nml"] = np.frombuffer(repr(nml).encode(), np.uint8)
+        out[f"{case}_header"] = np.array(hdr, np.int32)
+        out[f"{case}_noisy"] = res["noisy"]
+        out[f"{case}_clean"] = res["clean"]
+    out["Ehistmin"], out["Ebinsize"], out["MCsig"], out["ekevs"] = 18.0, 1.0, 70.0, np.array([18.0, 19.0, 20.0], np.float32)
+    fn = os.path.join(HERE, "ebsd_det.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -198,6 +268,8 @@ def main():
             print(fn, [d[f"dyn{j}_lgh"].shape[0] for j in keep])
         elif w == "sym":
             print(golden_sym())
+        elif w == "ebsd":
+            print(golden_ebsd())
         else:
             raise SystemExit(f"unknown workload {w}")
 

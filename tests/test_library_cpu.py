@@ -1,5 +1,5 @@
 """CPU-side checks of the HIP library: it loads, exports exactly the C-ABI that
-include/ctem_ebsd.h declares, reports a clean error without a GPU, and its host-side
+include/*.h declare, reports a clean error without a GPU, and its host-side
 functions (RoscaLambert k-vectors, Laue equivalents) equal the reference's."""
 import ctypes
 import os
@@ -22,11 +22,12 @@ def _all_headers():
 
 
 def test_library_exports_every_declared_symbol():
-    from ctemsoft_amd import ebsd, xtal
+    from ctemsoft_amd import detector, ebsd, xtal
     L = ebsd.lib()
     decl = _declared()
     assert decl == sorted(ebsd.EXPORTS)
     assert _declared(("ctem_setup.h",)) == sorted(xtal.EXPORTS)
+    assert _declared(("ctem_detector.h",)) == sorted(detector.EXPORTS)
     for name in _declared(_all_headers()):
         assert hasattr(L, name), name
 
