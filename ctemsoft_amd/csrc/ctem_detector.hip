@@ -4,22 +4,26 @@
 //
 // Set-up (host, once per detector; CTEMEBSD.f90:299-343,411-518): the pixel direction cosines in
 // float32 as the program forms them, the Monte Carlo exit-energy histograms interpolated to the
-// pixels, num_el and the prefactor. Device layout: the three direction planes [3][npix], the
-// detector histograms of the energy window [nwin][npix] (energy-major, so that the threads of a
-// workgroup read one bin's weights coalesced), the site-summed master [nE][2npy+1][2npx+1] in the
-// master file's order.
+// pixels, num_el and the prefactor. Device layout: the three direction planes [3][npix]; the
+// detector histograms of the energy window as [kw/4][npix] float4 (four consecutive bins of a
+// pixel in one 16-byte load, coalesced across the threads; kw = the window rounded up to 4,
+// zero-padded); the site-summed master energy-innermost, [2npy+1][2npx+1][kp] with the bins from
+// Emin on (kp rounded up to 4), so that the four bins of a corner are one 16-byte load and a
+// corner's whole energy run is contiguous (the master file stores an energy plane after the
+// other; the transposition is done once, at upload).
 //
 // Patterns (k_ebsd_patterns; CTEMEBSD.f90:525-556): one thread per pixel and group of P
 // orientations. Per orientation the thread rotates its direction (quat_mult twice), normalises
 // it, folds it into the northern hemisphere, maps it to the Lambert square and forms the
-// bilinear weights; then it walks the energy window once, reading the pixel's histogram weight
-// for the bin once for all P orientations and gathering the four master values of each. Every
+// bilinear weights; then it walks the energy window four bins at a time, reading the pixel's
+// histogram weights once for all P orientations and the four corners' bins of each. Every
 // float32 step follows the program's order (contraction off; division, square root and atanf
-// bit-equal to the host's, ctem_lambert.h), so the patterns equal the program's bit for bit.
-// Reads past the end of a row of the master wrap into the next row as the compiled program's
-// do; a read outside the whole array (a direction on the equator maps to the square's edge, and
-// the top row's upper neighbour of the last energy bin lies past the array) gives 0, where the
-// program reads whatever memory follows its array.
+// bit-equal to the host's, ctem_lambert.h; bins summed in order), so the patterns equal the
+// program's bit for bit. On the square's right or top edge a corner lies past the end of a row
+// of the master; the compiled program's read then wraps into the next row (or energy plane), and
+// such positions take a slower path that finds that element; a read outside the whole array (a
+// direction on the equator, the top row of the last bin) gives 0, where the program reads
+// whatever memory follows its array.
 //
 // Where the program calls SUM (the direction normalisation, num_el, the site sum of the master,
 // the binning), the compiled program -- built at -O0, where each SUM is a call into the Fortran
@@ -93,8 +97,8 @@ struct ctem_detector {
   double prefactor = 0.0;
   float* d_rg = nullptr;
   float* d_w = nullptr;
-  float* d_sr = nullptr;
-  int32_t npx = -1, npy = -1;
+  float* d_sr = nullptr;  // [2npy+1][2npx+1][kp]
+  int32_t npx = -1, npy = -1, kp = 0;
   float* d_q = nullptr;
   size_t qcap = 0;
   float* d_out = nullptr;
@@ -105,20 +109,21 @@ struct ctem_detector {
 namespace {
 
 template <int P>
-__global__ __launch_bounds__(256) void k_ebsd_patterns(int npix, int nwin, int e0, int npx, int npy, int nE,
-                                                        const float* __restrict__ rg, const float* __restrict__ w,
-                                                        const float* __restrict__ sr, const float* __restrict__ quats,
+__global__ __launch_bounds__(256) void k_ebsd_patterns(int npix, int nwin, int npx, int npy, int nE, int e0, int kp,
+                                                        const float* __restrict__ rg, const float4* __restrict__ w4,
+                                                        const float* __restrict__ srt, const float* __restrict__ quats,
                                                         int n, float scl, float spi2, double prefactor,
                                                         float* __restrict__ out) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int a0 = blockIdx.y * P;
   if (p >= npix) return;
-  const int nxr = 2 * npx + 1;
-  const int64_t plane = (int64_t)nxr * (2 * npy + 1);
-  const int64_t total = plane * nE;
+  const int nxr = 2 * npx + 1, nyr = 2 * npy + 1;
+  const float4* sr4 = (const float4*)srt;
+  const int kp4 = kp >> 2;
   const float qq[4] = {0.0f, rg[p], rg[npix + p], rg[2 * npix + p]};
-  int64_t o00[P];
+  int cell[P];
+  bool fast[P];
   float dx[P], dy[P], dxm[P], dym[P];
   bool live[P];
 #pragma unroll
@@ -148,20 +153,53 @@ __global__ __launch_bounds__(256) void k_ebsd_patterns(int npix, int nwin, int e
     dy[a] = ixy[1] - (float)niy;
     dxm[a] = 1.0f - dx[a];
     dym[a] = 1.0f - dy[a];
-    o00[a] = (int64_t)(nix + npx) + (int64_t)nxr * (niy + npy) + plane * e0;
+    // int() truncates toward 0, so nix >= -npx; all four corners inside the plane unless on
+    // the right or top edge (or a NaN direction)
+    fast[a] = nix >= -npx && nix < npx && niy >= -npy && niy < npy;
+    cell[a] = (niy + npy) * nxr + (nix + npx);
   }
+  // a corner's bin k of the window through the program's column-major offset (row ends wrap)
+  const int64_t plane = (int64_t)nxr * nyr, total = plane * nE;
+  auto slow = [&](int a, int ox, int oy, int k) -> float {
+    const int cx = cell[a] % nxr, cy = cell[a] / nxr;  // cell of corner (nix, niy)
+    const int64_t o = (int64_t)(cx + ox) + (int64_t)nxr * (cy + oy) + plane * (e0 + k);
+    if (o < 0 || o >= total) return 0.0f;
+    const int64_t x = o % nxr, yk = o / nxr, y = yk % nyr, kb = yk / nyr;
+    return srt[(y * nxr + x) * kp + (kb - e0)];
+  };
   float acc[P];
 #pragma unroll
   for (int a = 0; a < P; a++) acc[a] = 0.0f;
-  auto at = [&](int64_t o) { return (o >= 0 && o < total) ? sr[o] : 0.0f; };
-  for (int k = 0; k < nwin; k++) {
-    const float wk = w[(int64_t)k * npix + p];
+  for (int kk = 0; 4 * kk < nwin; kk++) {
+    const float4 w = w4[(int64_t)kk * npix + p];
+    const float wv[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (int a = 0; a < P; a++) {
-      const int64_t o = o00[a] + plane * k;
-      const float v = at(o) * dxm[a] * dym[a] + at(o + 1) * dx[a] * dym[a] + at(o + nxr) * dxm[a] * dy[a] +
-                      at(o + nxr + 1) * dx[a] * dy[a];
-      acc[a] = acc[a] + wk * v;
+      float c[4][4];  // corners (nix,niy), (nix+1,niy), (nix,niy+1), (nix+1,niy+1) x 4 bins
+      if (fast[a]) {
+        const int64_t b = (int64_t)cell[a] * kp4 + kk;
+        const float4 v0 = sr4[b], v1 = sr4[b + kp4], v2 = sr4[b + (int64_t)nxr * kp4], v3 = sr4[b + (int64_t)(nxr + 1) * kp4];
+        c[0][0] = v0.x; c[0][1] = v0.y; c[0][2] = v0.z; c[0][3] = v0.w;
+        c[1][0] = v1.x; c[1][1] = v1.y; c[1][2] = v1.z; c[1][3] = v1.w;
+        c[2][0] = v2.x; c[2][1] = v2.y; c[2][2] = v2.z; c[2][3] = v2.w;
+        c[3][0] = v3.x; c[3][1] = v3.y; c[3][2] = v3.z; c[3][3] = v3.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int k = 4 * kk + e < nwin ? 4 * kk + e : 0;
+          c[0][e] = slow(a, 0, 0, k);
+          c[1][e] = slow(a, 1, 0, k);
+          c[2][e] = slow(a, 0, 1, k);
+          c[3][e] = slow(a, 1, 1, k);
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (4 * kk + e < nwin) {
+          const float v = c[0][e] * dxm[a] * dym[a] + c[1][e] * dx[a] * dym[a] + c[2][e] * dxm[a] * dy[a] +
+                          c[3][e] * dx[a] * dy[a];
+          acc[a] = acc[a] + wv[e] * v;
+        }
     }
   }
 #pragma unroll
@@ -182,8 +220,9 @@ int32_t launch_patterns(ctem_detector* d, int32_t n, const float* dq, float* dou
   const dim3 grid((npix + 255) / 256, (n + kP - 1) / kP);
   if (grid.y > 65535u) return dfail(CTEM_E_RANGE, "ctem_detector_patterns: more than %d orientations per call", 65535 * kP);
   DHIP(hipEventRecord(d->ev0, d->stream));
-  hipLaunchKernelGGL(k_ebsd_patterns<kP>, grid, dim3(256), 0, d->stream, npix, nwin, d->Emin - 1, d->npx, d->npy,
-                     d->numEbins, d->d_rg, d->d_w, d->d_sr, dq, n, scl, lps_spi2_f(), d->prefactor, dout);
+  hipLaunchKernelGGL(k_ebsd_patterns<kP>, grid, dim3(256), 0, d->stream, npix, nwin, d->npx, d->npy, d->numEbins,
+                     d->Emin - 1, d->kp, d->d_rg, (const float4*)d->d_w, d->d_sr, dq, n, scl, lps_spi2_f(),
+                     d->prefactor, dout);
   DHIP(hipGetLastError());
   DHIP(hipEventRecord(d->ev1, d->stream));
   d->timed = true;
@@ -248,8 +287,9 @@ CTEM_API int32_t ctem_detector_create(int32_t device, const ctem_ebsd_geometry* 
   }
   // exit-energy histograms at the pixels (:469-503): Lambert position with the axes swapped,
   // bilinear weights, bins Emin..Emax; accum_e read at its Fortran offset (a row's end wraps)
-  const int nwin = Emax - Emin + 1;
-  std::vector<float> wdet((size_t)nwin * npix);
+  const int nwin = Emax - Emin + 1, kw = (nwin + 3) & ~3;
+  std::vector<float> wdet((size_t)kw * npix, 0.0f);  // [kw/4][npix][4]
+  auto widx = [&](int64_t p, int k) { return ((size_t)(k >> 2) * npix + p) * 4 + (k & 3); };
   const float scl = (float)((double)(float)nsx / lps_spio2());
   const float spi2 = lps_spi2_f();
   const int64_t mtot = (int64_t)nEb * (2 * nsx + 1) * (2 * nsy + 1);
@@ -276,7 +316,7 @@ CTEM_API int32_t ctem_detector_create(int32_t device, const ctem_ebsd_geometry* 
       const float dx = ixy[0] - (float)nix, dy = ixy[1] - (float)niy;
       const float dxm = 1.0f - dx, dym = 1.0f - dy;
       for (int k = Emin - 1; k < Emax; k++)
-        wdet[(size_t)(k - (Emin - 1)) * npix + p] =
+        wdet[widx(p, k - (Emin - 1))] =
             (acc_e(k, nix, niy) * dxm * dym + acc_e(k, nix + 1, niy) * dx * dym + acc_e(k, nix, niy + 1) * dxm * dy +
              acc_e(k, nix + 1, niy + 1) * dx * dy) * 0.25f;
     }
@@ -284,7 +324,7 @@ CTEM_API int32_t ctem_detector_create(int32_t device, const ctem_ebsd_geometry* 
   // outside the window zero) and the prefactor (:508,525)
   KahanF tot;
   for (int64_t p = 0; p < npix; p++)
-    for (int k = 0; k < nEb; k++) tot.add((k >= Emin - 1 && k < Emax) ? wdet[(size_t)(k - (Emin - 1)) * npix + p] : 0.0f);
+    for (int k = 0; k < nEb; k++) tot.add((k >= Emin - 1 && k < Emax) ? wdet[widx(p, k - (Emin - 1))] : 0.0f);
   // (the program's num_el is a default integer: past 2^31 its NINT is undefined; here 64 bits)
   const int64_t num_el = (int64_t)llroundf(tot.s);
   const double nAmpere = 6.241e18;
@@ -333,12 +373,16 @@ CTEM_API int32_t ctem_detector_set_master(ctem_detector* d, int32_t npx, int32_t
   if (nE != d->numEbins)
     return dfail(CTEM_E_ARG, "ctem_detector_set_master: %d energy bins, the histograms have %d", nE, d->numEbins);
   const int64_t plane = (int64_t)(2 * npx + 1) * (2 * npy + 1), per = plane * nE;
-  std::vector<float> sr((size_t)per);
-  for (int64_t e = 0; e < per; e++) {  // sr = sum(srtmp, 4), each element over the sites
-    KahanF s;
-    for (int q = 0; q < numset; q++) s.add(srtmp[e + per * q]);
-    sr[e] = s.s;
-  }
+  // bins Emin..nE (a row's end wraps into later bins only), energy innermost
+  const int e0 = d->Emin - 1, kp = (nE - e0 + 3) & ~3;
+  std::vector<float> sr((size_t)plane * kp, 0.0f);
+  for (int k = e0; k < nE; k++)
+    for (int64_t c = 0; c < plane; c++) {  // sr = sum(srtmp, 4), each element over the sites
+      const int64_t e = c + plane * k;
+      KahanF s;
+      for (int q = 0; q < numset; q++) s.add(srtmp[e + per * q]);
+      sr[(size_t)c * kp + (k - e0)] = s.s;
+    }
   DHIP(hipSetDevice(d->device));
   if (d->d_sr) (void)hipFree(d->d_sr);
   d->d_sr = nullptr;
@@ -347,6 +391,7 @@ CTEM_API int32_t ctem_detector_set_master(ctem_detector* d, int32_t npx, int32_t
   DHIP(hipMemcpy(d->d_sr, sr.data(), sizeof(float) * sr.size(), hipMemcpyHostToDevice));
   d->npx = npx;
   d->npy = npy;
+  d->kp = kp;
   return CTEM_OK;
 }
 
