@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ctem_ebsd.h"
+#include "../../include/ctem_ecp.h"
 #include "ctem_internal.h"
 #include "ctem_laue.h"
 #include "ctem_sg.h"
@@ -95,6 +96,11 @@ struct Crystal {
   // far below the mixed form's storage resolution
   bool centro = false;
   std::vector<int32_t> h_hkl;
+  // host copies of the sites for the structure-matrix tables of later plans
+  std::vector<int32_t> h_site_z, h_norbit;
+  std::vector<float> h_site_b;
+  std::vector<double> h_orbit;
+  double h_rmt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
   ~Crystal() {
     (void)hipSetDevice(device);
@@ -202,6 +208,13 @@ struct ctem_plan {
   int32_t graph_mode = 0;
   hipGraphExec_t graph = nullptr;
   std::vector<uint64_t> graph_key;
+  // electron channeling pattern plan (ctem_ecp_plan_create): mode, thicknesses, the foil normal
+  // of the selection, the channeling structure matrix table, ScatMat's store map and the results
+  int32_t ecp_mode = -1;  // -1: a master-pattern plan
+  int32_t ecp_nt = 0, ecp_numt = 0;
+  float ecp_t0 = 0.0f, ecp_dt = 0.0f, ecp_zint = 0.0f;
+  double ecp_nat = 1.0, ecp_s0 = 1.0;
+  DBuf fn, ecp_slut, ecp_store, ecp_thick, ecp_sr, ecp_part;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -433,6 +446,13 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   cr->sgcutoff = x->sgcutoff;
   cr->nat = (float)nat;
   cr->h_hkl.assign(x->hkl, x->hkl + 3 * n);
+  if (x->nsite > 0) {
+    cr->h_site_z.assign(x->site_z, x->site_z + x->nsite);
+    cr->h_site_b.assign(x->site_b, x->site_b + x->nsite);
+    cr->h_norbit.assign(x->site_norbit, x->site_norbit + x->nsite);
+    cr->h_orbit.assign(x->orbit, x->orbit + 3 * (size_t)nat);
+  }
+  for (int i = 0; i < 9; i++) cr->h_rmt[i] = x->rmt[i];
   c->cr = cr;
   return CTEM_OK;
 }
@@ -597,7 +617,8 @@ extern "C" int32_t ctem_plan_destroy(ctem_plan* p) {
     p->ctx->mgraph_key.clear();
   }
   for (DBuf* b : {&p->k, &p->kn, &p->lam, &p->ij, &p->nns, &p->nnw, &p->soff, &p->woff, &p->sidx,
-                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->dirs})
+                  &p->ssg, &p->widx, &p->wsg, &p->err, &p->dots, &p->keep_lgh, &p->dirs, &p->fn, &p->ecp_slut,
+                  &p->ecp_store, &p->ecp_thick, &p->ecp_sr, &p->ecp_part})
     b->release();
   p->cr.reset();
   for (auto e : p->evpool) (void)hipEventDestroy(e);
@@ -641,7 +662,8 @@ extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw
   SelectOut o{};
   o.nns = p->nns.as<int32_t>(); o.nnw = p->nnw.as<int32_t>(); o.err = p->err.as<int32_t>();
   DevRefl R = p->cr->refl();
-  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), o, 0);
+  const float* fnp = p->fn.p ? p->fn.as<float>() : nullptr;
+  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), fnp, o, 0);
   HIPCHK(hipGetLastError());
   if ((rc = check_err_flag(p->err, s, "beam selection"))) return rc;
   p->h_nns.resize(nk); p->h_nnw.resize(nk);
@@ -660,7 +682,7 @@ extern "C" int32_t ctem_plan_select(ctem_plan* p, int32_t* nns_out, int32_t* nnw
   o.soff = p->soff.as<int64_t>(); o.woff = p->woff.as<int64_t>();
   o.sidx = p->sidx.as<int32_t>(); o.ssg = p->ssg.as<float>();
   o.widx = p->widx.as<int32_t>(); o.wsg = p->wsg.as<float>();
-  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), o, 1);
+  launch_select(s, p->k.as<double>(), nk, R, p->cr->rmt.as<double>(), fnp, o, 1);
   HIPCHK(hipGetLastError());
   // bucket layout: directions grouped by padded size, in list order within a bucket
   {
@@ -1262,6 +1284,7 @@ std::vector<uint64_t> graph_key_of(const ctem_plan* p, int32_t precision) {
 static int32_t run_prepare(ctem_plan* p, int32_t precision, int32_t keep_lgh_ik) {
   if (precision != CTEM_PREC_FP64 && precision != CTEM_PREC_MIXED)
     return fail(CTEM_E_ARG, "unknown precision %d", precision);
+  if (p->ecp_mode >= 0) return fail(CTEM_E_STATE, "a channeling-pattern plan runs with ctem_ecp_run");
   ctem_ctx* c = p->ctx;
   if (!p->cr) p->cr = c->cr;
   if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_run: no crystal uploaded");
@@ -1496,6 +1519,7 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
   if (!p->cr) p->cr = c->cr;
   if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_adjoint: no crystal uploaded");
   if (p->cr->nsite > 0 && !gb) return fail(CTEM_E_ARG, "ctem_plan_adjoint: gb is NULL");
+  if (p->ecp_mode >= 0) return fail(CTEM_E_STATE, "ctem_plan_adjoint: not for a channeling-pattern plan");
   int32_t rc;
   if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
   HIPCHK(hipSetDevice(c->device));
@@ -1870,5 +1894,319 @@ extern "C" int32_t ctem_calclgh3(ctem_ctx* c, const double* dmat, double* lgh, d
       lgh[2 * ((size_t)j * nn + i)] = h[2 * ((size_t)i * nn + j)];
       lgh[2 * ((size_t)j * nn + i) + 1] = h[2 * ((size_t)i * nn + j) + 1];
     }
+  return CTEM_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// electron channeling patterns (include/ctem_ecp.h)
+
+// The channeling structure matrix Sgh(d) for every d = g-h of the crystal's difference box, all
+// sites in one table (CTEMECP.f90:318-350): per site Znsq = float(Z)**2 * occ and
+// DBWF = Znsq * exp(-B kkl) with kkl = 0.25 CalcLength(float(d), 'r')**2, all float32; per
+// orbit position the float32 complex product cmplx(DBWF, 0) * cmplx(cos(arg), sin(arg)) with
+// arg = tpi * sum(d * r) (tpi = fl32(2 pi), the sum in double), added in double site after
+// site. *s0 receives Sgh(000).
+static void build_slut_ecp(const Crystal& cr, const float* occ, std::vector<double>& out, double* s0) {
+#pragma clang fp contract(off)
+  const int32_t* lo = cr.slut_lo;
+  const int32_t* dim = cr.slut_dim;
+  const size_t ncell = (size_t)dim[0] * dim[1] * dim[2];
+  out.assign(2 * ncell, 0.0);
+  const float tpi = (float)(2.0 * kPi);
+  std::vector<float> znsq(cr.nsite);
+  for (int p = 0; p < cr.nsite; p++) {
+    const float zf = (float)cr.h_site_z[p];
+    znsq[p] = (zf * zf) * occ[p];
+  }
+  for (int l = 0; l < dim[2]; l++)
+    for (int kk = 0; kk < dim[1]; kk++)
+      for (int h = 0; h < dim[0]; h++) {
+        const int d[3] = {h + lo[0], kk + lo[1], l + lo[2]};
+        const float df[3] = {(float)d[0], (float)d[1], (float)d[2]};
+        const float len = std::sqrt(dot_r(df, df, cr.h_rmt));
+        const float kkl = 0.25f * (len * len);
+        double sr = 0.0, si = 0.0;
+        size_t obase = 0;
+        for (int p = 0; p < cr.nsite; p++) {
+          const float dbwf = znsq[p] * ::expf(-cr.h_site_b[p] * kkl);
+          const double* orb = cr.h_orbit.data() + 3 * obase;
+          for (int a = 0; a < cr.h_norbit[p]; a++) {
+            double sd = 0.0;
+            for (int t = 0; t < 3; t++) sd += (double)d[t] * orb[3 * a + t];
+            const double arg = (double)tpi * sd;
+            double sn, cs;
+            ::sincos(arg, &sn, &cs);
+            sr += (double)(dbwf * (float)cs);
+            si += (double)(dbwf * (float)sn);
+          }
+          obase += cr.h_norbit[p];
+        }
+        const size_t cell = (size_t)h + (size_t)dim[0] * ((size_t)kk + (size_t)dim[1] * l);
+        out[2 * cell] = sr;
+        out[2 * cell + 1] = si;
+        if (d[0] == 0 && d[1] == 0 && d[2] == 0) *s0 = sr;
+      }
+}
+
+// E = MatrixExponential(A, z0, 'Pade') (math.f90:187-315) for the nb matrices of A (holding
+// A z0, overwritten with the scaled B); W holds 5 nb npad^2 entries of work space, d_q and d_j nb
+// integers each. Returns the slice of W that holds E.
+static double2* pade_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W, int32_t* d_q,
+                          int32_t* d_j, int32_t* d_err, int32_t& rc_out) {
+  rc_out = CTEM_OK;
+  const int64_t per = (int64_t)npad * npad;
+  double2* C0 = W;
+  double2* C1 = W + per * nb;
+  double2* N = W + 2 * per * nb;
+  double2* D = W + 3 * per * nb;
+  double2* T = W + 4 * per * nb;
+  auto prod = [&](double2* C, const double2* X, const double2* Y, const int32_t* active, int step) -> bool {
+    ZGemm g{};
+    g.m = g.n = g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = X; g.sx = per; g.Y = Y; g.sy = per; g.C = C; g.sc = per; g.sm = per;
+    g.alpha = 1.0; g.betaI = 0.0;
+    g.active = active; g.step = step; g.nb = nb;
+    GemmTimer t{tp, s};
+    t.begin();
+    launch_zgemm(s, g);
+    t.end();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rc_out = fail(CTEM_E_HIP, "Pade product: %s", hipGetErrorString(e)); return false; }
+    return true;
+  };
+  launch_pade_prep(s, npad, nb, A, N, D, C0, d_q, d_j, d_err);
+  double2* cb[2] = {C0, C1};
+  for (int icnt = 1; icnt <= 4; icnt++) {
+    launch_pade_accum(s, npad, nb, icnt, d_q, cb[(icnt - 1) & 1], N, D);
+    // C_{icnt+1} = C_icnt B for the matrices with icnt < q (active = q - 1, step = icnt - 1)
+    if (icnt < 4 && !prod(cb[icnt & 1], cb[(icnt - 1) & 1], A, d_q, icnt - 1)) return nullptr;
+  }
+  launch_zsolve(s, npad, nb, D, N, d_err);
+  std::vector<int32_t> hj(nb);
+  hipError_t e = hipMemcpyAsync(hj.data(), d_j, 4 * (size_t)nb, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { rc_out = fail(CTEM_E_HIP, "Pade squaring counts: %s", hipGetErrorString(e)); return nullptr; }
+  int jmax = 0;
+  for (int b = 0; b < nb; b++) jmax = std::max(jmax, hj[b]);
+  double2* buf[2] = {N, T};
+  for (int i = 0; i < jmax; i++)
+    if (!prod(buf[(i + 1) & 1], buf[i & 1], buf[i & 1], d_j, i)) return nullptr;
+  launch_parity_copy(s, npad, nb, d_j, T, N);
+  return N;
+}
+
+extern "C" int32_t ctem_ecp_plan_create(ctem_ctx* c, int32_t numk, const double* k, const double* kn,
+                                        const ctem_ecp_params* prm, ctem_plan** out) {
+#pragma clang fp contract(off)
+  if (!c || !prm || !out) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: null argument");
+  *out = nullptr;
+  if (!c->cr) return fail(CTEM_E_STATE, "ctem_ecp_plan_create: no crystal uploaded");
+  const Crystal& cr = *c->cr;
+  if (prm->mode != CTEM_ECP_BLOCHWV && prm->mode != CTEM_ECP_SCATMAT)
+    return fail(CTEM_E_ARG, "ctem_ecp_plan_create: unknown mode %d", prm->mode);
+  if (prm->numthick < 1 || prm->numthick > 4096) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: 1..4096 thicknesses");
+  if (cr.nsite < 1 || !prm->site_occ) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: site occupancies needed");
+  for (int p = 0; p < cr.nsite; p++)
+    if (!std::isfinite(prm->site_occ[p])) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: non-finite occupancy");
+  for (int i = 0; i < 3; i++)
+    if (!std::isfinite(prm->fn[i])) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: non-finite foil normal");
+  const int nt = prm->numthick;
+  // thick = startthick + thickinc * (/ (i-1, i=1,nt) /) (CTEMECP.f90:270), float32
+  std::vector<float> thick(nt);
+  for (int i = 0; i < nt; i++) thick[i] = prm->startthick + prm->thickinc * (float)i;
+  std::vector<int32_t> store;
+  int numt = 0;
+  if (prm->mode == CTEM_ECP_BLOCHWV) {
+    if (!(prm->startthick > 0.0f) || !(prm->thickinc > 0.0f) || !std::isfinite(thick[nt - 1]))
+      return fail(CTEM_E_ARG, "ctem_ecp_plan_create: Blochwv needs startthick > 0 and thickinc > 0");
+  } else {
+    if (!(prm->zintstep > 0.0f)) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: zintstep must be positive");
+    // tvals = nint(thick/zintstep), numt = nint(maxval(thick)/zintstep); step i stores the running
+    // sum at thickness tval when i == tvals(tval), then tval advances (CTEMECP.f90:596-616)
+    float tmax = thick[0];
+    for (int i = 1; i < nt; i++) tmax = std::max(tmax, thick[i]);
+    const float qmax = tmax / prm->zintstep;
+    if (!(qmax < 65536.5f)) return fail(CTEM_E_RANGE, "ctem_ecp_plan_create: more than 65536 ScatMat steps");
+    numt = (int)std::lround(qmax);
+    if (numt < 1) return fail(CTEM_E_ARG, "ctem_ecp_plan_create: no ScatMat step below the largest thickness");
+    std::vector<int> tvals(nt);
+    for (int i = 0; i < nt; i++) tvals[i] = (int)std::lround(thick[i] / prm->zintstep);
+    store.assign(numt, -1);
+    int tval = 0;
+    for (int i = 1; i <= numt; i++)
+      if (tval < nt && i == tvals[tval]) store[i - 1] = tval++;
+  }
+  static const float one = 1.0f;
+  int32_t rc = ctem_plan_create(c, numk, k, kn, 1, &one, 1.0, 1.0, out);
+  if (rc) return rc;
+  ctem_plan* p = *out;
+  p->ecp_mode = prm->mode;
+  p->ecp_nt = nt;
+  p->ecp_numt = numt;
+  p->ecp_t0 = prm->startthick;
+  p->ecp_dt = prm->thickinc;
+  p->ecp_zint = prm->zintstep;
+  int nat = 0;
+  for (int q = 0; q < cr.nsite; q++) nat += cr.h_norbit[q];
+  p->ecp_nat = (double)(float)nat;
+  std::vector<double> slut;
+  build_slut_ecp(cr, prm->site_occ, slut, &p->ecp_s0);
+  hipStream_t s = c->stream;
+  if ((rc = upload(p->fn, prm->fn, 3, s)) || (rc = upload(p->ecp_slut, slut.data(), slut.size(), s)) ||
+      (rc = upload(p->ecp_thick, thick.data(), thick.size(), s)) ||
+      (numt > 0 && (rc = upload(p->ecp_store, store.data(), store.size(), s))) ||
+      (rc = p->ecp_sr.ensure(sizeof(float) * (size_t)nt * std::max(numk, 1)))) {
+    ctem_plan_destroy(p);
+    *out = nullptr;
+    return rc;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    ctem_plan_destroy(p);
+    *out = nullptr;
+    return fail(CTEM_E_HIP, "ECP plan upload failed");
+  }
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_ecp_run(ctem_plan* p) {
+  if (!p) return fail(CTEM_E_ARG, "null plan");
+  if (p->ecp_mode < 0) return fail(CTEM_E_STATE, "ctem_ecp_run: not a channeling-pattern plan");
+  ctem_ctx* c = p->ctx;
+  int32_t rc;
+  if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const bool bloch = p->ecp_mode == CTEM_ECP_BLOCHWV;
+  const int nt = p->ecp_nt, numt = p->ecp_numt;
+  for (auto& kv : p->buckets)
+    if (bloch && kv.first > 512) return fail(CTEM_E_RANGE, "ctem_ecp_run: Blochwv supports up to 512 strong beams");
+  // work space per direction: ScatMat A, 5 matrices of the exponential, Psi at every step and the
+  // per-tile depth sums; Blochwv the assembled A, the two block matrices of size 2 npad and the
+  // 5 x 2 matrices of their exponentials
+  auto per_dir = [&](int npad) -> size_t {
+    const size_t per = (size_t)npad * npad;
+    if (bloch) return (per + 8 * per + 40 * per) * sizeof(double2) + 64;
+    const size_t nzp = (size_t)(numt + 31) / 32 * 32;
+    return (6 * per + (size_t)npad * numt) * sizeof(double2) + (size_t)(npad / 32) * nzp * 8 + 64;
+  };
+  size_t a = 0, w = 0, ps = 0, sq = 0, part = 0;
+  for (auto& kv : p->buckets) {
+    const int npad = kv.first, nbk = (int)kv.second.size();
+    const size_t per = (size_t)npad * npad;
+    const size_t nb = (size_t)std::max(1, (int)std::min<size_t>(nbk, std::max<size_t>(1, chunk_budget(p) / per_dir(npad))));
+    if (bloch) {
+      a = std::max(a, 8 * per * nb * sizeof(double2));
+      w = std::max(w, 40 * per * nb * sizeof(double2));
+      ps = std::max(ps, per * nb * sizeof(double2));
+      sq = std::max(sq, 8 * nb);
+    } else {
+      const size_t nzp = (size_t)(numt + 31) / 32 * 32;
+      a = std::max(a, per * nb * sizeof(double2));
+      w = std::max(w, 5 * per * nb * sizeof(double2));
+      ps = std::max(ps, (size_t)npad * numt * nb * sizeof(double2));
+      sq = std::max(sq, 8 * nb);
+      part = std::max(part, (size_t)(npad / 32) * nzp * nb * 8);
+    }
+  }
+  const void* before[4] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p};
+  if ((rc = c->w.A.ensure(a)) || (rc = c->w.W.ensure(w)) || (rc = c->w.Psi.ensure(ps)) || (rc = c->w.sq.ensure(sq)) ||
+      (part && (rc = p->ecp_part.ensure(part))))
+    return rc;
+  const void* after[4] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p};
+  for (int i = 0; i < 4; i++)
+    if (before[i] != after[i]) c->wgen++;
+  reset_timing(p);
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  launch_fill_zero(s, p->err.p, 4);
+  launch_fill_zero(s, p->ecp_sr.p, sizeof(float) * (size_t)nt * std::max(p->numk, 1));
+  BeamView V{p->nns.as<int32_t>(), p->nnw.as<int32_t>(), p->soff.as<int64_t>(), p->woff.as<int64_t>(),
+             p->sidx.as<int32_t>(), p->ssg.as<float>(), p->widx.as<int32_t>(), p->wsg.as<float>()};
+  DevRefl R = p->cr->refl();
+  DevLut L = p->cr->lutv(p->err.as<int32_t>());
+  DevLut LE = L;
+  LE.slut = p->ecp_slut.as<double2>();
+  LE.nsite = 1;
+  size_t off = 0;
+  for (auto& kv : p->buckets) {
+    const int npad = kv.first;
+    const int nbk = (int)kv.second.size();
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, chunk_budget(p) / per_dir(npad)));
+    for (int c0 = 0; c0 < nbk; c0 += chunk) {
+      const int nb = std::min(chunk, nbk - c0);
+      Bucket B{npad, nb, p->dirs.as<int32_t>() + off + c0};
+      std::vector<int> hn;
+      std::vector<int32_t> hsq;
+      if (!bloch) {
+        for (int b = 0; b < nb; b++) hn.push_back(p->h_nns[kv.second[c0 + b]]);
+        // Minp = DynMat * dcmplx(0, cPi mLambda), Azz = MatrixExponential(Minp, zintstep, 'Pade')
+        launch_assemble_scatmat(s, B, V, R, L, p->cr->mlambda, p->cr->dynupz, (double)p->ecp_zint, c->w.A.as<double2>());
+        HIPCHK(hipGetLastError());
+        double2* E = pade_fp64(p, s, npad, nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.sq.as<int32_t>(),
+                               c->w.sq.as<int32_t>() + nb, p->err.as<int32_t>(), rc);
+        if (!E) return rc;
+        const int64_t spsi = (int64_t)npad * numt;
+        launch_depth(s, npad, nb, numt, E, 0, 0, c->w.Psi.as<double2>(), spsi, c->cu_count);
+        launch_ecp_depth_dots(s, B, V, R, LE, c->w.Psi.as<double2>(), spsi, numt, p->ecp_part.as<double>());
+        launch_ecp_scatmat_out(s, B, numt, p->ecp_part.as<double>(), p->ecp_store.as<int32_t>(), p->ecp_nat, p->numk,
+                               p->ecp_sr.as<float>());
+        HIPCHK(hipGetLastError());
+      } else {
+        for (int b = 0; b < nb; b++) hn.push_back(2 * p->h_nns[kv.second[c0 + b]]);
+        for (int b = 0; b < nb; b++) hn.push_back(2 * p->h_nns[kv.second[c0 + b]]);
+        // A1 = 2 pi i DynMat / fl32(2 kn); eps scales Sgh^T to entries of order 1/npad
+        launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, 1.0, c->w.Psi.as<double2>());
+        const double eps = std::ldexp(1.0, -std::ilogb(std::max(p->ecp_s0, 1e-300) * (double)npad));
+        launch_ecp_augment(s, B, V, R, LE, c->w.Psi.as<double2>(), eps, (double)p->ecp_t0, (double)p->ecp_dt,
+                           c->w.A.as<double2>());
+        HIPCHK(hipGetLastError());
+        double2* E = expm_fp64(p, s, 2 * npad, 2 * nb, c->w.A.as<double2>(), c->w.W.as<double2>(), c->w.sq.as<int32_t>(),
+                               p->err.as<int32_t>(), hn, hsq, rc);
+        if (!E) return rc;
+        launch_ecp_bloch_steps(s, B, V, E, nt, p->ecp_thick.as<float>(), eps, p->ecp_nat, p->numk, p->ecp_sr.as<float>());
+        HIPCHK(hipGetLastError());
+      }
+    }
+    off += nbk;
+  }
+  HIPCHK(hipEventRecord(c->ev[1], s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  p->t_total = ms;
+  for (size_t i = 0; i + 1 < p->evused; i += 2) {
+    float g = 0.0f;
+    HIPCHK(hipEventElapsedTime(&g, p->evpool[i], p->evpool[i + 1]));
+    p->t_gemm += g;
+  }
+  if ((rc = check_err_flag(p->err, s, "ctem_ecp_run"))) return rc;
+  p->ran = true;
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_ecp_get_sr(ctem_plan* p, float* sr) {
+  if (!p || !sr) return fail(CTEM_E_ARG, "ctem_ecp_get_sr: null argument");
+  if (p->ecp_mode < 0 || !p->ran) return fail(CTEM_E_STATE, "ctem_ecp_get_sr: run ctem_ecp_run first");
+  const size_t n = (size_t)p->ecp_nt * p->numk;
+  if (n) {
+    HIPCHK(hipMemcpyAsync(sr, p->ecp_sr.p, sizeof(float) * n, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+  }
+  return CTEM_OK;
+}
+
+extern "C" int32_t ctem_ecp_image(ctem_plan* p, const int32_t* ij, int32_t npx, float* img) {
+  if (!p || !img || (p->numk > 0 && !ij)) return fail(CTEM_E_ARG, "ctem_ecp_image: null argument");
+  if (npx < 0 || npx > 4096) return fail(CTEM_E_ARG, "ctem_ecp_image: npx must be 0..4096");
+  const int64_t w = 2 * (int64_t)npx + 1;
+  std::vector<float> sr((size_t)p->ecp_nt * p->numk);
+  int32_t rc;
+  if ((rc = ctem_ecp_get_sr(p, sr.data()))) return rc;
+  std::fill(img, img + (size_t)p->ecp_nt * w * w, 0.0f);
+  for (int ik = 0; ik < p->numk; ik++) {
+    const int64_t x = ij[2 * ik] + npx, y = ij[2 * ik + 1] + npx;
+    if (x < 0 || x >= w || y < 0 || y >= w) return fail(CTEM_E_RANGE, "ctem_ecp_image: direction %d outside the image", ik);
+    for (int it = 0; it < p->ecp_nt; it++) img[((size_t)it * w + y) * w + x] = sr[(size_t)it * p->numk + ik];
+  }
   return CTEM_OK;
 }

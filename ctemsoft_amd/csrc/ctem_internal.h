@@ -134,9 +134,10 @@ struct SelectOut {       // per-direction beam sets (CSR over directions)
   int32_t* err;          // set to 1 when an angle leaves the reproduced cosf branch
 };
 
-// Beam selection. fill=0 counts, fill=1 writes the lists.
+// Beam selection. fill=0 counts, fill=1 writes the lists. fn (float[3] on the device, or null):
+// the foil normal of the excitation errors, null for FN = k per direction.
 void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& R,
-                   const double* rmt9, const SelectOut& out, int fill);
+                   const double* rmt9, const float* fn, const SelectOut& out, int fill);
 
 // Per-bucket descriptors for the dense stages. All matrices of a bucket are n_pad x n_pad,
 // column-major, with batch stride n_pad*n_pad; dirs[b] names the plan direction of slot b.
@@ -157,6 +158,9 @@ struct BeamView {        // read-only view of the selection lists
 void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                      const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
                      double2* A);
+// A = i pi mLambda dz DynMat, FP64 (the scattering matrix of the channeling-pattern program)
+void launch_assemble_scatmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                             const DevLut& L, double mlambda, float dynupz, double dz, double2* A);
 // the same scaled matrix rounded to FP32 (mixed form)
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
                        const DevLut& L, const double* kn, double mlambda, float dynupz, double dz,
@@ -319,6 +323,33 @@ void launch_norm_scale32(hipStream_t s, int32_t npad, int32_t nb, float2* A, dou
                          int32_t smax, int32_t* sq, int32_t* err, float* amax);
 void launch_cparity_copy(hipStream_t s, int32_t npad, int32_t nb, const int32_t* sq,
                          const float2* src, float2* dst);
+
+// ---- electron channeling patterns (ctem_ecp.hip) ----
+// L.slut: the channeling structure matrix table Sgh(g-h) (one plane)
+// ScatMat: per-depth Re psi_z^T Sgh conj(psi_z), partial sums per 32-row tile:
+// partial[(b * npad/32 + tile) * nzp + z], nzp = nz rounded up to 32
+void launch_ecp_depth_dots(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                           const double2* Psi, int64_t spsi, int32_t nz, double* partial);
+// running sums over the depths, sr[t * numk + dir] = fl32(sum / nat) where store[z] = t >= 0
+void launch_ecp_scatmat_out(hipStream_t s, const Bucket& B, int32_t nz, const double* partial, const int32_t* store,
+                            double nat, int32_t numk, float* sr);
+// Blochwv: M[b] = t0 X_b, M[nb + b] = dt X_b, X = [[A1, 0], [eps Sgh^T, -A1^H]] (2 npad square)
+void launch_ecp_augment(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
+                        const double2* A1, double eps, double t0, double dt, double2* M);
+// y = E[b] e_1, y <- E[nb + b] y per further thickness; sr[it * numk + dir] = fl32(Re psi^H phi / eps
+// / thick[it] / nat)
+void launch_ecp_bloch_steps(hipStream_t s, const Bucket& B, const BeamView& V, const double2* E, int32_t nt,
+                            const float* thick, double eps, double nat, int32_t numk, float* sr);
+
+// MatrixExponential 'Pade' (math.f90:187-315), batched over nb npad x npad matrices A (holding
+// A z0): prep scales A to B = A / 2^j in place and sets N = D = I, C = B, qm1[b] = q - 1, jj[b] = j;
+// accum adds step icnt (1..4) of the Pade sums for the matrices with icnt <= q (C = B^icnt
+// supplied by the caller's products); zsolve overwrites X with D^-1 X (D is destroyed).
+void launch_pade_prep(hipStream_t s, int32_t npad, int32_t nb, double2* A, double2* N, double2* D, double2* C,
+                      int32_t* qm1, int32_t* jj, int32_t* err);
+void launch_pade_accum(hipStream_t s, int32_t npad, int32_t nb, int32_t icnt, const int32_t* qm1, const double2* C,
+                       double2* N, double2* D);
+void launch_zsolve(hipStream_t s, int32_t npad, int32_t nb, double2* D, double2* X, int32_t* err);
 
 // ---- reverse mode (ctem_adjoint.hip) ----
 void launch_adj_m(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,

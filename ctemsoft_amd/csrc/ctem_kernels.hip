@@ -28,9 +28,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------------------------
 // beam selection
+// fn: the foil normal (float32, reciprocal components) shared by all directions, or null for
+// FN = k of each direction
 __global__ __launch_bounds__(256) void k_select(const double* __restrict__ kvec, int32_t numk,
                                                 DevRefl R, const double* __restrict__ rmt_g,
-                                                SelectOut out, int fill) {
+                                                const float* __restrict__ fn_g, SelectOut out, int fill) {
   const int ik = blockIdx.x;
   if (ik >= numk) return;
   __shared__ double rmt[9];
@@ -41,6 +43,7 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ kvec,
   if (tid == 0) { base_s = 0; base_w = 0; }
   __syncthreads();
   const float k[3] = {(float)kvec[3 * ik], (float)kvec[3 * ik + 1], (float)kvec[3 * ik + 2]};
+  const float fn[3] = {fn_g ? fn_g[0] : k[0], fn_g ? fn_g[1] : k[1], fn_g ? fn_g[2] : k[2]};
   const int64_t soff = fill ? out.soff[ik] : 0;
   const int64_t woff = fill ? out.woff[ik] : 0;
   bool range_ok = true;
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ kvec,
       cls = 2;  // the transmitted beam is always strong, with s_g = 0
     } else if (ig < R.nref) {
       const float g[3] = {R.g[3 * ig], R.g[3 * ig + 1], R.g[3 * ig + 2]};
-      sg = calcsg(g, k, rmt, &range_ok);
+      sg = calcsg_fn(g, k, fn, rmt, &range_ok);
       const float sgp = fabsf(sg);
       if (R.dbd[ig]) {
         if (sgp <= R.sgcutoff) cls = 2;
@@ -93,9 +96,9 @@ __global__ __launch_bounds__(256) void k_select(const double* __restrict__ kvec,
 }
 
 void launch_select(hipStream_t s, const double* k, int32_t numk, const DevRefl& R,
-                   const double* rmt9, const SelectOut& out, int fill) {
+                   const double* rmt9, const float* fn, const SelectOut& out, int fill) {
   if (numk <= 0) return;
-  hipLaunchKernelGGL(k_select, dim3(numk), dim3(256), 0, s, k, numk, R, rmt9, out, fill);
+  hipLaunchKernelGGL(k_select, dim3(numk), dim3(256), 0, s, k, numk, R, rmt9, fn, out, fill);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -225,7 +228,11 @@ __global__ __launch_bounds__(256) void k_assemble(Bucket B, BeamView V, DevRefl 
   // write the tile: lane l holds rows wr + l/16 + 4e (f64 MFMA) or wr + 4(l/16) + e (f32),
   // column wc + l%16
   const double half = 0.5 * mlambda;
-  const double pre = scaled ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]) : 0.0;
+  // scaled 1: 2 pi dz / fl32(2 kn) (Bloch waves, W / cmplx(2 kn)); 2: pi mLambda dz (the
+  // scattering matrix of CalcLghSM, DynMat * dcmplx(0, cPi * mLambda), CTEMECP.f90:586)
+  const double pre = scaled == 1   ? 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir])
+                     : scaled == 2 ? 3.141592653589793238 * mlambda * dz
+                                   : 0.0;
   for (int e = 0; e < 4; e++) {
     const int r = F32 ? wr + 4 * l4 + e : wr + l4 + 4 * e, c = wc + l16;
     const int gi = ti * 32 + r, gj = tj * 32 + c;
@@ -258,6 +265,13 @@ void launch_assemble(hipStream_t s, const Bucket& B, const BeamView& V, const De
   const int tiles = B.npad / 32;
   hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L, kn,
                      mlambda, dynupz, dz, A, (float2*)nullptr, 1);
+}
+
+void launch_assemble_scatmat(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,
+                             const DevLut& L, double mlambda, float dynupz, double dz, double2* A) {
+  const int tiles = B.npad / 32;
+  hipLaunchKernelGGL(k_assemble<false>, dim3(tiles * tiles * B.nb), dim3(256), 0, s, B, V, R, L,
+                     (const double*)nullptr, mlambda, dynupz, dz, A, (float2*)nullptr, 2);
 }
 
 void launch_assemble32(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R,

@@ -916,15 +916,23 @@ static int32_t fill_box_gpu(ctem_setup* T, int device, double* ms) {
   return CTEM_OK;
 }
 
-static int32_t setup_build(const ctem_xtal* x, float ekev, float dmin, float weakcutoff, float cutoff,
-                           float sgcutoff, int device, double* gpu_ms, ctem_setup** out) {
+// voltage [V] enters CalcWaveLength as the programs pass it (the master program 1000 ekev in
+// float32, the channeling-pattern program its float32 `voltage`). kzone (may be null) and
+// convang >= 0: the convergence-angle window of Compute_ReflectionList (gvectors.f90:1295-1301,
+// 1349-1395) for zone axis kzone; null: no window (every angle lies in 0..pi).
+static int32_t setup_build(const ctem_xtal* x, float ekev, double voltage, const int32_t* kzone, float convang,
+                           float dmin, float weakcutoff, float cutoff, float sgcutoff, int device, double* gpu_ms,
+                           ctem_setup** out) {
 #pragma clang fp contract(off)
   if (!x || !out) return setup_fail(CTEM_E_ARG, "ctem_setup_create: null argument");
   *out = nullptr;
   if (x->ntype < 1 || x->ntype > 250 || !x->atom_type || !x->atom_pos)
     return setup_fail(CTEM_E_ARG, "ctem_setup_create: 1..250 sites needed");
   if (x->sgnum < 1 || x->sgnum > 237) return setup_fail(CTEM_E_ARG, "ctem_setup_create: space group not in 1..237");
-  if (!(ekev > 0.0f) || !(dmin > 0.0f)) return setup_fail(CTEM_E_ARG, "ctem_setup_create: energy and dmin must be positive");
+  if (!(voltage > 0.0) || !(dmin > 0.0f)) return setup_fail(CTEM_E_ARG, "ctem_setup_create: energy and dmin must be positive");
+  if (kzone && (kzone[0] == 0 && kzone[1] == 0 && kzone[2] == 0))
+    return setup_fail(CTEM_E_ARG, "ctem_setup_create_ecp: zone axis [000]");
+  if (kzone && !(convang >= 0.0f)) return setup_fail(CTEM_E_ARG, "ctem_setup_create_ecp: convergence angle must be >= 0");
   for (int i = 0; i < x->ntype; i++)
     if (x->atom_type[i] < 1 || x->atom_type[i] > 98 || !(x->atom_pos[5 * i + 4] >= 0.0f))
       return setup_fail(CTEM_E_ARG, "ctem_setup_create: atomic numbers 1..98 and B >= 0 needed");
@@ -942,9 +950,34 @@ static int32_t setup_build(const ctem_xtal* x, float ekev, float dmin, float wea
   }
   calc_positions(S);
   T->isym = laue_isym(S.sgnum, &T->usehex);
-  calc_wavelength(S, (double)(ekev * 1000.0f));
-  // Compute_ReflectionList 'ALL' with k = [001] and no angular window (every angle lies in the
-  // default window 0..pi, so every family member is kept)
+  calc_wavelength(S, voltage);
+  // Compute_ReflectionList 'ALL': the angular window of the family members (gvectors.f90:1295-1301):
+  // minang = pi/2 - 3 convang .. maxang = pi/2 + 3 convang about the zone axis kr (reciprocal
+  // components of k, float32), or 0..pi without a window
+  float kr[3] = {0.0f, 0.0f, 1.0f}, minang = 0.0f, maxang = (float)M_PI;
+  if (kzone) {
+    for (int j = 0; j < 3; j++) {
+      double d = 0.0;
+      for (int i = 0; i < 3; i++) d = d + (double)(float)kzone[i] * S.dmt[i][j];  // TransSpace d -> r
+      kr[j] = (float)d;
+    }
+    const float c3 = 3.0f * convang;
+    minang = (float)(M_PI * 0.5 - (double)c3);
+    maxang = (float)(M_PI * 0.5 + (double)c3);
+  }
+  const double rm9[9] = {S.rmt[0][0], S.rmt[0][1], S.rmt[0][2], S.rmt[1][0], S.rmt[1][1], S.rmt[1][2],
+                         S.rmt[2][0], S.rmt[2][1], S.rmt[2][2]};
+  // CalcAngle(float(g), kr, 'r') (crystal.f90:632-666)
+  auto angle_r = [&](const int g[3]) -> float {
+    const float p[3] = {(float)g[0], (float)g[1], (float)g[2]};
+    const float xx = dot_r(p, kr, rm9);
+    const float yy = std::sqrt(dot_r(p, p, rm9));
+    const float zz = std::sqrt(dot_r(kr, kr, rm9));
+    const float t = xx / (yy * zz);
+    if (t >= 1.0f) return 0.0f;
+    if (t <= -1.0f) return (float)M_PI;
+    return ::acosf(t);
+  };
   int im[3];
   for (int d = 0; d < 3; d++) {
     im[d] = 1;
@@ -1004,8 +1037,24 @@ static int32_t setup_build(const ctem_xtal* x, float ekev, float dmin, float wea
             delete T;
             return setup_fail(CTEM_E_RANGE, "ctem_setup_create: a family member lies outside the LUT box");
           }
+        // members inside the angular window are added in family order; then the zone's own
+        // members (g.k = 0) that the window missed (gvectors.f90:1364-1410)
+        std::vector<uint8_t> inrange(numr, 1);
+        if (kzone)
+          for (int ir = 0; ir < numr; ir++) {
+            const float sa = angle_r(S.itmp[ir]);
+            inrange[ir] = (sa >= minang) && (sa <= maxang);
+          }
         for (int ir = 0; ir < numr; ir++) refdone[cell_of(S.itmp[ir])] = 1;
+        std::vector<int> order;
+        for (int ir = 0; ir < numr; ir++)
+          if (inrange[ir]) order.push_back(ir);
         for (int ir = 0; ir < numr; ir++) {
+          const int* gi = S.itmp[ir];
+          const int nz = kzone ? gi[0] * kzone[0] + gi[1] * kzone[1] + gi[2] * kzone[2] : 1;
+          if (!inrange[ir] && nz == 0) order.push_back(ir);
+        }
+        for (int ir : order) {
           const int g[3] = {S.itmp[ir][0], S.itmp[ir][1], S.itmp[ir][2]};
           // the family comes from the point group of the first space group of the crystal's
           // point group, whose operators need not keep the lattice centring condition (R-3c in
@@ -1022,6 +1071,19 @@ static int32_t setup_build(const ctem_xtal* x, float ekev, float dmin, float wea
           }
           add_reflection(g);
           refdone[cell_of(g)] = 2;
+        }
+      }
+  // LUT cells of the inner box that no list entry took keep the reference's initial 0
+  // (gvectors.f90:1326-1327: only AddReflection's branch writes inner-box cells)
+  for (int l = -im[2]; l <= im[2]; l++)
+    for (int kk = -im[1]; kk <= im[1]; kk++)
+      for (int h = -im[0]; h <= im[0]; h++) {
+        const int g[3] = {h, kk, l};
+        const size_t c = cell_of(g);
+        if (refdone[c] != 2) {
+          T->lut[2 * c] = 0.0;
+          T->lut[2 * c + 1] = 0.0;
+          T->lut_dbd[c] = 0;
         }
       }
   const int nref = (int)T->hkl.size() / 3;
@@ -1041,7 +1103,16 @@ static int32_t setup_build(const ctem_xtal* x, float ekev, float dmin, float wea
 
 extern "C" int32_t ctem_setup_create(const ctem_xtal* x, float ekev, float dmin, float weakcutoff,
                                      float cutoff, float sgcutoff, ctem_setup** out) {
-  return setup_build(x, ekev, dmin, weakcutoff, cutoff, sgcutoff, -1, nullptr, out);
+  return setup_build(x, ekev, (double)(ekev * 1000.0f), nullptr, 0.0f, dmin, weakcutoff, cutoff, sgcutoff, -1,
+                     nullptr, out);
+}
+
+extern "C" int32_t ctem_setup_create_ecp(const ctem_xtal* x, float voltage, float dmin, const int32_t k[3],
+                                         float convang, float weakcutoff, float cutoff, float sgcutoff,
+                                         ctem_setup** out) {
+  if (!k) return setup_fail(CTEM_E_ARG, "ctem_setup_create_ecp: null zone axis");
+  return setup_build(x, voltage / 1000.0f, (double)voltage, k, convang, dmin, weakcutoff, cutoff, sgcutoff, -1,
+                     nullptr, out);
 }
 
 extern "C" int32_t ctem_setup_create_gpu(int32_t device, const ctem_xtal* x, float ekev, float dmin,
@@ -1050,7 +1121,8 @@ extern "C" int32_t ctem_setup_create_gpu(int32_t device, const ctem_xtal* x, flo
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return setup_fail(CTEM_E_HIP, "no HIP device available");
   if (device < 0 || device >= ndev) return setup_fail(CTEM_E_ARG, "ctem_setup_create_gpu: device out of range");
-  return setup_build(x, ekev, dmin, weakcutoff, cutoff, sgcutoff, device, lut_ms, out);
+  return setup_build(x, ekev, (double)(ekev * 1000.0f), nullptr, 0.0f, dmin, weakcutoff, cutoff, sgcutoff, device,
+                     lut_ms, out);
 }
 
 extern "C" int32_t ctem_setup_crystal(const ctem_setup* T, ctem_crystal* c) {

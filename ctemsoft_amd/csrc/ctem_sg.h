@@ -127,9 +127,12 @@ CTEM_HD inline float dot_r(const float p[3], const float q[3], const double* rmt
   return (float)c;
 }
 
-// CalcsgSingle with FN = k. *range_ok is cleared when the angle leaves the cosf branch
-// reproduced here (|q2| >= pi/4, i.e. |g| comparable to |k|).
-CTEM_HD inline float calcsg(const float g[3], const float k[3], const double* rmt, bool* range_ok) {
+// CalcsgSingle (diffraction.f90:649-674) with foil normal fn: sg = -g.(2k+g) / (2|k+g| cos angle(k+g,
+// fn)), every quantity in float32 through CalcDot/CalcLength/CalcAngle. *range_ok is cleared when
+// the angle leaves the cosf branch reproduced here (|angle| >= pi/4, i.e. |g| comparable to |k| or
+// a foil normal far from the beam).
+CTEM_HD inline float calcsg_fn(const float g[3], const float k[3], const float fn[3], const double* rmt,
+                               bool* range_ok) {
 #pragma clang fp contract(off)
   float kpg[3], tkpg[3];
   for (int i = 0; i < 3; i++) {
@@ -139,10 +142,10 @@ CTEM_HD inline float calcsg(const float g[3], const float k[3], const double* rm
   }
   float xnom = -dot_r(g, tkpg, rmt);
   float q1 = CTEM_FSQRT(dot_r(kpg, kpg, rmt));
-  // CalcAngleSingle(kpg, k)
-  float x = dot_r(kpg, k, rmt);
+  // CalcAngleSingle(kpg, fn)
+  float x = dot_r(kpg, fn, rmt);
   float y = q1;
-  float z = CTEM_FSQRT(dot_r(k, k, rmt));
+  float z = CTEM_FSQRT(dot_r(fn, fn, rmt));
   float yz = y * z;
   float t = CTEM_FDIV(x, yz);
   float q2;
@@ -153,6 +156,12 @@ CTEM_HD inline float calcsg(const float g[3], const float k[3], const double* rm
   float tq = 2.0f * q1;
   float xden = tq * cosf_ref(q2);
   return CTEM_FDIV(xnom, xden);
+}
+
+// CalcsgSingle with FN = k (the master-pattern program sets DynFN to each beam direction,
+// CTEMEBSDmaster.f90:414)
+CTEM_HD inline float calcsg(const float g[3], const float k[3], const double* rmt, bool* range_ok) {
+  return calcsg_fn(g, k, k, rmt, range_ok);
 }
 
 }  // namespace ctem

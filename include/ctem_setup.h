@@ -65,6 +65,15 @@ CTEM_API int32_t ctem_setup_create(const ctem_xtal* x, float ekev, float dmin, f
 CTEM_API int32_t ctem_setup_create_gpu(int32_t device, const ctem_xtal* x, float ekev, float dmin,
                                        float weakcutoff, float cutoff, float sgcutoff, double* lut_ms,
                                        ctem_setup** out);
+/* The set-up of the electron channeling pattern program (CTEMECP.f90:158-238): the same crystal
+ * data with the wavelength from `voltage` in V (CalcWaveLength(dble(voltage)), :197-198) and the
+ * master reflection list of Compute_ReflectionList 'ALL' for zone axis k with the convergence-angle
+ * window convang = thetac/1000 [rad] (:238; gvectors.f90:1295-1301,1349-1410): of each family
+ * only the members at angles pi/2 -+ 3 convang to k, then the zone's own members (g.k = 0), are
+ * listed; LUT cells of the inner box that no listed entry took stay 0, as in the reference. */
+CTEM_API int32_t ctem_setup_create_ecp(const ctem_xtal* x, float voltage, float dmin, const int32_t k[3],
+                                       float convang, float weakcutoff, float cutoff, float sgcutoff,
+                                       ctem_setup** out);
 /* A ctem_crystal that points into the setup's arrays (valid until ctem_setup_destroy), for
  * ctem_upload_crystal. */
 CTEM_API int32_t ctem_setup_crystal(const ctem_setup* s, ctem_crystal* out);

@@ -68,19 +68,38 @@ float orc_calcsg(const float g[3], const float k[3], const float fn[3], const do
 /* Compute_DynMat 'BLOCHBETHE' beam selection and ordering (gvectors.f90:988-1120):
  * |s_g| against weakcutoff*lambda|U_g| (strong) and cutoff*lambda|U_g| (weak), with the
  * sg cutoff for double-diffraction reflections; 000 is always strong and first. */
+int orc_select_beams_fn(int nref, const int32_t* hkl, const double* ucg, const uint8_t* dbdiff,
+                        double mlambda, float weakcutoff, float cutoff, float sgcutoff,
+                        const double rmt[9], const double kvec[3], const float* fnv, int32_t* strong,
+                        float* strongsg, int32_t* nns, int32_t* weak, float* weaksg,
+                        int32_t* nnw);
+
 int orc_select_beams(int nref, const int32_t* hkl, const double* ucg, const uint8_t* dbdiff,
                      double mlambda, float weakcutoff, float cutoff, float sgcutoff,
                      const double rmt[9], const double kvec[3], int32_t* strong,
                      float* strongsg, int32_t* nns, int32_t* weak, float* weaksg,
                      int32_t* nnw) {
-  float k[3] = {(float)kvec[0], (float)kvec[1], (float)kvec[2]}; /* sngl(kk), DynFN */
+  return orc_select_beams_fn(nref, hkl, ucg, dbdiff, mlambda, weakcutoff, cutoff, sgcutoff, rmt, kvec, NULL,
+                             strong, strongsg, nns, weak, weaksg, nnw);
+}
+
+/* the same with a fixed foil normal fnv (DynFN of the channeling-pattern program,
+ * CTEMECP.f90:204-205; gvectors.f90:1014); fnv = NULL: DynFN = k of each direction
+ * (CTEMEBSDmaster.f90:414) */
+int orc_select_beams_fn(int nref, const int32_t* hkl, const double* ucg, const uint8_t* dbdiff,
+                        double mlambda, float weakcutoff, float cutoff, float sgcutoff,
+                        const double rmt[9], const double kvec[3], const float* fnv, int32_t* strong,
+                        float* strongsg, int32_t* nns, int32_t* weak, float* weaksg,
+                        int32_t* nnw) {
+  float k[3] = {(float)kvec[0], (float)kvec[1], (float)kvec[2]}; /* sngl(kk) */
+  const float* fn = fnv ? fnv : k;
   int ns = 0, nw = 0;
   strong[ns] = 0;
   strongsg[ns] = 0.0f;
   ns++;
   for (int ig = 1; ig < nref; ig++) {
     float g[3] = {(float)hkl[3 * ig], (float)hkl[3 * ig + 1], (float)hkl[3 * ig + 2]};
-    float sg = orc_calcsg(g, k, k, rmt);
+    float sg = orc_calcsg(g, k, fn, rmt);
     float sgp = fabsf(sg);
     float lug = (float)(cabs(ucg[2 * ig] + I * ucg[2 * ig + 1]) * mlambda);
     float cut1 = cutoff * lug;
@@ -190,6 +209,45 @@ int orc_sgh(int nns, const int32_t* strong, const int32_t* hkl, const double rmt
         float im = pre * sn;
         sr += (double)re;
         si += (double)im;
+      }
+      sgh[2 * (ir * nns + ic)] = sr;
+      sgh[2 * (ir * nns + ic) + 1] = si;
+    }
+  }
+  return 0;
+}
+
+/* The channeling structure matrix of the channeling-pattern program (CTEMECP.f90:318-350), all
+ * sites in one matrix: per site Znsq = float(Z)**2 * occ, kkl = 0.25 CalcLength(float(g-h))**2
+ * and DBWF = Znsq * exp(-B kkl), all float32 (expf); per orbit position the float32 product
+ * cmplx(DBWF, 0) * cmplx(cos(arg), sin(arg)) with arg = fl32(2 pi) * sum((g-h) * r) in double,
+ * added in double site after site. sgh is row-major [ir][ic]. */
+int orc_sgh_ecp(int nns, const int32_t* strong, const int32_t* hkl, const double rmt[9], int nsite,
+                const int32_t* z, const float* b_nm2, const float* occ, const int32_t* norbit,
+                const double* orbit, double* sgh) {
+  const float tpi = (float)(2.0 * CPI);
+  for (int ir = 0; ir < nns; ir++) {
+    for (int ic = 0; ic < nns; ic++) {
+      int kk[3];
+      for (int t = 0; t < 3; t++) kk[t] = hkl[3 * strong[ir] + t] - hkl[3 * strong[ic] + t];
+      float kf[3] = {(float)kk[0], (float)kk[1], (float)kk[2]};
+      float L = len_r_single(kf, rmt);
+      float kkl = 0.25f * (L * L);
+      double sr = 0.0, si = 0.0;
+      const double* orb = orbit;
+      for (int p = 0; p < nsite; p++) {
+        float zf = (float)z[p];
+        float znsq = (zf * zf) * occ[p];
+        float dbwf = znsq * expf(-b_nm2[p] * kkl);
+        for (int a = 0; a < norbit[p]; a++) {
+          double sd = 0.0;
+          for (int t = 0; t < 3; t++) sd += (double)kk[t] * orb[3 * a + t];
+          double arg = (double)tpi * sd;
+          float c = (float)cos(arg), sn = (float)sin(arg);
+          sr += (double)(dbwf * c);
+          si += (double)(dbwf * sn);
+        }
+        orb += 3 * norbit[p];
       }
       sgh[2 * (ir * nns + ic)] = sr;
       sgh[2 * (ir * nns + ic) + 1] = si;

@@ -30,6 +30,13 @@ int orc_select_beams(int nref, const int32_t* hkl, const double* ucg /* 2*nref *
                      const double kvec[3], int32_t* strong, float* strongsg, int32_t* nns,
                      int32_t* weak, float* weaksg, int32_t* nnw);
 
+/* the same with the foil normal fn (float32 reciprocal components; NULL: FN = k), as the
+ * channeling-pattern program selects with its DynFN (CTEMECP.f90:204-205,299) */
+int orc_select_beams_fn(int nref, const int32_t* hkl, const double* ucg, const uint8_t* dbdiff,
+                        double mlambda, float weakcutoff, float cutoff, float sgcutoff,
+                        const double rmt[9], const double kvec[3], const float* fn, int32_t* strong,
+                        float* strongsg, int32_t* nns, int32_t* weak, float* weaksg, int32_t* nnw);
+
 /* Bethe-corrected dynamical matrix (n x n complex): gvectors.f90:1123-1173 */
 int orc_dynmat(int nns, const int32_t* strong, const float* strongsg, int nnw,
                const int32_t* weak, const float* weaksg, const int32_t* hkl,
@@ -40,6 +47,11 @@ int orc_dynmat(int nns, const int32_t* strong, const float* strongsg, int nnw,
 int orc_sgh(int nns, const int32_t* strong, const int32_t* hkl, const double rmt[9],
             int32_t z, float b_nm2, int norbit, const double* orbit /* norbit x 3 */,
             double* sgh);
+
+/* the channeling structure matrix, all sites (occupancy included): CTEMECP.f90:318-350 */
+int orc_sgh_ecp(int nns, const int32_t* strong, const int32_t* hkl, const double rmt[9], int nsite,
+                const int32_t* z, const float* b_nm2, const float* occ, const int32_t* norbit,
+                const double* orbit, double* sgh);
 
 /* depth-integrated Lgh = (dz/thick) sum_k lam_k psi_k psi_k^H with
  * psi_k = expm(2 pi i k dz DynMat / fl32(2 kn)) e_1, evaluated in extended precision by

@@ -244,3 +244,159 @@ def bin_pattern(pattern, binning: int, gamma: bool, gammavalue: float) -> np.nda
     out = np.zeros((ny // binning, nx // binning), np.float32)
     lib().orc_bin_pattern(nx, ny, binning, int(gamma), float(gammavalue), _p(p), _p(out))
     return out
+
+
+# ----------------------------------------------------------------------------------------
+# electron channeling patterns (src_programs/CTEMECP.f90)
+
+class EcpCrystal:
+    """The set-up the channeling-pattern beam loop reads (reflection list, LUT, sites), from the
+    arrays of the reference's set-up dump (oracle/ref_ecp.f90) or of the native set-up."""
+
+    def __init__(self, hkl, ucg, dbdiff_ref, lut, lut_lo, mlambda, dynupz, cutoffs, rmt, sites):
+        self.hkl = np.ascontiguousarray(hkl, np.int32).reshape(-1, 3)
+        u = np.asarray(ucg, np.complex128)
+        self.ucg = np.ascontiguousarray(np.stack([u.real, u.imag], -1).ravel())
+        self.dbdiff_ref = np.ascontiguousarray(dbdiff_ref, np.uint8)
+        lutc = np.asarray(lut, np.complex128)  # [l][k][h]
+        self.lut_dim = np.array(lutc.shape[::-1], np.int32)
+        self.lut_lo = np.ascontiguousarray(lut_lo, np.int32)
+        self.lut = np.ascontiguousarray(np.stack([lutc.real, lutc.imag], -1).ravel())
+        self.mlambda, self.dynupz = float(mlambda), float(dynupz)
+        self.weakcutoff, self.cutoff, self.sgcutoff = (float(np.float32(c)) for c in cutoffs)
+        self.rmt = np.ascontiguousarray(rmt, np.float64).ravel()
+        self.sites = sites  # dict(Z, B, occ, orbit [n][3])
+        self.nref = len(self.hkl)
+        self.nat = int(sum(len(s["orbit"]) for s in sites))
+
+
+def select_beams_fn(cr, kvec, fn):
+    """orc_select_beams_fn: the beam sets with the foil normal fn (None: FN = k)."""
+    n = cr.nref
+    st = np.zeros(n, np.int32); ssg = np.zeros(n, np.float32)
+    wk = np.zeros(n, np.int32); wsg = np.zeros(n, np.float32)
+    ns = C.c_int32(); nw = C.c_int32()
+    kv = np.ascontiguousarray(kvec, np.float64)
+    f = None if fn is None else _p(np.ascontiguousarray(fn, np.float32))
+    L = lib()
+    L.orc_select_beams_fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_float,
+                                      C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_select_beams_fn(n, _p(cr.hkl), _p(cr.ucg), _p(cr.dbdiff_ref), cr.mlambda, cr.weakcutoff, cr.cutoff,
+                          cr.sgcutoff, _p(cr.rmt), _p(kv), f, _p(st), _p(ssg), C.byref(ns), _p(wk), _p(wsg),
+                          C.byref(nw))
+    return st[:ns.value].copy(), ssg[:ns.value].copy(), wk[:nw.value].copy(), wsg[:nw.value].copy()
+
+
+def sgh_ecp(cr, strong):
+    """orc_sgh_ecp: the channeling structure matrix (all sites, occupancies) of the strong beams."""
+    n = len(strong)
+    out = np.zeros(2 * n * n)
+    s = np.ascontiguousarray(strong, np.int32)
+    z = np.array([st["Z"] for st in cr.sites], np.int32)
+    b = np.array([st["B"] for st in cr.sites], np.float32)
+    occ = np.array([st["occ"] for st in cr.sites], np.float32)
+    norb = np.array([len(st["orbit"]) for st in cr.sites], np.int32)
+    orb = np.ascontiguousarray(np.concatenate([np.asarray(st["orbit"], np.float64).ravel() for st in cr.sites]))
+    L = lib()
+    L.orc_sgh_ecp.argtypes = [C.c_int] + [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 6
+    L.orc_sgh_ecp(n, _p(s), _p(cr.hkl), _p(cr.rmt), len(cr.sites), _p(z), _p(b), _p(occ), _p(norb), _p(orb),
+                  _p(out))
+    return out.view(np.complex128).reshape(n, n)
+
+
+def matrix_exponential_pade(A, z0):
+    """MatrixExponential with 'Pade' (math.f90:187-315): the diagonal Pade approximant of degree q
+    with 2^j scaling, q and j from ilev = nint(log10(max |a_ij| z0)) + 3; the inverse of the
+    denominator through LAPACK (numpy's), then j squarings."""
+    q_tab, j_tab = (2, 3, 4, 4, 4, 4), (0, 0, 1, 5, 8, 11)
+    modA = float(np.max(np.abs(A) * z0))
+    if modA > 1000.0:
+        raise ValueError("MatrixExponential: ||A|| > 1000")
+    x = float(np.log10(modA)) if modA > 0 else -100.0
+    ilev = int(np.copysign(np.floor(abs(x) + 0.5), x)) + 3  # nint: halves away from zero
+    ilev = min(max(ilev, 1), 6)
+    q, j = q_tab[ilev - 1], j_tab[ilev - 1]
+    B = (A * z0) / 2.0 ** j
+    C_ = B.copy()
+    n = len(A)
+    N = np.eye(n, dtype=np.complex128)
+    D = N.copy()
+    pref, sgn = 1.0, -1.0
+    for icnt in range(1, q + 1):
+        pref = pref * float(q - icnt + 1) / float(icnt) / float(2 * q - icnt + 1)
+        N = N + pref * C_
+        D = D + sgn * pref * C_
+        sgn = -sgn
+        C_ = C_ @ B
+    E = np.linalg.inv(D) @ N
+    for _ in range(j):
+        E = E @ E
+    return E
+
+
+def ecp_lgh_scatmat(dm, mlambda, thick, zintstep, exact=False):
+    """CalcLghSM (CTEMECP.f90:559-618): Azz = expm(i pi mLambda zintstep DynMat) (the reference's
+    Pade rule, or with exact=True scipy's expm), psi_i = Azz^i e_1, Lgh(t) = sum_{i <= tvals(t)}
+    psi_i psi_i^H with tvals = nint(thick/zintstep) met in order; [nt][n][n]."""
+    import scipy.linalg
+    thick = np.asarray(thick, np.float32)
+    zs = np.float32(zintstep)
+    Minp = dm * complex(0.0, np.pi * mlambda)
+    Azz = scipy.linalg.expm(Minp * float(zs)) if exact else matrix_exponential_pade(Minp, float(zs))
+    n = len(dm)
+    nt = len(thick)
+    tvals = [int(np.floor(abs(float(v)) + 0.5) * np.sign(float(v))) for v in thick / zs]
+    numt = int(np.floor(float(np.max(thick) / zs) + 0.5))
+    psi = np.zeros(n, np.complex128)
+    psi[0] = 1.0
+    lsum = np.zeros((n, n), np.complex128)
+    out = np.zeros((nt, n, n), np.complex128)
+    tval = 0
+    for i in range(1, numt + 1):
+        psi = Azz @ psi
+        lsum = lsum + np.outer(psi, np.conj(psi))
+        if tval < nt and i == tvals[tval]:
+            out[tval] = lsum
+            tval += 1
+    return out
+
+
+def ecp_lgh_blochwv(dm, kn, thick, round_q=True):
+    """CalcLgh (CTEMECP.f90:472-534): the Bloch waves of DynMat (numpy's zgeev), W / fl32(2 kn),
+    Ijk(i,j) = conj(a_i) (1 - exp(-q)) / q a_j with a = CGinv(:, 1) and
+    q = 2 pi thick cmplx(Im W_i + Im W_j, Re W_i - Re W_j) -- the cmplx of default kind rounds both
+    parts to float32 (round_q=False keeps them in double) -- and Lgh = CG Ijk^T CG^H; [nt][n][n]."""
+    W, CG = np.linalg.eig(np.asarray(dm, np.complex128))
+    CGinv = np.linalg.inv(CG)
+    W = W / complex(float(np.float32(2.0 * kn)), 0.0)
+    a = CGinv[:, 0]
+    out = []
+    for t in np.asarray(thick, np.float32):
+        re = W.imag[:, None] + W.imag[None, :]
+        im = W.real[:, None] - W.real[None, :]
+        if round_q:
+            re = re.astype(np.float32).astype(np.float64)
+            im = im.astype(np.float32).astype(np.float64)
+        q = (2.0 * np.pi * float(t)) * (re + 1j * im)
+        Ijk = ((np.conj(a)[:, None] * (1.0 - np.exp(-q))) / q) * a[None, :]
+        out.append(CG @ Ijk.T @ np.conj(CG).T)
+    return np.array(out)
+
+
+def ecp_direction(cr, kvec, kn, fn, mode, thick, zintstep=1.0, exact=False):
+    """One incident direction of the channeling-pattern beam loop (CTEMECP.f90:292-369): the
+    beam sets with the foil normal, DynMat, Sgh, Lgh per thickness in `mode` ('Blochwv' or
+    'ScatMat'; exact=True: double-precision q, scipy's expm) and
+    sr(t) = fl32(real(sum(Lgh(t) * Sgh)) / float(sum(nat))); returns (sr float32 [nt], the double
+    sums, the strong beams, DynMat)."""
+    st, ssg, wk, wsg = select_beams_fn(cr, kvec, fn)
+    dm = dynmat(cr, st, ssg, wk, wsg)
+    S = sgh_ecp(cr, st)
+    if mode == "Blochwv":
+        L = ecp_lgh_blochwv(dm, kn, thick, round_q=not exact)
+    else:
+        L = ecp_lgh_scatmat(dm, cr.mlambda, thick, zintstep, exact=exact)
+    dots = np.array([np.sum(Lt * S).real for Lt in L])
+    sr = (dots / float(np.float32(cr.nat))).astype(np.float32)
+    return sr, dots, st, dm

@@ -332,3 +332,130 @@ def run_ebsd_program(work: str, nml: dict, clean: bool, timeout: int = 300) -> n
         off += 8 + n
     nx, ny, num = struct.unpack("<3i", recs[0])
     return np.stack([np.frombuffer(r, np.float32).copy() for r in recs[1:1 + num]]), (nx, ny, num)
+
+
+def run_ecp_program(work: str, xtal: dict, nml: dict, cutoffs=(4.0, 8.0, 0.05), timeout: int = 600) -> dict:
+    """Run CTEMECP in `work` on &ECPlist `nml` (xtalname is set here) with the crystal `xtal` and
+    the &BetheList cutoffs (weakcutoff, cutoff, sgcutoff); returns the output file's records
+    (CTEMECP.f90:385-449) and the DynMat of the first direction the program writes to
+    ECPmatrix.data (:301-306). The program's automatic arrays need an unlimited stack."""
+    write_xtal(os.path.join(work, "crystal.xtal"), xtal)
+    with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
+        f.write(" &BetheList\n weakcutoff = %r,\n cutoff = %r,\n sgcutoff = %r,\n /\n" % tuple(float(c) for c in cutoffs))
+
+    def fmt(v):
+        if isinstance(v, str):
+            return f"'{v}'"
+        if isinstance(v, (list, tuple)):
+            return ", ".join(repr(x) for x in v)
+        return repr(v)
+    body = dict(nml, xtalname="crystal.xtal", outname="ecp.data")
+    with open(os.path.join(work, "CTEMECP.nml"), "w") as f:
+        f.write(" &ECPlist\n" + "".join(f" {k} = {fmt(v)},\n" for k, v in body.items()) + " /\n")
+    exe = os.path.join(REF_BIN, "CTEMECP")
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec timeout -k 5 {int(timeout)} {exe} CTEMECP.nml"],
+                   cwd=work, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+    def records(path):
+        with open(path, "rb") as f:
+            data = f.read()
+        out, off = [], 0
+        while off < len(data):
+            (n,) = struct.unpack_from("<i", data, off)
+            out.append(data[off + 4: off + 4 + n])
+            off += 8 + n
+        return out
+    r = records(os.path.join(work, "ecp.data"))
+    nx, ny, nt = struct.unpack("<3i", r[2])
+    m = records(os.path.join(work, "ECPmatrix.data"))
+    (nn,) = struct.unpack("<i", m[0])
+    return dict(
+        mode=struct.unpack("<i", r[4])[0], thetac=struct.unpack("<f", r[8])[0], ktmax=struct.unpack("<f", r[9])[0],
+        numk=struct.unpack("<i", r[12])[0], ga=np.array(struct.unpack("<3i", r[14]), np.int32),
+        galen=struct.unpack("<f", r[15])[0], delta=struct.unpack("<f", r[16])[0],
+        gperp=np.frombuffer(r[17], np.float32).copy(), symlabels=np.frombuffer(r[18], np.int32).copy(),
+        thick0=np.frombuffer(r[19], np.float32).copy(),
+        sr=np.frombuffer(r[20], np.float32).reshape(nt, ny, nx).copy(),
+        dynmat=np.frombuffer(m[1], np.complex128).reshape(nn, nn, order="F").copy())
+
+
+def run_ecp_setup(work: str, xtal: dict, voltage: float, k, fn, dmin: float, thetac: float, npix: int,
+                  cutoffs=(4.0, 8.0, 0.05), ndyn: int = 1, ktmax: float = 0.0, timeout: int = 300) -> dict:
+    """Run oracle/_ref/ctemref_ecp (the channeling-pattern program's set-up, ref_ecp.f90) in `work`
+    and parse its dump: the foil normal, ga/gb, ktmax/thetac, the reflection list with its LUT,
+    the sites and orbits, every incident direction, and the beam sets and DynMat of the first
+    `ndyn` directions."""
+    write_xtal(os.path.join(work, "crystal.xtal"), xtal)
+    with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
+        f.write(" &BetheList\n weakcutoff = %r,\n cutoff = %r,\n sgcutoff = %r,\n /\n" % tuple(float(c) for c in cutoffs))
+    with open(os.path.join(work, "refecp.nml"), "w") as f:
+        f.write(" &refecp\n xtalname = 'crystal.xtal',\n voltage = %r,\n k = %d, %d, %d,\n fn = %d, %d, %d,\n"
+                " dmin = %r,\n thetac = %r,\n ktmax = %r,\n npix = %d,\n outname = 'ecpdump.bin',\n ndyn = %d,\n /\n"
+                % (float(voltage), *[int(v) for v in k], *[int(v) for v in fn], float(dmin), float(thetac),
+                   float(ktmax), int(npix), int(ndyn)))
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec timeout -k 5 {int(timeout)} "
+                    f"{os.path.join(REF_BIN, 'ctemref_ecp')}"], cwd=work, check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    with open(os.path.join(work, "ecpdump.bin"), "rb") as f:
+        buf = f.read()
+    off = 0
+
+    def take(fmt):
+        nonlocal off
+        v = struct.unpack_from("<" + fmt, buf, off)
+        off += struct.calcsize("<" + fmt)
+        return v
+
+    def arr(dtype, n):
+        nonlocal off
+        a = np.frombuffer(buf, dtype, n, off).copy()
+        off += a.nbytes
+        return a
+    magic, numk, nref, ntype, sgnum = take("5i")
+    assert magic == 20261
+    ga = np.array(take("3i"), np.int32)
+    gb = np.array(take("3i"), np.int32)
+    dims = take("3i")
+    lbs = take("3i")
+    ktmax_, thetac_, bragg, galen = take("4f")
+    dynfn = arr(np.float32, 3)
+    mlambda, dynupz, vol = take("3d")
+    dmt = arr(np.float64, 9).reshape(3, 3, order="F")
+    rmt = arr(np.float64, 9).reshape(3, 3, order="F")
+    ncell = dims[0] * dims[1] * dims[2]
+    lut = arr(np.complex128, ncell)
+    dbd = arr(np.int32, ncell)
+    hkl = np.zeros((nref, 3), np.int32)
+    ucg = np.zeros(nref, np.complex128)
+    for i in range(nref):
+        hkl[i] = take("3i")
+        ucg[i] = complex(*take("2d"))
+    sites = []
+    for _ in range(ntype):
+        z = take("i")[0]
+        pos = arr(np.float32, 5)
+        n = take("i")[0]
+        orb = arr(np.float64, 3 * n).reshape(3, n).T.copy()
+        sites.append(dict(Z=z, pos=pos, orbit=orb))
+    ij = np.zeros((numk, 2), np.int32)
+    kv = np.zeros((numk, 3))
+    kt = np.zeros((numk, 3))
+    kn = np.zeros(numk)
+    for i in range(numk):
+        ij[i] = take("2i")
+        kv[i] = take("3d")
+        kt[i] = take("3d")
+        kn[i] = take("d")[0]
+    dyn = []
+    for _ in range(min(ndyn, numk)):
+        nns, nnw = take("2i")
+        shkl = arr(np.int32, 3 * nns).reshape(nns, 3)
+        ssg = arr(np.float32, nns)
+        D = arr(np.complex128, nns * nns).reshape(nns, nns, order="F")
+        dyn.append(dict(nns=nns, nnw=nnw, hkl=shkl, sg=ssg, dynmat=D))
+    assert off == len(buf)
+    return dict(numk=numk, nref=nref, sgnum=sgnum, ga=ga, gb=gb, lut_dim=np.array(dims, np.int32),
+                lut_lo=np.array(lbs, np.int32), ktmax=ktmax_, thetac=thetac_, bragg=bragg, galen=galen,
+                dynfn=dynfn, mlambda=mlambda, dynupz=dynupz, vol=vol, dmt=dmt, rmt=rmt,
+                lut=lut.reshape(dims, order="F"), dbdiff=dbd.reshape(dims, order="F"), hkl=hkl, ucg=ucg,
+                sites=sites, ij=ij, k=kv, kt=kt, kn=kn, dyn=dyn)

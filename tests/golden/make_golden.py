@@ -229,6 +229,85 @@ def golden_ebsd() -> str:
     return fn
 
 
+GAAS = dict(system=1, abc=(0.56533, 0.56533, 0.56533), angles=(90.0, 90.0, 90.0), sg=216, sgset=1,
+            sites=[dict(Z=31, xyz=(0.0, 0.0, 0.0), occ=1.0, B=0.0060), dict(Z=33, xyz=(0.25, 0.25, 0.25), occ=0.95, B=0.0050)])
+
+# channeling-pattern set-ups (crystal, voltage [V], zone axis, foil normal, dmin, thetac, npix,
+# Bethe cutoffs) and runs (set-up, compmode, startthick, thickinc, numthick, zintstep)
+ECP_SETUPS = {
+    "s1": ("ni", 20000.0, (0, 0, 1), (0, 0, 1), 0.05, 120.0, 6, (4.0, 8.0, 0.05)),
+    "s2": ("gaas", 15000.0, (0, 1, 1), (0, 1, 2), 0.06, 90.0, 5, (20.0, 40.0, 0.05)),
+    "s3": ("ni", 20000.0, (1, 1, 2), (1, 1, 3), 0.05, 100.0, 3, (4.0, 8.0, 0.05)),
+}
+ECP_RUNS = {
+    "a": ("s1", "Blochwv", 2.0, 2.0, 10, 1.0),
+    "b": ("s1", "ScatMat", 2.0, 2.0, 10, 1.0),
+    "c": ("s2", "ScatMat", 3.0, 2.5, 4, 0.5),
+    "d": ("s2", "Blochwv", 3.0, 2.0, 6, 0.5),
+    "e": ("s3", "ScatMat", 0.1, 1.0, 5, 0.25),   # nint(0.1/0.25) = 0: no thickness is ever stored
+    "f": ("s3", "Blochwv", 0.5, 7.5, 3, 1.0),
+}
+
+
+def golden_ecp() -> str:
+    """The channeling-pattern program (src_programs/CTEMECP.f90, oracle/_ref/CTEMECP) on three
+    set-ups -- fcc Ni [001], zinc-blende GaAs [011] with foil normal [012] and 95% As occupancy,
+    Ni [112] with foil normal [113] -- in both solution modes: the output file's records (sr on
+    the disk's directions, ga, ktmax, galen, delta, gperp, the symmetry labels) and the first
+    direction's DynMat (ECPmatrix.data); and the set-up driver (oracle/ref_ecp.f90) dump: foil
+    normal, ga, convergence, reflection list with its LUT, sites, and every incident direction."""
+    from oracle.refrun import run_ecp_program, run_ecp_setup
+    from ctemsoft_amd import workloads
+    xtals = dict(ni=workloads.get("ni_c0").xtal, gaas=GAAS)
+    out = {}
+    setups = {}
+    with tempfile.TemporaryDirectory(prefix="ctemecp_") as work:
+        for key, (xn, volt, k, fn, dmin, th, npix, cut) in ECP_SETUPS.items():
+            d = run_ecp_setup(work, xtals[xn], volt, k, fn, dmin, th, npix, cutoffs=cut, ndyn=1)
+            setups[key] = d
+            out[f"{key}_xtal"] = np.frombuffer(repr(xtals[xn]).encode(), np.uint8)
+            out[f"{key}_params"] = np.array([volt, dmin, th, npix, *cut], np.float64)
+            out[f"{key}_zone"] = np.array(k, np.int32)
+            out[f"{key}_fn"] = np.array(fn, np.int32)
+            for f in ("ga", "dynfn", "hkl", "lut_lo", "ij", "kt", "kn", "dmt", "rmt"):
+                out[f"{key}_{f}"] = np.asarray(d[f])
+            out[f"{key}_kvec"] = np.asarray(d["k"])
+            out[f"{key}_conv"] = np.array([d["ktmax"], d["thetac"], d["bragg"], d["galen"]], np.float32)
+            out[f"{key}_scalars"] = np.array([d["mlambda"], d["dynupz"], d["vol"]], np.float64)
+            out[f"{key}_ucg"] = d["ucg"].astype(np.complex64)
+            out[f"{key}_lut"] = d["lut"].astype(np.complex64)          # Fortran LUT(h,k,l), float32 values
+            out[f"{key}_dbdiff"] = d["dbdiff"].astype(np.uint8)
+            for i, st in enumerate(d["sites"]):
+                out[f"{key}_site{i}_Z"] = np.int32(st["Z"])
+                out[f"{key}_site{i}_pos"] = st["pos"]
+                out[f"{key}_site{i}_orbit"] = st["orbit"]
+            out[f"{key}_nsite"] = np.int32(len(d["sites"]))
+            dy = d["dyn"][0]
+            out[f"{key}_dyn0_hkl"] = dy["hkl"]
+            out[f"{key}_dyn0_sg"] = dy["sg"]
+            out[f"{key}_dyn0_nnw"] = np.int32(dy["nnw"])
+            out[f"{key}_dyn0_dynmat"] = dy["dynmat"]
+        for run, (key, mode, t0, dt, nt, zint) in ECP_RUNS.items():
+            xn, volt, k, fn, dmin, th, npix, cut = ECP_SETUPS[key]
+            nml = dict(voltage=volt, k=list(k), fn=list(fn), dmin=dmin, thetac=th, startthick=t0, thickinc=dt,
+                       numthick=nt, npix=npix, compmode=mode, zintstep=zint)
+            r = run_ecp_program(work, xtals[xn], nml, cutoffs=cut)
+            d = setups[key]
+            assert r["numk"] == d["numk"] and np.array_equal(r["ga"], d["ga"])
+            assert np.array_equal(r["dynmat"], d["dyn"][0]["dynmat"])
+            ij = d["ij"]
+            # sr on the disk's directions only: the program leaves the pixels outside unassigned
+            out[f"{run}_sr"] = np.stack([r["sr"][:, j + npix, i + npix] for i, j in ij], axis=1)
+            out[f"{run}_run"] = np.frombuffer(repr(dict(setup=key, compmode=mode, startthick=t0, thickinc=dt,
+                                                         numthick=nt, zintstep=zint)).encode(), np.uint8)
+            out[f"{run}_header"] = np.array([r["mode"], r["numk"]], np.int32)
+            out[f"{run}_file"] = np.array([r["thetac"], r["ktmax"], r["galen"], r["delta"], *r["gperp"]], np.float32)
+            out[f"{run}_symlabels"] = r["symlabels"]
+    fn = os.path.join(HERE, "ecp.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -270,6 +349,8 @@ def main():
             print(golden_sym())
         elif w == "ebsd":
             print(golden_ebsd())
+        elif w == "ecp":
+            print(golden_ecp())
         else:
             raise SystemExit(f"unknown workload {w}")
 
