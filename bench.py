@@ -157,6 +157,8 @@ def main():
 
     if a.workload == "ebsd_det640":
         return bench_detector(a)
+    if a.workload == "ecp_ni30":
+        return bench_ecp(a)
     wl, jobs = load_jobs(a.workload)
     tt, ts = (int(v) for v in a.terms.split(","))
     # one plan per energy bin, each holding its bin's crystal, all resident before timing; the
@@ -410,6 +412,111 @@ def bench_detector(a):
     }
     print(json.dumps(line), flush=True)
     det.close()
+
+
+# ---- electron channeling patterns (include/ctem_ecp.h) -----------------------------------------
+
+# the reference's own template (templatefolder/CTEMECP.template: 30 kV, dmin 0.025, ktmax 5,
+# npix 256, ten thicknesses 2..20 nm, Blochwv) with the SEM Bethe cutoffs
+# (BetheParametersSEM.template) on fcc Ni, zone axis and foil normal [001]
+ECP_NML = dict(voltage=30000.0, k=[0, 0, 1], fn=[0, 0, 1], dmin=0.025, ktmax=5.0, startthick=2.0, thickinc=2.0,
+               numthick=10, npix=256, compmode="Blochwv", zintstep=1.0)
+ECP_CUTOFFS = (30.0, 60.0, 0.05)
+
+
+def ecp_cpu_baseline(xtal, seconds=20.0):
+    """The reference program itself (oracle/_ref/CTEMECP: CTEMECP.f90 built from its sources) on a
+    bounded disk of the same workload (npix grown until the run takes about `seconds` / 4..
+    `seconds`), one core (the program's beam loop is serial); directions per second of the whole
+    run, set-up and file output included."""
+    import subprocess
+    import tempfile
+    from oracle.refrun import REF_BIN, run_ecp_program
+    if not os.path.exists(os.path.join(REF_BIN, "CTEMECP")):
+        return None
+    npix = 2
+    with tempfile.TemporaryDirectory(prefix="ctemecp_") as work:
+        while True:
+            nml = dict(ECP_NML, npix=npix)
+            t0 = time.perf_counter()
+            try:
+                r = run_ecp_program(work, xtal, nml, cutoffs=ECP_CUTOFFS, timeout=300)
+            except (subprocess.CalledProcessError, OSError):
+                return None
+            dt = time.perf_counter() - t0
+            if dt > seconds / 4 or npix >= 64:
+                break
+            npix = min(64, int(npix * max(1.5, min(4.0, (seconds / 2 / max(dt, 1e-3)) ** 0.5))))
+    return {"value": round(r["numk"] / dt, 3), "unit": "directions/s", "cores": 1, "kind": "reference",
+            "sample": f"the reference program CTEMECP on the same workload with npix {npix} ({r['numk']} "
+                      f"directions, set-up and file output included), {dt:.1f} s"}
+
+
+def bench_ecp(a):
+    import torch  # noqa: F401
+    from ctemsoft_amd import ebsd, ecp, workloads, xtal as xtal_mod
+    x = workloads.get("ni_c0").xtal
+    nml = ECP_NML
+    # the program's set-up (host, outside the timed step, as the per-crystal set-up of the master bench)
+    st0, v0 = xtal_mod.native_setup(x, nml["voltage"] / 1000.0, nml["dmin"], ECP_CUTOFFS, ecp=(nml["voltage"], nml["k"], 0.0))
+    dmt, rmt, mlambda = v0["dmt"], v0["rmt"], v0["mLambda"]
+    dynfn = ecp.foil_normal(nml["fn"], dmt, rmt)
+    ga = ecp.shortest_g(nml["k"], dmt, rmt)
+    thetac, ktmax, bragg, galen = ecp.convergence(ga, rmt, mlambda, 0.0, nml["ktmax"])
+    st, _ = xtal_mod.native_setup(x, nml["voltage"] / 1000.0, nml["dmin"], ECP_CUTOFFS,
+                                  ecp=(nml["voltage"], nml["k"], np.float32(thetac) / np.float32(1000.0)))
+    kv = ecp.kvectors(nml["k"], ga, ktmax, nml["npix"], dmt, rmt, mlambda)
+    ctx = ebsd.Context(0)
+    ctx.upload_crystal(st)
+    occ = [s["occ"] for s in x["sites"]]
+    numk = len(kv["kn"])
+    res = {}
+    for mode in ("Blochwv", "ScatMat"):
+        p = ecp.EcpPlan(ctx, kv["k"], kv["kn"], mode, nml["startthick"], nml["thickinc"], nml["numthick"],
+                        nml["zintstep"], dynfn, occ)
+        ns, nw = p.select()
+        for _ in range(a.warmup):
+            p.select()
+            p.run()
+        t0 = time.perf_counter()
+        tm = []
+        for _ in range(a.steps):
+            p.select()  # beam selection is part of the step
+            p.run()
+            tm.append(p.timing())
+        wall = (time.perf_counter() - t0) / a.steps
+        sr = p.sr()
+        assert np.all(np.isfinite(sr)) and np.all(sr > 0)
+        res[mode] = dict(wall=wall, gemm_ms=float(np.mean([t["gemm_ms"] for t in tm])),
+                         gemm_flops=tm[-1]["gemm_flops"], total_ms=float(np.mean([t["total_ms"] for t in tm])),
+                         ns=ns, nw=nw)
+        log(f"{mode}: {numk} directions, strong {ns.min()}..{ns.max()} (mean {ns.mean():.1f}), weak mean "
+            f"{nw.mean():.1f}; {wall * 1e3:.1f} ms per step, run {res[mode]['total_ms']:.1f} ms, GEMM "
+            f"{res[mode]['gemm_ms']:.1f} ms / {res[mode]['gemm_flops'] / 1e12:.3f} TFLOP")
+        p.close()
+    ctx.close()
+    b = res["Blochwv"]
+    ach = b["gemm_flops"] / (b["gemm_ms"] * 1e-3) / 1e12
+    cpu = None if a.no_cpu else ecp_cpu_baseline(x)
+    line = {
+        "metric": "ECP directions/s, 1 GPU (Blochwv, intensities at 10 thicknesses; beam selection included)",
+        "value": round(numk / b["wall"], 1), "unit": "directions/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(b["wall"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: the reference's own ECP template on fcc Ni",
+        "config": {"workload": f"ecp_ni30: fcc Ni, 30 kV, zone axis and foil normal [001], dmin 0.025 nm, ktmax 5 "
+                               f"(thetac {thetac:.1f} mrad), npix 256 ({numk} directions), thicknesses 2..20 nm "
+                               f"(10), Bethe cutoffs 30/60/0.05",
+                   "directions": numk, "scatmat_directions_per_s": round(numk / res["ScatMat"]["wall"], 1),
+                   "scatmat_ms_per_step": round(res["ScatMat"]["wall"] * 1e3, 3)},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(ach / 78.6, 4), "traffic": None, "kernel": "k_zgemm",
+                     "alg_flops_per_step": b["gemm_flops"], "kernel_ms_per_step": round(b["gemm_ms"], 3),
+                     "kernel_ms_source": "HIP events around the GEMM launches on the context's stream",
+                     "note": "FP64 complex products of the block-matrix exponentials (2n = 2 x strong beams), "
+                             "8 (2n)^3 flops per product on the true sizes"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":
