@@ -18,7 +18,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libctem_ebsd.so")
 SOURCES = ["ctem_kernels.hip", "ctem_mixed.hip", "ctem_split.hip", "ctem_adjoint.hip", "ctem_api.hip", "ctem_detector.hip",
-           "ctem_ecp.hip",
+           "ctem_ecp.hip", "ctem_mc.hip",
            "ctem_setup.hip"]
 ARCH = os.environ.get("CTEM_OFFLOAD_ARCH", "gfx950")
 

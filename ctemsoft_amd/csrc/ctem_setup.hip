@@ -1168,6 +1168,26 @@ extern "C" int32_t ctem_setup_info(const ctem_setup* T, ctem_setup_values* v) {
   return CTEM_OK;
 }
 
+extern "C" int32_t ctem_setup_density(const ctem_setup* T, float* dens, float* avZ, float* avA) {
+#pragma clang fp contract(off)
+  if (!T || !dens || !avZ || !avA) return setup_fail(CTEM_E_ARG, "ctem_setup_density: null argument");
+  const State& S = T->S;
+  // CalcDensity (crystal.f90:1388-1417): numat are the orbit sizes of CalcPositions, float32 sums
+  float aw = 0.0f, z = 0.0f;
+  int nat = 0;
+  for (int p = 0; p < S.ntype; p++) nat += T->site_norbit[p];
+  for (int p = 0; p < S.ntype; p++) {
+    const float n = (float)T->site_norbit[p];
+    aw = aw + (n * kAtomWeights[S.atom_type[p] - 1]) * S.atom_pos[5 * p + 3];
+    z = z + n * (float)S.atom_type[p];
+  }
+  if (nat == 0) return setup_fail(CTEM_E_ARG, "ctem_setup_density: no atoms");
+  *avA = aw / (float)nat;
+  *avZ = z / (float)nat;
+  *dens = aw / (float)(S.vol * 1.0e-21 * 6.0221413e23);  // cAvogadro of constants.f90:75
+  return CTEM_OK;
+}
+
 extern "C" int32_t ctem_setup_destroy(ctem_setup* T) {
   delete T;
   return CTEM_OK;

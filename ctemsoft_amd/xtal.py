@@ -29,7 +29,7 @@ class _Values(C.Structure):
 
 
 EXPORTS = ["ctem_setup_create", "ctem_setup_create_gpu", "ctem_setup_create_ecp", "ctem_setup_crystal",
-           "ctem_setup_info", "ctem_setup_destroy", "ctem_setup_last_error"]
+           "ctem_setup_density", "ctem_setup_info", "ctem_setup_destroy", "ctem_setup_last_error"]
 
 _BOUND = False
 
@@ -41,7 +41,7 @@ def _lib():
         P, I32, F = C.c_void_p, C.c_int32, C.c_float
         for name, args in {"ctem_setup_create": [P, F, F, F, F, F, P],
                            "ctem_setup_create_gpu": [I32, P, F, F, F, F, F, P, P], "ctem_setup_crystal": [P, P],
-                           "ctem_setup_create_ecp": [P, F, F, P, F, F, F, F, P],
+                           "ctem_setup_create_ecp": [P, F, F, P, F, F, F, F, P], "ctem_setup_density": [P, P, P, P],
                            "ctem_setup_info": [P, P], "ctem_setup_destroy": [P]}.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -111,5 +111,47 @@ def native_setup(xtal: dict, ekev: float, dmin: float, cutoffs=(80.0, 160.0, 0.0
         st = ebsd.Setup(hkl, ucg, dbd, lut, vals["lut_lo"], v.mlambda, v.dynupz, cr.weakcutoff, cr.cutoff,
                         cr.sgcutoff, vals["rmt"], sites, dsm=vals["dsm"], isym=v.isym, usehex=v.usehex)
         return st, vals
+    finally:
+        L.ctem_setup_destroy(h)
+
+
+def read_xtal(path: str) -> dict:
+    """The .xtal crystal file (files.f90:271-277, two Fortran sequential records: crystal system
+    and lattice parameters; ATOM_type(250), ATOM_pos(250,5) float32, ntype, SGnum, SGset) as the
+    dict that native_setup takes."""
+    import struct
+    with open(path, "rb") as f:
+        data = f.read()
+    recs, off = [], 0
+    while off < len(data):
+        (n,) = struct.unpack_from("<i", data, off)
+        recs.append(data[off + 4: off + 4 + n])
+        off += 8 + n
+    if len(recs) < 2 or len(recs[0]) != 52:
+        raise ValueError(f"{path}: not a .xtal file")
+    system, *lat = struct.unpack("<i6d", recs[0])
+    types = np.frombuffer(recs[1], np.int32, 250, 0)
+    pos = np.frombuffer(recs[1], np.float32, 1250, 1000).reshape(5, 250).T
+    ntype, sg, sgset = struct.unpack_from("<3i", recs[1], 6000)
+    sites = [dict(Z=int(types[i]), xyz=tuple(float(v) for v in pos[i, :3]), occ=float(pos[i, 3]), B=float(pos[i, 4]))
+             for i in range(ntype)]
+    return dict(system=int(system), abc=tuple(lat[:3]), angles=tuple(lat[3:]), sg=int(sg), sgset=int(sgset),
+                sites=sites)
+
+
+def density(xtal: dict):
+    """CalcDensity (crystal.f90:1388-1417) through the native set-up: (dens, avZ, avA) float32"""
+    L = _lib()
+    types = np.array([s["Z"] for s in xtal["sites"]], np.int32)
+    pos = np.array([[*s["xyz"], s["occ"], s["B"]] for s in xtal["sites"]], np.float32)
+    x = _Xtal(int(xtal["system"]), *[float(v) for v in xtal["abc"]], *[float(v) for v in xtal["angles"]],
+              int(xtal["sg"]), int(xtal.get("sgset", 1)), len(types), types.ctypes.data, pos.ctypes.data)
+    h = C.c_void_p()
+    # the density needs the cell and the orbits only: a coarse dmin keeps the reflection list tiny
+    _check(L.ctem_setup_create(C.byref(x), 20.0, 10.0, 4.0, 8.0, 0.05, C.byref(h)))
+    try:
+        d, z, a = C.c_float(), C.c_float(), C.c_float()
+        _check(L.ctem_setup_density(h, C.byref(d), C.byref(z), C.byref(a)))
+        return np.float32(d.value), np.float32(z.value), np.float32(a.value)
     finally:
         L.ctem_setup_destroy(h)

@@ -78,6 +78,10 @@ CTEM_API int32_t ctem_setup_create_ecp(const ctem_xtal* x, float voltage, float 
  * ctem_upload_crystal. */
 CTEM_API int32_t ctem_setup_crystal(const ctem_setup* s, ctem_crystal* out);
 CTEM_API int32_t ctem_setup_info(const ctem_setup* s, ctem_setup_values* out);
+/* CalcDensity (crystal.f90:1388-1417) of the setup's crystal, float32 as the reference returns them:
+ * density [g/cm^3], average atomic number and average atomic weight [g/mol] over the atoms of the
+ * cell (occupancy-weighted weights), as the Monte Carlo program takes them (CTEMMC.f90:191-194). */
+CTEM_API int32_t ctem_setup_density(const ctem_setup* s, float* dens, float* avZ, float* avA);
 CTEM_API int32_t ctem_setup_destroy(ctem_setup* s);
 /* Message of the last failed ctem_setup_* call of this thread. */
 CTEM_API const char* ctem_setup_last_error(void);

@@ -928,3 +928,199 @@ void orc_bin_pattern(int numsx, int numsy, int binning, int gamma, float gammava
   if (gamma)
     for (int64_t e = 0; e < (int64_t)bx * by; e++) binned[e] = powf(binned[e], gammavalue);
 }
+
+/* ---------------- Monte Carlo electron trajectories (CTEMMC.f90) --------------------------- */
+
+/* the generator of rng.f95: a lag-3 subtraction modulo 2147483579 (s1 - s3) plus the LCG
+ * 69069 s4 + 1013904243 (32-bit wrap), u = fl32(0.5 + 0.23283064e-9 (imz + s4)) */
+
+void orc_mcrng_seed(orc_mcrng* r, int32_t seed) {
+  r->s[0] = seed; r->s[1] = 362436069; r->s[2] = 16163801; r->s[3] = 1131199299;
+}
+
+float orc_mcrng_uniform(orc_mcrng* r) {
+  int32_t imz = r->s[0] - r->s[2];
+  if (imz < 0) imz += 2147483579;
+  r->s[0] = r->s[1];
+  r->s[1] = r->s[2];
+  r->s[2] = imz;
+  r->s[3] = (int32_t)(69069u * (uint32_t)r->s[3] + 1013904243u);
+  imz = (int32_t)((uint32_t)imz + (uint32_t)r->s[3]);
+  return (float)(0.5 + 0.23283064e-9 * (double)imz);
+}
+
+/* jump of the generator by n draws: the lag-3 recurrence x_k = x_{k-3} - x_{k-1} (mod m) as a 3x3
+ * matrix power, the LCG as an affine map power */
+#define ORC_MCM 2147483579LL
+static void m3mul(const int64_t a[9], const int64_t b[9], int64_t c[9]) {
+  int64_t t[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      int64_t s = 0;
+      for (int k = 0; k < 3; k++) s = (s + a[3 * i + k] * b[3 * k + j]) % ORC_MCM;
+      t[3 * i + j] = s;
+    }
+  memcpy(c, t, sizeof t);
+}
+void orc_mcrng_jump(orc_mcrng* r, uint64_t n) {
+  /* state vector (s1, s2, s3) = (x_{k-2}, x_{k-1}, x_k): next = (s2, s3, s1 - s3) */
+  int64_t P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, B[9] = {0, 1, 0, 0, 0, 1, 1, 0, ORC_MCM - 1};
+  uint32_t A = 1, C = 0, a = 69069u, c = 1013904243u;
+  while (n) {
+    if (n & 1) {
+      m3mul(B, P, P);
+      C = a * C + c;
+      A = a * A;
+    }
+    m3mul(B, B, B);
+    c = a * c + c;
+    a = a * a;
+    n >>= 1;
+  }
+  int64_t v[3] = {r->s[0], r->s[1], r->s[2]}, w[3];
+  for (int i = 0; i < 3; i++) w[i] = ((P[3 * i] * v[0]) % ORC_MCM + (P[3 * i + 1] * v[1]) % ORC_MCM + (P[3 * i + 2] * v[2]) % ORC_MCM) % ORC_MCM;
+  for (int i = 0; i < 3; i++) r->s[i] = (int32_t)w[i];
+  r->s[3] = (int32_t)(A * (uint32_t)r->s[3] + C);
+}
+
+static double orc_sum3_kahan_d(double a, double b, double c) {
+  const double v[3] = {a, b, c};
+  double s = 0.0, comp = 0.0;
+  for (int i = 0; i < 3; i++) {
+    const double y = v[i] - comp;
+    const double t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  }
+  return s;
+}
+
+int64_t orc_mc_block = 16384; /* draws per electron in the blocked mode (>= 1 + 3 * 5000) */
+
+/* single_run (CTEMMC.f90:304-526) for num_el electrons. mode 0: one stream seeded primeseed + 1,
+ * electrons in order (the program); mode 1: electron e draws from that stream's block starting at
+ * e * 16384. kahan: the normalisation's SUM compensated (the Fortran runtime's). accum_e
+ * [numsy][numsx][numEbins], accum_z [nz][nz][numzbins][numEbins] (energy fastest), added to. Returns
+ * the number of hits that fell outside the energy bins or the depth map's pixel box (the program
+ * writes past its arrays there: an energy window that is not a whole number of bins, or
+ * (numsx - 1) / 2 mod 10 >= 5). */
+int64_t orc_mc_run(double sig, double omega, int32_t numsx, int32_t primeseed, int64_t num_el, double EkeV,
+                   double Ehistmin, double Ebinsize, double depthmax, double depthstep, double Ze, double density,
+                   double at_wt, int mode, int kahan, int32_t* accum_e, int32_t* accum_z) {
+  const int numEbins = (int)((EkeV - Ehistmin) / Ebinsize) + 1;
+  const int numzbins = (int)(depthmax / depthstep) + 1;
+  const int nx = (numsx - 1) / 2, nzx = nx / 10, nzw = 2 * nzx + 1;
+  const double scaled = 1.0e8, min_energy = 1.0, presig = 1.5273987e19;
+  const double cPi = 3.141592653589, cAvogadro = 6.0221415e23, cDtoR = 0.017453293;
+  const double dpi = 4.0 * atan(1.0);
+  const double sPio2 = sqrt(dpi * 0.5), sPi2 = sqrt(dpi) * 0.5;
+  const double Emin = Ehistmin - Ebinsize / 2.0;
+  const double pre = at_wt / cAvogadro / density;
+  const double prealpha = 3.4e-3 * pow(Ze, (double)0.67f);
+  double J = (9.76 * Ze + 58.5 / pow(Ze, 0.19)) * 1.0e-3 / (double)1.166f;
+  J = 1.0 / J;
+  const double predEds = -78500.0 * density * Ze / at_wt;
+  const int num = 5000;
+  const double tpi = 2.0 * cPi;
+  const double tano = tan(omega * cDtoR);
+  const double delta = (double)nx / sPio2;
+  const double cxstart = cos((90.0 - sig) * cDtoR);
+  const double czstart = -sin((90.0 - sig) * cDtoR);
+  int64_t dropped = 0;
+  orc_mcrng r0, r;
+  orc_mcrng_seed(&r0, primeseed + 1);
+  r = r0;
+  for (int64_t el = 0; el < num_el; el++) {
+    if (mode == 1) {
+      r = r0;
+      orc_mcrng_jump(&r, (uint64_t)el * (uint64_t)orc_mc_block);
+    }
+    double Ec = EkeV;
+    double xyz[3] = {0.0, 0.0, 0.0};
+    double alpha = prealpha / Ec;
+    double step = Ec * (Ec + 1024.0) / Ze / (Ec + 511.0);
+    double sige = presig * step * step * alpha * (1.0 + alpha);
+    double lambda = pre * sige;
+    double rr = (double)orc_mcrng_uniform(&r);
+    step = -lambda * log(rr);
+    double cxyz[3] = {cxstart, 0.0, czstart};
+    for (int i = 0; i < 3; i++) xyz[i] = xyz[i] + step * scaled * cxyz[i];
+    int traj = 0;
+    while (traj < num) {
+      const double dEds = predEds * log(Ec * J + 1.0) / Ec;
+      const double dE = dEds * step;
+      Ec = Ec + dE;
+      if (Ec < min_energy) break;
+      rr = (double)orc_mcrng_uniform(&r);
+      const double cphi = 1.0 - 2.0 * alpha * rr / (1.0 + alpha - rr);
+      const double sphi = sin(acos(cphi));
+      rr = (double)orc_mcrng_uniform(&r);
+      const double psi = tpi * rr;
+      const double spsi = sin(psi), cpsi = cos(psi);
+      double cxyzp[3];
+      if (fabs(cxyz[2]) > 0.99999) {
+        cxyzp[0] = sphi * cpsi;
+        cxyzp[1] = sphi * spsi;
+        cxyzp[2] = (cxyz[2] / fabs(cxyz[2])) * cphi;
+      } else {
+        const double dsq = sqrt(1.0 - cxyz[2] * cxyz[2]);
+        const double dsqi = 1.0 / dsq;
+        cxyzp[0] = sphi * (cxyz[0] * cxyz[2] * cpsi - cxyz[1] * spsi) * dsqi + cxyz[0] * cphi;
+        cxyzp[1] = sphi * (cxyz[1] * cxyz[2] * cpsi + cxyz[0] * spsi) * dsqi + cxyz[1] * cphi;
+        cxyzp[2] = -sphi * cpsi * dsq + cxyz[2] * cphi;
+      }
+      const double q0 = cxyzp[0] * cxyzp[0], q1 = cxyzp[1] * cxyzp[1], q2 = cxyzp[2] * cxyzp[2];
+      const double ss = kahan ? orc_sum3_kahan_d(q0, q1, q2) : (q0 + q1) + q2;
+      const double dd = 1.0 / sqrt(ss);
+      for (int i = 0; i < 3; i++) cxyzp[i] = cxyzp[i] * dd;
+      alpha = prealpha / Ec;
+      step = Ec * (Ec + 1024.0) / Ze / (Ec + 511.0);
+      sige = presig * step * step * alpha * (1.0 + alpha);
+      lambda = pre * sige;
+      rr = (double)orc_mcrng_uniform(&r);
+      step = -lambda * log(rr);
+      double xyzn[3];
+      for (int i = 0; i < 3; i++) xyzn[i] = xyz[i] + step * scaled * cxyzp[i];
+      const double znmax = xyzn[1] * tano;
+      if (xyzn[2] > znmax) {
+        /* Lambert2DSquareInverseDouble (Lambert.f90:452-483) times delta */
+        double res0 = 0.0, res1 = 0.0;
+        const double s2 = (cxyzp[0] * cxyzp[0] + cxyzp[1] * cxyzp[1]) + cxyzp[2] * cxyzp[2];
+        if (!(fabs(1.0 - s2) > 1.0e-12) && fabs(cxyzp[2]) != 1.0) {
+          if (fabs(cxyzp[1]) <= fabs(cxyzp[0])) {
+            const double q = fabs(cxyzp[0]) / cxyzp[0] * sqrt(2.0 * (1.0 - cxyzp[2]));
+            res0 = q * sPi2;
+            res1 = q * atan(cxyzp[1] / cxyzp[0]) / sPi2;
+          } else {
+            const double q = fabs(cxyzp[1]) / cxyzp[1] * sqrt(2.0 * (1.0 - cxyzp[2]));
+            res0 = q * atan(cxyzp[0] / cxyzp[1]) / sPi2;
+            res1 = q * sPi2;
+          }
+        }
+        const double dx0 = delta * res0, dx1 = delta * res1;
+        const long ix = lround(dx1), iy = lround(-dx0);
+        if (labs(ix) <= nx && labs(iy) <= nx && Ec > Emin) {
+          const long iE = lround((Ec - Ehistmin) / Ebinsize) + 1;
+          if (iE < 1 || iE > numEbins) {
+            dropped++;
+          } else {
+            const double edis = fabs(xyz[2] / cxyzp[2]);
+            const long iz = lround(edis * 0.1 / depthstep) + 1;
+            if (iz > 0 && iz <= numzbins) {
+              const long px = lroundf((float)ix / 10.0f), py = lroundf((float)iy / 10.0f);
+              if (labs(px) <= nzx && labs(py) <= nzx)
+                accum_z[(((py + nzx) * nzw + (px + nzx)) * numzbins + (iz - 1)) * numEbins + (iE - 1)] += 1;
+              else
+                dropped++;
+            }
+            accum_e[((iy + nx) * numsx + (ix + nx)) * numEbins + (iE - 1)] += 1;
+          }
+        }
+        break;
+      }
+      for (int i = 0; i < 3; i++) { cxyz[i] = cxyzp[i]; xyz[i] = xyzn[i]; }
+      traj++;
+    }
+  }
+  return dropped;
+}

@@ -114,6 +114,18 @@ void orc_bin_pattern(int numsx, int numsy, int binning, int gamma, float gammava
 /* Euler angles (radians, Bunge) to a quaternion, eu2qu (rotations.f90:1322-1351), float32 */
 void orc_eu2qu(const float e[3], float q[4]);
 
+/* ---- Monte Carlo (CTEMMC.f90) ---- */
+/* the generator of rng.f95 (seed: rng_seed; uniform: rng_uniform, float32) and its jump by n draws */
+typedef struct { int32_t s[4]; } orc_mcrng;
+void orc_mcrng_seed(orc_mcrng* r, int32_t seed);
+float orc_mcrng_uniform(orc_mcrng* r);
+void orc_mcrng_jump(orc_mcrng* r, uint64_t n);
+/* single_run (CTEMMC.f90:304-526): mode 0 the program's one stream, mode 1 one block of 16384
+ * draws of that stream per electron; kahan: compensated SUM in the direction normalisation */
+int64_t orc_mc_run(double sig, double omega, int32_t numsx, int32_t primeseed, int64_t num_el, double EkeV,
+                   double Ehistmin, double Ebinsize, double depthmax, double depthstep, double Ze, double density,
+                   double at_wt, int mode, int kahan, int32_t* accum_e, int32_t* accum_z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -400,3 +400,43 @@ def ecp_direction(cr, kvec, kn, fn, mode, thick, zintstep=1.0, exact=False):
     dots = np.array([np.sum(Lt * S).real for Lt in L])
     sr = (dots / float(np.float32(cr.nat))).astype(np.float32)
     return sr, dots, st, dm
+
+
+# ----------------------------------------------------------------------------------------
+# Monte Carlo electron trajectories (src_programs/CTEMMC.f90)
+
+class _McRng(C.Structure):
+    _fields_ = [("s", C.c_int32 * 4)]
+
+
+def mc_rng_stream(seed, n, jump=0):
+    """n draws of rng.f95's generator seeded with `seed` (rng_seed), after a jump of `jump` draws"""
+    L = lib()
+    L.orc_mcrng_uniform.restype = C.c_float
+    r = _McRng()
+    L.orc_mcrng_seed(C.byref(r), C.c_int32(seed))
+    if jump:
+        L.orc_mcrng_jump.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_mcrng_jump(C.byref(r), jump)
+    return np.array([L.orc_mcrng_uniform(C.byref(r)) for _ in range(n)], np.float32), tuple(r.s)
+
+
+def mc_run(p: dict, mode=0, kahan=True):
+    """orc_mc_run on the &MCdata values p (sig, omega, numsx, primeseed, num_el, EkeV, Ehistmin,
+    Ebinsize, depthmax, depthstep) and CalcDensity's (Ze, density, at_wt, double): accum_e
+    [numsx][numsx][numEbins], accum_z [nz][nz][numzbins][numEbins] int32, dropped hits"""
+    L = lib()
+    D = C.c_double
+    L.orc_mc_run.restype = C.c_int64
+    L.orc_mc_run.argtypes = [D, D, C.c_int32, C.c_int32, C.c_int64, D, D, D, D, D, D, D, D, C.c_int, C.c_int,
+                             C.c_void_p, C.c_void_p]
+    nE = int((p["EkeV"] - p["Ehistmin"]) / p["Ebinsize"]) + 1
+    nzb = int(p["depthmax"] / p["depthstep"]) + 1
+    nx = (p["numsx"] - 1) // 2
+    nzw = 2 * (nx // 10) + 1
+    ae = np.zeros((p["numsx"], p["numsx"], nE), np.int32)
+    az = np.zeros((nzw, nzw, nzb, nE), np.int32)
+    dropped = L.orc_mc_run(p["sig"], p["omega"], p["numsx"], p["primeseed"], p["num_el"], p["EkeV"], p["Ehistmin"],
+                           p["Ebinsize"], p["depthmax"], p["depthstep"], p["Ze"], p["density"], p["at_wt"], mode,
+                           1 if kahan else 0, _p(ae), _p(az))
+    return ae, az, int(dropped)

@@ -459,3 +459,66 @@ def run_ecp_setup(work: str, xtal: dict, voltage: float, k, fn, dmin: float, the
                 dynfn=dynfn, mlambda=mlambda, dynupz=dynupz, vol=vol, dmt=dmt, rmt=rmt,
                 lut=lut.reshape(dims, order="F"), dbdiff=dbd.reshape(dims, order="F"), hkl=hkl, ucg=ucg,
                 sites=sites, ij=ij, k=kv, kt=kt, kn=kn, dyn=dyn)
+
+
+def run_mc_program(work: str, xtal: dict, nml: dict, timeout: int = 600) -> dict:
+    """Run CTEMMC (oracle/_ref/CTEMMC, src_programs/CTEMMC.f90 built from its sources) in `work` on
+    &MCdata `nml` (xtalname and dataname are set here) and parse its output file
+    (CTEMMC.f90:264-276): the header values and accum_e [numsy][numsx][numEbins],
+    accum_z [ny][nx][numzbins][numEbins] (the Fortran arrays, energy fastest)."""
+    write_xtal(os.path.join(work, "crystal.xtal"), xtal)
+
+    def fmt(v):
+        if isinstance(v, str):
+            return f"'{v}'"
+        return repr(v)
+    body = dict(nml, xtalname="crystal.xtal", dataname="mc.data")
+    with open(os.path.join(work, "CTEMMC.nml"), "w") as f:
+        f.write(" &MCdata\n" + "".join(f" {k} = {fmt(v)},\n" for k, v in body.items()) + " /\n")
+    subprocess.run(["timeout", "-k", "5", str(int(timeout)), os.path.join(REF_BIN, "CTEMMC"), "CTEMMC.nml"],
+                   cwd=work, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return read_mc_file(os.path.join(work, "mc.data"))
+
+
+def read_mc_file(path: str) -> dict:
+    """The Monte Carlo output file of CTEMMC.f90:264-276."""
+    with open(path, "rb") as f:
+        data = f.read()
+    recs, off = [], 0
+    while off < len(data):
+        (n,) = struct.unpack_from("<i", data, off)
+        recs.append(data[off + 4: off + 4 + n])
+        off += 8 + n
+    numEbins, numzbins, numsx, numsy = struct.unpack_from("<4i", recs[3], 0)
+    num_el, nthreads = struct.unpack_from("<qi", recs[3], 16)
+    ekev, ehistmin, ebinsize, depthmax, depthstep = struct.unpack("<5d", recs[4])
+    sig, omega = struct.unpack("<2d", recs[5])
+    nx = (numsx - 1) // 2
+    nz = 2 * (nx // 10) + 1
+    acc_e = np.frombuffer(recs[7], np.int32).reshape(numsy, numsx, numEbins).copy()
+    acc_z = np.frombuffer(recs[8], np.int32).reshape(nz, nz, numzbins, numEbins).copy()
+    return dict(progname=recs[0].decode().strip(), version=recs[1].decode().strip(), xtalname=recs[2].decode().strip(),
+                numEbins=numEbins, numzbins=numzbins, numsx=numsx, numsy=numsy, num_el=num_el, nthreads=nthreads,
+                EkeV=ekev, Ehistmin=ehistmin, Ebinsize=ebinsize, depthmax=depthmax, depthstep=depthstep, sig=sig,
+                omega=omega, MCmode=recs[6].decode(), accum_e=acc_e, accum_z=acc_z)
+
+
+def run_mc_setup(work: str, xtal: dict, ekev: float, primeseed: int, ndraw: int = 1000, timeout: int = 120) -> dict:
+    """oracle/_ref/ctemref_mc (ref_mc.f90): CalcDensity's density, avZ, avA (float32), the Lambert
+    constants, the cell volume, numat and the first ndraw numbers of rng_uniform after
+    rng_seed(primeseed + 1)."""
+    write_xtal(os.path.join(work, "crystal.xtal"), xtal)
+    with open(os.path.join(work, "refmc.nml"), "w") as f:
+        f.write(" &refmc\n xtalname = 'crystal.xtal',\n EkeV = %r,\n primeseed = %d,\n ndraw = %d,\n"
+                " outname = 'mcdump.bin',\n /\n" % (float(ekev), int(primeseed), int(ndraw)))
+    subprocess.run(["timeout", "-k", "5", str(int(timeout)), os.path.join(REF_BIN, "ctemref_mc")], cwd=work,
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    with open(os.path.join(work, "mcdump.bin"), "rb") as f:
+        buf = f.read()
+    magic, nd, ntype = struct.unpack_from("<3i", buf, 0)
+    assert magic == 20262
+    dens, avz, ava = np.frombuffer(buf, np.float32, 3, 12)
+    spio2, spi2, vol = struct.unpack_from("<3d", buf, 24)
+    numat = np.frombuffer(buf, np.int32, ntype, 48).copy()
+    u = np.frombuffer(buf, np.float32, nd, 48 + 4 * ntype).copy()
+    return dict(dens=dens, avZ=avz, avA=ava, sPio2=spio2, sPi2=spi2, vol=vol, numat=numat, u=u)

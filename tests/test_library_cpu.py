@@ -22,13 +22,14 @@ def _all_headers():
 
 
 def test_library_exports_every_declared_symbol():
-    from ctemsoft_amd import detector, ebsd, ecp, xtal
+    from ctemsoft_amd import detector, ebsd, ecp, mc, xtal
     L = ebsd.lib()
     decl = _declared()
     assert decl == sorted(ebsd.EXPORTS)
     assert _declared(("ctem_setup.h",)) == sorted(xtal.EXPORTS)
     assert _declared(("ctem_detector.h",)) == sorted(detector.EXPORTS)
     assert _declared(("ctem_ecp.h",)) == sorted(ecp.EXPORTS)
+    assert _declared(("ctem_mc.h",)) == sorted(mc.EXPORTS)
     for name in _declared(_all_headers()):
         assert hasattr(L, name), name
 
