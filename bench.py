@@ -159,6 +159,8 @@ def main():
         return bench_detector(a)
     if a.workload == "ecp_ni30":
         return bench_ecp(a)
+    if a.workload == "mc_ni30":
+        return bench_mc(a)
     wl, jobs = load_jobs(a.workload)
     tt, ts = (int(v) for v in a.terms.split(","))
     # one plan per energy bin, each holding its bin's crystal, all resident before timing; the
@@ -514,6 +516,94 @@ def bench_ecp(a):
                      "kernel_ms_source": "HIP events around the GEMM launches on the context's stream",
                      "note": "FP64 complex products of the block-matrix exponentials (2n = 2 x strong beams), "
                              "8 (2n)^3 flops per product on the true sizes"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+# ---- Monte Carlo electron trajectories (include/ctem_mc.h) --------------------------------------
+
+# the reference's own template (templatefolder/CTEMMC.template) on fcc Ni: 30 kV, 70 degree tilt,
+# a 1501 x 1501 Lambert grid, 51 energy bins 5..30 keV, 101 depth bins, 12.5 million electrons
+MC_NML = dict(sig=70.0, omega=0.0, numsx=1501, primeseed=932117, num_el=12_500_000, EkeV=30.0, Ehistmin=5.0,
+              Ebinsize=0.5, depthmax=100.0, depthstep=1.0)
+# double-precision arithmetic of one trajectory step (CTEMMC.f90:434-521) besides its six library
+# calls (two log, acos, two sin, cos): energy loss 6, deflection angle 6, azimuth 1, the new
+# direction 24, its normalisation with the compensated SUM 20, the next step length 14, the new
+# position 9, the surface test 1
+MC_FLOPS_PER_STEP = 81
+
+
+def mc_cpu_baseline(xtal, seconds=20.0):
+    """The reference program itself (oracle/_ref/CTEMMC: CTEMMC.f90 built from its sources) on a
+    bounded number of electrons of the same workload with a 301 x 301 grid (the histograms' size
+    does not change the per-electron work; the template's 1501 x 1501 output file alone is 930 MB),
+    one core (the program runs its electrons on one stream)."""
+    import subprocess
+    import tempfile
+    from oracle.refrun import REF_BIN, run_mc_program
+    if not os.path.exists(os.path.join(REF_BIN, "CTEMMC")):
+        return None
+    n = 2000
+    with tempfile.TemporaryDirectory(prefix="ctemmc_") as work:
+        while True:
+            nml = dict(MC_NML, numsx=301, num_el=n, nthreads=1)
+            t0 = time.perf_counter()
+            try:
+                run_mc_program(work, xtal, nml, timeout=300)
+            except (subprocess.CalledProcessError, OSError):
+                return None
+            dt = time.perf_counter() - t0
+            if dt > seconds / 4 or n >= 2_000_000:
+                break
+            n = min(2_000_000, int(n * max(1.5, min(8.0, seconds / 2 / max(dt, 1e-3)))))
+    return {"value": round(n / dt, 1), "unit": "electrons/s", "cores": 1, "kind": "reference",
+            "sample": f"the reference program CTEMMC on {n} electrons of the same workload with a 301 x 301 grid "
+                      f"(set-up and file output included), {dt:.1f} s"}
+
+
+def bench_mc(a):
+    import torch  # noqa: F401
+    from ctemsoft_amd import mc, workloads, xtal as xtal_mod
+    x = workloads.get("ni_c0").xtal
+    p = MC_NML
+    dens, avz, ava = xtal_mod.density(x)
+    m = mc.MonteCarlo(p["sig"], p["omega"], p["numsx"], p["primeseed"], p["num_el"], p["EkeV"], p["Ehistmin"],
+                      p["Ebinsize"], p["depthmax"], p["depthstep"], float(avz), float(dens), float(ava))
+    for _ in range(a.warmup):
+        m.reset()
+        m.run("blocked")
+    kms, steps = [], 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        m.reset()  # a fresh simulation: the histograms are zeroed in the step
+        m.run("blocked")
+        kms.append(m.last_ms())
+        steps = m.steps()
+    wall = (time.perf_counter() - t0) / a.steps
+    ae, az, dropped = m.histograms()
+    hits = int(ae.sum())
+    m.close()
+    kms_mean = float(np.mean(kms))
+    ach = MC_FLOPS_PER_STEP * steps / (kms_mean * 1e-3) / 1e12
+    log(f"mc_ni30: {p['num_el']} electrons, {steps} steps ({steps / p['num_el']:.1f} per electron), {hits} hits, "
+        f"{dropped} dropped; kernel {kms_mean:.1f} ms, step {wall * 1e3:.1f} ms")
+    cpu = None if a.no_cpu else mc_cpu_baseline(x)
+    line = {
+        "metric": "Monte Carlo electrons/s, 1 GPU (backscattered-electron energy and depth histograms)",
+        "value": round(p["num_el"] / wall, 1), "unit": "electrons/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: the reference's own MC template on fcc Ni",
+        "config": {"workload": "mc_ni30: fcc Ni, 30 kV, tilt 70 deg, Lambert grid 1501 x 1501, energy bins 5..30 keV "
+                               "by 0.5, depth bins 0..100 nm by 1, 12.5 million electrons (blocked random streams)",
+                   "electrons": p["num_el"], "trajectory_steps": steps, "hits": hits},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(ach / 78.6, 4), "traffic": None, "kernel": "k_mc",
+                     "alg_flops_per_step": MC_FLOPS_PER_STEP, "kernel_ms_per_step": round(kms_mean, 3),
+                     "kernel_ms_source": "HIP events around the launch on the object's stream",
+                     "note": "FP64 vector work (78.6 TFLOP/s on MI355X as for the matrix cores); the count is "
+                             "the steps' arithmetic without their six library calls (log, acos, sin, cos), "
+                             "which take most of the executed instructions"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

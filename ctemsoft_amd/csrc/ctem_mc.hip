@@ -139,7 +139,8 @@ __device__ __forceinline__ double sum3_kahan_d(double a, double b, double c) {
 __global__ __launch_bounds__(256) void k_mc(McDev P, const int32_t* __restrict__ lane_state, JumpDev JG,
                                             int64_t first, int64_t count, int sequential,
                                             int32_t* __restrict__ ae, int32_t* __restrict__ az,
-                                            unsigned long long* __restrict__ dropped) {
+                                            unsigned long long* __restrict__ dropped,
+                                            unsigned long long* __restrict__ steps) {
   const int64_t G = (int64_t)gridDim.x * blockDim.x;
   const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (sequential && lane > 0) return;
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_mc(McDev P, const int32_t* __restrict__
   double xyz[3] = {0.0, 0.0, 0.0}, cxyz[3] = {0.0, 0.0, 0.0};
   int traj = 0;
   bool fresh = true;
-  unsigned long long drop = 0;
+  unsigned long long drop = 0, nstep = 0;
   while (true) {
     if (fresh) {
       if (e >= end) break;
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(256) void k_mc(McDev P, const int32_t* __restrict__
     }
     bool done = traj >= kNumSteps;
     if (!done) {
+      nstep++;
       // one step (:434-521)
       const double dEds = P.predEds * log(Ec * P.J + 1.0) / Ec;
       const double dE = dEds * step;
@@ -264,6 +266,7 @@ __global__ __launch_bounds__(256) void k_mc(McDev P, const int32_t* __restrict__
     }
   }
   if (drop) atomicAdd(dropped, drop);
+  atomicAdd(steps, nstep);
 }
 
 }  // namespace ctem
@@ -279,6 +282,7 @@ struct ctem_mc {
   int32_t* az = nullptr;
   int32_t* lanes = nullptr;
   unsigned long long* dropped = nullptr;
+  unsigned long long* steps = nullptr;
   size_t ne = 0, nz = 0;
   int lanes_n = 0;
   double last_ms = 0.0;
@@ -354,6 +358,7 @@ extern "C" CTEM_API int32_t ctem_mc_create(int32_t device, const ctem_mc_params*
   if (e == hipSuccess) e = hipMalloc((void**)&m->ae, sizeof(int32_t) * m->ne);
   if (e == hipSuccess) e = hipMalloc((void**)&m->az, sizeof(int32_t) * m->nz);
   if (e == hipSuccess) e = hipMalloc((void**)&m->dropped, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc((void**)&m->steps, sizeof(unsigned long long));
   if (e != hipSuccess) {
     ctem_mc_destroy(m);
     return mc_fail(CTEM_E_HIP, std::string("ctem_mc_create: ") + hipGetErrorString(e));
@@ -380,6 +385,7 @@ extern "C" CTEM_API int32_t ctem_mc_reset(ctem_mc* m) {
   MCHIP(hipMemsetAsync(m->ae, 0, sizeof(int32_t) * m->ne, m->stream));
   MCHIP(hipMemsetAsync(m->az, 0, sizeof(int32_t) * m->nz, m->stream));
   MCHIP(hipMemsetAsync(m->dropped, 0, sizeof(unsigned long long), m->stream));
+  MCHIP(hipMemsetAsync(m->steps, 0, sizeof(unsigned long long), m->stream));
   MCHIP(hipStreamSynchronize(m->stream));
   return CTEM_OK;
 }
@@ -431,7 +437,7 @@ extern "C" CTEM_API int32_t ctem_mc_run(ctem_mc* m, int32_t mode, int64_t first,
   MCHIP(hipEventRecord(m->ev[0], m->stream));
   if (count > 0)
     hipLaunchKernelGGL(ctem::k_mc, dim3(blocks), dim3(256), 0, m->stream, m->d, m->lanes, jd, first, count,
-                       mode == CTEM_MC_SEQUENTIAL ? 1 : 0, m->ae, m->az, m->dropped);
+                       mode == CTEM_MC_SEQUENTIAL ? 1 : 0, m->ae, m->az, m->dropped, m->steps);
   MCHIP(hipGetLastError());
   MCHIP(hipEventRecord(m->ev[1], m->stream));
   MCHIP(hipStreamSynchronize(m->stream));
@@ -460,6 +466,16 @@ extern "C" CTEM_API int32_t ctem_mc_device(ctem_mc* m, int32_t** accum_e, int32_
   return CTEM_OK;
 }
 
+extern "C" CTEM_API int32_t ctem_mc_steps(ctem_mc* m, int64_t* steps) {
+  if (!m || !steps) return mc_fail(CTEM_E_ARG, "ctem_mc_steps: null argument");
+  MCHIP(hipSetDevice(m->device));
+  unsigned long long v = 0;
+  MCHIP(hipMemcpyAsync(&v, m->steps, sizeof v, hipMemcpyDeviceToHost, m->stream));
+  MCHIP(hipStreamSynchronize(m->stream));
+  *steps = (int64_t)v;
+  return CTEM_OK;
+}
+
 extern "C" CTEM_API int32_t ctem_mc_last_ms(const ctem_mc* m, double* ms) {
   if (!m || !ms) return mc_fail(CTEM_E_ARG, "ctem_mc_last_ms: null argument");
   *ms = m->last_ms;
@@ -474,6 +490,7 @@ extern "C" CTEM_API int32_t ctem_mc_destroy(ctem_mc* m) {
   if (m->az) (void)hipFree(m->az);
   if (m->lanes) (void)hipFree(m->lanes);
   if (m->dropped) (void)hipFree(m->dropped);
+  if (m->steps) (void)hipFree(m->steps);
   for (auto& e : m->ev)
     if (e) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);

@@ -27,7 +27,7 @@ from . import ebsd, xtal as xtal_mod
 MODES = {"blocked": 0, "sequential": 1}
 
 EXPORTS = ["ctem_mc_create", "ctem_mc_dims", "ctem_mc_run", "ctem_mc_reset", "ctem_mc_get", "ctem_mc_device",
-           "ctem_mc_last_ms", "ctem_mc_destroy", "ctem_mc_last_error"]
+           "ctem_mc_steps", "ctem_mc_last_ms", "ctem_mc_destroy", "ctem_mc_last_error"]
 
 # &MCdata defaults (CTEMMC.f90:153-166)
 NML_DEFAULTS = dict(xtalname="undefined", sig=70.0, omega=0.0, numsx=1501, primeseed=932117, num_el=10000000,
@@ -52,7 +52,8 @@ def _lib():
         P, I32, I64 = C.c_void_p, C.c_int32, C.c_int64
         for name, args in {"ctem_mc_create": [I32, P, P], "ctem_mc_dims": [P, P, P, P, P],
                            "ctem_mc_run": [P, I32, I64, I64], "ctem_mc_reset": [P], "ctem_mc_get": [P, P, P, P],
-                           "ctem_mc_device": [P, P, P], "ctem_mc_last_ms": [P, P], "ctem_mc_destroy": [P]}.items():
+                           "ctem_mc_device": [P, P, P], "ctem_mc_steps": [P, P], "ctem_mc_last_ms": [P, P],
+                           "ctem_mc_destroy": [P]}.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = I32
@@ -94,6 +95,12 @@ class MonteCarlo:
         dr = C.c_int64()
         _check(_lib().ctem_mc_get(self._h, ebsd._p(ae), ebsd._p(az), C.byref(dr)))
         return ae, az, dr.value
+
+    def steps(self):
+        """trajectory steps since the last reset"""
+        n = C.c_int64()
+        _check(_lib().ctem_mc_steps(self._h, C.byref(n)))
+        return n.value
 
     def last_ms(self):
         ms = C.c_double()

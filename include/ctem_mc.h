@@ -70,6 +70,9 @@ CTEM_API int32_t ctem_mc_reset(ctem_mc* mc);
 CTEM_API int32_t ctem_mc_get(ctem_mc* mc, int32_t* accum_e, int32_t* accum_z, int64_t* dropped);
 /* device addresses of the histograms, valid until ctem_mc_destroy (for a consumer on the GPU) */
 CTEM_API int32_t ctem_mc_device(ctem_mc* mc, int32_t** accum_e, int32_t** accum_z);
+/* No counterpart in the reference (measurement): trajectory steps taken since the last reset,
+ * the scattering events of :434-521 (the entry of each electron not counted) */
+CTEM_API int32_t ctem_mc_steps(ctem_mc* mc, int64_t* steps);
 /* milliseconds of the last ctem_mc_run's kernel (HIP events) */
 CTEM_API int32_t ctem_mc_last_ms(const ctem_mc* mc, double* ms);
 CTEM_API int32_t ctem_mc_destroy(ctem_mc* mc);
