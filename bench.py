@@ -161,6 +161,8 @@ def main():
         return bench_ecp(a)
     if a.workload == "mc_ni30":
         return bench_mc(a)
+    if a.workload == "ni_c4w_adjoint":
+        return bench_adjoint(a)
     wl, jobs = load_jobs(a.workload)
     tt, ts = (int(v) for v in a.terms.split(","))
     # one plan per energy bin, each holding its bin's crystal, all resident before timing; the
@@ -519,6 +521,73 @@ def bench_ecp(a):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+# ---- reverse mode (ctem_plan_adjoint) on the adjoint workload ---------------------------------
+
+def bench_adjoint(a):
+    """The adjoint workload ni_c4w (fcc Ni as ni_c0w on its own npx 100 grid, 5,151 directions): the
+    master-pattern residual against the perturbed target model and its gradient with respect to the
+    LUT and the site B (ctemsoft_amd.fit.master_loss_grad): one step = FP64 forward, scatter and
+    residual, reverse pass."""
+    import torch  # noqa: F401
+    from ctemsoft_amd import ebsd, fit, workloads
+    fx = np.load(os.path.join(REPO, "tests", "golden", "ni_c0w.npz"))
+    st = ebsd.Setup.from_fixture(fx)
+    wl = workloads.get("ni_c4w")
+    ij, k, kn = ebsd.kvectors(wl.npx, wl.npx, st.isym, st.usehex, st.dsm, st.mlambda, 1)
+    npx = int(ij[:, 0].max())
+    isym = int(st.isym)
+    lam = wl.depth_weights()
+    thick = float(int(wl.nz * wl.dz))
+    ctx = ebsd.Context(0)
+    ctx.upload_crystal(fit.perturbed_target_setup(st, wl.seed))
+    pt = ctx.plan(k, kn, lam, wl.dz, thick)
+    pt.run("fp64")
+    target = pt.scatter(ij, npx, npx, isym)
+    pt.close()
+    ctx.upload_crystal(st)
+    plan = ctx.plan(k, kn, lam, wl.dz, thick)
+    ns, nw = plan.select()
+    owner = fit.pixel_owner(ij, npx, npx, isym)
+    for _ in range(a.warmup):
+        fit.master_loss_grad(plan, ij, npx, npx, isym, target, owner)
+    t_rev, t_all = [], []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        loss, sr = fit.master_loss(plan, ij, npx, npx, isym, target)
+        t1 = time.perf_counter()
+        loss, glut, gb = fit.master_loss_grad(plan, ij, npx, npx, isym, target, owner)
+        t2 = time.perf_counter()
+        t_all.append(t2 - t1)
+        t_rev.append((t2 - t1) - (t1 - t0))
+    fwd = {}
+    for prec in ("fp64", "mixed"):
+        plan.run(prec)
+        t0 = time.perf_counter()
+        plan.run(prec)
+        fwd[prec] = time.perf_counter() - t0
+    step = float(np.median(t_all))
+    rev = float(np.median(t_rev))
+    nk = len(k)
+    log(f"ni_c4w adjoint: {nk} directions, strong {ns.min()}..{ns.max()}; gradient step {step * 1e3:.1f} ms "
+        f"(reverse {rev * 1e3:.1f} ms), forward fp64 {fwd['fp64'] * 1e3:.1f} ms, mixed {fwd['mixed'] * 1e3:.1f} ms; "
+        f"loss {loss:.6e}, |grad LUT| {np.abs(glut).max():.3e}, grad B {gb}")
+    line = {
+        "metric": "master-pattern gradient directions/s, 1 GPU (FP64 forward + reverse: dloss/dLUT and dloss/dB)",
+        "value": round(nk / step, 1), "unit": "directions/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(step * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: the reference-built Ni setup, a perturbed target model",
+        "config": {"workload": f"ni_c4w: fcc Ni 20 kV, npx {wl.npx} ({nk} directions), strong beams "
+                               f"{ns.min()}-{ns.max()}, residual against B x 1.05 and LUT x (1 + 0.01 N(0,1))",
+                   "reverse_ms": round(rev * 1e3, 3), "forward_fp64_ms": round(fwd["fp64"] * 1e3, 3),
+                   "forward_mixed_ms": round(fwd["mixed"] * 1e3, 3),
+                   "reverse_over_forward_fp64": round(rev / fwd["fp64"], 2)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
 
 
 # ---- Monte Carlo electron trajectories (include/ctem_mc.h) --------------------------------------

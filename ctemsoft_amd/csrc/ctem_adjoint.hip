@@ -95,57 +95,59 @@ __global__ void k_adj_hm(int npad, int nz, const double2* __restrict__ Psi, int6
 }
 
 // backward depth recursion: Phi[:,k-1] = phi_k, phi_Z = lam_Z Q_Z, phi_k = lam_k Q_k + E^H phi_{k+1}
-// (Q = M Psi). One workgroup per matrix; row r of E^H is column r of E (contiguous).
+// (Q = M Psi). One workgroup per matrix, phi in LDS; (E^H phi)_r = sum_c conj(E(c, r)) phi_c is the
+// dot product of column r of E (contiguous) with phi: each wave takes rows r = wave, wave + 4, ..,
+// its lanes read the column in 64-entry runs and the wave sums them in a fixed order.
 __global__ __launch_bounds__(256) void k_adj_depth(int npad, int nz, const double2* __restrict__ E,
                                                    const double2* __restrict__ Q, const double* __restrict__ lam,
                                                    double2* __restrict__ Phi) {
-  extern __shared__ double2 ph[];  // [npad]
+  extern __shared__ double2 ph[];  // [2 npad]: phi_{k+1} and phi_k
   const int b = blockIdx.x;
   const double2* Eb = E + (int64_t)b * npad * npad;
   const double2* Qb = Q + (int64_t)b * npad * nz;
   double2* Pb = Phi + (int64_t)b * npad * nz;
-  for (int r = threadIdx.x; r < npad; r += blockDim.x) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double2* cur = ph;
+  double2* nxt = ph + npad;
+  for (int r = tid; r < npad; r += blockDim.x) {
     const double2 q = Qb[(int64_t)(nz - 1) * npad + r];
     const double2 v = make_double2(lam[nz - 1] * q.x, lam[nz - 1] * q.y);
-    ph[r] = v;
+    cur[r] = v;
     Pb[(int64_t)(nz - 1) * npad + r] = v;
   }
   __syncthreads();
   for (int k = nz - 2; k >= 0; k--) {
-    double2 y[2];
-    int cnt = 0;
-    for (int r = threadIdx.x; r < npad; r += blockDim.x, cnt++) {
-      double sr = 0.0, si = 0.0;
+    const double lk = lam[k];
+    for (int r = wave; r < npad; r += 4) {
       const double2* col = Eb + (int64_t)r * npad;
-      for (int c = 0; c < npad; c++) {
-        const double2 e = col[c], x = ph[c];
+      double sr = 0.0, si = 0.0;
+      for (int c = lane; c < npad; c += 64) {
+        const double2 e = col[c], x = cur[c];
         // conj(e) * x
         sr += e.x * x.x + e.y * x.y;
         si += e.x * x.y - e.y * x.x;
       }
-      const double2 q = Qb[(int64_t)k * npad + r];
-      y[cnt] = make_double2(lam[k] * q.x + sr, lam[k] * q.y + si);
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      if (lane == 0) {
+        const double2 q = Qb[(int64_t)k * npad + r];
+        const double2 v = make_double2(lk * q.x + sr, lk * q.y + si);
+        nxt[r] = v;
+        Pb[(int64_t)k * npad + r] = v;
+      }
     }
     __syncthreads();
-    cnt = 0;
-    for (int r = threadIdx.x; r < npad; r += blockDim.x, cnt++) {
-      ph[r] = y[cnt];
-      Pb[(int64_t)k * npad + r] = y[cnt];
-    }
-    __syncthreads();
+    double2* t = cur; cur = nxt; nxt = t;
   }
 }
 
-// sigma_b = 2^(floor(log2 ||A||_1) - ceil(log2 ||Ebar||_1)) (0 when Ebar = 0), then
-// Mblk = [[A^H, sigma Ebar], [0, A^H]] (2 npad x 2 npad, column-major)
-__global__ __launch_bounds__(256) void k_adj_block(int npad, const double2* __restrict__ A,
-                                                   const double2* __restrict__ Eb, double* __restrict__ sigma,
-                                                   double2* __restrict__ Mblk) {
+__global__ __launch_bounds__(256) void k_adj_pair(int npad, const double2* __restrict__ A,
+                                                  const double2* __restrict__ Eb, double* __restrict__ sigma,
+                                                  double2* __restrict__ PX, double2* __restrict__ PY) {
   const int b = blockIdx.x;
   const int64_t per = (int64_t)npad * npad;
   const double2* Ab = A + b * per;
   const double2* Gb = Eb + b * per;
-  double2* Mb = Mblk + 4 * b * per;
   __shared__ double red[2][4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double na = 0.0, ng = 0.0;
@@ -172,41 +174,83 @@ __global__ __launch_bounds__(256) void k_adj_block(int npad, const double2* __re
     sg = ldexp(1.0, (ea - 1) - eg);
   }
   if (tid == 0) sigma[b] = sg;
-  const int n2 = 2 * npad;
-  for (int64_t e = tid; e < (int64_t)n2 * n2; e += 256) {
-    const int i = (int)(e % n2), j = (int)(e / n2);
-    double2 v = make_double2(0.0, 0.0);
-    if (i < npad && j < npad) {
-      const double2 a = Ab[(int64_t)i * npad + j];  // A^H(i,j) = conj(A(j,i))
-      v = make_double2(a.x, -a.y);
-    } else if (i >= npad && j >= npad) {
-      const double2 a = Ab[(int64_t)(i - npad) * npad + (j - npad)];
-      v = make_double2(a.x, -a.y);
-    } else if (i < npad && j >= npad) {
-      const double2 g = Gb[(int64_t)(j - npad) * npad + i];
-      v = make_double2(sg * g.x, sg * g.y);
-    }
-    Mb[e] = v;
+  for (int64_t e = tid; e < per; e += 256) {
+    const int i = (int)(e % npad), j = (int)(e / npad);
+    const double2 a = Ab[(int64_t)i * npad + j];  // A^H(i,j) = conj(A(j,i))
+    PX[b * per + e] = make_double2(a.x, -a.y);
+    const double2 g = Gb[e];
+    PY[b * per + e] = make_double2(sg * g.x, sg * g.y);
   }
 }
 
-// D_bar = conj(alpha) A_bar / sigma from the upper-right block of expm(Mblk); conj(alpha) =
+// ||[[X, Y], [0, X]]||_1 = max_j (sum_i |X_ij| + |Y_ij|), the squaring count s with
+// norm / 2^s <= theta, and X, Y scaled by 2^-s in place (error bits as k_norm_scale)
+__global__ __launch_bounds__(256) void k_norm_scale_pair(int npad, double2* __restrict__ X, double2* __restrict__ Y,
+                                                         double theta, int smax, int* __restrict__ sq,
+                                                         int* __restrict__ err) {
+  const int b = blockIdx.x;
+  const int64_t per = (int64_t)npad * npad;
+  double2* Xb = X + b * per;
+  double2* Yb = Y + b * per;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double wmax[4];
+  __shared__ int s_sh;
+  double cmax = 0.0;
+  for (int j = wave; j < npad; j += 4) {
+    double sum = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const double2 u = Xb[(int64_t)j * npad + i], v = Yb[(int64_t)j * npad + i];
+      sum += hypot(u.x, u.y) + hypot(v.x, v.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    cmax = fmax(cmax, sum);
+    if (!(sum == sum)) cmax = sum;
+  }
+  if (lane == 0) wmax[wave] = cmax;
+  __syncthreads();
+  if (tid == 0) {
+    double nrm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    for (int w = 0; w < 4; w++) if (!(wmax[w] == wmax[w])) nrm = wmax[w];
+    int s = 0;
+    if (!isfinite(nrm)) {
+      atomicOr(err, 2);
+    } else {
+      while (nrm > theta && s < smax) { nrm *= 0.5; s++; }
+      if (nrm > theta) atomicOr(err, 4);
+    }
+    s_sh = s;
+    sq[b] = s;
+  }
+  __syncthreads();
+  const double f = ldexp(1.0, -s_sh);
+  if (s_sh > 0)
+    for (int64_t e = tid; e < per; e += 256) {
+      const double2 u = Xb[e], v = Yb[e];
+      Xb[e] = make_double2(u.x * f, u.y * f);
+      Yb[e] = make_double2(v.x * f, v.y * f);
+    }
+}
+
+// D_bar = conj(alpha) A_bar / sigma from the derivative block of the exponential; conj(alpha) =
 // -i pre with pre = 2 pi dz / fl32(2 kn); zero outside the n x n block
+// (ldt = 2 npad: T2 is the dense block matrix, the block at column offset npad; ldt = npad: T2 is
+// the Y part of the pair form)
 __global__ void k_adj_extract(Bucket B, BeamView V, const double* __restrict__ kn, double dz,
-                              const double2* __restrict__ T2, const double* __restrict__ sigma,
+                              const double2* __restrict__ T2, int ldt, const double* __restrict__ sigma,
                               double2* __restrict__ Dbar) {
   const int npad = B.npad, b = blockIdx.y;
   const int dir = B.dirs[b];
   const int n = V.nns[dir];
   const int64_t per = (int64_t)npad * npad;
-  const int n2 = 2 * npad;
+  const int64_t tstride = (int64_t)ldt * ldt;
+  const int coff = ldt == npad ? 0 : npad;
   const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
   const double sg = sigma[b];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(e % npad), j = (int)(e / npad);
     double2 v = make_double2(0.0, 0.0);
     if (i < n && j < n && sg > 0.0) {
-      const double2 a = T2[(int64_t)b * 4 * per + (int64_t)(j + npad) * n2 + i];
+      const double2 a = T2[(int64_t)b * tstride + (int64_t)(j + coff) * ldt + i];
       // (-i pre) (a.x + i a.y) / sigma
       v = make_double2(pre * a.y / sg, -pre * a.x / sg);
     }
@@ -386,16 +430,21 @@ void launch_adj_hm(hipStream_t s, int npad, int nb, int nz, const double2* Psi, 
 }
 void launch_adj_depth(hipStream_t s, int npad, int nb, int nz, const double2* E, const double2* Q,
                       const double* lam, double2* Phi) {
-  hipLaunchKernelGGL(k_adj_depth, dim3(nb), dim3(256), npad * sizeof(double2), s, npad, nz, E, Q, lam, Phi);
+  hipLaunchKernelGGL(k_adj_depth, dim3(nb), dim3(256), 2 * npad * sizeof(double2), s, npad, nz, E, Q, lam, Phi);
 }
-void launch_adj_block(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar,
-                      double* sigma, double2* Mblk) {
-  hipLaunchKernelGGL(k_adj_block, dim3(nb), dim3(256), 0, s, npad, A, Ebar, sigma, Mblk);
+void launch_adj_pair(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar, double* sigma,
+                     double2* PX, double2* PY) {
+  hipLaunchKernelGGL(k_adj_pair, dim3(nb), dim3(256), 0, s, npad, A, Ebar, sigma, PX, PY);
+}
+void launch_norm_scale_pair(hipStream_t s, int npad, int nb, double2* X, double2* Y, double theta, int smax,
+                            int32_t* sq, int32_t* err) {
+  hipLaunchKernelGGL(k_norm_scale_pair, dim3(nb), dim3(256), 0, s, npad, X, Y, theta, smax, sq, err);
 }
 void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
-                        const double2* T2, const double* sigma, double2* Dbar) {
+                        const double2* T2, int ldt, const double* sigma, double2* Dbar) {
   const int64_t per = (int64_t)B.npad * B.npad;
-  hipLaunchKernelGGL(k_adj_extract, dim3(grid_rows(per), B.nb), dim3(256), 0, s, B, V, kn, dz, T2, sigma, Dbar);
+  hipLaunchKernelGGL(k_adj_extract, dim3(grid_rows(per), B.nb), dim3(256), 0, s, B, V, kn, dz, T2, ldt, sigma,
+                     Dbar);
 }
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw) {

@@ -828,6 +828,94 @@ double2* expm_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, d
   return T;
 }
 
+// expm of the block matrix [[X, Y], [0, X]] held as the pair (X, Y) (npad x npad each): the same
+// degree-16 Taylor evaluation and squarings as expm_fp64, every product of two pairs
+// (X1 X2, X1 Y2 + Y1 X2) taken as three n-size GEMMs (the dense 2n-size product is eight). X and
+// Y are scaled in place; W holds 10 nb npad^2 entries. Returns the pair (E, L) in W: E at the
+// returned pointer, L at E + 5 nb npad^2 (L = the Frechet derivative of expm at X in the direction
+// Y when X, Y are unscaled on entry).
+double2* expm_pair_fp64(hipStream_t s, int npad, int nb, double2* X, double2* Y, double2* W, int32_t* d_sq,
+                        int32_t* d_err, std::vector<int32_t>& h_sq, int32_t& rc_out) {
+  rc_out = CTEM_OK;
+  const int64_t per = (int64_t)npad * npad;
+  const int64_t half = 5 * per * nb;  // X parts in W[0, half), Y parts in W[half, 2 half)
+  double2* X2 = W;
+  double2* X3 = W + per * nb;
+  double2* X4 = W + 2 * per * nb;
+  double2* XH = W + 3 * per * nb;
+  double2* XT = W + 4 * per * nb;
+  double2* Y2 = X2 + half;
+  double2* Y3 = X3 + half;
+  double2* Y4 = X4 + half;
+  double2* YH = XH + half;
+  double2* YT = XT + half;
+#define PAIR_CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { rc_out = fail(CTEM_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); return nullptr; } } while (0)
+  launch_norm_scale_pair(s, npad, nb, X, Y, kTheta64, kSmax, d_sq, d_err);
+  PAIR_CHK(hipGetLastError());
+  h_sq.resize(nb);
+  PAIR_CHK(hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s));
+  PAIR_CHK(hipStreamSynchronize(s));
+  int smax = 0;
+  for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  // C = X*Y + betaI I + sum beta_j M_j over the nb matrices (M may be C itself: the epilogue reads
+  // each addend element before it writes that element of C)
+  auto gemm = [&](double2* C, const double2* A_, const double2* B_, double betaI,
+                  std::initializer_list<std::pair<const double2*, double>> ms, const int32_t* active,
+                  int step) -> bool {
+    ZGemm g{};
+    g.m = g.n = g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = A_; g.sx = per; g.Y = B_; g.sy = per; g.C = C; g.sc = per; g.sm = per;
+    g.alpha = 1.0; g.betaI = betaI;
+    int q = 0;
+    for (auto& m : ms) { g.M[q] = m.first; g.beta[q] = m.second; q++; }
+    g.active = active; g.step = step; g.nb = nb;
+    launch_zgemm(s, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rc_out = fail(CTEM_E_HIP, "pair product: %s", hipGetErrorString(e)); return false; }
+    return true;
+  };
+  // (CX, CY) = (AX, AY)(BX, BY) + betaI (I, 0) + sum beta_j (MX_j, MY_j)
+  auto pmul = [&](double2* CX, double2* CY, const double2* AX, const double2* AY, const double2* BX,
+                  const double2* BY, double betaI, std::initializer_list<std::pair<const double2*, double>> mx,
+                  std::initializer_list<std::pair<const double2*, double>> my, const int32_t* active,
+                  int step) -> bool {
+    if (!gemm(CX, AX, BX, betaI, mx, active, step)) return false;
+    if (!gemm(CY, AX, BY, 0.0, my, active, step)) return false;
+    return gemm(CY, AY, BX, 0.0, {{CY, 1.0}}, active, step);
+  };
+  if (!pmul(X2, Y2, X, Y, X, Y, 0.0, {}, {}, nullptr, 0)) return nullptr;
+  if (!pmul(X3, Y3, X, Y, X2, Y2, 0.0, {}, {}, nullptr, 0)) return nullptr;
+  if (!pmul(X4, Y4, X2, Y2, X2, Y2, 0.0, {}, {}, nullptr, 0)) return nullptr;
+  {
+    const double beta[4] = {inv_fact(13), inv_fact(14), inv_fact(15), inv_fact(16)};
+    const double2* MX[4] = {X, X2, X3, X4};
+    const double2* MY[4] = {Y, Y2, Y3, Y4};
+    launch_combo(s, npad, nb, XH, inv_fact(12), MX, beta);
+    launch_combo(s, npad, nb, YH, 0.0, MY, beta);
+    PAIR_CHK(hipGetLastError());
+  }
+  if (!pmul(XT, YT, X4, Y4, XH, YH, inv_fact(8), {{X, inv_fact(9)}, {X2, inv_fact(10)}, {X3, inv_fact(11)}},
+            {{Y, inv_fact(9)}, {Y2, inv_fact(10)}, {Y3, inv_fact(11)}}, nullptr, 0))
+    return nullptr;
+  if (!pmul(XH, YH, X4, Y4, XT, YT, inv_fact(4), {{X, inv_fact(5)}, {X2, inv_fact(6)}, {X3, inv_fact(7)}},
+            {{Y, inv_fact(5)}, {Y2, inv_fact(6)}, {Y3, inv_fact(7)}}, nullptr, 0))
+    return nullptr;
+  if (!pmul(XT, YT, X4, Y4, XH, YH, 1.0, {{X, 1.0}, {X2, inv_fact(2)}, {X3, inv_fact(3)}},
+            {{Y, 1.0}, {Y2, inv_fact(2)}, {Y3, inv_fact(3)}}, nullptr, 0))
+    return nullptr;
+  double2* bx[2] = {XT, XH};
+  double2* by[2] = {YT, YH};
+  for (int i = 0; i < smax; i++)
+    if (!pmul(bx[(i + 1) & 1], by[(i + 1) & 1], bx[i & 1], by[i & 1], bx[i & 1], by[i & 1], 0.0, {}, {}, d_sq, i))
+      return nullptr;
+  launch_parity_copy(s, npad, nb, d_sq, XH, XT);
+  launch_parity_copy(s, npad, nb, d_sq, YH, YT);
+  PAIR_CHK(hipGetLastError());
+#undef PAIR_CHK
+  return XT;
+}
+
 // FP64 form: expm, then psi_k = E^k e_1 by the depth recursion
 int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2* A, double2* W,
                         double2* Psi, int64_t spsi, int nz, int32_t* d_sq, int32_t* d_err,
@@ -1554,7 +1642,7 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
     const int nbk = (int)kv.second.size();
     const int64_t per = (int64_t)npad * npad;
     const int64_t spsi = (int64_t)npad * nz;
-    const size_t per_dir = (size_t)(33 * per + 4 * spsi) * sizeof(double2) + 256;
+    const size_t per_dir = (size_t)(21 * per + 4 * spsi) * sizeof(double2) + 256;
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
     for (int c0 = 0; c0 < nbk; c0 += chunk) {
       const int nb = std::min(chunk, nbk - c0);
@@ -1563,17 +1651,16 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
           (rc = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
           (rc = c->w.aM.ensure((size_t)nb * per * 16)) || (rc = c->w.aQ.ensure((size_t)nb * spsi * 16)) ||
           (rc = c->w.aPhi.ensure((size_t)nb * spsi * 16)) || (rc = c->w.aHm.ensure((size_t)nb * spsi * 16)) ||
-          (rc = c->w.aEbar.ensure((size_t)nb * per * 16)) || (rc = c->w.aBlk.ensure((size_t)4 * nb * per * 16)) ||
-          (rc = c->w.aW2.ensure((size_t)20 * nb * per * 16)) || (rc = c->w.aDbar.ensure((size_t)nb * per * 16)) ||
+          (rc = c->w.aEbar.ensure((size_t)nb * per * 16)) || (rc = c->w.aBlk.ensure((size_t)2 * nb * per * 16)) ||
+          (rc = c->w.aW2.ensure((size_t)10 * nb * per * 16)) || (rc = c->w.aDbar.ensure((size_t)nb * per * 16)) ||
           (rc = c->w.aSig.ensure(8 * (size_t)nb)))
         return rc;
       Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
-      std::vector<int> hn(nb), hn2(nb);
+      std::vector<int> hn(nb);
       int max_nw = 0;
       for (int b = 0; b < nb; b++) {
         const int d = kv.second[c0 + b];
         hn[b] = p->h_nns[d];
-        hn2[b] = 2 * p->h_nns[d];
         max_nw = std::max(max_nw, p->h_nnw[d]);
       }
       double2* A = c->w.A.as<double2>();
@@ -1618,14 +1705,18 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
         launch_zgemm(s, g);
         HIPCHK(hipGetLastError());
       }
-      // A_bar = L_exp(A^H, Ebar) from expm([[A^H, sigma Ebar], [0, A^H]]); A rebuilt unscaled
+      // A_bar = L_exp(A^H, Ebar), the upper-right block of expm([[A^H, sigma Ebar], [0, A^H]]) taken in
+      // the pair form; A rebuilt unscaled
       launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, A);
-      launch_adj_block(s, npad, nb, A, c->w.aEbar.as<double2>(), c->w.aSig.as<double>(), c->w.aBlk.as<double2>());
+      double2* PX = c->w.aBlk.as<double2>();
+      double2* PY = PX + per * nb;
+      launch_adj_pair(s, npad, nb, A, c->w.aEbar.as<double2>(), c->w.aSig.as<double>(), PX, PY);
       HIPCHK(hipGetLastError());
-      double2* T2 = expm_fp64(nullptr, s, 2 * npad, nb, c->w.aBlk.as<double2>(), c->w.aW2.as<double2>(),
-                              c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn2, h_sq2, rc);
+      double2* T2 = expm_pair_fp64(s, npad, nb, PX, PY, c->w.aW2.as<double2>(), c->w.sq.as<int32_t>(),
+                                   p->err.as<int32_t>(), h_sq2, rc);
       if (!T2) return rc;
-      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2, c->w.aSig.as<double>(), c->w.aDbar.as<double2>());
+      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2 + 5 * per * nb, npad, c->w.aSig.as<double>(),
+                         c->w.aDbar.as<double2>());
       HIPCHK(hipGetLastError());
       // LUT entries
       launch_adj_lut(s, B, V, R, L, p->cr->mlambda, c->w.aDbar.as<double2>(), c->w.g_lut.as<double>(), max_nw);

@@ -357,10 +357,15 @@ void launch_adj_m(hipStream_t s, const Bucket& B, const BeamView& V, const DevRe
 void launch_adj_hm(hipStream_t s, int npad, int nb, int nz, const double2* Psi, int64_t spsi, double2* Hm);
 void launch_adj_depth(hipStream_t s, int npad, int nb, int nz, const double2* E, const double2* Q,
                       const double* lam, double2* Phi);
-void launch_adj_block(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar,
-                      double* sigma, double2* Mblk);
+// X = A^H, Y = sigma Ebar: the pair form of [[A^H, sigma Ebar], [0, A^H]]
+void launch_adj_pair(hipStream_t s, int npad, int nb, const double2* A, const double2* Ebar, double* sigma,
+                     double2* PX, double2* PY);
+// ||[[X, Y], [0, X]]||_1 per matrix, its squaring count and the scaling of X and Y in place
+void launch_norm_scale_pair(hipStream_t s, int npad, int nb, double2* X, double2* Y, double theta, int smax,
+                            int32_t* sq, int32_t* err);
+// D_bar from the derivative block: T2 with leading dimension ldt (npad: the pair's Y part)
 void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
-                        const double2* T2, const double* sigma, double2* Dbar);
+                        const double2* T2, int ldt, const double* sigma, double2* Dbar);
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw);
 
