@@ -1,4 +1,5 @@
-// Reverse-mode kernels of the FP64 master-pattern path: the gradient of a real function
+// Reverse-mode kernels of the master-pattern path (FP64 form, and the mixed form's own kernels
+// below): the gradient of a real function
 // f(dots) of the per-direction, per-site intensities dots_p = Re sum_gh Lgh(g,h) S_p(g,h)
 // with respect to the complex LUT entries U(g) and the per-site Debye-Waller factors B_p.
 //
@@ -418,6 +419,178 @@ __global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRef
 }
 
 // ------------------------------------------------------------------------------------------
+// Mixed form: the forward exponential as E = I + F (FP32 F), the Frechet derivative from FP32
+// matrices on split-FP16 products. It is taken as L_exp(A^H, Ebar) = L_exp(A, Ebar^H)^H, so that
+// the pair exponential starts from A itself.
+
+// E = I + F in FP64 for the backward depth recursion
+__global__ void k_adj_e64(int npad, const float2* __restrict__ F, double2* __restrict__ E) {
+  const int b = blockIdx.y;
+  const int64_t per = (int64_t)npad * npad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    const float2 f = F[b * per + e];
+    const bool diag = (e % npad) == (e / npad);
+    E[b * per + e] = make_double2((double)f.x + (diag ? 1.0 : 0.0), (double)f.y);
+  }
+}
+
+__device__ __forceinline__ double dnanmax(double a, double b) { return (a != a) ? a : ((b != b) ? b : fmax(a, b)); }
+
+// max over the workgroup (256 threads), a NaN kept
+__device__ __forceinline__ double block_nanmax(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = dnanmax(v, __shfl_xor(v, o));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return dnanmax(dnanmax(red[0], red[1]), dnanmax(red[2], red[3]));
+}
+
+// The pair (X, Y) = (A 2^-s, sigma 2^-s Ebar^H) from the FP32 A (read and scaled in place:
+// X may alias A) and the FP64 Ebar: sigma the power of two that brings ||Ebar^H||_1 (the largest
+// row sum of |Ebar|) just below ||A||_1, s from ||[[X, Y], [0, X]]||_1 = max_j (colsum_j |A| +
+// sigma rowsum_j |Ebar|) / 2^s <= theta (error bits as k_norm_scale), and the max |re|, |im| of X
+// and Y into xmax[b], ymax[b] (zeroed by the caller). One workgroup per matrix; the transpose
+// goes through 32 x 32 LDS tiles so that reads and writes are both column runs.
+__global__ __launch_bounds__(256) void k_adj_pair32(int npad, float2* __restrict__ A, const double2* __restrict__ Eb,
+                                                    double theta, int smax, int* __restrict__ sq,
+                                                    int* __restrict__ err, double* __restrict__ sigma,
+                                                    float2* __restrict__ Y, float* __restrict__ xmax,
+                                                    float* __restrict__ ymax) {
+  extern __shared__ double cs[];  // [2 npad]: column sums of |A|, row sums of |Ebar|
+  __shared__ double2 tile[32][33];
+  __shared__ double red[4];
+  __shared__ int s_sh;
+  __shared__ double sg_sh;
+  const int b = blockIdx.x;
+  const int64_t per = (int64_t)npad * npad;
+  float2* Ab = A + b * per;
+  const double2* Gb = Eb + b * per;
+  double* cA = cs;
+  double* rE = cs + npad;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = wave; j < npad; j += 4) {
+    double sa = 0.0;
+    for (int i = lane; i < npad; i += 64) {
+      const float2 a = Ab[(int64_t)j * npad + i];
+      sa += (double)hypotf(a.x, a.y);
+    }
+    sa = wave_sum(sa);
+    if (lane == 0) cA[j] = sa;
+  }
+  // row sums: consecutive threads take consecutive rows, so each column is read in runs
+  for (int j = tid; j < npad; j += 256) {
+    double se = 0.0;
+    for (int i = 0; i < npad; i++) {
+      const double2 g = Gb[(int64_t)i * npad + j];
+      se += hypot(g.x, g.y);
+    }
+    rE[j] = se;
+  }
+  __syncthreads();
+  double na = 0.0, ng = 0.0;
+  for (int j = tid; j < npad; j += 256) {
+    na = dnanmax(na, cA[j]);
+    ng = dnanmax(ng, rE[j]);
+  }
+  na = block_nanmax(na, red);
+  ng = block_nanmax(ng, red);
+  if (tid == 0) {
+    double sg = 0.0;
+    if (ng > 0.0 && isfinite(ng)) {
+      int ea, eg;
+      frexp((na > 0.0 && isfinite(na)) ? na : 1.0, &ea);
+      frexp(ng, &eg);
+      sg = ldexp(1.0, (ea - 1) - eg);
+    }
+    sg_sh = sg;
+    sigma[b] = sg;
+  }
+  __syncthreads();
+  const double sg = sg_sh;
+  double nrm = 0.0;
+  for (int j = tid; j < npad; j += 256) nrm = dnanmax(nrm, cA[j] + sg * rE[j]);
+  nrm = block_nanmax(nrm, red);
+  if (tid == 0) {
+    int sc = 0;
+    if (!isfinite(nrm)) {
+      atomicOr(err, 2);
+    } else {
+      while (nrm > theta && sc < smax) { nrm *= 0.5; sc++; }
+      if (nrm > theta) atomicOr(err, 4);
+    }
+    s_sh = sc;
+    sq[b] = sc;
+  }
+  __syncthreads();
+  const float fx = ldexpf(1.0f, -s_sh);
+  float vx = 0.0f;
+  for (int64_t e = tid; e < per; e += 256) {
+    const float2 a = Ab[e];
+    const float2 w = make_float2(a.x * fx, a.y * fx);
+    Ab[e] = w;
+    vx = nmax(vx, nmax(fabsf(w.x), fabsf(w.y)));
+  }
+  wave_max_to(vx, xmax + b);
+  // Y(r, c) = f conj(Ebar(c, r)); tile[rl][cl] holds Ebar(c0 + cl, r0 + rl)
+  const double fy = sg * ldexp(1.0, -s_sh);
+  float vy = 0.0f;
+  const int tx = tid & 31, ty = tid >> 5;
+  const int nt = npad / 32;
+  for (int t = 0; t < nt * nt; t++) {
+    const int r0 = (t % nt) * 32, c0 = (t / nt) * 32;
+    __syncthreads();
+    for (int q = 0; q < 4; q++) {
+      const int rl = ty + 8 * q;
+      tile[rl][tx] = Gb[(int64_t)(r0 + rl) * npad + c0 + tx];
+    }
+    __syncthreads();
+    for (int q = 0; q < 4; q++) {
+      const int cl = ty + 8 * q;
+      const double2 g = tile[tx][cl];
+      const float2 w = make_float2((float)(fy * g.x), (float)(-fy * g.y));
+      Y[b * per + (int64_t)(c0 + cl) * npad + r0 + tx] = w;
+      vy = nmax(vy, nmax(fabsf(w.x), fabsf(w.y)));
+    }
+  }
+  wave_max_to(vy, ymax + b);
+}
+
+// D_bar = conj(alpha) A_bar from the mixed pair exponential's derivative part
+// G = L_exp(A, sigma Ebar^H) = (sigma A_bar)^H: D_bar(i,j) = conj(alpha) conj(G(j,i)) / sigma, zero
+// outside the n x n block. One workgroup per 32 x 32 tile of D_bar (G read through LDS).
+__global__ __launch_bounds__(256) void k_adj_extract32(Bucket B, BeamView V, const double* __restrict__ kn,
+                                                       double dz, const float2* __restrict__ G,
+                                                       const double* __restrict__ sigma, double2* __restrict__ Dbar) {
+  __shared__ float2 tile[32][33];
+  const int npad = B.npad, b = blockIdx.y, nt = npad / 32;
+  const int i0 = (blockIdx.x % nt) * 32, j0 = (blockIdx.x / nt) * 32;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir];
+  const int64_t per = (int64_t)npad * npad;
+  const double pre = 2.0 * 3.141592653589793238 * dz / (double)(float)(2.0 * kn[dir]);
+  const double sg = sigma[b];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // tile[il][jl] = G(j0 + jl, i0 + il)
+  for (int q = 0; q < 4; q++) {
+    const int il = ty + 8 * q;
+    tile[il][tx] = G[b * per + (int64_t)(i0 + il) * npad + j0 + tx];
+  }
+  __syncthreads();
+  for (int q = 0; q < 4; q++) {
+    const int jl = ty + 8 * q;
+    const int i = i0 + tx, j = j0 + jl;
+    double2 v = make_double2(0.0, 0.0);
+    if (i < n && j < n && sg > 0.0) {
+      const float2 g = tile[tx][jl];
+      // a = conj(g) = sigma A_bar(i, j);  (-i pre) a / sigma
+      v = make_double2(-pre * (double)g.y / sg, -pre * (double)g.x / sg);
+    }
+    Dbar[b * per + (int64_t)j * npad + i] = v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 static int grid_rows(int64_t per) { return (int)((per + 255) / 256 < 256 ? (per + 255) / 256 : 256); }
 
 void launch_adj_m(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
@@ -445,6 +618,19 @@ void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const
   const int64_t per = (int64_t)B.npad * B.npad;
   hipLaunchKernelGGL(k_adj_extract, dim3(grid_rows(per), B.nb), dim3(256), 0, s, B, V, kn, dz, T2, ldt, sigma,
                      Dbar);
+}
+void launch_adj_e64(hipStream_t s, int npad, int nb, const float2* F, double2* E) {
+  hipLaunchKernelGGL(k_adj_e64, dim3(grid_rows((int64_t)npad * npad), nb), dim3(256), 0, s, npad, F, E);
+}
+void launch_adj_pair32(hipStream_t s, int npad, int nb, float2* A, const double2* Ebar, double theta, int smax,
+                       int32_t* sq, int32_t* err, double* sigma, float2* Y, float* xmax, float* ymax) {
+  hipLaunchKernelGGL(k_adj_pair32, dim3(nb), dim3(256), 2 * npad * sizeof(double), s, npad, A, Ebar, theta, smax, sq,
+                     err, sigma, Y, xmax, ymax);
+}
+void launch_adj_extract32(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
+                          const float2* G, const double* sigma, double2* Dbar) {
+  const int nt = B.npad / 32;
+  hipLaunchKernelGGL(k_adj_extract32, dim3(nt * nt, B.nb), dim3(256), 0, s, B, V, kn, dz, G, sigma, Dbar);
 }
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw) {

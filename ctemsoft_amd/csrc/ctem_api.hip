@@ -128,10 +128,10 @@ struct Crystal {
 // on one context are sequential); it grows to the largest chunk run and is kept.
 struct Work {
   DBuf A, W, Psi, sq, dirs, partial, wmax, colsum, order, amod;
-  DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig;
+  DBuf g_dots, g_lut, g_dots2, aM, aQ, aPhi, aHm, aEbar, aBlk, aW2, aDbar, aSig, aMax;
   void release() {
     for (DBuf* b : {&A, &W, &Psi, &sq, &dirs, &partial, &wmax, &colsum, &order, &amod, &g_dots, &g_lut, &g_dots2, &aM, &aQ, &aPhi,
-                    &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig})
+                    &aHm, &aEbar, &aBlk, &aW2, &aDbar, &aSig, &aMax})
       b->release();
   }
 };
@@ -1183,6 +1183,96 @@ int32_t expm_depth_split(ctem_plan* tp, hipStream_t s, int npad, int nb, float2*
   return CTEM_OK;
 }
 
+// The Frechet derivative L_exp(A, D0) in the mixed form, from the pair (B, D) = (A 2^-s, D0 2^-s)
+// in slots 0 and 1 of W32 (maxima in wmax slots 0 and 1): the degree-8 Taylor polynomial of
+// expm_depth_mixed and its derivative in the direction D by the product rule,
+//   B2 = B B                 dB2 = B D + D B
+//   P  = x2 B2 + x1 B        dP  = x2 dB2 + x1 D
+//   L  = B2 P + x3 B2        dL  = B2 dP + dB2 P + x3 dB2
+//   R  = x7 B2 P + x4 I + x6 B2 + x5 B
+//                            dR  = x7 (B2 dP + dB2 P) + x6 dB2 + x5 D
+//   F0 = L R + B + y2 B2     G0  = L dR + dL R + D + y2 dB2
+// (the upper-right block of the same polynomial of [[B, D], [0, B]]), then s squarings of the pair
+// in the E = I + F form, F' = F F + 2F, G' = F G + G F + 2G, so that every product acts on the
+// small part. Products on split-FP16 MFMA from FP32 storage (k_cgemm), nine for the polynomial and
+// three per squaring, each matrix leaving the squarings after its own count. W32: 16 slots of nb
+// matrices; wmax: 16 maxima per matrix (slots 0 and 1 set, the rest zeroed here). Returns G
+// (slot 13) or null with rc_out set.
+float2* expm_pair_mixed(hipStream_t s, int npad, int nb, float2* W32, float* wmax, int32_t* d_sq, int32_t* d_err,
+                        std::vector<int32_t>& h_sq, int32_t& rc_out) {
+  const int64_t per = (int64_t)npad * npad;
+  float2* S[16];
+  float* m[16];
+  for (int i = 0; i < 16; i++) {
+    S[i] = W32 + i * per * nb;
+    m[i] = wmax + i * nb;
+  }
+  rc_out = CTEM_OK;
+  launch_fill_zero(s, m[2], 14 * sizeof(float) * (size_t)nb);
+  h_sq.resize(nb);
+  if (hipMemcpyAsync(h_sq.data(), d_sq, 4 * (size_t)nb, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    rc_out = fail(CTEM_E_HIP, "expm_pair_mixed: reading the squaring counts");
+    return nullptr;
+  }
+  int smax = 0;
+  for (int b = 0; b < nb; b++) smax = std::max(smax, h_sq[b]);
+  // C = X Y + sum beta_j M_j (+ C2 = alpha2 X Y + betaI2 I + sum beta2_j M_j)
+  struct Term { int slot; float beta, beta2; };
+  auto gemm = [&](int c, bool cmax, int x, int y, std::initializer_list<Term> ms, int c2 = -1, float alpha2 = 0.f,
+                  float betaI2 = 0.f, const int32_t* active = nullptr, int step = 0) -> int32_t {
+    CGemm g{};
+    g.m = g.n = g.k = npad;
+    g.ldx = g.ldy = g.ldc = g.ldm = npad;
+    g.X = S[x]; g.sx = per; g.Y = S[y]; g.sy = per; g.C = S[c]; g.sc = per; g.sm = per;
+    g.alpha = 1.0f;
+    int q = 0;
+    for (const Term& t : ms) { g.M[q] = S[t.slot]; g.beta[q] = t.beta; g.beta2[q] = t.beta2; q++; }
+    g.nb = nb; g.terms = CTEM_SPLIT_FP16X3; g.err = d_err;
+    g.xmax = m[x]; g.ymax = m[y]; g.cmax = cmax ? m[c] : nullptr;
+    if (c2 >= 0) { g.C2 = S[c2]; g.cmax2 = m[c2]; g.alpha2 = alpha2; g.betaI2 = betaI2; }
+    g.active = active; g.step = step;
+    launch_cgemm(s, g);
+    HIPCHK(hipGetLastError());
+    return CTEM_OK;
+  };
+  const float x1 = kBbc8[0], x2 = kBbc8[1], x3 = kBbc8[2], x4 = kBbc8[3], x5 = kBbc8[4], x6 = kBbc8[5],
+              x7 = kBbc8[6], y2 = kBbc8[7];
+  enum { sB = 0, sD, sB2, sP, sT, sdB2, sdP, sL, sR, sdL, sdR, sF0, sT3, sG0, sF1, sG1 };
+  int32_t rc;
+  if ((rc = gemm(sB2, true, sB, sB, {{sB, 0.f, x1}}, sP, x2)) ||                              // B2, P
+      (rc = gemm(sT, false, sD, sB, {})) ||                                                 // D B
+      (rc = gemm(sdB2, true, sB, sD, {{sT, 1.f, x2}, {sD, 0.f, x1}}, sdP, x2)) ||             // dB2, dP
+      (rc = gemm(sL, true, sB2, sP, {{sB2, x3, x6}, {sB, 0.f, x5}}, sR, x7, x4)) ||            // L, R
+      (rc = gemm(sT, false, sdB2, sP, {})) ||                                               // dB2 P
+      (rc = gemm(sdL, true, sB2, sdP, {{sT, 1.f, x7}, {sdB2, x3, x6}, {sD, 0.f, x5}}, sdR, x7)) ||  // dL, dR
+      (rc = gemm(sF0, true, sL, sR, {{sB, 1.f, 0.f}, {sB2, y2, 0.f}})) ||                     // F0
+      (rc = gemm(sT3, false, sdL, sR, {})) ||                                               // dL R
+      (rc = gemm(sG0, true, sL, sdR, {{sT3, 1.f, 0.f}, {sD, 1.f, 0.f}, {sdB2, y2, 0.f}}))) {  // G0
+    rc_out = rc;
+    return nullptr;
+  }
+  const int fb[2] = {sF0, sF1}, gb[2] = {sG0, sG1};
+  for (int i = 0; i < smax; i++) {
+    const int f = fb[i & 1], fo = fb[(i + 1) & 1], gi = gb[i & 1], go = gb[(i + 1) & 1];
+    // the output slots' maxima were those of two steps back
+    launch_fill_zero(s, m[fo], sizeof(float) * (size_t)nb);
+    launch_fill_zero(s, m[go], sizeof(float) * (size_t)nb);
+    if ((rc = gemm(fo, true, f, f, {{f, 2.f, 0.f}}, -1, 0.f, 0.f, d_sq, i)) ||            // F F + 2F
+        (rc = gemm(sT, false, gi, f, {}, -1, 0.f, 0.f, d_sq, i)) ||                        // G F
+        (rc = gemm(go, true, f, gi, {{sT, 1.f, 0.f}, {gi, 2.f, 0.f}}, -1, 0.f, 0.f, d_sq, i))) {  // F G + G F + 2G
+      rc_out = rc;
+      return nullptr;
+    }
+  }
+  launch_cparity_copy(s, npad, nb, d_sq, S[sG1], S[sG0]);
+  if (hipGetLastError() != hipSuccess) {
+    rc_out = fail(CTEM_E_HIP, "expm_pair_mixed: launch failed");
+    return nullptr;
+  }
+  return S[sG0];
+}
+
 }  // namespace
 
 
@@ -1601,20 +1691,26 @@ extern "C" int32_t ctem_plan_get_lgh(ctem_plan* p, double* lgh, int32_t cap, int
 
 // ------------------------------------------------------------------------------------------
 // reverse mode (FP64): gradient of f(dots) with respect to the LUT and the site B factors
-extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* glut, double* gb) {
-  if (!p || !gdots || !glut) return fail(CTEM_E_ARG, "ctem_plan_adjoint: null argument");
+// The reverse pass (both forms). FP64: forward and Frechet exponential in FP64 (the pair form of
+// expm_pair_fp64). Mixed: the forward is the plan's own mixed form (so psi, and the intensities
+// the caller's loss saw, are those of ctem_plan_run in mixed precision), the backward depth
+// recursion and E_bar stay FP64, and the Frechet exponential runs on split-FP16 products
+// (expm_pair_mixed). Squaring stages are recomputed, not stored.
+static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, double* glut, double* gb) {
+  const char* who = mixed ? "ctem_plan_adjoint_mixed" : "ctem_plan_adjoint";
+  if (!p || !gdots || !glut) return fail(CTEM_E_ARG, "%s: null argument", who);
   ctem_ctx* c = p->ctx;
   if (!p->cr) p->cr = c->cr;
-  if (!p->cr) return fail(CTEM_E_STATE, "ctem_plan_adjoint: no crystal uploaded");
-  if (p->cr->nsite > 0 && !gb) return fail(CTEM_E_ARG, "ctem_plan_adjoint: gb is NULL");
-  if (p->ecp_mode >= 0) return fail(CTEM_E_STATE, "ctem_plan_adjoint: not for a channeling-pattern plan");
+  if (!p->cr) return fail(CTEM_E_STATE, "%s: no crystal uploaded", who);
+  if (p->cr->nsite > 0 && !gb) return fail(CTEM_E_ARG, "%s: gb is NULL", who);
+  if (p->ecp_mode >= 0) return fail(CTEM_E_STATE, "%s: not for a channeling-pattern plan", who);
   int32_t rc;
   if (!p->selected && (rc = ctem_plan_select(p, nullptr, nullptr))) return rc;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   const int nk = p->numk, ns = std::max(1, p->cr->nsite), nz = p->nz;
   for (int i = 0; i < nk; i++)
-    if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "ctem_plan_adjoint: at most 512 strong beams per direction");
+    if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "%s: at most 512 strong beams per direction", who);
   if ((rc = upload(c->w.g_dots, gdots, (size_t)nk * p->cr->nsite, s)) || (rc = c->w.g_lut.ensure(16 * (size_t)p->cr->ncell)) ||
       (rc = c->w.g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
       (rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * ns)))
@@ -1634,6 +1730,9 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
   DevLut LdB = L;
   LdB.slut = p->cr->dslut.as<double2>();
   const double dzt = p->dz / p->thick;
+  const int form = run_form(p, CTEM_PREC_MIXED);
+  const bool split = mixed && form == 1;
+  const bool sym = split && p->symmetry == 1 && p->cr->centro;
   const size_t budget = (size_t)16 << 30;
   size_t off = 0;
   std::vector<int32_t> h_sq, h_sq2;
@@ -1646,15 +1745,30 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nbk, budget / per_dir));
     for (int c0 = 0; c0 < nbk; c0 += chunk) {
       const int nb = std::min(chunk, nbk - c0);
-      if ((rc = c->w.A.ensure((size_t)nb * per * 16)) || (rc = c->w.W.ensure((size_t)5 * nb * per * 16)) ||
-          (rc = c->w.Psi.ensure((size_t)nb * spsi * 16)) || (rc = c->w.sq.ensure(4 * (size_t)nb)) ||
-          (rc = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
-          (rc = c->w.aM.ensure((size_t)nb * per * 16)) || (rc = c->w.aQ.ensure((size_t)nb * spsi * 16)) ||
-          (rc = c->w.aPhi.ensure((size_t)nb * spsi * 16)) || (rc = c->w.aHm.ensure((size_t)nb * spsi * 16)) ||
-          (rc = c->w.aEbar.ensure((size_t)nb * per * 16)) || (rc = c->w.aBlk.ensure((size_t)2 * nb * per * 16)) ||
-          (rc = c->w.aW2.ensure((size_t)10 * nb * per * 16)) || (rc = c->w.aDbar.ensure((size_t)nb * per * 16)) ||
-          (rc = c->w.aSig.ensure(8 * (size_t)nb)))
-        return rc;
+      // buffers shared with ctem_plan_run: a move changes the addresses its captured graphs hold
+      const void* before[7] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p, c->w.partial.p, c->w.wmax.p, c->w.colsum.p};
+      auto grow = [&]() -> int32_t {
+        int32_t r;
+        if ((r = c->w.A.ensure((size_t)nb * per * 16)) || (r = c->w.W.ensure((size_t)5 * nb * per * 16)) ||
+            (r = c->w.Psi.ensure((size_t)nb * spsi * 16)) || (r = c->w.sq.ensure(4 * (size_t)nb)) ||
+            (r = c->w.partial.ensure(8 * (size_t)nb * (npad / 32) * (npad / 32) * ns)) ||
+            (r = c->w.aM.ensure((size_t)nb * per * 16)) || (r = c->w.aQ.ensure((size_t)nb * spsi * 16)) ||
+            (r = c->w.aPhi.ensure((size_t)nb * spsi * 16)) || (r = c->w.aHm.ensure((size_t)nb * spsi * 16)) ||
+            (r = c->w.aEbar.ensure((size_t)nb * per * 16)) || (r = c->w.aBlk.ensure((size_t)2 * nb * per * 16)) ||
+            (r = c->w.aW2.ensure((size_t)10 * nb * per * 16)) || (r = c->w.aDbar.ensure((size_t)nb * per * 16)) ||
+            (r = c->w.aSig.ensure(8 * (size_t)nb)))
+          return r;
+        if (mixed && ((r = c->w.wmax.ensure(13 * sizeof(float) * (size_t)nb)) ||
+                      (r = c->w.colsum.ensure(sizeof(double) * (size_t)nb * npad * ((npad + 63) / 64))) ||
+                      (r = c->w.aMax.ensure(16 * sizeof(float) * (size_t)nb))))
+          return r;
+        return CTEM_OK;
+      };
+      rc = grow();
+      const void* after[7] = {c->w.A.p, c->w.W.p, c->w.Psi.p, c->w.sq.p, c->w.partial.p, c->w.wmax.p, c->w.colsum.p};
+      for (int i = 0; i < 7; i++)
+        if (before[i] != after[i]) c->wgen++;
+      if (rc) return rc;
       Bucket B{npad, nb, c->w.dirs.as<int32_t>() + off + c0};
       std::vector<int> hn(nb);
       int max_nw = 0;
@@ -1663,14 +1777,40 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
         hn[b] = p->h_nns[d];
         max_nw = std::max(max_nw, p->h_nnw[d]);
       }
-      double2* A = c->w.A.as<double2>();
+      const double* kn = p->kn.as<double>();
       // forward: E and psi_1..psi_Z
-      launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, A);
-      HIPCHK(hipGetLastError());
-      double2* E = expm_fp64(nullptr, s, npad, nb, A, c->w.W.as<double2>(), c->w.sq.as<int32_t>(),
-                             p->err.as<int32_t>(), hn, h_sq, rc);
-      if (!E) return rc;
-      launch_depth(s, npad, nb, nz, E, 0, 0, c->w.Psi.as<double2>(), spsi, c->cu_count);
+      double2* E = nullptr;
+      if (mixed) {
+        float2* W32 = c->w.W.as<float2>();
+        float* wmax = c->w.wmax.as<float>();
+        ChunkSched cs;
+        if (split) {
+          float* amax = wmax + 12 * (size_t)nb;
+          launch_fill_zero(s, amax, sizeof(float) * (size_t)nb);
+          launch_assemble_split(s, B, V, R, L, kn, p->cr->mlambda, p->cr->dynupz, p->dz, p->cr->lutmax, max_nw, W32,
+                                c->w.colsum.as<double>(), amax, sym);
+          HIPCHK(hipGetLastError());
+          rc = expm_depth_split(nullptr, s, npad, nb, W32, c->w.colsum.as<double>(), amax, wmax, c->w.Psi.as<double2>(),
+                                spsi, nz, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn, h_sq, c->cu_count, sym, &cs);
+        } else {
+          launch_assemble32(s, B, V, R, L, kn, p->cr->mlambda, p->cr->dynupz, p->dz, W32);
+          HIPCHK(hipGetLastError());
+          rc = expm_depth_mixed(nullptr, s, npad, nb, W32, wmax, c->w.Psi.as<double2>(), spsi, nz, c->w.sq.as<int32_t>(),
+                                p->err.as<int32_t>(), hn, h_sq, p->terms_taylor, p->terms_square, c->cu_count, &cs);
+        }
+        if (rc) return rc;
+        // the final F = E - I in FP32: buffer 3 of the split form, buffer 4 of the other
+        E = c->w.A.as<double2>();
+        launch_adj_e64(s, npad, nb, W32 + (split ? 3 : 4) * per * nb, E);
+      } else {
+        double2* A = c->w.A.as<double2>();
+        launch_assemble(s, B, V, R, L, kn, p->cr->mlambda, p->cr->dynupz, p->dz, A);
+        HIPCHK(hipGetLastError());
+        E = expm_fp64(nullptr, s, npad, nb, A, c->w.W.as<double2>(), c->w.sq.as<int32_t>(), p->err.as<int32_t>(), hn,
+                      h_sq, rc);
+        if (!E) return rc;
+        launch_depth(s, npad, nb, nz, E, 0, 0, c->w.Psi.as<double2>(), spsi, c->cu_count);
+      }
       HIPCHK(hipGetLastError());
       // B: d dots_p / d B_p = Re sum Lgh dS_p/dB_p
       launch_lgh_dots(s, B, V, R, LdB, c->w.Psi.as<double2>(), spsi, nz, p->lam.as<double>(), dzt,
@@ -1705,18 +1845,33 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
         launch_zgemm(s, g);
         HIPCHK(hipGetLastError());
       }
-      // A_bar = L_exp(A^H, Ebar), the upper-right block of expm([[A^H, sigma Ebar], [0, A^H]]) taken in
-      // the pair form; A rebuilt unscaled
-      launch_assemble(s, B, V, R, L, p->kn.as<double>(), p->cr->mlambda, p->cr->dynupz, p->dz, A);
-      double2* PX = c->w.aBlk.as<double2>();
-      double2* PY = PX + per * nb;
-      launch_adj_pair(s, npad, nb, A, c->w.aEbar.as<double2>(), c->w.aSig.as<double>(), PX, PY);
-      HIPCHK(hipGetLastError());
-      double2* T2 = expm_pair_fp64(s, npad, nb, PX, PY, c->w.aW2.as<double2>(), c->w.sq.as<int32_t>(),
-                                   p->err.as<int32_t>(), h_sq2, rc);
-      if (!T2) return rc;
-      launch_adj_extract(s, B, V, p->kn.as<double>(), p->dz, T2 + 5 * per * nb, npad, c->w.aSig.as<double>(),
-                         c->w.aDbar.as<double2>());
+      if (mixed) {
+        // A_bar = L_exp(A, Ebar^H)^H: the pair (A, sigma Ebar^H) from A rebuilt unscaled in FP32
+        float2* PW = c->w.aW2.as<float2>();
+        float* pm = c->w.aMax.as<float>();
+        launch_fill_zero(s, pm, 2 * sizeof(float) * (size_t)nb);
+        launch_assemble32(s, B, V, R, L, kn, p->cr->mlambda, p->cr->dynupz, p->dz, PW);
+        launch_adj_pair32(s, npad, nb, PW, c->w.aEbar.as<double2>(), kThetaSplit, kSmax, c->w.sq.as<int32_t>(),
+                          p->err.as<int32_t>(), c->w.aSig.as<double>(), PW + per * nb, pm, pm + nb);
+        HIPCHK(hipGetLastError());
+        float2* G = expm_pair_mixed(s, npad, nb, PW, pm, c->w.sq.as<int32_t>(), p->err.as<int32_t>(), h_sq2, rc);
+        if (!G) return rc;
+        launch_adj_extract32(s, B, V, kn, p->dz, G, c->w.aSig.as<double>(), c->w.aDbar.as<double2>());
+      } else {
+        // A_bar = L_exp(A^H, Ebar), the upper-right block of expm([[A^H, sigma Ebar], [0, A^H]]) taken in
+        // the pair form; A rebuilt unscaled
+        double2* A = c->w.A.as<double2>();
+        launch_assemble(s, B, V, R, L, kn, p->cr->mlambda, p->cr->dynupz, p->dz, A);
+        double2* PX = c->w.aBlk.as<double2>();
+        double2* PY = PX + per * nb;
+        launch_adj_pair(s, npad, nb, A, c->w.aEbar.as<double2>(), c->w.aSig.as<double>(), PX, PY);
+        HIPCHK(hipGetLastError());
+        double2* T2 = expm_pair_fp64(s, npad, nb, PX, PY, c->w.aW2.as<double2>(), c->w.sq.as<int32_t>(),
+                                     p->err.as<int32_t>(), h_sq2, rc);
+        if (!T2) return rc;
+        launch_adj_extract(s, B, V, kn, p->dz, T2 + 5 * per * nb, npad, c->w.aSig.as<double>(),
+                           c->w.aDbar.as<double2>());
+      }
       HIPCHK(hipGetLastError());
       // LUT entries
       launch_adj_lut(s, B, V, R, L, p->cr->mlambda, c->w.aDbar.as<double2>(), c->w.g_lut.as<double>(), max_nw);
@@ -1728,13 +1883,21 @@ extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* 
   HIPCHK(hipMemcpyAsync(glut, c->w.g_lut.p, 16 * (size_t)p->cr->ncell, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(d2.data(), c->w.g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  if ((rc = check_err_flag(p->err, s, "ctem_plan_adjoint"))) return rc;
+  if ((rc = check_err_flag(p->err, s, who))) return rc;
   for (int q = 0; q < p->cr->nsite; q++) {
     double acc = 0.0;
     for (int i = 0; i < nk; i++) acc += gdots[(size_t)i * p->cr->nsite + q] * d2[(size_t)i * p->cr->nsite + q];
     gb[q] = acc;
   }
   return CTEM_OK;
+}
+
+extern "C" int32_t ctem_plan_adjoint(ctem_plan* p, const double* gdots, double* glut, double* gb) {
+  return plan_adjoint(p, false, gdots, glut, gb);
+}
+
+extern "C" int32_t ctem_plan_adjoint_mixed(ctem_plan* p, const double* gdots, double* glut, double* gb) {
+  return plan_adjoint(p, true, gdots, glut, gb);
 }
 
 extern "C" int32_t ctem_plan_dynmat(ctem_plan* p, int32_t ik, double* dynmat, int32_t cap, int32_t* n) {

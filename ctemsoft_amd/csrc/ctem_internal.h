@@ -368,5 +368,14 @@ void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const
                         const double2* T2, int ldt, const double* sigma, double2* Dbar);
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw);
+// mixed form: E = I + F in FP64 from the FP32 F
+void launch_adj_e64(hipStream_t s, int npad, int nb, const float2* F, double2* E);
+// the pair (A 2^-s, sigma 2^-s Ebar^H) in FP32 (A scaled in place), its squaring count and the
+// max |re|, |im| of both parts (xmax, ymax zeroed by the caller)
+void launch_adj_pair32(hipStream_t s, int npad, int nb, float2* A, const double2* Ebar, double theta, int smax,
+                       int32_t* sq, int32_t* err, double* sigma, float2* Y, float* xmax, float* ymax);
+// D_bar from the derivative part G = L_exp(A, sigma Ebar^H) (FP32)
+void launch_adj_extract32(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
+                          const float2* G, const double* sigma, double2* Dbar);
 
 }  // namespace ctem

@@ -32,7 +32,7 @@ EXPORTS = ["ctem_ctx_create", "ctem_ctx_destroy", "ctem_last_error", "ctem_devic
            "ctem_plan_destroy", "ctem_plan_select", "ctem_plan_get_beams", "ctem_plan_run",
            "ctem_plan_get_dots", "ctem_plan_get_svals", "ctem_plan_get_lgh", "ctem_plan_dynmat",
            "ctem_plan_scatter", "ctem_plan_timing", "ctem_calclgh3", "ctem_plan_set_mixed_terms",
-           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_alg_flops", "ctem_calclgh3_static",
+           "ctem_plan_set_symmetry", "ctem_plan_adjoint", "ctem_plan_adjoint_mixed", "ctem_plan_alg_flops", "ctem_calclgh3_static",
            "ctem_plan_set_graph", "ctem_plans_run", "ctem_plan_set_tuning"]
 
 
@@ -77,7 +77,8 @@ def lib():
             "ctem_plan_set_symmetry": [P, I32, P],
             "ctem_calclgh3": [P, P, P, D, D, I32, I32, D, P, I32],
             "ctem_calclgh3_static": [P, P, P, D, D, I32, I32, D, P, I32],
-            "ctem_plan_adjoint": [P, P, P, P], "ctem_plan_alg_flops": [P, P, P, P],
+            "ctem_plan_adjoint": [P, P, P, P], "ctem_plan_adjoint_mixed": [P, P, P, P],
+            "ctem_plan_alg_flops": [P, P, P, P],
             "ctem_plan_set_graph": [P, I32], "ctem_plans_run": [P, I32, I32],
             "ctem_plan_set_tuning": [P, I32, D],
         }
@@ -337,15 +338,20 @@ class Plan:
         _check(lib().ctem_plan_get_dots(self._h, _p(d)))
         return d.reshape(self.numk, self.nsite)
 
-    def adjoint(self, gdots):
-        """FP64 reverse mode: gdots[ik, p] = df/ddots -> (df/dU as a complex array shaped like
-        the uploaded LUT, [l][k][h], with df/dRe U + i df/dIm U; df/dB per site)."""
+    def adjoint(self, gdots, precision="fp64"):
+        """Reverse mode: gdots[ik, p] = df/ddots -> (df/dU as a complex array shaped like the
+        uploaded LUT, [l][k][h], with df/dRe U + i df/dIm U; df/dB per site). precision "fp64"
+        (ctem_plan_adjoint) or "mixed" (ctem_plan_adjoint_mixed: the mixed forward, split-FP16
+        products in the derivative of the exponential)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         st = self.setup
         g = np.ascontiguousarray(gdots, np.float64).reshape(self.numk, self.nsite)
         ncell = int(np.prod(st.lut_dim))
         glut = np.zeros(2 * ncell)
         gb = np.zeros(max(1, self.nsite))
-        _check(lib().ctem_plan_adjoint(self._h, _p(g), _p(glut), _p(gb)))
+        f = lib().ctem_plan_adjoint if precision == "fp64" else lib().ctem_plan_adjoint_mixed
+        _check(f(self._h, _p(g), _p(glut), _p(gb)))
         d = st.lut_dim
         return glut.view(np.complex128).reshape(d[2], d[1], d[0]), gb[:self.nsite].copy()
 

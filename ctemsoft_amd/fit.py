@@ -1,6 +1,6 @@
 """Master-pattern residual and its gradient with respect to the LUT (complex Ucg) and the site
 Debye-Waller factors, on top of the C-ABI (ctem_plan_run / ctem_plan_scatter /
-ctem_plan_adjoint).
+ctem_plan_adjoint, ctem_plan_adjoint_mixed).
 
     loss = sum over sites and Lambert pixels of (sr - sr_target)^2
 
@@ -39,8 +39,9 @@ def master_loss(plan, ij, npx, npy, isym, target, precision="fp64"):
 
 
 def master_loss_grad(plan, ij, npx, npy, isym, target, owner=None, precision="fp64"):
-    """(loss, dloss/dLUT shaped like the LUT [l][k][h], dloss/dB per site). The loss value
-    comes from the chosen forward precision, the gradient from the FP64 reverse pass."""
+    """(loss, dloss/dLUT shaped like the LUT [l][k][h], dloss/dB per site). precision "fp64":
+    FP64 forward and reverse pass; "mixed": the loss from the mixed forward and the gradient from
+    the mixed reverse pass (ctem_plan_adjoint_mixed, which reruns the same mixed forward)."""
     loss, sr = master_loss(plan, ij, npx, npy, isym, target, precision)
     if owner is None:
         owner = pixel_owner(ij, npx, npy, isym)
@@ -51,7 +52,7 @@ def master_loss_grad(plan, ij, npx, npy, isym, target, owner=None, precision="fp
     m = owner >= 0
     for p in range(nsite):
         np.add.at(gsv[:, p], owner[m], 2.0 * r[p, m])
-    glut, gb = plan.adjoint(gsv / nat)
+    glut, gb = plan.adjoint(gsv / nat, precision)
     return loss, glut, gb
 
 

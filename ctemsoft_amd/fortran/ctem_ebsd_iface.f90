@@ -157,6 +157,16 @@ interface
     integer(c_int32_t) :: rc
   end function
 
+  ! the same gradient with the mixed forward and split-FP16 products in the exponential's derivative
+  function ctem_plan_adjoint_mixed(plan, gdots, glut, gb) bind(C, name='ctem_plan_adjoint_mixed') result(rc)
+    import :: c_int32_t, c_double, c_double_complex, c_ptr
+    type(c_ptr), value :: plan
+    real(c_double), intent(in) :: gdots(*)
+    complex(c_double_complex), intent(out) :: glut(*)
+    real(c_double), intent(out) :: gb(*)
+    integer(c_int32_t) :: rc
+  end function
+
   ! same argument list and meaning as the reference's external CalcLgh3
   ! (CTEMEBSDmaster.f90:610-630), with the context first and no kin/debug
   function ctem_calclgh3(ctx, dmat, lgh, thick, kn, nn, gzero, depthstep, lambdae, izz) &

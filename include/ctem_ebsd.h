@@ -183,9 +183,15 @@ CTEM_API int32_t ctem_plan_dynmat(ctem_plan* plan, int32_t ik, double* dynmat, i
  * glut[2*cell+1] imaginary part, cells in the Fortran order of the uploaded LUT) and df/dB_p
  * per site (gb[p], B in nm^2). The beam sets are held fixed; the forward matrices are
  * recomputed, not stored. The reference has no derivatives; this entry serves the fitting of
- * Ucg and Debye-Waller factors to a measured master pattern (the mixed forward is the
- * intended inner loop, this gradient the outer step). Up to 512 strong beams per direction. */
+ * Ucg and Debye-Waller factors to a measured master pattern. Up to 512 strong beams per
+ * direction. */
 CTEM_API int32_t ctem_plan_adjoint(ctem_plan* plan, const double* gdots, double* glut, double* gb);
+/* The same gradient with the mixed forward: psi (and so the intensities the function of gdots was
+ * evaluated on) from the plan's mixed form of ctem_plan_run, the backward depth recursion in FP64,
+ * and the Frechet derivative of the exponential on split-FP16 products with FP32 accumulation,
+ * its squaring stages recomputed. No counterpart in the reference, as ctem_plan_adjoint. Same
+ * arguments and limits. */
+CTEM_API int32_t ctem_plan_adjoint_mixed(ctem_plan* plan, const double* gdots, double* glut, double* gb);
 
 /* Scatter svals into a Lambert master array sr[nsite][2*npy+1][2*npx+1] (float32), applying
  * the Laue-symmetry equivalents per direction (CTEMEBSDmaster.f90:462-479). ij holds the

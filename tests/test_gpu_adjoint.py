@@ -181,3 +181,42 @@ def test_gradient_matches_complex_step(ctx, oracle_mod):
     refb = sum(c[2] * cs.derivative_b(*args(c), st["lut"], st["site_b"], 0)[0] for c in cases)
     _say(f"B: adjoint {gb[0]:.12e} complex step {refb:.12e}")
     assert abs(gb[0] - refb) <= 1e-8 * abs(refb)
+
+
+# the mixed reverse pass (ctem_plan_adjoint_mixed) against the FP64 one on the same directions and
+# weights: the plan's mixed forward in each of its forms (split-FP16 storage with symmetric
+# products, with full products, and the split-BF16 options on FP32 storage), a bcc and a two-site
+# trigonal crystal
+MIXED_CASES = [("ni_c0w", 1, 16), ("ni_c0w", 0, 16), ("ni_c0w", 1, 3), ("fe_c1w", 1, 16), ("al2o3_c2w", 1, 16)]
+
+
+@pytest.mark.parametrize("name,symmetry,terms", MIXED_CASES)
+def test_mixed_adjoint_matches_fp64(ctx, name, symmetry, terms):
+    from ctemsoft_amd import ebsd
+    fx = load_golden(name)
+    rng = np.random.default_rng(21)
+    ks = np.sort(rng.choice(len(fx["kvec"]), 12, replace=False))
+    # corundum: the full-box LUT (a reference-sized one does not hold every g-h of its list)
+    st = ebsd.Setup.from_fixture(fx, load_golden("al2o3_c2") if name.startswith("al2o3") else None)
+    ctx.upload_crystal(st)
+    plan = ctx.plan(fx["kvec"][ks], fx["kn"][ks], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+    plan.set_symmetry(symmetry)
+    plan.set_mixed_terms(terms, terms)
+    plan.run("fp64")
+    d64 = plan.dots().copy()
+    w = rng.standard_normal(d64.shape)
+    g64, b64 = plan.adjoint(w)
+    plan.run("mixed")
+    dmx = plan.dots().copy()
+    gmx, bmx = plan.adjoint(w, "mixed")
+    # the mixed reverse pass reruns the plan's mixed forward: the same intensities
+    plan.run("mixed")
+    assert np.array_equal(plan.dots(), dmx)
+    plan.close()
+    assert np.isfinite(gmx).all() and np.isfinite(bmx).all()
+    eg = np.abs(gmx - g64).max() / np.abs(g64).max()
+    eb = np.abs(bmx - b64).max() / np.abs(b64).max()
+    ed = np.abs(dmx - d64).max() / np.abs(d64).max()
+    _say(f"{name} symmetry {symmetry} terms {terms}: forward {ed:.2e}, grad LUT {eg:.2e}, grad B {eb:.2e}")
+    tol = 2e-3 if terms == 3 else 2e-4
+    assert eg <= tol and eb <= tol, (eg, eb)

@@ -25,8 +25,11 @@ def _problem(npx=20):
     return fx, st, wl, ij, k, kn, fit.perturbed_target_setup(st, wl.seed)
 
 
-@pytest.mark.parametrize("npx", [20, 100])
-def test_master_gradient_directional_and_descent(npx):
+@pytest.mark.parametrize("npx,precision", [(20, "fp64"), (100, "fp64"), (100, "mixed")])
+def test_master_gradient_directional_and_descent(npx, precision):
+    """The residual's gradient against a central difference of the FP64 loss along a random
+    direction of every LUT entry; precision "mixed" takes loss and gradient from the mixed forward
+    and reverse pass (the workload's mixed form), checked against the same FP64 difference."""
     from ctemsoft_amd import ebsd, fit
     fx, st, wl, ij, k, kn, st_t = _problem(npx)
     npx = int(ij[:, 0].max())
@@ -44,16 +47,17 @@ def test_master_gradient_directional_and_descent(npx):
     owner = fit.pixel_owner(ij, npx, npx, isym)
     import time
     t0 = time.perf_counter()
-    loss, glut, gb = fit.master_loss_grad(plan, ij, npx, npx, isym, target, owner)
+    loss, glut, gb = fit.master_loss_grad(plan, ij, npx, npx, isym, target, owner, precision)
     t1 = time.perf_counter()
-    _say(f"{len(k)} directions, loss {loss:.6e}, |grad LUT| {np.abs(glut).max():.3e}, grad B {gb}, "
+    _say(f"{precision}: {len(k)} directions, loss {loss:.6e}, |grad LUT| {np.abs(glut).max():.3e}, grad B {gb}, "
          f"forward + reverse {t1 - t0:.2f} s")
     assert loss > 0 and np.isfinite(glut).all() and np.isfinite(gb).all()
     # the loss from the mixed forward (the inner loop of a fit) against the FP64 one: the
     # residuals are ~1e-2 of the intensities, the mixed form's error ~1e-5 of them
     loss_m, _ = fit.master_loss(plan, ij, npx, npx, isym, target, precision="mixed")
-    _say(f"loss fp64 {loss:.8e} mixed {loss_m:.8e}")
-    assert abs(loss_m - loss) <= 1e-2 * loss
+    loss_d, _ = fit.master_loss(plan, ij, npx, npx, isym, target, precision="fp64")
+    _say(f"loss fp64 {loss_d:.8e} mixed {loss_m:.8e}")
+    assert abs(loss_m - loss_d) <= 1e-2 * loss_d
     plan.close()
 
     d = st.lut_dim
