@@ -28,17 +28,19 @@ use kvectors
 
 IMPLICIT NONE
 
-character(fnlen)  :: xtalname, lamfile, kselfile, outname
+character(fnlen)  :: xtalname, lamfile, kselfile, outname, kfile
 real(kind=sgl)    :: ekev, dmin
 integer(kind=irg) :: npx, izz, dolgh, nsel, i, j, ik, isel, npy, npyhex, isym, numk, ijmax, nref, ip, n
 real(kind=dbl)    :: depthstep, thick, kin
+real(kind=dbl)    :: kcur(3), ktcur(3), kncur
+real(kind=dbl),allocatable :: kf(:,:), knf(:)
 logical           :: usehex, switchmirror
 integer(kind=irg),parameter :: LaueTest(11) = (/ 149, 151, 153, 156, 158, 160, 161, 164, 165, 166, 167 /)
 real(kind=sgl),allocatable :: lam(:)
 integer(kind=irg),allocatable :: ksel(:), kdone(:)
 complex(kind=dbl),allocatable :: Lgh(:,:)
 real(kind=dbl) :: ctmp(192,3)
-integer(kind=irg) :: gbox(3), hm(3), gg(3), ix, iy, iz, lutfull
+integer(kind=irg) :: gbox(3), hm(3), gg(3), ix, iy, iz, lutfull, nkf
 complex(kind=dbl),allocatable :: lutx(:,:,:)
 real(kind=sgl) :: gtmp(3)
 integer, parameter :: ou = 77, iu = 78
@@ -46,11 +48,14 @@ type(kvectorlist),pointer :: kp
 
 ! dolgh: 0 = beam sets only, 1 = beam sets + DynMat, 2 = beam sets + DynMat + Lgh
 ! lutfull: 1 = also dump CalcUcg over the box that holds every difference of two list entries
+! kfile: a stream file (int32 count, then per direction k(3) and kn as doubles) whose wave vectors
+!        replace the selected ones of the list (e.g. the library's southern hemisphere, which the
+!        program itself does not compute); the output numbers them 1..count
 namelist /refdrv/ xtalname, ekev, dmin, npx, izz, depthstep, thick, lamfile, kselfile, outname, dolgh, &
-                  lutfull
+                  lutfull, kfile
 
 xtalname = 'undefined'; ekev = 20.0; dmin = 0.1; npx = 50; izz = 1; depthstep = 1.D0; thick = 1.D0
-lamfile = 'undefined'; kselfile = 'undefined'; outname = 'refdump.bin'; dolgh = 0; lutfull = 0
+lamfile = 'undefined'; kselfile = 'undefined'; outname = 'refdump.bin'; dolgh = 0; lutfull = 0; kfile = 'undefined'
 
 open(unit=iu, file='refdrv.nml', status='old', DELIM='APOSTROPHE')
 read(unit=iu, nml=refdrv)
@@ -129,6 +134,22 @@ else
   close(iu)
 end if
 
+if (trim(kfile).ne.'undefined') then
+  open(unit=iu, file=trim(kfile), access='stream', form='unformatted', status='old')
+  read(iu) nkf
+  allocate(kf(3,nkf), knf(nkf))
+  do i = 1, nkf
+    read(iu) kf(1:3,i), knf(i)
+  end do
+  close(iu)
+  deallocate(ksel)
+  nsel = nkf
+  allocate(ksel(nsel))
+  do i = 1, nsel
+    ksel(i) = i
+  end do
+end if
+
 open(unit=ou, file=trim(outname), access='stream', form='unformatted', status='replace')
 ! header
 write(ou) int(20260, 4), isym, merge(1,0,usehex), numk, nref, nsel, cell%ATOM_ntype, cell%SYM_SGnum
@@ -191,12 +212,21 @@ allocate(kdone(numk))
 kp => khead
 ik = 1
 do isel = 1, nsel
-  do while (ik.lt.ksel(isel))
-    kp => kp%next
-    ik = ik + 1
-  end do
-  DynFN = kp%k
-  call Compute_DynMat('BLOCHBETHE', kp%k, kp%kt, .FALSE.)
+  if (allocated(kf)) then
+    kcur = kf(1:3,isel)
+    kncur = knf(isel)
+    ktcur = 0.D0
+  else
+    do while (ik.lt.ksel(isel))
+      kp => kp%next
+      ik = ik + 1
+    end do
+    kcur = kp%k
+    kncur = kp%kn
+    ktcur = kp%kt
+  end if
+  DynFN = kcur
+  call Compute_DynMat('BLOCHBETHE', kcur, ktcur, .FALSE.)
   write(ou) ksel(isel), BetheParameter%nns, BetheParameter%nnw
   write(ou) BetheParameter%strongID(1:BetheParameter%nns)
   write(ou) BetheParameter%strongsg(1:BetheParameter%nns)
@@ -205,7 +235,7 @@ do isel = 1, nsel
   if (dolgh.ge.1) write(ou) DynMat
   if (dolgh.ge.2) then
     allocate(Lgh(BetheParameter%nns, BetheParameter%nns))
-    call CalcLgh3(DynMat, Lgh, thick, kp%kn, BetheParameter%nns, 1, kin, 0, depthstep, lam, izz)
+    call CalcLgh3(DynMat, Lgh, thick, kncur, BetheParameter%nns, 1, kin, 0, depthstep, lam, izz)
     write(ou) Lgh
     deallocate(Lgh)
   end if

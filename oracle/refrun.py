@@ -166,7 +166,10 @@ def parse_dump(path: str, dolgh: int, lutfull: bool = False) -> dict:
 
 
 def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: int,
-               ekev: float | None = None, lutfull: bool = False) -> dict:
+               ekev: float | None = None, lutfull: bool = False, kvec=None) -> dict:
+    """The reference's beam loop body (ref_driver.f90) on the directions ksel (indices of its own
+    list, ascending), or, with kvec = (k [n][3], kn [n]), on those wave vectors instead (e.g. the
+    library's southern hemisphere, which the program does not compute); ik then counts them."""
     xt = os.path.join(work, "crystal.xtal")
     write_xtal(xt, wl.xtal)
     with open(os.path.join(work, "BetheParameters.nml"), "w") as f:
@@ -179,6 +182,12 @@ def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: 
         ks = np.asarray(ksel, np.int32) + 1
         with open(os.path.join(work, ksf), "wb") as f:
             f.write(struct.pack("<i", len(ks)) + ks.tobytes())
+    kf = "undefined"
+    if kvec is not None:
+        kf = "kvec.bin"
+        kk, kn = np.asarray(kvec[0], np.float64), np.asarray(kvec[1], np.float64)
+        with open(os.path.join(work, kf), "wb") as f:
+            f.write(struct.pack("<i", len(kn)) + np.concatenate([kk, kn[:, None]], axis=1).tobytes())
     ek = wl.ekev if ekev is None else ekev
     dz = wl.dz_for(ekev)
     thick = float(int(len(lam) * dz))  # integer thickness (CTEMEBSDmaster.f90:142,306)
@@ -186,7 +195,7 @@ def run_driver(work: str, wl: workloads.Workload, lam: np.ndarray, ksel, dolgh: 
         f.write(f" &refdrv\n xtalname = 'crystal.xtal',\n ekev = {ek!r},\n dmin = {wl.dmin!r},\n"
                 f" npx = {wl.npx},\n izz = {len(lam)},\n depthstep = {dz!r},\n"
                 f" thick = {thick!r},\n lamfile = 'lam.bin',\n kselfile = '{ksf}',\n"
-                f" outname = 'dump.bin',\n dolgh = {dolgh},\n lutfull = {int(lutfull)},\n /\n")
+                f" outname = 'dump.bin',\n dolgh = {dolgh},\n lutfull = {int(lutfull)},\n kfile = '{kf}',\n /\n")
     env = dict(os.environ, MKL_NUM_THREADS="1", OMP_NUM_THREADS="1")
     subprocess.run(["bash", "-c", f"ulimit -s unlimited && {REF_BIN}/ctemref_driver > driver.log 2>&1"],
                    cwd=work, check=True, env=env)

@@ -347,6 +347,47 @@ def golden_mc() -> str:
     return fn
 
 
+def golden_south() -> str:
+    """The southern hemisphere of the corundum workload with the wide cutoffs (al2o3_c2w). The
+    reference program computes the northern hemisphere only; the library's southern half takes
+    the same Lambert points with the Cartesian z negated before the crystal-frame transform
+    (ctem_kvectors, hemisphere -1). Here the reference's own beam loop body (ref_driver.f90, given
+    those wave vectors) on 32 seeded southern directions and on the two (list indices 23995 and
+    23996, 0-based) where a weak beam lies within ~3e-6 of the Bragg condition, so that the Bethe
+    terms blow up: its beam sets on all, and on the two its DynMat maximum and whether CalcLgh3's
+    Lgh is finite, also for their northern twins."""
+    from ctemsoft_amd import ebsd
+    wl = workloads.get("al2o3_c2w")
+    lam, _, _ = wl.depth_weights_for_program()
+    blow = np.array([23995, 23996])
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="ctemsouth_") as work:
+        setup = run_driver(work, wl, lam, [0], 0)
+        numk = setup["numk"]
+        npx, npy = ebsd.lambert_dims(wl.npx, setup["usehex"])
+        ij, k, kn = ebsd.kvectors(npx, npy, setup["isym"], setup["usehex"], setup["dsm"], setup["mLambda"], -1)
+        assert len(k) == numk
+        rng = np.random.default_rng(17)
+        kb = np.sort(np.array(sorted(set(rng.choice(numk, 32, replace=False).tolist()) | set(blow.tolist()))))
+        beams = run_driver(work, wl, lam, None, 0, kvec=(k[kb], kn[kb]))
+        per = beams["per"]
+        for j, p in enumerate(per):
+            p["ik"] = int(kb[j])
+        for key, v in _beams_csr(per).items():
+            out["beams_" + key] = v
+        out["kvec"], out["kn"], out["kij"] = k[kb], kn[kb], ij[kb]
+        for tag, kk in (("south", (k[blow], kn[blow])), ("north", (setup["kvec"][blow], setup["kn"][blow]))):
+            d = run_driver(work, wl, lam, None, 2, kvec=kk)
+            out[f"blow_{tag}_lgh_finite"] = np.array([np.isfinite(p["lgh"]).all() for p in d["per"]])
+            out[f"blow_{tag}_dynmat_max"] = np.array([np.abs(p["dynmat"]).max() for p in d["per"]])
+            out[f"blow_{tag}_min_sw"] = np.array([np.abs(p["weaksg"]).min() for p in d["per"]], np.float32)
+        out["blow_ik"] = blow.astype(np.int32)
+        out["numk"] = np.int32(numk)
+    fn = os.path.join(HERE, "al2o3_c2w_south.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -392,6 +433,8 @@ def main():
             print(golden_ecp())
         elif w == "mc":
             print(golden_mc())
+        elif w == "south":
+            print(golden_south())
         else:
             raise SystemExit(f"unknown workload {w}")
 

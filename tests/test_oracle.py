@@ -40,6 +40,27 @@ def test_beam_sets_bit_exact(name, oracle_mod):
     assert not bad, f"{len(bad)} directions with different beam sets: {bad[:10]}"
 
 
+def test_southern_hemisphere_beam_sets_match_reference(oracle_mod):
+    """The library's southern half of the corundum workload (al2o3_c2w; wave vectors from
+    ctem_kvectors with hemisphere -1, the program itself computes the northern half only): the
+    oracle's beam sets equal the reference's beam loop body on the same wave vectors
+    (tests/golden/al2o3_c2w_south.npz), including the two directions with a weak beam ~3e-6
+    from the Bragg condition. On those two the reference's CalcLgh3 gives a non-finite Lgh (its
+    DynMat reaches ~1e4), on their northern twins a finite one."""
+    fx = load_golden("al2o3_c2w")
+    z = load_golden("al2o3_c2w_south")
+    cr = oracle_mod.Crystal(fx)
+    ns, nw = z["beams_nns"], z["beams_nnw"]
+    for j in range(len(ns)):
+        st, ssg, wk, wsg = oracle_mod.select_beams(cr, z["kvec"][j])
+        rs, rssg, rw, rwsg = _split(z, j, ns, nw)
+        assert np.array_equal(st, rs) and np.array_equal(wk, rw), int(z["beams_ik"][j])
+        assert np.array_equal(ssg.view(np.int32), rssg.view(np.int32))
+        assert np.array_equal(wsg.view(np.int32), rwsg.view(np.int32))
+    assert not z["blow_south_lgh_finite"].any() and z["blow_north_lgh_finite"].all()
+    assert (z["blow_south_min_sw"] < 1e-5).all() and (z["blow_south_dynmat_max"] > 1e4).all()
+
+
 @pytest.mark.parametrize("name", FIXTURES)
 def test_dynmat_and_lgh_match_reference(name, oracle_mod):
     fx = load_golden(name)
@@ -252,3 +273,52 @@ def test_kvectors_every_laue_group_match_reference(isym, hexgrid, oracle_mod):
     assert np.array_equal(ij, fx[f"kv{isym}_{hexgrid}_ij"])
     assert np.array_equal(k, fx[f"kv{isym}_{hexgrid}_k"], equal_nan=True)
     assert np.array_equal(kn, fx[f"kv{isym}_{hexgrid}_kn"])
+
+
+@pytest.mark.parametrize("name", ["fe_c1w", "ni_c0w", "al2o3_c2w", "fe_c3w_10.0kV"])
+def test_mixed_form_truncation_at_its_scaling_threshold(name, oracle_mod):
+    """The mixed form scales A until ||A / 2^s||_1 <= 3/2 and takes the degree-8 Taylor polynomial
+    of exp - I there (F' = F^2 + 2F per squaring). In FP64 arithmetic, without the split
+    products' rounding, that truncation moves the intensities Re sum Lgh S by less than 1e-5
+    relative to the exact exponential (scipy) on sampled directions of each crystal, while one
+    squaring fewer (threshold 3) would move them by up to ~1e-3."""
+    import math
+    fx = load_golden(name)
+    cr = oracle_mod.Crystal(fx)
+    dz, thick = float(fx["dz"]), float(fx["thick"])
+    lam = np.asarray(fx["lam"], float)
+    rng = np.random.default_rng(2)
+
+    def taylor8(B):
+        F = np.zeros_like(B)
+        P = np.eye(len(B), dtype=complex)
+        for k in range(1, 9):
+            P = P @ B / k
+            F = F + P
+        return F
+
+    def dots(E, S):
+        psi = np.zeros(len(E), complex)
+        psi[0] = 1
+        L = np.zeros(E.shape, complex)
+        for lk in lam:
+            psi = E @ psi
+            L += lk * np.outer(psi, psi.conj())
+        return float(np.real(np.sum(L * S))) * dz / thick
+
+    worst = {1.5: 0.0, 3.0: 0.0}
+    for k in rng.choice(len(fx["kvec"]), 4, replace=False):
+        st, ssg, wk, wsg = oracle_mod.select_beams(cr, fx["kvec"][k])
+        A = 2j * np.pi * dz * oracle_mod.dynmat(cr, st, ssg, wk, wsg) / np.float32(2 * float(fx["kn"][k]))
+        S = oracle_mod.sgh(cr, st, 0)
+        ref = dots(scipy.linalg.expm(A), S)
+        nrm = np.abs(A).sum(axis=0).max()
+        for theta in worst:
+            s = max(0, math.ceil(math.log2(nrm / theta)))
+            F = taylor8(A / 2.0 ** s)
+            for _ in range(s):
+                F = F @ F + 2 * F
+            worst[theta] = max(worst[theta], abs(dots(np.eye(len(A)) + F, S) - ref) / abs(ref))
+    print(f"{name}: truncation at threshold 3/2 {worst[1.5]:.2e}, at 3 {worst[3.0]:.2e}")
+    assert worst[1.5] < 1e-5
+    assert worst[3.0] > 10 * worst[1.5]
