@@ -951,14 +951,15 @@ int32_t expm_depth_fp64(ctem_plan* tp, hipStream_t s, int npad, int nb, double2*
 // The depth powers use the same form, psi_{m+j} = psi_j + F_m psi_j with F_m = E^m - I.
 // Products run on split-FP16 (or BF16) MFMA with FP32 accumulation (ctem_mixed.hip); the
 // depth recursion accumulates in FP64 and the final Lgh/Sgh contraction runs in FP64.
-// theta: 1/2 for the split-BF16 options (truncation theta^9/9! = 5.4e-9). 3/2 for split-FP16
-// storage: one squaring fewer than theta = 3/4 for every matrix (8 products instead of 9 on
-// bcc Fe at 20 kV). The truncation's effect on the intensities, measured in exact (FP64)
-// arithmetic against the exact exponential over sampled directions, is at most 6e-6 relative
-// (bcc Fe at 10 and 20 kV, fcc Ni, corundum; 2e-9 at theta = 3/4 and up to 1.2e-3 at theta = 3),
-// the size of the split products' own rounding error and far below the mixed form's 1e-4 bar.
+// theta: 1/2 for the split-BF16 options (truncation theta^9/9! = 5.4e-9). 5/4 for split-FP16
+// storage: one squaring fewer than theta = 3/4 for about three quarters of the matrices (all of
+// bcc Fe at 20 kV, whose norms lie between 31 and 37: 8 products instead of 9). The truncation's
+// effect on the intensities, measured in FP64 arithmetic against the exact exponential over
+// sampled directions, stays below 5e-6 relative (bcc Fe at 10 and 20 kV, fcc Ni, corundum;
+// 2e-9 at theta = 3/4, up to 6e-6 at 3/2 and 1.2e-3 at 3), the size of the split products' own
+// rounding error and far below the mixed form's 1e-4 bar.
 constexpr double kThetaMixed = 0.5;
-constexpr double kThetaSplit = 1.5;
+constexpr double kThetaSplit = 1.25;
 // x1..x7, y2 of the three-product degree-8 Taylor evaluation (the structure of Bader, Blanes and
 // Casas, "Computing the matrix exponential with an optimized Taylor polynomial approximation",
 // Mathematics 7 (2019) 1174), solved here for the coefficients of B^2..B^8 of exp(B):
