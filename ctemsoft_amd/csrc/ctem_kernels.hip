@@ -828,12 +828,15 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       ch[tid][d] = (c < n) ? R.hkl[3 * sidx[c] + d] : 0;
     }
   }
+  // S_p(h - g) = conj S_p(g - h) bit for bit (the table's sine terms are odd in d, the rest
+  // depends on |d|), so the mirrored entry conj Lgh(g, h) of an off-diagonal tile meets
+  // conj S_p(g - h) and adds the same term Re(Lgh(g, h) S_p(g - h)) again: one gather per entry
   constexpr int NS1 = NSP > 0 ? NSP : 1;
-  double2 sp[NS1][4], sm[NS1][4];  // S_p(g - h) and S_p(h - g) of the lane's entries
+  double2 sp[NS1][4];  // S_p(g - h) of the lane's entries
 #pragma unroll
   for (int p = 0; p < NS1; p++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) sp[p][q] = sm[p][q] = make_double2(0.0, 0.0);
+    for (int q = 0; q < 4; q++) sp[p][q] = make_double2(0.0, 0.0);
   if constexpr (NSP > 0) {
     __syncthreads();  // rh, ch
     if (live) {
@@ -845,11 +848,6 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
           const int64_t cell = slut_index(L, dh, dk, dl);
 #pragma unroll
           for (int p = 0; p < NSP; p++) sp[p][q] = L.slut[(int64_t)p * L.sncell + cell];
-          if (offd) {
-            const int64_t cell2 = slut_index(L, -dh, -dk, -dl);
-#pragma unroll
-            for (int p = 0; p < NSP; p++) sm[p][q] = L.slut[(int64_t)p * L.sncell + cell2];
-          }
         }
       }
     }
@@ -906,25 +904,19 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
       const double lr = dzt * (double)accr[q], li = dzt * (double)acci[q];
       if constexpr (NSP > 0) {
 #pragma unroll
-        for (int p = 0; p < NSP; p++) part[p] += lr * sp[p][q].x - li * sp[p][q].y;
-        // the mirrored entry Lgh(h, g) = conj Lgh(g, h) meets S_p(h - g)
-        if (offd)
-#pragma unroll
-          for (int p = 0; p < NSP; p++) part[p] += lr * sm[p][q].x + li * sm[p][q].y;
+        for (int p = 0; p < NSP; p++) {
+          const double v = lr * sp[p][q].x - li * sp[p][q].y;
+          part[p] += v;
+          if (offd) part[p] += v;  // the mirrored entry
+        }
       } else {
         const int dh = rh[r][0] - ch[c][0], dk = rh[r][1] - ch[c][1], dl = rh[r][2] - ch[c][2];
         const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
         for (int p = 0; p < nsite && p < 4; p++) {
           const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
-          part[p] += lr * sv.x - li * sv.y;
-        }
-        if (offd) {
-          // the mirrored entry Lgh(h, g) = conj Lgh(g, h) meets S_p(h - g)
-          const int64_t cell2 = nsite ? slut_index(L, -dh, -dk, -dl) : 0;
-          for (int p = 0; p < nsite && p < 4; p++) {
-            const double2 sv = L.slut[(int64_t)p * L.sncell + cell2];
-            part[p] += lr * sv.x + li * sv.y;
-          }
+          const double v = lr * sv.x - li * sv.y;
+          part[p] += v;
+          if (offd) part[p] += v;  // the mirrored entry
         }
       }
       if (dir == keep_dir) {
