@@ -46,6 +46,16 @@ __device__ __forceinline__ void lut_add(double* glut, int idx, double re, double
   atomicAdd(glut + 2 * idx + 1, im);
 }
 
+// The LUT gradient is accumulated in kLutReplicas copies (glut[replica][2 ncell]), the workgroup's
+// linear index choosing the copy: every direction adds into the same few hundred cells of small
+// g - h, and one copy for the whole GPU serialises those atomics (the direct term alone took
+// three times the rest of its kernel). Workgroups are dispatched to the XCDs round-robin, so with
+// a multiple of 8 copies each copy is written from one XCD. launch_adj_lut_reduce sums them.
+__device__ __forceinline__ double* lut_replica(double* glut, int ncell) {
+  const unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
+  return glut + (size_t)(lin % kLutReplicas) * 2 * (size_t)ncell;
+}
+
 __device__ __forceinline__ int64_t s_index(const DevLut& L, int h, int k, int l) {
   return (int64_t)(h - L.slo[0]) + (int64_t)L.sdim[0] * ((k - L.slo[1]) + (int64_t)L.sdim[1] * (l - L.slo[2]));
 }
@@ -275,6 +285,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRef
   const int32_t* widx = V.widx + V.woff[dir];
   const float* wsg = V.wsg + V.woff[dir];
   const double2* Db = Dbar + (int64_t)b * npad * npad;
+  double* gl = lut_replica(glut, L.ncell);
   __shared__ int gh[32][3], wh[32][3], hh[32][3];
   __shared__ double2 Dt[32][33], Uc[32][33];
   const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRef
     const double f = -0.5 * mlambda / sw;
     const double2 u = lut_val(L, idx);
     const double fd = -mlambda / sw * dgg.x;  // diagonal |U(g-w)|^2 / s_w term
-    lut_add(glut, idx, f * xr[q] + fd * u.x, f * xi[q] + fd * u.y);
+    lut_add(gl, idx, f * xr[q] + fd * u.x, f * xi[q] + fd * u.y);
   }
 }
 
@@ -348,17 +359,21 @@ __global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRef
   const int32_t* widx = V.widx + V.woff[dir];
   const float* wsg = V.wsg + V.woff[dir];
   const double2* Db = Dbar + (int64_t)b * npad * npad;
+  double* gl = lut_replica(glut, L.ncell);
   const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
   if (wc < 0) {
-    // direct term: column tile tj, all rows g != h
-    const int h = tj * 32 + r;
-    if (h >= n) return;
-    const int32_t* hv = R.hkl + 3 * sidx[h];
-    for (int g = cq; g < n; g += 8) {
-      if (g == h) continue;
-      const int32_t* gv = R.hkl + 3 * sidx[g];
-      const double2 d = Db[(int64_t)h * npad + g];
-      lut_add(glut, lut_cell(L, gv[0] - hv[0], gv[1] - hv[1], gv[2] - hv[2]), d.x, d.y);
+    // direct term: the columns h of tile tj, all rows g != h; the 32 lanes of a half wave run
+    // over consecutive rows of one column
+    for (int q = 0; q < 4; q++) {
+      const int h = tj * 32 + cq + 8 * q;
+      if (h >= n) continue;
+      const int32_t* hv = R.hkl + 3 * sidx[h];
+      for (int g = r; g < n; g += 32) {
+        if (g == h) continue;
+        const int32_t* gv = R.hkl + 3 * sidx[g];
+        const double2 d = Db[(int64_t)h * npad + g];
+        lut_add(gl, lut_cell(L, gv[0] - hv[0], gv[1] - hv[1], gv[2] - hv[2]), d.x, d.y);
+      }
     }
     return;
   }
@@ -414,7 +429,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRef
     if (w >= nw) continue;
     const int idx = lut_cell(L, wh[wl][0] - hh[r][0], wh[wl][1] - hh[r][1], wh[wl][2] - hh[r][2]);
     const double f = -0.5 * mlambda / (double)wsg[w];
-    lut_add(glut, idx, f * yr[q], f * yi[q]);
+    lut_add(gl, idx, f * yr[q], f * yi[q]);
   }
 }
 
@@ -631,6 +646,18 @@ void launch_adj_extract32(hipStream_t s, const Bucket& B, const BeamView& V, con
                           const float2* G, const double* sigma, double2* Dbar) {
   const int nt = B.npad / 32;
   hipLaunchKernelGGL(k_adj_extract32, dim3(nt * nt, B.nb), dim3(256), 0, s, B, V, kn, dz, G, sigma, Dbar);
+}
+// glut[0][e] = sum over the copies r of glut[r][e], in the order r = 0, 1, ..
+__global__ void k_adj_lut_reduce(int64_t n2, double* __restrict__ glut) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (int64_t)gridDim.x * blockDim.x) {
+    double v = glut[e];
+    for (int r = 1; r < kLutReplicas; r++) v += glut[(int64_t)r * n2 + e];
+    glut[e] = v;
+  }
+}
+void launch_adj_lut_reduce(hipStream_t s, int ncell, double* glut) {
+  const int64_t n2 = 2 * (int64_t)ncell;
+  hipLaunchKernelGGL(k_adj_lut_reduce, dim3(grid_rows(n2)), dim3(256), 0, s, n2, glut);
 }
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw) {

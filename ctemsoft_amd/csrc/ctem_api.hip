@@ -1729,11 +1729,12 @@ static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, doubl
   const int nk = p->numk, ns = std::max(1, p->cr->nsite), nz = p->nz;
   for (int i = 0; i < nk; i++)
     if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "%s: at most 512 strong beams per direction", who);
-  if ((rc = upload(c->w.g_dots, gdots, (size_t)nk * p->cr->nsite, s)) || (rc = c->w.g_lut.ensure(16 * (size_t)p->cr->ncell)) ||
+  const size_t glut_bytes = 16 * (size_t)p->cr->ncell * kLutReplicas;
+  if ((rc = upload(c->w.g_dots, gdots, (size_t)nk * p->cr->nsite, s)) || (rc = c->w.g_lut.ensure(glut_bytes)) ||
       (rc = c->w.g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
       (rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * ns)))
     return rc;
-  HIPCHK(hipMemsetAsync(c->w.g_lut.p, 0, 16 * (size_t)p->cr->ncell, s));
+  HIPCHK(hipMemsetAsync(c->w.g_lut.p, 0, glut_bytes, s));
   HIPCHK(hipMemsetAsync(c->w.g_dots2.p, 0, 8 * (size_t)nk * ns, s));
   HIPCHK(hipMemsetAsync(p->err.p, 0, 4, s));
   std::map<int, std::vector<int32_t>> buckets;
@@ -1898,6 +1899,8 @@ static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, doubl
     off += nbk;
   }
   std::vector<double> d2((size_t)nk * ns);
+  launch_adj_lut_reduce(s, p->cr->ncell, c->w.g_lut.as<double>());
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(glut, c->w.g_lut.p, 16 * (size_t)p->cr->ncell, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(d2.data(), c->w.g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
