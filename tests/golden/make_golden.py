@@ -388,6 +388,38 @@ def golden_south() -> str:
     return fn
 
 
+def golden_refdots() -> str:
+    """The reference's intensities on many directions, for the FP64 path's parity beyond the few
+    directions whose Lgh the workload fixtures hold: the reference's own DynMat and CalcLgh3
+    (ref_driver.f90, eigen-based) on 128 seeded directions of fe_c1w and 64 of fe_c3w at 10 kV,
+    each Lgh contracted here with the oracle's S_p (its per-site structure matrix, pinned to the
+    reference through the fcc-Ni master pattern) to dots_p = Re sum_gh Lgh(g,h) S_p(g,h); only the
+    dots are stored."""
+    from oracle import oracle as O
+    out = {}
+    for name, fixture, count, ekev in (("fe_c1w", "fe_c1w", 128, None), ("fe_c3w", "fe_c3w_10.0kV", 64, 10.0)):
+        wl = workloads.get(name)
+        fx = np.load(os.path.join(HERE, f"{fixture}.npz"))
+        cr = O.Crystal(fx)
+        lam, _, _ = wl.depth_weights_for_program(ekev)
+        assert np.array_equal(lam, fx["lam"])
+        rng = np.random.default_rng(23)
+        ks = np.sort(rng.choice(len(fx["kvec"]), count, replace=False))
+        with tempfile.TemporaryDirectory(prefix="ctemrefdots_") as work:
+            d = run_driver(work, wl, lam, ks, 2, ekev)
+        nsite = int(fx["nsite"])
+        dots = np.zeros((len(ks), nsite))
+        for j, p in enumerate(d["per"]):
+            assert p["ik"] == ks[j]
+            for site in range(nsite):
+                dots[j, site] = float(np.real(np.sum(p["lgh"] * O.sgh(cr, p["strong"], site))))
+        out[f"{fixture}_ik"] = ks.astype(np.int32)
+        out[f"{fixture}_dots"] = dots
+    fn = os.path.join(HERE, "ref_dots.npz")
+    np.savez_compressed(fn, **out)
+    return fn
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", nargs="*", default=["ni_c0", "ni_c0w", "fe_c1", "fe_c1w"])
@@ -435,6 +467,8 @@ def main():
             print(golden_mc())
         elif w == "south":
             print(golden_south())
+        elif w == "refdots":
+            print(golden_refdots())
         else:
             raise SystemExit(f"unknown workload {w}")
 

@@ -258,6 +258,23 @@ def test_dots_many_directions_against_oracle(ctx, oracle_mod, name, count):
     assert np.array_equal(np.isfinite(d), np.isfinite(ref))
 
 
+@pytest.mark.parametrize("name", ["fe_c1w", "fe_c3w_10.0kV"])
+def test_fp64_dots_against_reference_on_many_directions(ctx, name):
+    """FP64 intensities against the reference's own CalcLgh3 (eigen-based) on 128 seeded fe_c1w
+    directions and 64 at 10 kV (tests/golden/ref_dots.npz: the reference's Lgh contracted with
+    S_p), within 1e-10 relative."""
+    fx = load_golden(name)
+    z = load_golden("ref_dots")
+    ks, ref = z[f"{name}_ik"], z[f"{name}_dots"]
+    plan, _ = _plan_for(ctx, fx, ks)
+    plan.run("fp64")
+    d = plan.dots().reshape(ref.shape)
+    plan.close()
+    rel = np.abs(d - ref) / np.abs(ref)
+    _say(f"{name}: FP64 vs reference dots on {len(ks)} directions, max rel {rel.max():.2e}")
+    assert rel.max() <= 1e-10
+
+
 def test_hexagonal_scatter_keeps_last_direction(ctx):
     """Corundum (Laue group 12 on the hexagonal grid, where directions share pixels): the GPU
     scatter equals the reference's serial loop -- every direction in list order overwriting its

@@ -322,3 +322,24 @@ def test_mixed_form_truncation_at_its_scaling_threshold(name, oracle_mod):
     print(f"{name}: truncation at threshold 5/4 {worst[1.25]:.2e}, at 3 {worst[3.0]:.2e}")
     assert worst[1.25] < 5e-6
     assert worst[3.0] > 10 * worst[1.25]
+
+
+@pytest.mark.parametrize("name,count", [("fe_c1w", 12), ("fe_c3w_10.0kV", 6)])
+def test_oracle_dots_match_reference_on_many_directions(name, count, oracle_mod):
+    """The oracle's intensities (extended-precision Lgh) against the reference's own CalcLgh3 on
+    seeded directions of tests/golden/ref_dots.npz (the reference's Lgh contracted with the
+    per-site S_p), within 1e-10 relative."""
+    from concurrent.futures import ThreadPoolExecutor
+    fx = load_golden(name)
+    z = load_golden("ref_dots")
+    cr = oracle_mod.Crystal(fx)
+    ks, ref = z[f"{name}_ik"][:count], z[f"{name}_dots"][:count]
+
+    def one(k):
+        return oracle_mod.master_direction(cr, fx["kvec"][k], fx["kn"][k], fx["lam"], float(fx["dz"]),
+                                           float(fx["thick"]))[0]
+    with ThreadPoolExecutor(8) as ex:
+        got = np.array(list(ex.map(one, ks)))
+    rel = np.abs(got - ref) / np.abs(ref)
+    print(f"{name}: oracle vs reference dots on {len(ks)} directions, max rel {rel.max():.2e}")
+    assert rel.max() <= 1e-10
