@@ -168,10 +168,12 @@ def main():
     # one plan per energy bin, each holding its bin's crystal, all resident before timing; the
     # plans share the context's work space
     ctx = ebsd.Context(0)
-    plans, ndir = [], 0
+    plans, ndir, npix = [], 0, 0
     ns_all, nw_all = [], []
     for label, st, lam, dz, thick, _ in jobs:
         ij, k, kn = directions(wl, st)
+        lx, ly = ebsd.lambert_dims(wl.npx, st.usehex)
+        npix += wl.hemispheres * (2 * lx + 1) * (2 * ly + 1)
         ctx.upload_crystal(st)
         plan = ctx.plan(k, kn, lam, dz, thick)
         ns, nw = plan.select()
@@ -277,6 +279,10 @@ def main():
                               "autotuned": tuned is not None}},
         "fp64_directions_per_s": round(res["fp64"]["dps"], 2) if "fp64" in res else None,
         "mixed_directions_per_s": round(res["mixed"]["dps"], 2) if "mixed" in res else None,
+        # Lambert pixels of the master pattern(s) the step's directions determine through the Laue
+        # equivalents (every hemisphere and energy bin), per second of the step (scatter not timed)
+        "master_pixels_per_step": int(npix),
+        "pixels_per_s": round(npix / (r["ms"] * 1e-3), 1),
         "mixed_over_fp64": round(res["mixed"]["dps"] / res["fp64"]["dps"], 3) if err is not None else None,
         "mixed_max_rel_err_vs_fp64": err,
         "nonfinite_directions": nonfinite if err is not None else None,
