@@ -937,6 +937,97 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   }
 }
 
+// The mixed form's Lgh and contraction with one workgroup per direction: Psi (FP32) is staged in
+// LDS once, as Ps[k][g] (the depth-major layout of k_lgh_dots, rows of depths past nz zero), and
+// the 16 waves walk the 16 x 16 quadrants of the 32 x 32 tiles on and above the diagonal (the
+// same quadrants, MFMA sequence and per-entry terms as k_lgh_dots<.., true>), each lane adding its
+// entries' Re(Lgh S_p) terms to its own partial sums; the direction's dots are the fixed-order sum
+// of the lanes' and waves' partial sums. The tile kernel re-reads the Psi rows of every tile
+// pair from L2 (about eight times the matrix per direction); here they are read once.
+__global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                       const double2* __restrict__ Psi, int64_t spsi,
+                                                       int nz, const double* __restrict__ lam, double dzt,
+                                                       double* __restrict__ dots, int keep_dir,
+                                                       double2* __restrict__ keep_lgh) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int npad = B.npad, tiles = npad / 32, ntri = tiles * (tiles + 1) / 2;
+  const int nzp = (nz + 3) & ~3;
+  float2* Ps = (float2*)lds_raw;                              // [nzp][npad]
+  float* lw = (float*)(Ps + (size_t)nzp * npad);              // [nzp]
+  int* hk = (int*)(lw + nzp);                                 // [npad][3]
+  double* red = (double*)(((uintptr_t)(hk + 3 * npad) + 7) & ~(uintptr_t)7);  // [16][4]
+  const int b = blockIdx.x;
+  const int dir = B.dirs[b];
+  const int n = V.nns[dir];
+  const int32_t* sidx = V.sidx + V.soff[dir];
+  const double2* P = Psi + (int64_t)b * spsi;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
+  const int nsite = L.nsite;
+  for (int e = tid; e < nzp * npad; e += 1024) {
+    const int k = e / npad;
+    const double2 v = (k < nz) ? P[e] : make_double2(0.0, 0.0);
+    Ps[e] = make_float2((float)v.x, (float)v.y);
+  }
+  for (int k = tid; k < nzp; k += 1024) lw[k] = (k < nz) ? (float)lam[k] : 0.0f;
+  for (int g = tid; g < npad; g += 1024)
+    for (int d = 0; d < 3; d++) hk[3 * g + d] = (g < n) ? R.hkl[3 * sidx[g] + d] : 0;
+  __syncthreads();
+  double part[4] = {0, 0, 0, 0};
+  for (int task = wave; task < 4 * ntri; task += 16) {
+    const int t = task >> 2, quad = task & 3;
+    int tj = 0;
+    while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
+    const int ti = t - tj * (tj + 1) / 2;
+    const bool offd = (ti != tj);
+    const int r0 = ti * 32 + (quad >> 1) * 16, c0 = tj * 32 + (quad & 1) * 16;
+    if (r0 >= n || c0 >= n) continue;  // wave-uniform
+    f32x4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+    for (int k4 = 0; k4 < nzp; k4 += 4) {
+      const float2 a = Ps[(k4 + l4) * npad + r0 + l16];
+      const float w = lw[k4 + l4];
+      const float ar = w * a.x, ai = w * a.y;
+      const float2 h = Ps[(k4 + l4) * npad + c0 + l16];
+      accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, h.x, accr, 0, 0, 0);
+      accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, h.y, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, h.x, acci, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f32_16x16x4f32(-ar, h.y, acci, 0, 0, 0);
+    }
+    for (int q = 0; q < 4; q++) {
+      const int gi = r0 + 4 * l4 + q, gj = c0 + l16;
+      if (gi >= n || gj >= n) continue;
+      const double lr = dzt * (double)accr[q], li = dzt * (double)acci[q];
+      const int dh = hk[3 * gi] - hk[3 * gj], dk = hk[3 * gi + 1] - hk[3 * gj + 1], dl = hk[3 * gi + 2] - hk[3 * gj + 2];
+      const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
+      for (int p = 0; p < nsite && p < 4; p++) {
+        const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
+        const double v = lr * sv.x - li * sv.y;
+        part[p] += v;
+        if (offd) part[p] += v;  // the mirrored entry conj Lgh(g, h) meets conj S_p(g - h)
+      }
+      if (dir == keep_dir) {
+        keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);
+        if (offd) keep_lgh[(int64_t)gj * n + gi] = make_double2(lr, -li);
+      }
+    }
+  }
+  for (int p = 0; p < 4; p++) {
+    const double v = wave_sum(part[p]);
+    if (lane == 0) red[wave * 4 + p] = v;
+  }
+  __syncthreads();
+  if (tid < nsite && tid < 4) {
+    double v = 0.0;
+    for (int w = 0; w < 16; w++) v += red[w * 4 + tid];
+    dots[(int64_t)dir * nsite + tid] = v;
+  }
+}
+
+// LDS of k_lgh_dots_dir for one direction: Psi in FP32, the depth weights, the beams' indices
+static size_t lgh_dir_lds(int npad, int nz) {
+  const size_t nzp = (size_t)((nz + 3) & ~3);
+  return nzp * npad * sizeof(float2) + nzp * sizeof(float) + 3 * (size_t)npad * sizeof(int) + 8 + 16 * 4 * sizeof(double);
+}
+
 __global__ void k_reduce_dots(int nb, int ntile, int nsite, const int* __restrict__ dirs,
                               const double* __restrict__ partial, double* __restrict__ dots) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -953,6 +1044,12 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const double* lam, double dzt, double* partial, double* dots,
                      int32_t keep_dir, double2* keep_lgh, int32_t f32) {
   const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
+  // the mixed form, when one direction's Psi fits in LDS: one workgroup per direction
+  if (f32 && lgh_dir_lds(B.npad, nz) <= (size_t)150 * 1024 && L.nsite <= 4) {
+    hipLaunchKernelGGL(k_lgh_dots_dir, dim3(B.nb), dim3(1024), lgh_dir_lds(B.npad, nz), s, B, V, R, L, Psi, spsi,
+                       nz, lam, dzt, dots, keep_dir, keep_lgh);
+    return;
+  }
   const dim3 grid(ntri * B.nb);
 #define LGH_LAUNCH(NSP, F)                                                                          \
   hipLaunchKernelGGL((k_lgh_dots<NSP, F>), grid, dim3(256), 0, s, B, V, R, L, Psi, spsi, nz, lam, dzt, \
