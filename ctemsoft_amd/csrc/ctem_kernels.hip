@@ -944,6 +944,7 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
 // entries' Re(Lgh S_p) terms to its own partial sums; the direction's dots are the fixed-order sum
 // of the lanes' and waves' partial sums. The tile kernel re-reads the Psi rows of every tile
 // pair from L2 (about eight times the matrix per direction); here they are read once.
+template <int NSP>
 __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                        const double2* __restrict__ Psi, int64_t spsi,
                                                        int nz, const double* __restrict__ lam, double dzt,
@@ -981,6 +982,24 @@ __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, Dev
     const bool offd = (ti != tj);
     const int r0 = ti * 32 + (quad >> 1) * 16, c0 = tj * 32 + (quad & 1) * 16;
     if (r0 >= n || c0 >= n) continue;  // wave-uniform
+    // NSP = nsite (1 or 2): the S_p gathers of the lane's entries are issued before the depth
+    // loop and wait in registers; NSP = 0: gathered after it (any nsite)
+    constexpr int NS1 = NSP > 0 ? NSP : 1;
+    double2 sp[NS1][4];
+    if constexpr (NSP > 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int gi = r0 + 4 * l4 + q, gj = c0 + l16;
+#pragma unroll
+        for (int p = 0; p < NSP; p++) sp[p][q] = make_double2(0.0, 0.0);
+        if (gi < n && gj < n) {
+          const int64_t cell = slut_index(L, hk[3 * gi] - hk[3 * gj], hk[3 * gi + 1] - hk[3 * gj + 1],
+                                          hk[3 * gi + 2] - hk[3 * gj + 2]);
+#pragma unroll
+          for (int p = 0; p < NSP; p++) sp[p][q] = L.slut[(int64_t)p * L.sncell + cell];
+        }
+      }
+    }
     f32x4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
     for (int k4 = 0; k4 < nzp; k4 += 4) {
       const float2 a = Ps[(k4 + l4) * npad + r0 + l16];
@@ -996,13 +1015,22 @@ __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, Dev
       const int gi = r0 + 4 * l4 + q, gj = c0 + l16;
       if (gi >= n || gj >= n) continue;
       const double lr = dzt * (double)accr[q], li = dzt * (double)acci[q];
-      const int dh = hk[3 * gi] - hk[3 * gj], dk = hk[3 * gi + 1] - hk[3 * gj + 1], dl = hk[3 * gi + 2] - hk[3 * gj + 2];
-      const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
-      for (int p = 0; p < nsite && p < 4; p++) {
-        const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
-        const double v = lr * sv.x - li * sv.y;
-        part[p] += v;
-        if (offd) part[p] += v;  // the mirrored entry conj Lgh(g, h) meets conj S_p(g - h)
+      if constexpr (NSP > 0) {
+#pragma unroll
+        for (int p = 0; p < NSP; p++) {
+          const double v = lr * sp[p][q].x - li * sp[p][q].y;
+          part[p] += v;
+          if (offd) part[p] += v;  // the mirrored entry conj Lgh(g, h) meets conj S_p(g - h)
+        }
+      } else {
+        const int dh = hk[3 * gi] - hk[3 * gj], dk = hk[3 * gi + 1] - hk[3 * gj + 1], dl = hk[3 * gi + 2] - hk[3 * gj + 2];
+        const int64_t cell = nsite ? slut_index(L, dh, dk, dl) : 0;
+        for (int p = 0; p < nsite && p < 4; p++) {
+          const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
+          const double v = lr * sv.x - li * sv.y;
+          part[p] += v;
+          if (offd) part[p] += v;  // the mirrored entry
+        }
       }
       if (dir == keep_dir) {
         keep_lgh[(int64_t)gi * n + gj] = make_double2(lr, li);
@@ -1046,8 +1074,13 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
   const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
   // the mixed form, when one direction's Psi fits in LDS: one workgroup per direction
   if (f32 && lgh_dir_lds(B.npad, nz) <= (size_t)150 * 1024 && L.nsite <= 4) {
-    hipLaunchKernelGGL(k_lgh_dots_dir, dim3(B.nb), dim3(1024), lgh_dir_lds(B.npad, nz), s, B, V, R, L, Psi, spsi,
-                       nz, lam, dzt, dots, keep_dir, keep_lgh);
+#define LGH_DIR(NSP)                                                                                     \
+  hipLaunchKernelGGL(k_lgh_dots_dir<NSP>, dim3(B.nb), dim3(1024), lgh_dir_lds(B.npad, nz), s, B, V, R, L, Psi, \
+                     spsi, nz, lam, dzt, dots, keep_dir, keep_lgh)
+    if (L.nsite == 1) LGH_DIR(1);
+    else if (L.nsite == 2) LGH_DIR(2);
+    else LGH_DIR(0);
+#undef LGH_DIR
     return;
   }
   const dim3 grid(ntri * B.nb);
