@@ -1343,7 +1343,9 @@ int32_t run_reserve(ctem_plan* p, int32_t precision) {
     w = std::max(w, 5 * nb * per * sizeof(double2));
     psi = std::max(psi, nb * spsi * sizeof(double2));
     sq = std::max(sq, 4 * nb);
-    part = std::max(part, 8 * nb * (npad / 32) * (npad / 32) * ns);
+    // partial sums of the contraction: one per tile pair (FP64 form) or per depth part of a
+    // direction (mixed form, two parts)
+    part = std::max(part, 8 * nb * std::max(2, (npad / 32) * (npad / 32)) * ns);
     wmax = std::max(wmax, 13 * sizeof(float) * nb);
     col = std::max(col, sizeof(double) * nb * npad * ((npad + 63) / 64));
     amod = std::max(amod, 8 * nb);

@@ -937,44 +937,50 @@ __global__ __launch_bounds__(256) void k_lgh_dots(Bucket B, BeamView V, DevRefl 
   }
 }
 
-// The mixed form's Lgh and contraction with one workgroup per direction: Psi (FP32) is staged in
-// LDS once, as Ps[k][g] (the depth-major layout of k_lgh_dots, rows of depths past nz zero), and
-// the 16 waves walk the 16 x 16 quadrants of the 32 x 32 tiles on and above the diagonal (the
-// same quadrants, MFMA sequence and per-entry terms as k_lgh_dots<.., true>), each lane adding its
-// entries' Re(Lgh S_p) terms to its own partial sums; the direction's dots are the fixed-order sum
-// of the lanes' and waves' partial sums. The tile kernel re-reads the Psi rows of every tile
-// pair from L2 (about eight times the matrix per direction); here they are read once.
+// The mixed form's Lgh and contraction with whole directions per workgroup: Lgh = sum over the
+// depths, and the contraction is linear in it, so each direction's depths are split into nsplit
+// parts, one workgroup (8 waves) per (direction, part). The part's Psi (FP32) is staged in LDS
+// once, as Ps[k][g] (the depth-major layout of k_lgh_dots, rows past the part's depths zero), and
+// the waves walk the 16 x 16 quadrants of the 32 x 32 tiles on and above the diagonal (the same
+// quadrants, MFMA sequence and per-entry terms as k_lgh_dots<.., true>), each lane adding its
+// entries' Re(Lgh S_p) terms to its own partial sums; partial[(b nsplit + part) nsite + p] is the
+// fixed-order sum of the lanes' and waves' sums, k_reduce_dots adds the parts in order. The tile
+// kernel re-reads the Psi rows of every tile pair from L2 (about eight times the matrix per
+// direction); here they are read once, and with two parts three workgroups share a CU, so that
+// one stages its Psi while the others run their MFMAs. keep_dir needs nsplit = 1.
 template <int NSP>
-__global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, DevRefl R, DevLut L,
-                                                       const double2* __restrict__ Psi, int64_t spsi,
-                                                       int nz, const double* __restrict__ lam, double dzt,
-                                                       double* __restrict__ dots, int keep_dir,
-                                                       double2* __restrict__ keep_lgh) {
+__global__ __launch_bounds__(512) void k_lgh_dots_dir(Bucket B, BeamView V, DevRefl R, DevLut L,
+                                                      const double2* __restrict__ Psi, int64_t spsi,
+                                                      int nz, int nsplit, const double* __restrict__ lam,
+                                                      double dzt, double* __restrict__ partial, int keep_dir,
+                                                      double2* __restrict__ keep_lgh) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int npad = B.npad, tiles = npad / 32, ntri = tiles * (tiles + 1) / 2;
-  const int nzp = (nz + 3) & ~3;
+  const int kper = (nz + nsplit - 1) / nsplit;
+  const int nzp = (kper + 3) & ~3;
   float2* Ps = (float2*)lds_raw;                              // [nzp][npad]
   float* lw = (float*)(Ps + (size_t)nzp * npad);              // [nzp]
   int* hk = (int*)(lw + nzp);                                 // [npad][3]
-  double* red = (double*)(((uintptr_t)(hk + 3 * npad) + 7) & ~(uintptr_t)7);  // [16][4]
-  const int b = blockIdx.x;
+  double* red = (double*)(((uintptr_t)(hk + 3 * npad) + 7) & ~(uintptr_t)7);  // [8][4]
+  const int b = blockIdx.x / nsplit, part = blockIdx.x % nsplit;
+  const int k0 = part * kper, kn = min(kper, nz - k0);  // this part's depths k0 .. k0 + kn - 1
   const int dir = B.dirs[b];
   const int n = V.nns[dir];
   const int32_t* sidx = V.sidx + V.soff[dir];
-  const double2* P = Psi + (int64_t)b * spsi;
+  const double2* P = Psi + (int64_t)b * spsi + (int64_t)k0 * npad;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, l16 = lane & 15, l4 = lane >> 4;
   const int nsite = L.nsite;
-  for (int e = tid; e < nzp * npad; e += 1024) {
+  for (int e = tid; e < nzp * npad; e += 512) {
     const int k = e / npad;
-    const double2 v = (k < nz) ? P[e] : make_double2(0.0, 0.0);
+    const double2 v = (k < kn) ? P[e] : make_double2(0.0, 0.0);
     Ps[e] = make_float2((float)v.x, (float)v.y);
   }
-  for (int k = tid; k < nzp; k += 1024) lw[k] = (k < nz) ? (float)lam[k] : 0.0f;
-  for (int g = tid; g < npad; g += 1024)
+  for (int k = tid; k < nzp; k += 512) lw[k] = (k < kn) ? (float)lam[k0 + k] : 0.0f;
+  for (int g = tid; g < npad; g += 512)
     for (int d = 0; d < 3; d++) hk[3 * g + d] = (g < n) ? R.hkl[3 * sidx[g] + d] : 0;
   __syncthreads();
-  double part[4] = {0, 0, 0, 0};
-  for (int task = wave; task < 4 * ntri; task += 16) {
+  double acc[4] = {0, 0, 0, 0};
+  for (int task = wave; task < 4 * ntri; task += 8) {
     const int t = task >> 2, quad = task & 3;
     int tj = 0;
     while ((tj + 1) * (tj + 2) / 2 <= t) tj++;
@@ -1019,8 +1025,8 @@ __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, Dev
 #pragma unroll
         for (int p = 0; p < NSP; p++) {
           const double v = lr * sp[p][q].x - li * sp[p][q].y;
-          part[p] += v;
-          if (offd) part[p] += v;  // the mirrored entry conj Lgh(g, h) meets conj S_p(g - h)
+          acc[p] += v;
+          if (offd) acc[p] += v;  // the mirrored entry conj Lgh(g, h) meets conj S_p(g - h)
         }
       } else {
         const int dh = hk[3 * gi] - hk[3 * gj], dk = hk[3 * gi + 1] - hk[3 * gj + 1], dl = hk[3 * gi + 2] - hk[3 * gj + 2];
@@ -1028,8 +1034,8 @@ __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, Dev
         for (int p = 0; p < nsite && p < 4; p++) {
           const double2 sv = L.slut[(int64_t)p * L.sncell + cell];
           const double v = lr * sv.x - li * sv.y;
-          part[p] += v;
-          if (offd) part[p] += v;  // the mirrored entry
+          acc[p] += v;
+          if (offd) acc[p] += v;  // the mirrored entry
         }
       }
       if (dir == keep_dir) {
@@ -1039,21 +1045,23 @@ __global__ __launch_bounds__(1024) void k_lgh_dots_dir(Bucket B, BeamView V, Dev
     }
   }
   for (int p = 0; p < 4; p++) {
-    const double v = wave_sum(part[p]);
+    const double v = wave_sum(acc[p]);
     if (lane == 0) red[wave * 4 + p] = v;
   }
   __syncthreads();
   if (tid < nsite && tid < 4) {
     double v = 0.0;
-    for (int w = 0; w < 16; w++) v += red[w * 4 + tid];
-    dots[(int64_t)dir * nsite + tid] = v;
+    for (int w = 0; w < 8; w++) v += red[w * 4 + tid];
+    partial[((int64_t)b * nsplit + part) * nsite + tid] = v;
   }
 }
 
-// LDS of k_lgh_dots_dir for one direction: Psi in FP32, the depth weights, the beams' indices
-static size_t lgh_dir_lds(int npad, int nz) {
-  const size_t nzp = (size_t)((nz + 3) & ~3);
-  return nzp * npad * sizeof(float2) + nzp * sizeof(float) + 3 * (size_t)npad * sizeof(int) + 8 + 16 * 4 * sizeof(double);
+// LDS of k_lgh_dots_dir for one (direction, part): Psi in FP32, the depth weights, the beams'
+// indices, the wave sums
+static size_t lgh_dir_lds(int npad, int nz, int nsplit) {
+  const int kper = (nz + nsplit - 1) / nsplit;
+  const size_t nzp = (size_t)((kper + 3) & ~3);
+  return nzp * npad * sizeof(float2) + nzp * sizeof(float) + 3 * (size_t)npad * sizeof(int) + 8 + 8 * 4 * sizeof(double);
 }
 
 __global__ void k_reduce_dots(int nb, int ntile, int nsite, const int* __restrict__ dirs,
@@ -1072,15 +1080,19 @@ void launch_lgh_dots(hipStream_t s, const Bucket& B, const BeamView& V, const De
                      const double* lam, double dzt, double* partial, double* dots,
                      int32_t keep_dir, double2* keep_lgh, int32_t f32) {
   const int tiles = B.npad / 32, ntri = tiles * (tiles + 1) / 2;
-  // the mixed form, when one direction's Psi fits in LDS: one workgroup per direction
-  if (f32 && lgh_dir_lds(B.npad, nz) <= (size_t)150 * 1024 && L.nsite <= 4) {
-#define LGH_DIR(NSP)                                                                                     \
-  hipLaunchKernelGGL(k_lgh_dots_dir<NSP>, dim3(B.nb), dim3(1024), lgh_dir_lds(B.npad, nz), s, B, V, R, L, Psi, \
-                     spsi, nz, lam, dzt, dots, keep_dir, keep_lgh)
+  // the mixed form, when a direction's Psi fits in LDS: whole directions per workgroup, in two
+  // depth parts (one part when one Lgh is kept: each part only holds its depths' share)
+  const int nsplit = keep_dir >= 0 ? 1 : 2;
+  if (f32 && lgh_dir_lds(B.npad, nz, nsplit) <= (size_t)150 * 1024 && L.nsite <= 4) {
+#define LGH_DIR(NSP)                                                                                             \
+  hipLaunchKernelGGL(k_lgh_dots_dir<NSP>, dim3(B.nb * nsplit), dim3(512), lgh_dir_lds(B.npad, nz, nsplit), s, B, V, \
+                     R, L, Psi, spsi, nz, nsplit, lam, dzt, partial, keep_dir, keep_lgh)
     if (L.nsite == 1) LGH_DIR(1);
     else if (L.nsite == 2) LGH_DIR(2);
     else LGH_DIR(0);
 #undef LGH_DIR
+    hipLaunchKernelGGL(k_reduce_dots, dim3((B.nb + 127) / 128), dim3(128), 0, s, B.nb, nsplit, L.nsite, B.dirs,
+                       partial, dots);
     return;
   }
   const dim3 grid(ntri * B.nb);
