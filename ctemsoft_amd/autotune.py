@@ -25,7 +25,7 @@ import numpy as np
 
 from . import ebsd
 
-CANDIDATES = [(32, 24.0), (32, 64.0), (32, 8.0), (64, 24.0), (64, 64.0), (32, 160.0)]
+CANDIDATES = [(32, 24.0), (32, 64.0), (32, 8.0), (64, 24.0), (64, 64.0)]
 
 
 def cache_path() -> str:
