@@ -46,14 +46,14 @@ __device__ __forceinline__ void lut_add(double* glut, int idx, double re, double
   atomicAdd(glut + 2 * idx + 1, im);
 }
 
-// The LUT gradient is accumulated in kLutReplicas copies (glut[replica][2 ncell]), the workgroup's
+// The LUT gradient is accumulated in nrep copies (glut[replica][2 ncell]), the workgroup's
 // linear index choosing the copy: every direction adds into the same few hundred cells of small
 // g - h, and one copy for the whole GPU serialises those atomics (the direct term alone took
 // three times the rest of its kernel). Workgroups are dispatched to the XCDs round-robin, so with
 // a multiple of 8 copies each copy is written from one XCD. launch_adj_lut_reduce sums them.
-__device__ __forceinline__ double* lut_replica(double* glut, int ncell) {
+__device__ __forceinline__ double* lut_replica(double* glut, int ncell, int nrep) {
   const unsigned lin = blockIdx.x + gridDim.x * blockIdx.y;
-  return glut + (size_t)(lin % kLutReplicas) * 2 * (size_t)ncell;
+  return glut + (size_t)(lin % (unsigned)nrep) * 2 * (size_t)ncell;
 }
 
 __device__ __forceinline__ int64_t s_index(const DevLut& L, int h, int k, int l) {
@@ -273,7 +273,7 @@ __global__ void k_adj_extract(Bucket B, BeamView V, const double* __restrict__ k
 // 32-chunk of weak beams). X(g,w) = sum_{h != g} D_bar(g,h) conj(U(w-h)) by chunks of h.
 __global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                     double mlambda, const double2* __restrict__ Dbar,
-                                                    double* __restrict__ glut, int wchunks) {
+                                                    double* __restrict__ glut, int wchunks, int nrep) {
   const int npad = B.npad, tiles = npad / 32;
   const int b = blockIdx.y;
   const int ti = blockIdx.x / wchunks, wc = blockIdx.x % wchunks;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRef
   const int32_t* widx = V.widx + V.woff[dir];
   const float* wsg = V.wsg + V.woff[dir];
   const double2* Db = Dbar + (int64_t)b * npad * npad;
-  double* gl = lut_replica(glut, L.ncell);
+  double* gl = lut_replica(glut, L.ncell, nrep);
   __shared__ int gh[32][3], wh[32][3], hh[32][3];
   __shared__ double2 Dt[32][33], Uc[32][33];
   const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_gw(Bucket B, BeamView V, DevRef
 // the direct term D_bar(g,h) into U(g-h) (done by the first weak chunk).
 __global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRefl R, DevLut L,
                                                     double mlambda, const double2* __restrict__ Dbar,
-                                                    double* __restrict__ glut, int wchunks) {
+                                                    double* __restrict__ glut, int wchunks, int nrep) {
   const int npad = B.npad, tiles = npad / 32;
   const int b = blockIdx.y;
   const int tj = blockIdx.x / (wchunks + 1), wc = blockIdx.x % (wchunks + 1) - 1;
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_adj_lut_wh(Bucket B, BeamView V, DevRef
   const int32_t* widx = V.widx + V.woff[dir];
   const float* wsg = V.wsg + V.woff[dir];
   const double2* Db = Dbar + (int64_t)b * npad * npad;
-  double* gl = lut_replica(glut, L.ncell);
+  double* gl = lut_replica(glut, L.ncell, nrep);
   const int tid = threadIdx.x, r = tid & 31, cq = tid >> 5;
   if (wc < 0) {
     // direct term: the columns h of tile tj, all rows g != h; the 32 lanes of a half wave run
@@ -648,23 +648,29 @@ void launch_adj_extract32(hipStream_t s, const Bucket& B, const BeamView& V, con
   hipLaunchKernelGGL(k_adj_extract32, dim3(nt * nt, B.nb), dim3(256), 0, s, B, V, kn, dz, G, sigma, Dbar);
 }
 // glut[0][e] = sum over the copies r of glut[r][e], in the order r = 0, 1, ..
-__global__ void k_adj_lut_reduce(int64_t n2, double* __restrict__ glut) {
+__global__ void k_adj_lut_reduce(int64_t n2, int nrep, double* __restrict__ glut) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2; e += (int64_t)gridDim.x * blockDim.x) {
     double v = glut[e];
-    for (int r = 1; r < kLutReplicas; r++) v += glut[(int64_t)r * n2 + e];
+    for (int r = 1; r < nrep; r++) v += glut[(int64_t)r * n2 + e];
     glut[e] = v;
   }
 }
-void launch_adj_lut_reduce(hipStream_t s, int ncell, double* glut) {
+int adj_lut_replicas(int ncell) {
+  const size_t one = 16 * (size_t)ncell;
+  return (int)std::max<size_t>(1, std::min<size_t>(kLutReplicas, ((size_t)2 << 30) / std::max<size_t>(one, 1)));
+}
+void launch_adj_lut_reduce(hipStream_t s, int ncell, int nrep, double* glut) {
   const int64_t n2 = 2 * (int64_t)ncell;
-  hipLaunchKernelGGL(k_adj_lut_reduce, dim3(grid_rows(n2)), dim3(256), 0, s, n2, glut);
+  if (nrep > 1) hipLaunchKernelGGL(k_adj_lut_reduce, dim3(grid_rows(n2)), dim3(256), 0, s, n2, nrep, glut);
 }
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
-                    double mlambda, const double2* Dbar, double* glut, int max_nw) {
+                    double mlambda, const double2* Dbar, double* glut, int max_nw, int nrep) {
   const int tiles = B.npad / 32;
   const int wchunks = std::max(1, (max_nw + 31) / 32);
-  hipLaunchKernelGGL(k_adj_lut_gw, dim3(tiles * wchunks, B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut, wchunks);
-  hipLaunchKernelGGL(k_adj_lut_wh, dim3(tiles * (wchunks + 1), B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut, wchunks);
+  hipLaunchKernelGGL(k_adj_lut_gw, dim3(tiles * wchunks, B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut, wchunks,
+                     nrep);
+  hipLaunchKernelGGL(k_adj_lut_wh, dim3(tiles * (wchunks + 1), B.nb), dim3(256), 0, s, B, V, R, L, mlambda, Dbar, glut,
+                     wchunks, nrep);
 }
 
 }  // namespace ctem

@@ -1731,7 +1731,8 @@ static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, doubl
   const int nk = p->numk, ns = std::max(1, p->cr->nsite), nz = p->nz;
   for (int i = 0; i < nk; i++)
     if (npad_of(p->h_nns[i]) > 512) return fail(CTEM_E_RANGE, "%s: at most 512 strong beams per direction", who);
-  const size_t glut_bytes = 16 * (size_t)p->cr->ncell * kLutReplicas;
+  const int nrep = adj_lut_replicas(p->cr->ncell);
+  const size_t glut_bytes = 16 * (size_t)p->cr->ncell * nrep;
   if ((rc = upload(c->w.g_dots, gdots, (size_t)nk * p->cr->nsite, s)) || (rc = c->w.g_lut.ensure(glut_bytes)) ||
       (rc = c->w.g_dots2.ensure(8 * (size_t)nk * ns)) || (rc = p->err.ensure(4)) ||
       (rc = p->dots.ensure(8 * (size_t)std::max(1, nk) * ns)))
@@ -1895,13 +1896,13 @@ static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, doubl
       }
       HIPCHK(hipGetLastError());
       // LUT entries
-      launch_adj_lut(s, B, V, R, L, p->cr->mlambda, c->w.aDbar.as<double2>(), c->w.g_lut.as<double>(), max_nw);
+      launch_adj_lut(s, B, V, R, L, p->cr->mlambda, c->w.aDbar.as<double2>(), c->w.g_lut.as<double>(), max_nw, nrep);
       HIPCHK(hipGetLastError());
     }
     off += nbk;
   }
   std::vector<double> d2((size_t)nk * ns);
-  launch_adj_lut_reduce(s, p->cr->ncell, c->w.g_lut.as<double>());
+  launch_adj_lut_reduce(s, p->cr->ncell, nrep, c->w.g_lut.as<double>());
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(glut, c->w.g_lut.p, 16 * (size_t)p->cr->ncell, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(d2.data(), c->w.g_dots2.p, 8 * d2.size(), hipMemcpyDeviceToHost, s));

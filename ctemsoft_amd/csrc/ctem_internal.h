@@ -366,12 +366,14 @@ void launch_norm_scale_pair(hipStream_t s, int npad, int nb, double2* X, double2
 // D_bar from the derivative block: T2 with leading dimension ldt (npad: the pair's Y part)
 void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const double* kn, double dz,
                         const double2* T2, int ldt, const double* sigma, double2* Dbar);
-// glut: kLutReplicas copies of the 2 ncell gradient entries (zeroed by the caller); the copy a
-// workgroup adds into follows its index. launch_adj_lut_reduce leaves their sum in copy 0.
+// glut: nrep copies of the 2 ncell gradient entries (zeroed by the caller); the copy a workgroup
+// adds into follows its index. launch_adj_lut_reduce leaves their sum in copy 0. nrep =
+// adj_lut_replicas(ncell): kLutReplicas, fewer when the copies would pass 2 GiB.
 constexpr int kLutReplicas = 64;
+int adj_lut_replicas(int ncell);
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
-                    double mlambda, const double2* Dbar, double* glut, int max_nw);
-void launch_adj_lut_reduce(hipStream_t s, int ncell, double* glut);
+                    double mlambda, const double2* Dbar, double* glut, int max_nw, int nrep);
+void launch_adj_lut_reduce(hipStream_t s, int ncell, int nrep, double* glut);
 // mixed form: E = I + F in FP64 from the FP32 F
 void launch_adj_e64(hipStream_t s, int npad, int nb, const float2* F, double2* E);
 // the pair (A 2^-s, sigma 2^-s Ebar^H) in FP32 (A scaled in place), its squaring count and the
