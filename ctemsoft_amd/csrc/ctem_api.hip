@@ -369,7 +369,11 @@ static void build_slut(const ctem_crystal* c, const int32_t lo[3], const int32_t
     for (int q; (q = next.fetch_add(1)) < nplane;) plane(q / dim[2], q % dim[2]);
   };
   std::vector<std::thread> pool;
-  for (int t = 1; t < nthr; t++) pool.emplace_back(work);
+  try {
+    for (int t = 1; t < nthr; t++) pool.emplace_back(work);
+  } catch (...) {
+    // no more threads: the calling thread and those started share the planes
+  }
   work();
   for (auto& t : pool) t.join();
 }
