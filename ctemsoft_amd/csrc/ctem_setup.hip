@@ -32,6 +32,7 @@
 
 #include "../../include/ctem_ebsd.h"
 #include "../../include/ctem_setup.h"
+#include "ctem_libm.h"
 #include "ctem_sg.h"
 #include "ctem_tables.h"
 
@@ -46,21 +47,22 @@ namespace ctem {
 namespace su {
 
 // The Weickenmeier-Kohl routines below are written once and evaluated with the host C library
-// (HostMath: the reference's own calls, bit-equal results) or with the device's float32
-// functions (DevMath: the LUT build on the GPU, equal to the reference within a few float32
-// rounding steps).
+// (HostMath: the reference's own calls) or with the restatement of that library's algorithms
+// (DevMath: the LUT build on the GPU). Both return the same bits for every input, so the GPU's
+// LUT is bit-equal to the host's. powf is called only as Z^0.333333 for the atomic number of a
+// site; the host evaluates it once per site and hands the value in.
 struct HostMath {
   static float exp(float x) { return ::expf(x); }
   static float log(float x) { return ::logf(x); }
-  static float pow(float x, float y) { return ::powf(x, y); }
   static float sqrt(float x) { return ::sqrtf(x); }
 };
 struct DevMath {
-  __device__ static float exp(float x) { return expf(x); }
-  __device__ static float log(float x) { return logf(x); }
-  __device__ static float pow(float x, float y) { return powf(x, y); }
-  __device__ static float sqrt(float x) { return __fsqrt_rn(x); }
+  __device__ static float exp(float x) { return libm::expf(x); }
+  __device__ static float log(float x) { return libm::logf(x); }
+  __device__ static float sqrt(float x) { return sqrtf_cr(x); }
 };
+// Z^0.333333 of FCORE (others.f90), from the host C library
+inline float zcube_root(int z) { return ::powf((float)z, .333333f); }
 #define WK template <class MT> __host__ __device__
 
 constexpr double cPi = 3.141592653589793238, cLight = 299792458.0, cPlanck = 6.626075e-34,
@@ -567,12 +569,12 @@ WK float fphon(float g, float ul, const float A[4], const float B[4]) {
   return f;
 }
 
-WK float fcore(float g, int z, float accvlt) {
+WK float fcore(float g, int z, float zr, float accvlt) {
   const float PI = 3.1415927f, A0 = .5289f;
   const float k0 = .5068f * MT::sqrt(1022.f * accvlt + accvlt * accvlt);
   const float de = (float)(6 * z) * 1.e-3f;
   const float thetae = ((de / (2.f * accvlt)) * (2.f * accvlt + 1022.f)) / (accvlt + 1022.f);
-  const float r = (0.885f * A0) / MT::pow((float)z, .333333f);
+  const float r = (0.885f * A0) / zr;
   const float ta = 1.f / (k0 * r);
   const float tb = g / (2.f * k0);
   const float omega = (2.f * tb) / ta;
@@ -594,13 +596,15 @@ WK float fcore(float g, int z, float accvlt) {
   return ((((4.f / (A0 * A0)) * 2.f) * PI) / (k0 * k0)) * hi;
 }
 
-// FSCATT with absorption (ABSFLG = 3), ACCFLG and DWFLG set; returns (real, imaginary)
-WK void fscatt(float g, float ul, int z, const float* A, const float* B, float accvlt, float* fre, float* fim) {
+// FSCATT with absorption (ABSFLG = 3), ACCFLG and DWFLG set; returns (real, imaginary).
+// zr = Z^0.333333 (zcube_root).
+WK void fscatt(float g, float ul, int z, float zr, const float* A, const float* B, float accvlt, float* fre,
+               float* fim) {
   const float FOURPI = 12.56637062f;
   const float dewa = MT::exp(-((((.5f * ul) * ul) * g) * g));
   const float s = g / FOURPI;
   float freal = (FOURPI * dewa) * weko<MT>(A, B, s);
-  float fima = fcore<MT>(g, z, accvlt) * dewa + fphon<MT>(g, ul, A, B);
+  float fima = fcore<MT>(g, z, zr, accvlt) * dewa + fphon<MT>(g, ul, A, B);
   const float k0 = .5068f * MT::sqrt(1022.f * accvlt + accvlt * accvlt);
   const float gamma = (accvlt + 511.f) / 511.f;
   freal = freal * gamma;
@@ -632,7 +636,8 @@ Ucg calc_ucg(const State& S, const int hkl[3]) {
   for (int m = 0; m < S.ntype; m++) {
     const float ul = ::sqrtf(S.atom_pos[5 * m + 4] * dwwk);
     float sfr, sfi;
-    fscatt<HostMath>(s, ul, S.atom_type[m], kWKA[S.atom_type[m] - 1], kWKB[S.atom_type[m] - 1], accvlt, &sfr, &sfi);
+    fscatt<HostMath>(s, ul, S.atom_type[m], zcube_root(S.atom_type[m]), kWKA[S.atom_type[m] - 1],
+                     kWKB[S.atom_type[m] - 1], accvlt, &sfr, &sfi);
     const float occ = S.atom_pos[5 * m + 3];
     sfr = sfr * occ;
     sfi = sfi * occ;
@@ -682,11 +687,11 @@ void calc_wavelength(State& S, double voltage) {
 }
 
 // ---- the LUT box on the GPU ----------------------------------------------------------------
-// Inputs of CalcUcg that do not depend on g, per site: Z, ul, occupancy, the fit coefficients
-// and the float32-valued orbit positions.
+// Inputs of CalcUcg that do not depend on g, per site: Z, Z^0.333333, ul, occupancy, the fit
+// coefficients and the float32-valued orbit positions.
 struct DevSites {
   int ntype;
-  const int* z; const float* ul; const float* occ; const float* wka; const float* wkb;
+  const int* z; const float* zr; const float* ul; const float* occ; const float* wka; const float* wkb;
   const int* numat; const int* aoff; const double* apos;
 };
 struct DevBox {
@@ -709,7 +714,8 @@ __device__ bool dev_allowed(char lattice, int hexset, int h, int k, int l) {
 }
 
 // One thread per cell of the box: Ucg of every lattice-allowed g (0 elsewhere), in the order of
-// operations of CalcUcg 'WK' with absorption, with the device's float32 functions.
+// operations of CalcUcg 'WK' with absorption, with the C library's float32 functions restated
+// (bit-equal to the host's build).
 __global__ __launch_bounds__(256) void k_ucg_box(DevBox B, DevSites S, float2* __restrict__ lut,
                                                  uint8_t* __restrict__ dbd) {
 #pragma clang fp contract(off)
@@ -734,7 +740,7 @@ __global__ __launch_bounds__(256) void k_ucg_box(DevBox B, DevSites S, float2* _
     float ffr = 0.f, ffi = 0.f, ggr = 0.f, ggi = 0.f;
     for (int m = 0; m < S.ntype; m++) {
       float sfr, sfi;
-      fscatt<DevMath>(s, S.ul[m], S.z[m], S.wka + 4 * m, S.wkb + 4 * m, B.accvlt, &sfr, &sfi);
+      fscatt<DevMath>(s, S.ul[m], S.z[m], S.zr[m], S.wka + 4 * m, S.wkb + 4 * m, B.accvlt, &sfr, &sfi);
       sfr = sfr * S.occ[m];
       sfi = sfi * S.occ[m];
       float p1r = 0.f, p1i = 0.f;
@@ -744,7 +750,7 @@ __global__ __launch_bounds__(256) void k_ucg_box(DevBox B, DevSites S, float2* _
                            (double)(float)l * ap[3 * j + 2];
         const float arg = (float)((double)twopi * sum);
         float sn, cs;
-        sincosf(-arg, &sn, &cs);
+        libm::sincosf(-arg, &sn, &cs);
         p1r = p1r + cs;
         p1i = p1i + sn;
       }
@@ -754,7 +760,7 @@ __global__ __launch_bounds__(256) void k_ucg_box(DevBox B, DevSites S, float2* _
       ggi = ggi + p1i * sfi;
     }
     out = make_float2(B.pre * (ffr - ggi), B.pre * (ffi + ggr));
-    small = (h != 0 || k != 0 || l != 0) && sqrtf(out.x * out.x + out.y * out.y) <= 1.0e-10f;
+    small = (h != 0 || k != 0 || l != 0) && libm::cabsf(out.x, out.y) <= 1.0e-10f;
   }
   lut[c] = out;
   dbd[c] = small;
@@ -837,11 +843,12 @@ static int32_t fill_box_gpu(ctem_setup* T, int device, double* ms) {
   State& S = T->S;
   const size_t ncell = (size_t)T->lut_dim[0] * T->lut_dim[1] * T->lut_dim[2];
   std::vector<int> z, numat, aoff;
-  std::vector<float> ul, occ, wa, wb;
+  std::vector<float> ul, occ, wa, wb, zr;
   std::vector<double> ap;
   const float dwwk = (float)(100.0 / (8.0 * (cPi * cPi)));
   for (int m = 0; m < S.ntype; m++) {
     z.push_back(S.atom_type[m]);
+    zr.push_back(zcube_root(S.atom_type[m]));
     ul.push_back(::sqrtf(S.atom_pos[5 * m + 4] * dwwk));
     occ.push_back(S.atom_pos[5 * m + 3]);
     for (int i = 0; i < 4; i++) { wa.push_back(kWKA[S.atom_type[m] - 1][i]); wb.push_back(kWKB[S.atom_type[m] - 1][i]); }
@@ -863,7 +870,7 @@ static int32_t fill_box_gpu(ctem_setup* T, int device, double* ms) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return setup_fail(CTEM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
   // one allocation for every input and output
-  const size_t nb_i = sizeof(int) * (3 * (size_t)S.ntype), nb_f = sizeof(float) * (10 * (size_t)S.ntype),
+  const size_t nb_i = sizeof(int) * (3 * (size_t)S.ntype), nb_f = sizeof(float) * (11 * (size_t)S.ntype),
                nb_d = sizeof(double) * ap.size(), nb_l = sizeof(float2) * ncell, nb_b = ncell;
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t total = up(nb_i) + up(nb_f) + up(nb_d) + up(nb_l) + up(nb_b);
@@ -882,10 +889,12 @@ static int32_t fill_box_gpu(ctem_setup* T, int device, double* ms) {
   hf.insert(hf.end(), occ.begin(), occ.end());
   hf.insert(hf.end(), wa.begin(), wa.end());
   hf.insert(hf.end(), wb.begin(), wb.end());
+  hf.insert(hf.end(), zr.begin(), zr.end());
   DevSites D;
   D.ntype = S.ntype;
   D.z = d_i; D.numat = d_i + S.ntype; D.aoff = d_i + 2 * S.ntype;
   D.ul = d_f; D.occ = d_f + S.ntype; D.wka = d_f + 2 * S.ntype; D.wkb = d_f + 6 * S.ntype;
+  D.zr = d_f + 10 * S.ntype;
   D.apos = d_ap;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);

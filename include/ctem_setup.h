@@ -58,10 +58,10 @@ CTEM_API int32_t ctem_setup_create(const ctem_xtal* x, float ekev, float dmin, f
                                    float cutoff, float sgcutoff, ctem_setup** out);
 /* The same setup with the look-up-table box -- CalcUcg of every lattice-allowed g, the bulk of
  * the work (143,745 cells for corundum at dmin 0.05 nm) -- computed on GPU `device` with the
- * device's float32 functions (one thread per cell). The values equal the host's to a few
- * float32 rounding steps, not bit for bit (the reference's float32 expf/logf/powf/sincosf are
- * the C library's); the list order, 000 (and with it DynUpz and the wavelength) and everything
- * else are the host's. *lut_ms (may be NULL) receives the kernel time. */
+ * C library's float32 expf, logf, sincosf and cabsf restated for the device (one thread per
+ * cell). The result is bit-equal to ctem_setup_create's; the list order, 000 (and with it DynUpz
+ * and the wavelength) and everything else are the host's. *lut_ms (may be NULL) receives the
+ * kernel time. */
 CTEM_API int32_t ctem_setup_create_gpu(int32_t device, const ctem_xtal* x, float ekev, float dmin,
                                        float weakcutoff, float cutoff, float sgcutoff, double* lut_ms,
                                        ctem_setup** out);

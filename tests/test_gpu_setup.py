@@ -1,13 +1,16 @@
 """The look-up-table box of the per-energy setup built on the GPU (ctem_setup_create_gpu) against
-the host's bit-exact restatement of the reference (itself pinned by tests/test_setup_native.py),
-and the path run from the GPU-built setup against the reference's beam sets and the host-built
-setup's intensities."""
+the host's restatement of the reference (itself pinned by tests/test_setup_native.py) and against
+the reference's own setups, bit for bit, and the path run from the GPU-built setup against the
+reference's beam sets and the host-built setup's intensities. The kernel evaluates the C
+library's expf, logf, sincosf and cabsf through their restatement (ctemsoft_amd/csrc/ctem_libm.h,
+pinned exhaustively by tests/test_sg_math.py), so every float32 Ucg is the reference's."""
 import sys
 
 import numpy as np
 import pytest
 
 from conftest import load_golden
+from test_setup_native import FIXTURES, _compare
 
 pytestmark = pytest.mark.gpu
 
@@ -38,13 +41,39 @@ def test_gpu_lut_against_host(name, ekev):
     # list order, 000, wavelength: the host's
     assert np.array_equal(sg.hkl, sh.hkl)
     assert vg["mLambda"] == vh["mLambda"] and vg["DynUpz"] == vh["DynUpz"]
-    # a few float32 rounding steps of the device's expf/logf/powf/sincosf, relative to max|U|
-    assert dev.max() <= 2e-6
-    rel = np.abs(lg - lh) / np.maximum(np.abs(lh), 1e-30)
-    big = np.abs(lh) > 1e-4 * umax
-    assert rel[big].max() <= 2e-5
-    # double-diffraction flags of the list entries agree (no |U| lies near the 1e-10 threshold)
+    # every cell bit-equal, and the list's Ucg and double-diffraction flags with it
+    assert np.array_equal(lg.view(np.uint64), lh.view(np.uint64))
+    assert np.array_equal(sg.ucg.view(np.uint64), sh.ucg.view(np.uint64))
     assert np.array_equal(sg.dbdiff, sh.dbdiff)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_gpu_setup_against_reference(name):
+    """The GPU-built setup against the reference's dump of the same crystal and energy: every
+    quantity tests/test_setup_native.py compares, bit for bit."""
+    from ctemsoft_amd import workloads, xtal
+    fx = load_golden(name)
+    wl = workloads.get(name.split("_")[0] + "_" + name.split("_")[1])
+    st, v = xtal.native_setup(wl.xtal, float(fx["ekev"]) if "ekev" in fx.files else wl.ekev, wl.dmin,
+                              (wl.weakcutoff, wl.cutoff, wl.sgcutoff), device=0)
+    _say(f"{name}: {st.lut.size // 2} cells built on the GPU in {v['lut_ms']:.3f} ms")
+    _compare(st, v, fx)
+
+
+def test_gpu_setup_energy_series_against_reference():
+    """The 21 energy bins of the Fe series (10-20 kV), each setup built on the GPU, against the
+    reference's dumps."""
+    from ctemsoft_amd import workloads, xtal
+    wl = workloads.get("fe_c3w")
+    arc = load_golden("fe_c3w_setups")
+    ms = 0.0
+    for i in range(int(arc["nenergy"])):
+        view = {k[len(f"e{i}_"):]: arc[k] for k in arc.files if k.startswith(f"e{i}_")}
+        st, v = xtal.native_setup(wl.xtal, float(view["ekev"]), wl.dmin, (wl.weakcutoff, wl.cutoff, wl.sgcutoff),
+                                  device=0)
+        ms += v["lut_ms"]
+        _compare(st, v, view)
+    _say(f"fe_c3w: {int(arc['nenergy'])} setups built on the GPU, LUT kernels {ms:.3f} ms in all")
 
 
 @pytest.mark.parametrize("name", ["fe_c1w", "al2o3_c2"])
