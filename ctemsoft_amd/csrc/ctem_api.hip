@@ -80,11 +80,26 @@ int npad_of(int n, int multiple = 32) { return std::max(multiple, (n + multiple 
 
 }  // namespace
 
+// The per-site structure-matrix tables S_p(g-h) and their B-derivatives over the box of all
+// differences of list entries. They depend on the sites, the metric and the box only, not on the
+// energy, so the crystals of an energy series share one (the key holds those inputs' bytes).
+struct SiteTable {
+  int device = 0;
+  DBuf slut, dslut;
+  std::vector<unsigned char> key;
+  ~SiteTable() {
+    (void)hipSetDevice(device);
+    slut.release();
+    dslut.release();
+  }
+};
+
 // One uploaded crystal (one energy bin). A plan keeps the crystal that was current when it
 // was created, so that several bins can be resident at once under one context.
 struct Crystal {
   int device = 0;
-  DBuf g, hkl, cut1, cut2, dbd, lut, slut, dslut, rmt;
+  DBuf g, hkl, cut1, cut2, dbd, lut, rmt;
+  std::shared_ptr<SiteTable> st;
   int32_t nref = 0, nsite = 0, ncell = 0;
   int32_t lut_lo[3] = {0, 0, 0}, lut_dim[3] = {0, 0, 0};
   int32_t slut_lo[3] = {0, 0, 0}, slut_dim[3] = {0, 0, 0};
@@ -106,7 +121,7 @@ struct Crystal {
 
   ~Crystal() {
     (void)hipSetDevice(device);
-    for (DBuf* b : {&g, &hkl, &cut1, &cut2, &dbd, &lut, &slut, &dslut, &rmt}) b->release();
+    for (DBuf* b : {&g, &hkl, &cut1, &cut2, &dbd, &lut, &rmt}) b->release();
   }
   DevRefl refl() const {
     DevRefl R;
@@ -116,7 +131,7 @@ struct Crystal {
   }
   DevLut lutv(int32_t* err) const {
     DevLut L;
-    L.lut = lut.as<float2>(); L.slut = slut.as<double2>();
+    L.lut = lut.as<float2>(); L.slut = st->slut.as<double2>();
     for (int i = 0; i < 3; i++) {
       L.lo[i] = lut_lo[i]; L.dim[i] = lut_dim[i];
       L.slo[i] = slut_lo[i]; L.sdim[i] = slut_dim[i];
@@ -143,6 +158,7 @@ struct ctem_ctx {
   int cu_count = 256;
   hipStream_t stream = nullptr;
   std::shared_ptr<Crystal> cr;  // the crystal that plans created next will use
+  std::weak_ptr<SiteTable> last_st;  // the latest S table, reused while a crystal holds it
   Work w;
   // scratch for the CalcLgh3 drop-in
   DBuf s_A, s_W, s_Psi, s_sq, s_err, s_lam, s_lgh, s_dirs, s_nns, s_zero, s_off, s_part, s_dots;
@@ -386,6 +402,10 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   if (x->hkl[0] != 0 || x->hkl[1] != 0 || x->hkl[2] != 0)
     return fail(CTEM_E_ARG, "ctem_upload_crystal: the first list entry must be 000");
   if (x->nsite < 0 || x->nsite > 4) return fail(CTEM_E_RANGE, "ctem_upload_crystal: 0..4 sites supported (got %d)", x->nsite);
+  if (x->nsite > 0 && (!x->site_z || !x->site_b || !x->site_norbit || !x->orbit))
+    return fail(CTEM_E_ARG, "ctem_upload_crystal: null site arrays");
+  for (int p = 0; p < x->nsite; p++)
+    if (x->site_norbit[p] < 1) return fail(CTEM_E_ARG, "ctem_upload_crystal: empty orbit");
   for (int i = 0; i < 3; i++)
     if (x->lut_dim[i] <= 0) return fail(CTEM_E_ARG, "ctem_upload_crystal: bad LUT extent");
   HIPCHK(hipSetDevice(c->device));
@@ -432,8 +452,35 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
     }
     centro = !(dmax > std::ldexp(lmax, -20));  // NaN entries: not symmetric
   }
-  std::vector<double> slut, dslut;
-  build_slut(x, slo, sdim, slut, dslut);
+  // the S table's key: every input of build_slut
+  std::vector<unsigned char> key;
+  auto put = [&key](const void* v, size_t b) {
+    key.insert(key.end(), (const unsigned char*)v, (const unsigned char*)v + b);
+  };
+  put(slo, sizeof(slo));
+  put(sdim, sizeof(sdim));
+  put(x->rmt, 9 * sizeof(double));
+  put(&x->nsite, sizeof(x->nsite));
+  size_t norb = 0;
+  if (x->nsite > 0) {
+    put(x->site_z, x->nsite * sizeof(int32_t));
+    put(x->site_b, x->nsite * sizeof(float));
+    put(x->site_norbit, x->nsite * sizeof(int32_t));
+    for (int p = 0; p < x->nsite; p++) norb += (size_t)x->site_norbit[p];
+    put(x->orbit, 3 * norb * sizeof(double));
+  }
+  std::shared_ptr<SiteTable> st = c->last_st.lock();
+  if (!st || st->device != c->device || st->key != key) {
+    std::vector<double> slut, dslut;
+    build_slut(x, slo, sdim, slut, dslut);
+    st = std::make_shared<SiteTable>();
+    st->device = c->device;
+    st->key = std::move(key);
+    int32_t rs;
+    if ((rs = upload(st->slut, slut.data(), slut.size(), c->stream))) return rs;
+    if ((rs = upload(st->dslut, dslut.data(), dslut.size(), c->stream))) return rs;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   int nat = 0;
   for (int p = 0; p < x->nsite; p++) nat += x->site_norbit[p];
   int32_t rc;
@@ -445,8 +492,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
   if ((rc = upload(cr->cut2, cut2.data(), n, c->stream))) return rc;
   if ((rc = upload(cr->dbd, dbd.data(), n, c->stream))) return rc;
   if ((rc = upload(cr->lut, lutf.data(), lutf.size(), c->stream))) return rc;
-  if ((rc = upload(cr->slut, slut.data(), slut.size() ? slut.size() : 0, c->stream))) return rc;
-  if ((rc = upload(cr->dslut, dslut.data(), dslut.size() ? dslut.size() : 0, c->stream))) return rc;
+  cr->st = st;
   if ((rc = upload(cr->rmt, x->rmt, 9, c->stream))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   cr->nref = n;
@@ -457,6 +503,7 @@ extern "C" int32_t ctem_upload_crystal(ctem_ctx* c, const ctem_crystal* x) {
     cr->slut_lo[i] = slo[i]; cr->slut_dim[i] = sdim[i];
   }
   cr->sncell = sncell;
+  c->last_st = st;
   cr->lutmax = lutmax;
   cr->centro = centro;
   cr->mlambda = x->mlambda;
@@ -1754,7 +1801,7 @@ static int32_t plan_adjoint(ctem_plan* p, bool mixed, const double* gdots, doubl
   DevRefl R = p->cr->refl();
   DevLut L = p->cr->lutv(p->err.as<int32_t>());
   DevLut LdB = L;
-  LdB.slut = p->cr->dslut.as<double2>();
+  LdB.slut = p->cr->st->dslut.as<double2>();
   const double dzt = p->dz / p->thick;
   const int form = run_form(p, CTEM_PREC_MIXED);
   const bool split = mixed && form == 1;
