@@ -147,3 +147,45 @@ def test_argument_errors(ctx):
     assert L.ctem_plan_get_dots(h, P(dots)) == 0 and np.isfinite(dots).all()
     assert L.ctem_plan_destroy(h) == 0
     assert L.ctem_plan_run(None, 0, -1) == E_ARG
+    # a crystal whose site arrays are missing, or whose orbit is empty, is refused at upload
+    st = ctx.setup.struct()
+    st.site_z = None
+    assert L.ctem_upload_crystal(ctx._h, C.byref(st)) == E_ARG
+    assert b"null site arrays" in L.ctem_last_error()
+    st = ctx.setup.struct()
+    norb = np.zeros(max(1, st.nsite), np.int32)
+    st.site_norbit = P(norb)
+    assert L.ctem_upload_crystal(ctx._h, C.byref(st)) == E_ARG
+
+
+def test_site_tables_follow_the_crystal():
+    """A context that alternates between crystals (Fe, two-site corundum, Fe again) gives each plan
+    the structure-matrix table of its own sites: the dots equal those of a fresh context per
+    crystal, bit for bit (the table is shared only between uploads with equal sites and box)."""
+    from ctemsoft_amd import ebsd
+    names = ["fe_c1w", "al2o3_c2", "fe_c1w"]
+    shared = ebsd.Context(0)
+    plans, fxs = [], []
+    for name in names:
+        fx = _upload(shared, name)
+        n = min(16, len(fx["kn"]))
+        plans.append(shared.plan(fx["kvec"][:n], fx["kn"][:n], fx["lam"], float(fx["dz"]), float(fx["thick"])))
+        fxs.append(fx)
+    got = []
+    for p in plans:
+        p.run("fp64")
+        got.append(p.dots().copy())
+        p.close()
+    shared.close()
+    for name, fx, d in zip(names, fxs, got):
+        own = ebsd.Context(0)
+        _upload(own, name)
+        n = min(16, len(fx["kn"]))
+        p = own.plan(fx["kvec"][:n], fx["kn"][:n], fx["lam"], float(fx["dz"]), float(fx["thick"]))
+        p.run("fp64")
+        ref = p.dots().copy()
+        p.close()
+        own.close()
+        print(f"{name}: {n} directions, shared-context dots equal: {np.array_equal(d, ref)}", flush=True)
+        assert d.shape == ref.shape and np.array_equal(d, ref)
+    assert np.array_equal(got[0], got[2])
