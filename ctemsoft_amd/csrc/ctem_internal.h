@@ -369,7 +369,7 @@ void launch_adj_extract(hipStream_t s, const Bucket& B, const BeamView& V, const
 // glut: nrep copies of the 2 ncell gradient entries (zeroed by the caller); the copy a workgroup
 // adds into follows its index. launch_adj_lut_reduce leaves their sum in copy 0. nrep =
 // adj_lut_replicas(ncell): kLutReplicas, fewer when the copies would pass 2 GiB.
-constexpr int kLutReplicas = 64;
+constexpr int kLutReplicas = 256;
 int adj_lut_replicas(int ncell);
 void launch_adj_lut(hipStream_t s, const Bucket& B, const BeamView& V, const DevRefl& R, const DevLut& L,
                     double mlambda, const double2* Dbar, double* glut, int max_nw, int nrep);
